@@ -1,0 +1,121 @@
+// Internal graph handle: host copies of the loaded arrays plus their HBM
+// images and the per-query device workspaces.
+//
+// Replaces the reference's Graph (host vectors; src/sssp/base/Graph.h:16-51)
+// and DeviceMemory (cudaMalloc + H2D; src/sssp/base/DeviceMemory.cuh:22-99).
+//
+// HBM layout (all int32 unless stated):
+//   row   [V+1]        CSR offsets (the vlist file)
+//   adj   [E] int2     interleaved (dst, w) = (elist[e], wlist[e]): one 8-byte
+//                      load per edge instead of two 4-byte streams
+//   adjf  [E] int2     (dst, float bits of f) for PPR / adsorption
+//   hrow  [V+1], hadj [HE] int2   hidden lists (pruned variants)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <vector>
+#include <string>
+#include <memory>
+#include <mutex>
+#include "common.h"
+
+namespace c2g {
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess)                                                          \
+            return ::c2g::fail(C2G_EDEVICE, "%s failed: %s (%s:%d)", #expr,            \
+                               hipGetErrorString(_e), __FILE__, __LINE__);             \
+    } while (0)
+
+// Device workspace of single-query SSSP.
+struct SsspWs {
+    int32_t* dist = nullptr;     // [V]
+    int32_t* stamp = nullptr;    // [V] iteration for which v is queued
+    int4* q[2] = {nullptr, nullptr};  // frontier items (u, e_begin, e_end, flags)
+    int64_t qcap = 0;
+    int64_t* ctrl = nullptr;     // control block, see sssp.hip
+    int64_t* h_ctrl = nullptr;   // pinned mirror
+    uint8_t* hid = nullptr;      // [V] 1 = distance was set through a hidden edge
+    hipGraphExec_t exec = nullptr;
+    int exec_iters = 0, exec_block = 0, exec_grid = 0, exec_chunk = 0;
+};
+
+// Device workspace of batched SSSP (64 queries per group, lane = query).
+struct BatchWs {
+    int32_t* dist = nullptr;     // [V][64]
+    uint64_t* mask[2] = {nullptr, nullptr};  // [V] active lane masks cur/next
+    uint64_t* hmask = nullptr;   // [V] lanes whose value came through a hidden edge
+    int4* q[2] = {nullptr, nullptr};
+    int64_t qcap = 0;
+    int64_t* ctrl = nullptr;
+    int64_t* h_ctrl = nullptr;
+    int64_t* sums = nullptr;     // [64] checksum per lane
+    int32_t* reach = nullptr;    // [64]
+    hipGraphExec_t exec = nullptr;
+    int exec_iters = 0, exec_block = 0, exec_grid = 0, exec_chunk = 0;
+    int groups = 0;              // number of groups allocated
+};
+
+struct PprWs {
+    float* p = nullptr;
+    float* r = nullptr;
+    float* m = nullptr;
+    int32_t* stamp = nullptr;
+    int32_t* q[2] = {nullptr, nullptr};
+    int64_t* ctrl = nullptr;
+    int64_t* h_ctrl = nullptr;
+    int nq = 0;                  // queries allocated (batched: vertex-major [V][nq])
+};
+
+struct Graph {
+    int32_t V = 0;
+    int64_t E = 0, HE = 0;
+    int variant = C2G_VARIANT_ORIGIN;
+    std::vector<int32_t> vl, el, wl, hvl, hel, hwl;
+    std::vector<float> fl, hwf;
+
+    // device side
+    bool on_device = false;
+    int device = -1;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    int32_t* d_row = nullptr;
+    int2* d_adj = nullptr;
+    int2* d_adjf = nullptr;
+    int32_t* d_hrow = nullptr;
+    int2* d_hadj = nullptr;
+    int2* d_hadjf = nullptr;
+    SsspWs sssp;
+    BatchWs batch;
+    PprWs ppr;
+    c2g_tune tune{};
+    std::mutex mu;  // serialises device use of this handle (server vs caller)
+
+    bool has_w() const { return !wl.empty() || E == 0; }
+    bool has_f() const { return !fl.empty() || E == 0; }
+};
+
+int device_upload(Graph* g, int device);
+void device_release(Graph* g);
+void default_tune(Graph* g);
+int ensure_sssp_ws(Graph* g);
+int ensure_batch_ws(Graph* g);
+int ensure_ppr_ws(Graph* g, int nq);
+int64_t dev_read_i32(const std::string&, std::vector<int32_t>&);
+
+// Host-side helpers shared by lanes
+int host_bf(const Graph* g, int32_t* dist, const int32_t* frontier, int32_t count,
+            const uint8_t* via_hidden, c2g_stats* st);
+
+// Used by the server and batched callers.
+int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* frontier_out,
+             int32_t* count_out, uint8_t* hid_out, c2g_stats* st, bool copy_out);
+int sssp_batch_group(Graph* g, const int32_t* srcs, int32_t n, int32_t* dist_out,
+                     int64_t* sums, int32_t* reached, c2g_stats* st);
+
+}  // namespace c2g
+
+struct c2g_graph {
+    c2g::Graph g;
+};

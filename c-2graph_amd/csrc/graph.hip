@@ -1,0 +1,287 @@
+// Graph handle: dataset loading, validation and the one-off upload to HBM.
+//
+// Loader behaviour follows the reference's Graph constructors
+// (src/sssp/base/Graph.h:30-51, src/sssp/purn/Graph.h:33-48,
+// src/ppr/base/Graph.h:31-43, src/ppr/purn/Graph.h:34-46): V = len(vlist)-1,
+// E = len(elist).  Unlike them, every length is checked and a mismatch is an
+// error code (the reference trusts the files).
+#include "graph.h"
+#include <cstring>
+
+namespace c2g {
+
+int64_t read_vec_i32(const std::string& p, std::vector<int32_t>& v);
+int64_t read_vec_f32(const std::string& p, std::vector<float>& v);
+bool file_exists(const std::string& p);
+
+static int validate(Graph* g) {
+    const int32_t V = g->V;
+    if (V < 0) return fail(C2G_EFORMAT, "vlist is empty");
+    if ((int64_t)g->vl.size() != (int64_t)V + 1) return fail(C2G_EFORMAT, "vlist length");
+    if (g->vl[0] != 0) return fail(C2G_EFORMAT, "vlist[0] = %d, expected 0", g->vl[0]);
+    for (int32_t i = 0; i < V; ++i)
+        if (g->vl[i + 1] < g->vl[i]) return fail(C2G_EFORMAT, "vlist not monotone at %d", i);
+    if ((int64_t)g->vl[V] != (int64_t)g->el.size())
+        return fail(C2G_EFORMAT, "vlist[V] = %d but elist has %zu entries", g->vl[V], g->el.size());
+    g->E = (int64_t)g->el.size();
+    if (!g->wl.empty() && (int64_t)g->wl.size() != g->E)
+        return fail(C2G_EFORMAT, "wlist has %zu entries, elist %lld", g->wl.size(), (long long)g->E);
+    if (!g->fl.empty() && (int64_t)g->fl.size() != g->E)
+        return fail(C2G_EFORMAT, "flist has %zu entries, elist %lld", g->fl.size(), (long long)g->E);
+    for (int64_t e = 0; e < g->E; ++e)
+        if ((uint32_t)g->el[e] >= (uint32_t)V) return fail(C2G_EFORMAT, "elist[%lld] = %d out of range", (long long)e, g->el[e]);
+    for (int64_t e = 0; e < (int64_t)g->wl.size(); ++e)
+        if (g->wl[e] < 0) return fail(C2G_EFORMAT, "negative weight at %lld", (long long)e);
+    if (!g->hvl.empty()) {
+        if ((int64_t)g->hvl.size() != (int64_t)V + 1)
+            return fail(C2G_EFORMAT,
+                        "hidden vlist has %zu entries, expected V+1 = %lld (a compacted, "
+                        "prune-order hvlist is not supported; regenerate with purn-sssp of this library)",
+                        g->hvl.size(), (long long)V + 1);
+        if (g->hvl[0] != 0) return fail(C2G_EFORMAT, "hvlist[0] != 0");
+        for (int32_t i = 0; i < V; ++i)
+            if (g->hvl[i + 1] < g->hvl[i]) return fail(C2G_EFORMAT, "hvlist not monotone at %d", i);
+        g->HE = g->hvl[V];
+        if ((int64_t)g->hel.size() != g->HE) return fail(C2G_EFORMAT, "helist length %zu != hvlist[V] %lld", g->hel.size(), (long long)g->HE);
+        for (int64_t e = 0; e < g->HE; ++e)
+            if ((uint32_t)g->hel[e] >= (uint32_t)V) return fail(C2G_EFORMAT, "helist entry out of range");
+        if (!g->hwl.empty() && (int64_t)g->hwl.size() != g->HE) return fail(C2G_EFORMAT, "hwlist length");
+        if (!g->hwf.empty() && (int64_t)g->hwf.size() != g->HE) return fail(C2G_EFORMAT, "hwlist length");
+    }
+    return C2G_OK;
+}
+
+void default_tune(Graph* g) {
+    c2g_tune& t = g->tune;
+    memset(&t, 0, sizeof t);
+    t.block = 256;
+    t.grid_per_cu = 8;
+    t.chunk = 64;
+    t.batch_chunk = 32;
+    t.iters_per_graph = 8;
+    t.use_graph = 1;
+    t.delta = 0;
+}
+
+static int load_dir(Graph* g, const std::string& ds, int variant, const c2g_opts* o) {
+    std::string d;
+    int64_t n;
+    if (variant == C2G_VARIANT_ORIGIN) d = ds + "/origin/";
+    else if (variant == C2G_VARIANT_SSSP_PURN) d = ds + "/sssp/";
+    else if (variant == C2G_VARIANT_PPR_PURN) d = ds + "/ppr/";
+    else return fail(C2G_EINVAL, "unknown variant %d", variant);
+    if ((n = read_vec_i32(d + "vlist.bin", g->vl)) < 0) return (int)n;
+    if (n < 1) return fail(C2G_EFORMAT, "%svlist.bin is empty", d.c_str());
+    g->V = (int32_t)(n - 1);
+    if ((n = read_vec_i32(d + "elist.bin", g->el)) < 0) return (int)n;
+    if (variant == C2G_VARIANT_PPR_PURN) {
+        if ((n = read_vec_f32(d + "wlist.bin", g->fl)) < 0) return (int)n;
+    } else {
+        if (file_exists(d + "wlist.bin") && (n = read_vec_i32(d + "wlist.bin", g->wl)) < 0) return (int)n;
+        if (o && o->load_flist && variant == C2G_VARIANT_ORIGIN) {
+            if ((n = read_vec_f32(d + "flist.bin", g->fl)) < 0) return (int)n;
+        }
+    }
+    if (variant != C2G_VARIANT_ORIGIN) {
+        if ((n = read_vec_i32(d + "hvlist.bin", g->hvl)) < 0) return (int)n;
+        if ((n = read_vec_i32(d + "helist.bin", g->hel)) < 0) return (int)n;
+        if (variant == C2G_VARIANT_SSSP_PURN) {
+            if ((n = read_vec_i32(d + "hwlist.bin", g->hwl)) < 0) return (int)n;
+        } else {
+            if ((n = read_vec_f32(d + "hwlist.bin", g->hwf)) < 0) return (int)n;
+        }
+    }
+    g->variant = variant;
+    return validate(g);
+}
+
+static int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+int device_upload(Graph* g, int device) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail(C2G_EDEVICE, "no HIP device available");
+    if (device < 0) device = count - 1;  // the reference picks the last device (Main.cu:30)
+    if (device >= count) return fail(C2G_EINVAL, "device %d out of range (%d devices)", device, count);
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    g->device = device;
+    g->num_cus = prop.multiProcessorCount;
+    HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    const int64_t V = g->V, E = g->E;
+    HIP_TRY(hipMalloc(&g->d_row, sizeof(int32_t) * (V + 1)));
+    HIP_TRY(hipMemcpy(g->d_row, g->vl.data(), sizeof(int32_t) * (V + 1), hipMemcpyHostToDevice));
+    std::vector<int2> tmp;
+    auto pack = [&](const std::vector<int32_t>& e, const void* w, int64_t n, int2** dst) -> int {
+        HIP_TRY(hipMalloc(dst, sizeof(int2) * (size_t)std::max<int64_t>(n, 1)));
+        const int64_t step = 1 << 24;  // pack and copy in 128 MiB slices
+        tmp.resize((size_t)std::min<int64_t>(std::max<int64_t>(n, 1), step));
+        for (int64_t o = 0; o < n; o += step) {
+            int64_t m = std::min<int64_t>(step, n - o);
+            const int32_t* ww = (const int32_t*)w;
+            parallel_for(m, [&](int64_t lo, int64_t hi, int) {
+                for (int64_t i = lo; i < hi; ++i) tmp[i] = make_int2(e[o + i], ww[o + i]);
+            });
+            HIP_TRY(hipMemcpy(*dst + o, tmp.data(), sizeof(int2) * m, hipMemcpyHostToDevice));
+        }
+        return C2G_OK;
+    };
+    int rc;
+    // An edgeless graph still gets (1-element) edge arrays so every query runs.
+    if ((!g->wl.empty() || E == 0) && (rc = pack(g->el, g->wl.data(), E, &g->d_adj))) return rc;
+    if ((!g->fl.empty() || E == 0) && (rc = pack(g->el, g->fl.data(), E, &g->d_adjf))) return rc;
+    if (!g->hvl.empty()) {
+        HIP_TRY(hipMalloc(&g->d_hrow, sizeof(int32_t) * (V + 1)));
+        HIP_TRY(hipMemcpy(g->d_hrow, g->hvl.data(), sizeof(int32_t) * (V + 1), hipMemcpyHostToDevice));
+        if (!g->hwl.empty() && (rc = pack(g->hel, g->hwl.data(), g->HE, &g->d_hadj))) return rc;
+        if (!g->hwf.empty() && (rc = pack(g->hel, g->hwf.data(), g->HE, &g->d_hadjf))) return rc;
+    }
+    (void)align_up;
+    g->on_device = true;
+    return C2G_OK;
+}
+
+template <class T>
+static void dfree(T*& p) {
+    if (p) { (void)hipFree((void*)p); p = nullptr; }
+}
+
+void device_release(Graph* g) {
+    if (!g->on_device) return;
+    (void)hipSetDevice(g->device);
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    if (g->sssp.exec) (void)hipGraphExecDestroy(g->sssp.exec);
+    if (g->batch.exec) (void)hipGraphExecDestroy(g->batch.exec);
+    dfree(g->sssp.dist); dfree(g->sssp.stamp); dfree(g->sssp.q[0]); dfree(g->sssp.q[1]);
+    dfree(g->sssp.ctrl); dfree(g->sssp.hid);
+    if (g->sssp.h_ctrl) (void)hipHostFree(g->sssp.h_ctrl);
+    dfree(g->batch.dist); dfree(g->batch.mask[0]); dfree(g->batch.mask[1]); dfree(g->batch.hmask);
+    dfree(g->batch.q[0]); dfree(g->batch.q[1]); dfree(g->batch.ctrl); dfree(g->batch.sums);
+    dfree(g->batch.reach);
+    if (g->batch.h_ctrl) (void)hipHostFree(g->batch.h_ctrl);
+    dfree(g->ppr.p); dfree(g->ppr.r); dfree(g->ppr.m); dfree(g->ppr.stamp);
+    dfree(g->ppr.q[0]); dfree(g->ppr.q[1]); dfree(g->ppr.ctrl);
+    if (g->ppr.h_ctrl) (void)hipHostFree(g->ppr.h_ctrl);
+    dfree(g->d_row); dfree(g->d_adj); dfree(g->d_adjf); dfree(g->d_hrow); dfree(g->d_hadj);
+    dfree(g->d_hadjf);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    g->stream = nullptr;
+    g->on_device = false;
+}
+
+static int finish_open(c2g_graph* h, const c2g_opts* o, c2g_graph** out) {
+    default_tune(&h->g);
+    if (!(o && o->no_device)) {
+        int rc = device_upload(&h->g, o ? o->device : -1);
+        if (rc) { device_release(&h->g); delete h; return rc; }
+    }
+    if (o && o->verbose) {
+        printf("INIT--class Graph is constructed\n");
+        printf("The vertex num is:%d\n", h->g.V);
+        printf("The edge num is:%lld\n", (long long)h->g.E);
+    }
+    *out = h;
+    return C2G_OK;
+}
+
+}  // namespace c2g
+
+using namespace c2g;
+
+extern "C" {
+
+int c2g_device_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 0;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, 0) != hipSuccess) return 0;
+    return strncmp(p.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+int c2g_open(const char* ds, int32_t variant, const c2g_opts* opts, c2g_graph** out) {
+    if (!ds || !out) return fail(C2G_EINVAL, "c2g_open: null argument");
+    c2g_graph* h = new (std::nothrow) c2g_graph;
+    if (!h) return fail(C2G_ENOMEM, "c2g_open: out of memory");
+    int rc = load_dir(&h->g, ds, variant, opts);
+    if (rc) { delete h; return rc; }
+    return finish_open(h, opts, out);
+}
+
+int c2g_graph_from_csr(int32_t V, const int32_t* vlist, const int32_t* elist, const int32_t* wlist,
+                       const float* flist, const int32_t* hvlist, const int32_t* helist,
+                       const int32_t* hwlist, const float* hwf, int32_t variant,
+                       const c2g_opts* opts, c2g_graph** out) {
+    if (V < 0 || !vlist || !out) return fail(C2G_EINVAL, "c2g_graph_from_csr: bad arguments");
+    int64_t E = vlist[V];
+    if (E > 0 && !elist) return fail(C2G_EINVAL, "c2g_graph_from_csr: null elist");
+    c2g_graph* h = new (std::nothrow) c2g_graph;
+    if (!h) return fail(C2G_ENOMEM, "out of memory");
+    Graph& g = h->g;
+    g.V = V;
+    g.variant = variant;
+    g.vl.assign(vlist, vlist + V + 1);
+    g.el.assign(elist, elist + E);
+    if (wlist) g.wl.assign(wlist, wlist + E);
+    if (flist) g.fl.assign(flist, flist + E);
+    if (hvlist) {
+        int64_t HE = hvlist[V];
+        g.hvl.assign(hvlist, hvlist + V + 1);
+        if (HE > 0 && !helist) { delete h; return fail(C2G_EINVAL, "hidden lists need helist"); }
+        g.hel.assign(helist, helist + HE);
+        if (hwlist) g.hwl.assign(hwlist, hwlist + HE);
+        if (hwf) g.hwf.assign(hwf, hwf + HE);
+    }
+    int rc = validate(&g);
+    if (rc) { delete h; return rc; }
+    return finish_open(h, opts, out);
+}
+
+int c2g_close(c2g_graph* h) {
+    if (!h) return C2G_OK;
+    device_release(&h->g);
+    delete h;
+    return C2G_OK;
+}
+
+int c2g_graph_info(const c2g_graph* h, int32_t* V, int64_t* E, int64_t* HE, int32_t* variant) {
+    if (!h) return fail(C2G_EINVAL, "null graph");
+    if (V) *V = h->g.V;
+    if (E) *E = h->g.E;
+    if (HE) *HE = h->g.HE;
+    if (variant) *variant = h->g.variant;
+    return C2G_OK;
+}
+
+int c2g_graph_host_csr(const c2g_graph* h, const int32_t** vl, const int32_t** el,
+                       const int32_t** wl, const float** fl) {
+    if (!h) return fail(C2G_EINVAL, "null graph");
+    if (vl) *vl = h->g.vl.data();
+    if (el) *el = h->g.el.data();
+    if (wl) *wl = h->g.wl.empty() ? nullptr : h->g.wl.data();
+    if (fl) *fl = h->g.fl.empty() ? nullptr : h->g.fl.data();
+    return C2G_OK;
+}
+
+int c2g_set_tune(c2g_graph* h, const c2g_tune* t) {
+    if (!h || !t) return fail(C2G_EINVAL, "null argument");
+    if (t->block != 64 && t->block != 128 && t->block != 256 && t->block != 512 && t->block != 1024)
+        return fail(C2G_EINVAL, "block must be one of 64..1024 (power of two)");
+    if (t->grid_per_cu < 1 || t->grid_per_cu > 64) return fail(C2G_EINVAL, "grid_per_cu in [1,64]");
+    if (t->chunk < 1 || t->chunk > 4096) return fail(C2G_EINVAL, "chunk in [1,4096]");
+    if (t->batch_chunk < 1 || t->batch_chunk > 4096) return fail(C2G_EINVAL, "batch_chunk in [1,4096]");
+    if (t->iters_per_graph < 2 || t->iters_per_graph > 256 || (t->iters_per_graph & 1))
+        return fail(C2G_EINVAL, "iters_per_graph must be even, in [2,256]");
+    if (t->delta < 0) return fail(C2G_EINVAL, "delta >= 0");
+    std::lock_guard<std::mutex> lk(h->g.mu);
+    h->g.tune = *t;
+    return C2G_OK;
+}
+
+int c2g_get_tune(const c2g_graph* h, c2g_tune* t) {
+    if (!h || !t) return fail(C2G_EINVAL, "null argument");
+    *t = h->g.tune;
+    return C2G_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,502 @@
+// Single-source SSSP on gfx950: frontier relaxation with int32 atomicMin.
+//
+// Replaces CalcuSSSP (src/sssp/base/Main.cu:109-356) and, for pruned graphs,
+// CalcuSSSP + host seeding (src/sssp/purn/Main.cu:67-90, 136-193).
+//
+// Reference behaviour kept: label-correcting relaxation `cand = dis[u] + w;
+// if dis[v] > cand: atomicMin(dis[v], cand), v active` with dis[u] re-read at
+// relax time (Main.cu:188, 217-221), frontier = vertices improved in the
+// previous round, stop when the frontier is empty.  Changed on purpose: no
+// 1000-round cap (Main.cu:168), INT32_MAX as the unreached value (Main.cu:121
+// uses 1999999999), no software grid barrier (Main.cu:143-147, unsafe without
+// co-residency), no O(V) isactive rescan per round (Main.cu:313-340).
+//
+// Layout of one round (one kernel launch, `sssp_relax`):
+//   * The frontier is a queue of work items int4(u, e_begin, e_end, kind),
+//     kind 0 = live CSR edges, 1 = hidden-list edges.  A vertex of out-degree
+//     d gives ceil(d/chunk) items, so hubs are split across waves when they
+//     are queued (the reference's block/warp/thread tiers, Main.cu:197-293,
+//     become one balanced path).
+//   * A wave takes 64 items, scans their edge counts (wave64 shuffle scan),
+//     and walks the concatenated edge list 64 edges at a time; lane k finds
+//     its owner item with a 6-step shuffle search.  Edge (dst,w) pairs are one
+//     8-byte load from the interleaved `adj` array.
+//   * An improving lane claims v for the next round by exchanging v's stamp
+//     to the next round number (dedup without clearing any array); claimed
+//     vertices are appended with one atomicAdd per wave.
+//   * The last workgroup to finish publishes the next round's item count, so
+//     rounds are plain back-to-back launches (captured in a hipGraph).
+//
+// Pruned graphs: a vertex's hidden list is relaxed when the vertex is the
+// source or when its distance was ever lowered through a hidden edge
+// (flag `hid`).  Hidden edges stand for real paths, so extra relaxation can
+// never lower a distance below the true one.
+#include "graph.h"
+#include "wave.h"
+#include <cstring>
+
+namespace c2g {
+
+enum : int {
+    C_CUR = 0,    // items in the current round's queue
+    C_NEXT = 1,   // items appended for the next round
+    C_ITER = 2,   // rounds completed
+    C_DONE = 3,   // workgroups finished in this launch
+    C_FRONT = 4,  // sum of frontier vertices over rounds
+    C_EDGES = 5,  // edges queued for relaxation (R)
+    C_IMPR = 6,   // improving relaxations (U)
+    C_OVF = 7,    // queue overflow guard tripped (must stay 0)
+    C_HEDGES = 8, // hidden edges queued
+    C_N = 16
+};
+
+__global__ void sssp_init_k(int32_t* __restrict__ dist, int32_t* __restrict__ stamp,
+                            int32_t* __restrict__ hstamp, uint8_t* __restrict__ hid, int64_t V) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < V;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        dist[i] = INT32_MAX;
+        stamp[i] = -1;
+        if (hstamp) hstamp[i] = -1;
+        if (hid) hid[i] = 0;
+    }
+}
+
+// Queues the source: live items and (pruned graphs) hidden items.
+__global__ void sssp_seed_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
+                            int32_t* __restrict__ dist, int32_t* __restrict__ stamp,
+                            int32_t* __restrict__ hstamp, uint8_t* __restrict__ hid,
+                            int4* __restrict__ q, int64_t* __restrict__ ctrl, int32_t src,
+                            int chunk) {
+    const int r0 = row[src], r1 = row[src + 1];
+    const int nl = (r1 - r0 + chunk - 1) / chunk;
+    int h0 = 0, h1 = 0, nh = 0;
+    if (hrow) { h0 = hrow[src]; h1 = hrow[src + 1]; nh = (h1 - h0 + chunk - 1) / chunk; }
+    for (int j = threadIdx.x; j < nl + nh; j += blockDim.x) {
+        if (j < nl) {
+            int b = r0 + j * chunk;
+            q[j] = make_int4(src, b, min(b + chunk, r1), 0);
+        } else {
+            int b = h0 + (j - nl) * chunk;
+            q[j] = make_int4(src, b, min(b + chunk, h1), 1);
+        }
+    }
+    if (threadIdx.x == 0) {
+        dist[src] = 0;
+        stamp[src] = 0;
+        if (hstamp) hstamp[src] = 0;
+        if (hid) hid[src] = 1;  // the source's hidden list is always relaxed
+        for (int i = 0; i < C_N; ++i) ctrl[i] = 0;
+        ctrl[C_CUR] = nl + nh;
+        ctrl[C_FRONT] = 1;
+        ctrl[C_EDGES] = r1 - r0;
+        ctrl[C_HEDGES] = h1 - h0;
+    }
+}
+
+template <bool PRUNED>
+__global__ void __launch_bounds__(256)
+sssp_relax_k(const int32_t* __restrict__ row, const int2* __restrict__ adj,
+             const int32_t* __restrict__ hrow, const int2* __restrict__ hadj,
+             int32_t* __restrict__ dist, int32_t* __restrict__ stamp,
+             int32_t* __restrict__ hstamp, uint8_t* __restrict__ hid,
+             const int4* __restrict__ qin, int4* __restrict__ qout,
+             int64_t* __restrict__ ctrl, int chunk, int64_t qcap) {
+    const int64_t n = ctrl[C_CUR];
+    const int next_it = (int)ctrl[C_ITER] + 1;
+    const int lane = lane_id();
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    int64_t impr = 0, front = 0, qedges = 0, qhedges = 0;
+
+    for (int64_t base = wave * kWave; base < n; base += nwaves * kWave) {
+        const int64_t i = base + lane;
+        int u = 0, eb = 0, cnt = 0, du = 0, kind = 0;
+        if (i < n) {
+            int4 it = qin[i];
+            u = it.x;
+            eb = it.y;
+            cnt = it.z - it.y;
+            kind = it.w;
+            du = dist[u];
+        }
+        int tot;
+        const int pre = wave_excl_scan(cnt, &tot);
+        for (int s = 0; s < tot; s += kWave) {
+            const int k = s + lane;
+            const int o = wave_owner(pre, k);
+            const int e = __shfl(eb, o, 64) + (k - __shfl(pre, o, 64));
+            const int d_u = __shfl(du, o, 64);
+            const int okind = PRUNED ? __shfl(kind, o, 64) : 0;
+            bool claim_live = false, claim_hid = false;
+            int v = 0;
+            if (k < tot) {
+                const int2 a = (PRUNED && okind) ? hadj[e] : adj[e];
+                v = a.x;
+                const int w = a.y;
+                if (d_u <= INT32_MAX - w) {
+                    const int cand = d_u + w;
+                    if (cand < dist[v]) {
+                        const int old = atomicMin(&dist[v], cand);
+                        if (cand < old) {
+                            ++impr;
+                            claim_live = atomicExch(&stamp[v], next_it) != next_it;
+                            if (PRUNED) {
+                                if (okind) hid[v] = 1;
+                                if (okind || hid[v])
+                                    claim_hid = atomicExch(&hstamp[v], next_it) != next_it;
+                            }
+                        }
+                    }
+                }
+            }
+            // Append claimed vertices as chunked items (wave-aggregated).
+            if (__ballot(claim_live || claim_hid)) {
+                int r0 = 0, r1 = 0, h0 = 0, h1 = 0, nl = 0, nh = 0;
+                if (claim_live) {
+                    r0 = row[v];
+                    r1 = row[v + 1];
+                    nl = (r1 - r0 + chunk - 1) / chunk;
+                    front += 1;
+                    qedges += r1 - r0;
+                }
+                if (PRUNED && claim_hid) {
+                    h0 = hrow[v];
+                    h1 = hrow[v + 1];
+                    nh = (h1 - h0 + chunk - 1) / chunk;
+                    qhedges += h1 - h0;
+                }
+                int ntot;
+                const int off = wave_excl_scan(nl + nh, &ntot);
+                if (ntot > 0) {
+                    int64_t qb = 0;
+                    if (lane == 0)
+                        qb = (int64_t)atomicAdd((unsigned long long*)&ctrl[C_NEXT],
+                                                (unsigned long long)ntot);
+                    qb = shfl64(qb, 0);
+                    if (qb + ntot > qcap) {
+                        if (lane == 0) atomicOr((unsigned long long*)&ctrl[C_OVF], 1ull);
+                    } else {
+                        int4* dst = qout + qb + off;
+                        for (int j = 0; j < nl; ++j) {
+                            int b = r0 + j * chunk;
+                            dst[j] = make_int4(v, b, min(b + chunk, r1), 0);
+                        }
+                        for (int j = 0; j < nh; ++j) {
+                            int b = h0 + j * chunk;
+                            dst[nl + j] = make_int4(v, b, min(b + chunk, h1), 1);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    // Statistics: one atomic per wave.
+    impr = wave_sum64(impr);
+    front = wave_sum64(front);
+    qedges = wave_sum64(qedges);
+    if (PRUNED) qhedges = wave_sum64(qhedges);
+    if (lane == 0 && (impr | front | qedges | qhedges)) {
+        atomicAdd((unsigned long long*)&ctrl[C_IMPR], (unsigned long long)impr);
+        atomicAdd((unsigned long long*)&ctrl[C_FRONT], (unsigned long long)front);
+        atomicAdd((unsigned long long*)&ctrl[C_EDGES], (unsigned long long)qedges);
+        if (PRUNED) atomicAdd((unsigned long long*)&ctrl[C_HEDGES], (unsigned long long)qhedges);
+    }
+    // The last workgroup to finish turns the next queue into the current one.
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[C_DONE], 1ull);
+        if (prev == (unsigned long long)gridDim.x - 1) {
+            __threadfence();
+            int64_t nn = (int64_t)atomicAdd((unsigned long long*)&ctrl[C_NEXT], 0ull);
+            if (n > 0) ctrl[C_ITER] = next_it;
+            if (nn > qcap) {  // items past the queue were dropped: stop, report
+                ctrl[C_OVF] = 1;
+                nn = 0;
+            }
+            ctrl[C_CUR] = nn;
+            ctrl[C_NEXT] = 0;
+            ctrl[C_DONE] = 0;
+        }
+    }
+}
+
+// Checksum of a distance vector: sum of finite values and count of them.
+// Checksum of a distance vector (element stride `stride`): sum of finite
+// values, count of them, and the out-degree sum of reached vertices (m_s).
+__global__ void dist_checksum_k(const int32_t* __restrict__ row, const int32_t* __restrict__ dist,
+                                int64_t V, int64_t stride, int64_t* __restrict__ sum,
+                                int32_t* __restrict__ cnt, int64_t* __restrict__ medges) {
+    int64_t s = 0, c = 0, m = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < V;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t d = dist[i * stride];
+        if (d != INT32_MAX) { s += d; c += 1; m += row[i + 1] - row[i]; }
+    }
+    s = wave_sum64(s);
+    c = wave_sum64(c);
+    m = wave_sum64(m);
+    if (lane_id() == 0 && (s | c)) {
+        atomicAdd((unsigned long long*)sum, (unsigned long long)s);
+        atomicAdd((int*)cnt, (int)c);
+        atomicAdd((unsigned long long*)medges, (unsigned long long)m);
+    }
+}
+
+// ------------------------------------------------------------------ host
+int ensure_sssp_ws(Graph* g) {
+    SsspWs& w = g->sssp;
+    const int64_t V = g->V;
+    const int chunk = std::max(1, g->tune.chunk);
+    // Each vertex is claimed at most once per round for its live list and once
+    // for its hidden list, giving at most ceil(d/chunk) <= d/chunk + 1 items each.
+    const int64_t need = 2 * V + (g->E + g->HE) / chunk + 64;
+    HIP_TRY(hipSetDevice(g->device));
+    if (!w.dist) {
+        HIP_TRY(hipMalloc(&w.dist, sizeof(int32_t) * std::max<int64_t>(V, 1)));
+        HIP_TRY(hipMalloc(&w.stamp, sizeof(int32_t) * std::max<int64_t>(V, 1) * 2));
+        HIP_TRY(hipMalloc(&w.ctrl, sizeof(int64_t) * C_N));
+        HIP_TRY(hipHostMalloc((void**)&w.h_ctrl, sizeof(int64_t) * C_N, hipHostMallocDefault));
+        if (g->d_hrow) HIP_TRY(hipMalloc(&w.hid, std::max<int64_t>(V, 1)));
+    }
+    if (w.qcap < need) {
+        if (w.exec) { (void)hipGraphExecDestroy(w.exec); w.exec = nullptr; }
+        if (w.q[0]) { (void)hipFree(w.q[0]); (void)hipFree(w.q[1]); w.q[0] = w.q[1] = nullptr; }
+        HIP_TRY(hipMalloc(&w.q[0], sizeof(int4) * need));
+        HIP_TRY(hipMalloc(&w.q[1], sizeof(int4) * need));
+        w.qcap = need;
+    }
+    return C2G_OK;
+}
+
+static int relax_grid(const Graph* g) {
+    return std::max(1, g->num_cus) * g->tune.grid_per_cu;
+}
+
+// Launches rounds [0, k) of relax kernels starting with queue `par`.
+static int launch_rounds(Graph* g, int k, int par) {
+    SsspWs& w = g->sssp;
+    const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
+    const int block = 256;  // relax kernels are compiled for 256 threads
+    const int grid = relax_grid(g);
+    int32_t* hstamp = pruned ? w.stamp + g->V : nullptr;
+    for (int r = 0; r < k; ++r) {
+        const int4* qin = w.q[(par + r) & 1];
+        int4* qout = w.q[(par + r + 1) & 1];
+        if (pruned)
+            hipLaunchKernelGGL(sssp_relax_k<true>, dim3(grid), dim3(block), 0, g->stream, g->d_row,
+                               g->d_adj, g->d_hrow, g->d_hadj, w.dist, w.stamp, hstamp, w.hid, qin,
+                               qout, w.ctrl, g->tune.chunk, w.qcap);
+        else
+            hipLaunchKernelGGL(sssp_relax_k<false>, dim3(grid), dim3(block), 0, g->stream, g->d_row,
+                               g->d_adj, (const int32_t*)nullptr, (const int2*)nullptr, w.dist,
+                               w.stamp, (int32_t*)nullptr, (uint8_t*)nullptr, qin, qout, w.ctrl,
+                               g->tune.chunk, w.qcap);
+    }
+    HIP_TRY(hipGetLastError());
+    return C2G_OK;
+}
+
+// A hipGraph holding `iters_per_graph` rounds (even, so the queue parity is
+// the same before and after a replay).
+static int ensure_exec(Graph* g) {
+    SsspWs& w = g->sssp;
+    const int K = g->tune.iters_per_graph;
+    if (w.exec && w.exec_iters == K && w.exec_grid == relax_grid(g) && w.exec_chunk == g->tune.chunk)
+        return C2G_OK;
+    if (w.exec) { (void)hipGraphExecDestroy(w.exec); w.exec = nullptr; }
+    hipGraph_t graph;
+    HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_rounds(g, K, 0);
+    hipError_t e = hipStreamEndCapture(g->stream, &graph);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(C2G_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&w.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(C2G_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    w.exec_iters = K;
+    w.exec_grid = relax_grid(g);
+    w.exec_chunk = g->tune.chunk;
+    return C2G_OK;
+}
+
+static void fill_stats(c2g_stats* st, const int64_t* c, double ms, int lane) {
+    if (!st) return;
+    st->lane = lane;
+    st->iterations = (int32_t)c[C_ITER];
+    st->frontier_total = c[C_FRONT];
+    st->edges_relaxed = c[C_EDGES] + c[C_HEDGES];
+    st->improvements = c[C_IMPR];
+    st->ms = ms;
+}
+
+static int64_t reached_edges(const Graph* g, const int32_t* dist) {
+    int64_t m = 0;
+    for (int32_t v = 0; v < g->V; ++v)
+        if (dist[v] != INT32_MAX) m += g->vl[v + 1] - g->vl[v];
+    return m;
+}
+
+// Runs one query on the device.  budget > 0 stops after `budget` rounds and
+// leaves the remaining frontier in frontier_out (vertices, deduplicated).
+int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* frontier_out,
+             int32_t* count_out, uint8_t* hid_out, c2g_stats* st, bool copy_out) {
+    if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");
+    if (!g->d_adj) return fail(C2G_ENOTSUP, "graph has no int weights (wlist) for SSSP");
+    if (src < 0 || src >= g->V) return fail(C2G_EINVAL, "source %d out of range [0,%d)", src, g->V);
+    int rc = ensure_sssp_ws(g);
+    if (rc) return rc;
+    SsspWs& w = g->sssp;
+    const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
+    const double t0 = now_ms();
+    const int tb = 256, gridv = (int)std::min<int64_t>((g->V + tb - 1) / tb, 65535);
+    hipLaunchKernelGGL(sssp_init_k, dim3(std::max(gridv, 1)), dim3(tb), 0, g->stream, w.dist,
+                       w.stamp, pruned ? w.stamp + g->V : (int32_t*)nullptr,
+                       pruned ? w.hid : (uint8_t*)nullptr, (int64_t)g->V);
+    hipLaunchKernelGGL(sssp_seed_k, dim3(1), dim3(256), 0, g->stream, g->d_row,
+                       pruned ? g->d_hrow : (const int32_t*)nullptr, w.dist, w.stamp,
+                       pruned ? w.stamp + g->V : (int32_t*)nullptr,
+                       pruned ? w.hid : (uint8_t*)nullptr, w.q[0], w.ctrl, src, g->tune.chunk);
+    HIP_TRY(hipGetLastError());
+    const bool use_graph = g->tune.use_graph && budget <= 0;
+    if (use_graph && (rc = ensure_exec(g))) return rc;
+    int par = 0, done_rounds = 0;
+    for (;;) {
+        int k = g->tune.iters_per_graph;
+        if (budget > 0) k = std::min(k, budget - done_rounds);
+        if (k <= 0) break;
+        if (use_graph) HIP_TRY(hipGraphLaunch(w.exec, g->stream));
+        else if ((rc = launch_rounds(g, k, par))) return rc;
+        par = (par + k) & 1;
+        done_rounds += k;
+        HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, sizeof(int64_t) * C_N, hipMemcpyDeviceToHost,
+                               g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        if (w.h_ctrl[C_OVF]) return fail(C2G_EDEVICE, "frontier queue overflow (internal)");
+        if (w.h_ctrl[C_CUR] == 0) break;
+    }
+    const double t1 = now_ms();
+    int64_t c[C_N];
+    memcpy(c, w.h_ctrl, sizeof c);
+    if (copy_out && dist_out)
+        HIP_TRY(hipMemcpy(dist_out, w.dist, sizeof(int32_t) * g->V, hipMemcpyDeviceToHost));
+    if (count_out) {
+        // Remaining frontier (budgeted runs): distinct vertices of the current queue.
+        int64_t n = c[C_CUR];
+        std::vector<int4> items((size_t)n);
+        if (n) HIP_TRY(hipMemcpy(items.data(), w.q[par], sizeof(int4) * n, hipMemcpyDeviceToHost));
+        int32_t m = 0;
+        std::vector<uint8_t> seen((size_t)g->V, 0);
+        for (auto& it : items)
+            if (!seen[it.x]) { seen[it.x] = 1; if (frontier_out) frontier_out[m] = it.x; ++m; }
+        *count_out = m;
+        if (hid_out && pruned)
+            HIP_TRY(hipMemcpy(hid_out, w.hid, g->V, hipMemcpyDeviceToHost));
+    }
+    fill_stats(st, c, t1 - t0, C2G_LANE_GPU);
+    if (st && copy_out && dist_out) {
+        int32_t r = 0;
+        for (int32_t v = 0; v < g->V; ++v) r += dist_out[v] != INT32_MAX;
+        st->reached = r;
+        st->reached_edges = reached_edges(g, dist_out);
+    }
+    return C2G_OK;
+}
+
+}  // namespace c2g
+
+using namespace c2g;
+
+extern "C" {
+
+int c2g_sssp(c2g_graph* h, int32_t src, int32_t* dist_out, c2g_stats* st) {
+    if (!h || !dist_out) return fail(C2G_EINVAL, "c2g_sssp: null argument");
+    std::lock_guard<std::mutex> lk(h->g.mu);
+    if (st) memset(st, 0, sizeof *st);
+    return sssp_run(&h->g, src, 0, dist_out, nullptr, nullptr, nullptr, st, true);
+}
+
+int c2g_sssp_budget(c2g_graph* h, int32_t src, int32_t budget, int32_t* dist_out,
+                    int32_t* frontier_out, int32_t* count_out, c2g_stats* st) {
+    if (!h || !dist_out || !frontier_out || !count_out) return fail(C2G_EINVAL, "null argument");
+    if (budget < 1) return fail(C2G_EINVAL, "budget must be >= 1");
+    std::lock_guard<std::mutex> lk(h->g.mu);
+    if (st) memset(st, 0, sizeof *st);
+    return sssp_run(&h->g, src, budget, dist_out, frontier_out, count_out, nullptr, st, true);
+}
+
+int c2g_sssp_migrate(c2g_graph* h, int32_t src, int32_t budget, int32_t* dist_out, c2g_stats* st) {
+    if (!h || !dist_out) return fail(C2G_EINVAL, "null argument");
+    if (budget < 1) return fail(C2G_EINVAL, "budget must be >= 1");
+    std::lock_guard<std::mutex> lk(h->g.mu);
+    Graph* g = &h->g;
+    std::vector<int32_t> fr((size_t)g->V);
+    std::vector<uint8_t> hid((size_t)g->V, 0);
+    int32_t cnt = 0;
+    c2g_stats gs{};
+    int rc = sssp_run(g, src, budget, dist_out, fr.data(), &cnt, hid.data(), &gs, true);
+    if (rc) return rc;
+    c2g_stats hs{};
+    rc = host_bf(g, dist_out, fr.data(), cnt, g->hvl.empty() ? nullptr : hid.data(), &hs);
+    if (rc) return rc;
+    if (st) {
+        *st = gs;
+        st->lane = cnt ? C2G_LANE_MIXED : C2G_LANE_GPU;
+        st->host_iterations = hs.host_iterations;
+        st->edges_relaxed += hs.edges_relaxed;
+        st->improvements += hs.improvements;
+        int32_t r = 0;
+        for (int32_t v = 0; v < g->V; ++v) r += dist_out[v] != INT32_MAX;
+        st->reached = r;
+        st->reached_edges = reached_edges(g, dist_out);
+    }
+    return C2G_OK;
+}
+
+int c2g_sssp_each_resident(c2g_graph* h, const int32_t* srcs, int32_t n, int64_t* sums,
+                           int32_t* reached, c2g_stats* st) {
+    if (!h || !srcs || n < 0) return fail(C2G_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(h->g.mu);
+    Graph* g = &h->g;
+    if (st) memset(st, 0, sizeof *st);
+    if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");
+    int64_t* d_sum = nullptr;
+    int32_t* d_cnt = nullptr;
+    int64_t* d_m = nullptr;
+    HIP_TRY(hipSetDevice(g->device));
+    HIP_TRY(hipMalloc(&d_sum, sizeof(int64_t) * (std::max(n, 1) + 1)));
+    HIP_TRY(hipMalloc(&d_cnt, sizeof(int32_t) * std::max(n, 1)));
+    d_m = d_sum + std::max(n, 1);
+    HIP_TRY(hipMemsetAsync(d_sum, 0, sizeof(int64_t) * (std::max(n, 1) + 1), g->stream));
+    HIP_TRY(hipMemsetAsync(d_cnt, 0, sizeof(int32_t) * std::max(n, 1), g->stream));
+    const double t0 = now_ms();
+    int rc = C2G_OK;
+    for (int32_t q = 0; q < n && rc == C2G_OK; ++q) {
+        c2g_stats s{};
+        rc = sssp_run(g, srcs[q], 0, nullptr, nullptr, nullptr, nullptr, &s, false);
+        if (rc) break;
+        hipLaunchKernelGGL(dist_checksum_k, dim3(1024), dim3(256), 0, g->stream, g->d_row,
+                           g->sssp.dist, (int64_t)g->V, (int64_t)1, d_sum + q, d_cnt + q, d_m);
+        if (st) {
+            st->iterations += s.iterations;
+            st->frontier_total += s.frontier_total;
+            st->edges_relaxed += s.edges_relaxed;
+            st->improvements += s.improvements;
+        }
+    }
+    if (rc == C2G_OK) {
+        if (hipStreamSynchronize(g->stream) != hipSuccess) rc = fail(C2G_EDEVICE, "sync failed");
+    }
+    const double t1 = now_ms();
+    if (rc == C2G_OK && sums) (void)hipMemcpy(sums, d_sum, sizeof(int64_t) * n, hipMemcpyDeviceToHost);
+    if (rc == C2G_OK && reached) (void)hipMemcpy(reached, d_cnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost);
+    int64_t medges = 0;
+    if (rc == C2G_OK) (void)hipMemcpy(&medges, d_m, sizeof(int64_t), hipMemcpyDeviceToHost);
+    if (st) st->reached_edges = medges;
+    (void)hipFree(d_sum);
+    (void)hipFree(d_cnt);
+    if (st) { st->lane = C2G_LANE_GPU; st->ms = t1 - t0; }
+    return rc;
+}
+
+}  // extern "C"

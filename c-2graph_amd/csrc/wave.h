@@ -1,0 +1,62 @@
+// Wave64 primitives for gfx950: replace cub::BlockScan<int,512>::ExclusiveSum
+// and __any_sync / __syncthreads_or of the reference kernels
+// (src/sssp/base/Main.cu:197, 232, 265, 329).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace c2g {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
+
+// Exclusive prefix sum over the 64 lanes of a wave; *total gets the wave sum.
+// Every lane of the wave must call it (uniform control flow).
+__device__ __forceinline__ int wave_excl_scan(int x, int* total) {
+    const int lane = lane_id();
+    int v = x;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    *total = __shfl(v, 63, 64);
+    return v - x;
+}
+
+__device__ __forceinline__ int64_t wave_sum64(int64_t x) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        int lo = __shfl_xor((int)(uint32_t)x, d, 64);
+        int hi = __shfl_xor((int)(uint32_t)((uint64_t)x >> 32), d, 64);
+        x += (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+    }
+    return x;
+}
+
+__device__ __forceinline__ int64_t shfl64(int64_t x, int src) {
+    int lo = __shfl((int)(uint32_t)x, src, 64);
+    int hi = __shfl((int)(uint32_t)((uint64_t)x >> 32), src, 64);
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// Number of set bits of `mask` below this lane (ballot prefix).
+__device__ __forceinline__ int lanes_below(uint64_t mask) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Largest j in [0,64) with pre_j <= k, where pre is this wave's non-decreasing
+// exclusive prefix held one value per lane (six shuffles).
+__device__ __forceinline__ int wave_owner(int pre, int k) {
+    int o = 0;
+#pragma unroll
+    for (int st = 32; st > 0; st >>= 1) {
+        int pj = __shfl(pre, o + st, 64);
+        if (pj <= k) o += st;
+    }
+    return o;
+}
+
+}  // namespace c2g

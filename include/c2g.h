@@ -1,0 +1,337 @@
+/*
+ * c2g.h — C-ABI of the MI355X (gfx950) weighted-traversal library.
+ *
+ * The reference (lihuaibei7951/C-2graph) has no library interface: its
+ * boundary is the dataset directory, the CLI mains, and inside each main a
+ * `Graph` loader, a `DeviceMemory` owner and one persistent kernel launch per
+ * query.  Every entry point below replaces one of those pieces and cites it.
+ *
+ * Conventions
+ *  - Every function returns 0 (C2G_OK) on success or a negative C2G_E* code.
+ *    The message of the last failure on the calling thread is returned by
+ *    c2g_last_error().  Nothing in the library calls exit() (the reference
+ *    does: core/Util.cuh:17-24, core/io.hpp:27-30).
+ *  - Plain pointers and sizes only.  The caller owns every host buffer passed
+ *    in or out (sized V, or n*V for batched calls); the library owns all device
+ *    memory behind the opaque c2g_graph handle, mirroring DeviceMemory's RAII
+ *    (src/sssp/base/DeviceMemory.cuh:22-44, 103-122).
+ *  - One handle is used from one host thread at a time.  The query server
+ *    (c2g_server_*) owns its own handle-internal streams and host lane.
+ *  - Distances are int32; INT32_MAX marks an unreachable vertex (the reference
+ *    uses 1999999999, src/sssp/base/Main.cu:121).
+ *  - GPU entry points fail with C2G_EDEVICE when no HIP device or kernel image
+ *    is usable.  There is no silent CPU fallback: the host-core lane runs only
+ *    where a function says so (c2g_sssp_migrate, the server's fallback lane,
+ *    c2g_host_*), and c2g_stats.lane reports which lane produced a result.
+ */
+#ifndef C2G_H
+#define C2G_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+#define C2G_OK        0
+#define C2G_EINVAL   (-1)  /* bad argument (source out of range, null ptr) */
+#define C2G_EIO      (-2)  /* missing / unreadable / unwritable file        */
+#define C2G_EFORMAT  (-3)  /* file lengths inconsistent (E != vlist[V] ...) */
+#define C2G_ENOMEM   (-4)  /* host or device allocation failed              */
+#define C2G_EDEVICE  (-5)  /* no usable HIP device / kernel launch failed   */
+#define C2G_ENOTSUP  (-6)  /* graph lacks the data the call needs           */
+#define C2G_EBUSY    (-7)  /* admission refused (server)                    */
+#define C2G_ETIMEOUT (-8)  /* lane timeout / cancellation                   */
+
+/* ---- dataset variants (directory layout, SURVEY of reference) ---------- */
+/* <ds>/origin/{vlist,elist,wlist[,flist]}.bin  utils/convert2binary.cpp:122-125 */
+#define C2G_VARIANT_ORIGIN    0
+/* <ds>/sssp/{vlist,elist,wlist,hvlist,helist,hwlist}.bin  src/purn-sssp.cpp:72-87,342-344 */
+#define C2G_VARIANT_SSSP_PURN 1
+/* <ds>/ppr/{vlist,elist,wlist(f32),hvlist,helist,hwlist(f32)}.bin  src/purn-ppr.cpp:74-89,330-335 */
+#define C2G_VARIANT_PPR_PURN  2
+
+/* ---- lanes -------------------------------------------------------------- */
+#define C2G_LANE_GPU  0
+#define C2G_LANE_HOST 1
+#define C2G_LANE_MIXED 2   /* GPU head + host-core tail (migration) */
+
+/* ---- query kinds (server) ---------------------------------------------- */
+#define C2G_Q_SSSP 0
+#define C2G_Q_PPR  1
+#define C2G_Q_ADS  2
+
+typedef struct c2g_graph c2g_graph;
+typedef struct c2g_server c2g_server;
+
+typedef struct {
+    int32_t device;      /* HIP ordinal; -1 = last device, as the reference's
+                            cudaSetDevice(deviceCount-1), src/sssp/base/Main.cu:30 */
+    int32_t load_flist;  /* ORIGIN variant: also load origin/flist.bin (PPR)   */
+    int32_t verbose;     /* print loader lines like Graph.h:45-47              */
+    int32_t no_device;   /* 1 = host-only handle (host lanes, CLIs, CPU tests)  */
+} c2g_opts;
+
+typedef struct {
+    int32_t lane;            /* C2G_LANE_*                                        */
+    int32_t iterations;      /* frontier iterations executed on the GPU           */
+    int32_t host_iterations; /* frontier iterations executed on the host lane     */
+    int32_t reached;         /* vertices with a finite result                     */
+    int64_t frontier_total;  /* sum over iterations of |F|                        */
+    int64_t edges_relaxed;   /* R: edge relaxations attempted                     */
+    int64_t improvements;    /* U: relaxations that lowered a value               */
+    int64_t reached_edges;   /* m_s = sum of outdeg over reached vertices (GTEPS) */
+    double  ms;              /* wall time of the call's device work               */
+} c2g_stats;
+
+/* Tunables picked by the autotuner (c2g_autotune) or set by hand. */
+typedef struct {
+    int32_t block;          /* threads per workgroup for relax kernels         */
+    int32_t grid_per_cu;    /* workgroups per CU for grid-stride kernels       */
+    int32_t chunk;          /* max edges per single-query frontier item        */
+    int32_t batch_chunk;    /* max edges per batched frontier item             */
+    int32_t iters_per_graph;/* iterations captured per hipGraph replay         */
+    int32_t use_graph;      /* 1 = replay captured iterations, 0 = plain launches */
+    int32_t delta;          /* near/far split step for SSSP (0 = plain Bellman-Ford) */
+    int32_t reserved[9];
+} c2g_tune;
+
+/* ======================================================================= */
+/* Errors                                                                   */
+/* ======================================================================= */
+/* Message of the last failing call on this thread ("" if none). */
+const char* c2g_last_error(void);
+/* Library version string. */
+const char* c2g_version(void);
+/* 1 if a HIP device with a gfx950 kernel image is usable, else 0. */
+int c2g_device_available(void);
+
+/* ======================================================================= */
+/* Format (host only)                                                       */
+/* ======================================================================= */
+/* Replaces read_binary2vector<T> (core/io.hpp:25-40): element count is
+ * filesize/4.  Returns count >= 0, or a negative error.  If buf is NULL only
+ * the count is returned. */
+int64_t c2g_read_i32(const char* path, int32_t* buf, int64_t cap);
+int64_t c2g_read_f32(const char* path, float* buf, int64_t cap);
+/* Headerless little-endian writers (convert2binary.cpp:118-134). */
+int c2g_write_i32(const char* path, const int32_t* buf, int64_t n);
+int c2g_write_f32(const char* path, const float* buf, int64_t n);
+
+/* Seeded R-MAT generator (replaces convert2binary's text ingestion for the
+ * synthetic workloads, convert2binary.cpp:67,76-79: self-loops dropped, rows
+ * sorted, duplicates removed).  Edge k of edge_factor<<scale is drawn by
+ * quadrant recursion from a counter-based hash of (seed, k); vertex ids are
+ * then passed through a seeded bijection of [0, 2^scale).  Outputs are
+ * malloc'ed by the library (release with c2g_free); out_E receives the edge
+ * count after dedup.
+ *   want_w     : int weights uniform in [1,wmax], a hash of (seed, u, v)
+ *   want_f     : row-normalised float weights, U(0,1] / row sum
+ *   symmetric=1: add the reverse of every edge before dedup.             */
+int c2g_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
+             uint64_t seed, int32_t symmetric, int32_t wmax, int32_t want_f,
+             int32_t** vlist, int32_t** elist, int32_t** wlist, float** flist,
+             int64_t* out_E);
+
+/* Writes <ds>/origin/{vlist,elist,wlist[,flist]}.bin (creates directories). */
+int c2g_write_dataset(const char* ds, int32_t V, const int32_t* vlist,
+                      const int32_t* elist, const int32_t* wlist, const float* flist);
+
+/* convert2binary <adjlist.txt> <ds> (utils/convert2binary.cpp:86-135): reads
+ * "src dst1:dst2:..." lines; weights are (rand()%100)+1 from an unseeded
+ * glibc-compatible generator; flist = w / row sum.                      */
+int c2g_convert_adjlist(const char* txt, const char* ds);
+
+/* ======================================================================= */
+/* Pruning (C^2graph-P), host only                                          */
+/* ======================================================================= */
+/* purn-sssp <ds> (src/purn-sssp.cpp:61-352).  Reads origin/, writes
+ * sssp/{vlist,elist,wlist} with the reference's exact pruning decisions, and
+ * sssp/{hvlist,helist,hwlist} per vertex (hvlist has V+1 entries, the layout
+ * the loader sssp/purn/Graph.h:40 and sssp/purn/Main.cu:72 index), plus
+ * sssp/porder.bin (processed vertices in prune order) so the reference's
+ * compacted hidden-list layout can be rebuilt.  stats_out (nullable):
+ * [0]=edges kept, [1]=hidden edges, [2]=processed vertices.             */
+int c2g_prune_sssp(const char* ds, int64_t* stats_out);
+/* purn-ppr <ds> (src/purn-ppr.cpp:63-344): reads origin/{vlist,elist,flist},
+ * writes ppr/{vlist,elist,wlist(f32),hvlist,helist,hwlist(f32)}.        */
+int c2g_prune_ppr(const char* ds, int64_t* stats_out);
+/* In-memory forms used by the library and tests: inputs are an origin CSR;
+ * outputs are malloc'ed by the library and released with c2g_free.      */
+int c2g_prune_sssp_mem(int32_t V, const int32_t* vlist, const int32_t* elist,
+                       const int32_t* wlist, int32_t** out_vlist,
+                       int32_t** out_elist, int32_t** out_wlist,
+                       int32_t** out_hvlist, int32_t** out_helist,
+                       int32_t** out_hwlist, int32_t** out_porder,
+                       int64_t* out_nporder);
+int c2g_prune_ppr_mem(int32_t V, const int32_t* vlist, const int32_t* elist,
+                      const float* flist, int32_t** out_vlist,
+                      int32_t** out_elist, float** out_wlist,
+                      int32_t** out_hvlist, int32_t** out_helist,
+                      float** out_hwlist);
+void c2g_free(void* p);
+
+/* ======================================================================= */
+/* Graph handle (replaces Graph + DeviceMemory)                              */
+/* ======================================================================= */
+/* Graph(dir) + DeviceMemory(V,E) + CudaMemcpyGraph (src/sssp/base/Graph.h:30-51,
+ * src/sssp/purn/Graph.h:33-48, src/ppr/base/Graph.h:31-43,
+ * src/sssp/base/DeviceMemory.cuh:22-65).  Loads the variant's files, checks
+ * their lengths (vlist[V]==E, wlist length E, hvlist length V+1; a legacy
+ * compacted sssp/hvlist is rejected with C2G_EFORMAT) and uploads the graph
+ * to HBM once. */
+int c2g_open(const char* ds, int32_t variant, const c2g_opts* opts, c2g_graph** out);
+/* Same from caller arrays (copied).  wlist/flist/hidden arrays are nullable;
+ * for the PPR_PURN variant pass flist (live edge floats) and hwf (hidden
+ * floats).  */
+int c2g_graph_from_csr(int32_t V, const int32_t* vlist, const int32_t* elist,
+                       const int32_t* wlist, const float* flist,
+                       const int32_t* hvlist, const int32_t* helist,
+                       const int32_t* hwlist, const float* hwf,
+                       int32_t variant, const c2g_opts* opts, c2g_graph** out);
+int c2g_close(c2g_graph* g);
+int c2g_graph_info(const c2g_graph* g, int32_t* V, int64_t* E, int64_t* hidden_E,
+                   int32_t* variant);
+/* Host copies of the loaded arrays (nullable outputs; sized V+1 / E). */
+int c2g_graph_host_csr(const c2g_graph* g, const int32_t** vlist,
+                       const int32_t** elist, const int32_t** wlist,
+                       const float** flist);
+int c2g_set_tune(c2g_graph* g, const c2g_tune* t);
+int c2g_get_tune(const c2g_graph* g, c2g_tune* t);
+
+/* ======================================================================= */
+/* SSSP                                                                     */
+/* ======================================================================= */
+/* One query on the GPU.  Replaces CalcuSSSP (src/sssp/base/Main.cu:109-356)
+ * on ORIGIN graphs and CalcuSSSP + host seeding (src/sssp/purn/Main.cu:67-90,
+ * 136-193) on SSSP_PURN graphs.  Runs to convergence (no 1000-iteration cap).
+ * On SSSP_PURN graphs the hidden list of the source, and of every vertex
+ * whose distance was lowered through a hidden edge, is relaxed, so the
+ * distances equal those of the full graph.  dist_out: int32[V]. */
+int c2g_sssp(c2g_graph* g, int32_t src, int32_t* dist_out, c2g_stats* st);
+/* Batched multi-source SSSP: n sources in one captured launch sequence with
+ * per-query frontiers kept as a 64-bit lane mask per vertex.  dist_out is
+ * n*V int32 (query-major, row q = query q).  Results are bit-identical to
+ * n calls of c2g_sssp.  st (nullable) receives whole-batch totals. */
+int c2g_sssp_batch(c2g_graph* g, const int32_t* srcs, int32_t n,
+                   int32_t* dist_out, c2g_stats* st);
+/* Same as c2g_sssp_batch but leaves results on the device and only returns
+ * a checksum per query (sum of finite distances, count of reached) — used to
+ * time the kernels with inputs resident in HBM.  sums/reached nullable. */
+int c2g_sssp_batch_resident(c2g_graph* g, const int32_t* srcs, int32_t n,
+                            int64_t* sums, int32_t* reached, c2g_stats* st);
+/* n single-source queries, one launch sequence per query (the "per-query"
+ * comparison point for batching); same outputs as c2g_sssp_batch_resident. */
+int c2g_sssp_each_resident(c2g_graph* g, const int32_t* srcs, int32_t n,
+                           int64_t* sums, int32_t* reached, c2g_stats* st);
+
+/* CalcuSSSP with an iteration budget (src/sssp/base-perk/Main.cu:263-497,
+ * src/sssp/purn-perk/Main.cu:276-521): stops after `budget` GPU iterations
+ * and returns the resumable state (dist[V], frontier[*count], *count).
+ * frontier_out has room for V entries. */
+int c2g_sssp_budget(c2g_graph* g, int32_t src, int32_t budget, int32_t* dist_out,
+                    int32_t* frontier_out, int32_t* count_out, c2g_stats* st);
+/* GPU head for `budget` iterations, then the host-core lane finishes the
+ * frontier (the C^2graph-PM handoff, src/sssp/purn-perk/Main.cu:111-182 with
+ * Bellman_ford of src/sssp/base-perk/SSSP.cpp:88-119). */
+int c2g_sssp_migrate(c2g_graph* g, int32_t src, int32_t budget, int32_t* dist_out,
+                     c2g_stats* st);
+
+/* ======================================================================= */
+/* Host-core lane (the reference's CPU worker; fallback lane of the server) */
+/* ======================================================================= */
+/* Frontier Bellman-Ford from a (dist, frontier) snapshot, dist updated in
+ * place (src/sssp/base-perk/SSSP.cpp:88-119, the version that resets its flag).
+ * On SSSP_PURN graphs hidden lists are relaxed by the same rule as c2g_sssp. */
+int c2g_host_bellman_ford(const c2g_graph* g, int32_t* dist, const int32_t* frontier,
+                          int32_t count, const uint8_t* via_hidden, c2g_stats* st);
+/* Full single-source query on the host lane (seeding + Bellman-Ford). */
+int c2g_host_sssp(const c2g_graph* g, int32_t src, int32_t* dist_out, c2g_stats* st);
+/* Forward push from (p, r, frontier) (src/ppr/w-base-perk/PPR.cpp:178-236)
+ * with threshold r[v] > eps*outdeg(v); flag array zero-initialised. */
+int c2g_host_forward_push(const c2g_graph* g, float alpha, float eps, float* p,
+                          float* r, const int32_t* frontier, int32_t count,
+                          c2g_stats* st);
+int c2g_host_ppr(const c2g_graph* g, int32_t src, float alpha, float eps,
+                 float* p_out, float* r_out, c2g_stats* st);
+
+/* ======================================================================= */
+/* PPR (forward push)                                                        */
+/* ======================================================================= */
+/* calcuatePPR<float> (src/ppr/base/Main.cu:121-388; pruned seeding
+ * src/ppr/purn/Main.cu:78-101,171-207): r[s]=1, p[s]=alpha; each iteration
+ * every active u sends (1-alpha)*r[u]*f(u,v) to m[v] and zeroes r[u]; then
+ * r[v]+=m[v] and v becomes active (p[v]+=alpha*r[v]) when
+ * r[v] > eps*outdeg(v).  FP32 state.  p_out, r_out: float[V] (r nullable). */
+int c2g_ppr(c2g_graph* g, int32_t src, float alpha, float eps, float* p_out,
+            float* r_out, c2g_stats* st);
+/* n sources in one launch sequence; p_out/r_out are n*V (query-major). */
+int c2g_ppr_batch(c2g_graph* g, const int32_t* srcs, int32_t n, float alpha,
+                  float eps, float* p_out, float* r_out, c2g_stats* st);
+
+/* ======================================================================= */
+/* Adsorption                                                                */
+/* ======================================================================= */
+/* Multi-label adsorption (reference: adsorption/w-base/Main.cu:123-395 is a
+ * single-source push; the multi-label recurrence here follows Baluja et al.
+ * 2008 / Talukdar & Crammer 2009):
+ *   Y0[v][l] = 1 if (v,l) is a seed, else 0
+ *   Y_{t+1}[v] = p_inj*Y0[v] + p_cont * sum_{(v->u)} f(v,u) * Y_t[u]
+ * for `iters` Jacobi iterations starting from Y_0 = Y0.  p_abnd is the
+ * abandonment probability (the dummy label's share) and is checked to make
+ * p_inj+p_cont+p_abnd = 1.  y_out: float[V*L], vertex-major.  L <= 64. */
+int c2g_adsorption(c2g_graph* g, const int32_t* seed_v, const int32_t* seed_label,
+                   int32_t nseed, int32_t L, float p_inj, float p_cont, float p_abnd,
+                   int32_t iters, float* y_out, c2g_stats* st);
+
+/* ======================================================================= */
+/* Autotuner                                                                 */
+/* ======================================================================= */
+/* Times the SSSP kernels over a sweep of block shapes, chunk sizes and
+ * iterations-per-graph on the given sample sources and stores the fastest
+ * setting in the handle (returned in best_out, nullable). */
+int c2g_autotune(c2g_graph* g, const int32_t* srcs, int32_t n, int32_t batched,
+                 c2g_tune* best_out);
+
+/* ======================================================================= */
+/* Serving loop (continuous batching)                                        */
+/* ======================================================================= */
+typedef struct {
+    int32_t max_batch;        /* max SSSP queries per device batch (<=64)      */
+    int32_t max_inflight;     /* max queries admitted at once                  */
+    int64_t hbm_reserve;      /* bytes of HBM kept free                        */
+    int64_t fake_free_hbm;    /* >0: use this as free HBM (tests)              */
+    int32_t host_lane;        /* 1 = enable host-core fallback lane            */
+    int32_t device_failed;    /* 1 = treat device as failed (tests)            */
+    float   alpha, eps;       /* PPR parameters                                */
+    int32_t ads_labels, ads_iters;
+    float   p_inj, p_cont, p_abnd;
+    int32_t reserved[8];
+} c2g_server_opts;
+
+typedef struct {
+    int64_t id;
+    int32_t kind;     /* C2G_Q_* */
+    int32_t status;   /* 0 done, <0 error */
+    int32_t lane;     /* C2G_LANE_* */
+    int32_t pad;
+    double  t_submit_ms, t_admit_ms, t_done_ms;  /* server clock */
+} c2g_result_info;
+
+int c2g_server_start(c2g_graph* g, const c2g_server_opts* o, c2g_server** out);
+/* Submit one query; returns its id (>= 0) or a negative error (C2G_EBUSY when
+ * admission is refused).  For C2G_Q_ADS, src is the seed-set index. */
+int64_t c2g_server_submit(c2g_server* s, int32_t kind, int32_t src);
+/* Poll a finished query: 1 = done (info filled, result copied into out when
+ * non-null: int32[V] for SSSP, float[V] for PPR, float[V*L] for adsorption),
+ * 0 = not done, <0 error.  wait_ms < 0 blocks. */
+int c2g_server_poll(c2g_server* s, int64_t id, int32_t wait_ms, void* out,
+                    c2g_result_info* info);
+int c2g_server_stop(c2g_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* C2G_H */

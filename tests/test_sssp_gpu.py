@@ -1,0 +1,117 @@
+"""SSSP on the GPU through the C-ABI: bit-exact against the oracle's Dijkstra.
+
+Reference semantics: CalcuSSSP (src/sssp/base/Main.cu:109-356) converges to the
+unique shortest-path distances; unreached vertices are INT32_MAX here (the
+reference writes 1999999999, Main.cu:121).
+"""
+import numpy as np
+import pytest
+
+import c2g
+import oracle
+from conftest import INT_MAX, check_sssp_certificate, random_graph, rmat_graph
+
+pytestmark = pytest.mark.gpu
+
+
+def _graph(G):
+    return c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], device=0)
+
+
+def test_c1_max_degree_source(gpu):
+    # Workload C1: R-MAT scale 16, ef 16, seed 1, weights U[1,64], max out-degree source.
+    G = rmat_graph(16, 1, wmax=64)
+    src = int(np.argmax(np.diff(G["vlist"])))
+    with _graph(G) as g:
+        d, st = g.sssp(src)
+    ref = oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], src)
+    assert np.array_equal(d, ref)
+    assert st.lane == c2g.LANE_GPU and st.iterations > 0
+    assert st.reached == int((ref != INT_MAX).sum())
+
+
+@pytest.mark.parametrize("seed", [11, 12, 13])
+def test_c1_random_sources(gpu, seed):
+    G = rmat_graph(16, 1, wmax=64)
+    rng = np.random.default_rng(seed)
+    srcs = rng.integers(0, G["V"], 4)
+    with _graph(G) as g:
+        for s in srcs:
+            d, _ = g.sssp(int(s))
+            assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], int(s))), s
+
+
+def test_tiny_and_edge_cases(gpu):
+    # isolated source, zero-out-degree source, two-vertex graph, zero weights
+    G = random_graph(50, 120, seed=3, wmax=5, zero_ok=True)
+    with _graph(G) as g:
+        for s in range(G["V"]):
+            d, _ = g.sssp(s)
+            assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], s)), s
+    vl = np.array([0, 0, 0], np.int32)
+    with c2g.Graph.from_csr(vl, np.zeros(0, np.int32), np.zeros(0, np.int32), device=0) as g:
+        d, st = g.sssp(1)
+        assert d.tolist() == [INT_MAX, 0] and st.reached == 1
+
+
+def test_bad_source_is_error(gpu):
+    G = random_graph(20, 40, seed=1)
+    with _graph(G) as g:
+        with pytest.raises(c2g.C2GError) as e:
+            g.sssp(20)
+        assert e.value.code == c2g.EINVAL
+
+
+def test_hub_rows_split_across_chunks(gpu):
+    # a star with 10k leaves + chain: exercises items split at `chunk` edges
+    n = 10001
+    vl = np.zeros(n + 1, np.int32)
+    vl[1:] = n - 1
+    el = np.arange(1, n, dtype=np.int32)
+    w = (np.arange(1, n) % 7 + 1).astype(np.int32)
+    with c2g.Graph.from_csr(vl, el, w, device=0) as g:
+        for chunk in (1, 7, 64, 4096):
+            g.set_tune(chunk=chunk)
+            d, _ = g.sssp(0)
+            assert d[0] == 0 and np.array_equal(d[1:], w), chunk
+
+
+@pytest.mark.slow
+def test_scale20_against_dijkstra(gpu):
+    G = rmat_graph(20, 4, wmax=1023)
+    deg = np.diff(G["vlist"])
+    rng = np.random.default_rng(5)
+    srcs = [int(np.argmax(deg))] + [int(x) for x in rng.choice(np.flatnonzero(deg > 0), 3)]
+    with _graph(G) as g:
+        for s in srcs:
+            d, _ = g.sssp(s)
+            assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], s)), s
+
+
+@pytest.mark.slow
+def test_scale22_certificate(gpu):
+    # Workload C2's graph size (scale 22, weights U[1,255]); exactness by certificate.
+    G = rmat_graph(22, 2, wmax=255)
+    deg = np.diff(G["vlist"])
+    rng = np.random.default_rng(2)
+    srcs = rng.choice(np.flatnonzero(deg > 0), 3)
+    with _graph(G) as g:
+        for s in srcs:
+            d, _ = g.sssp(int(s))
+            check_sssp_certificate(G["vlist"], G["elist"], G["wlist"], int(s), d)
+
+
+def test_budget_and_migration(gpu):
+    # sssp-purn-M semantics: GPU head for `budget` rounds, host Bellman-Ford tail
+    G = rmat_graph(16, 1, wmax=64)
+    src = int(np.argmax(np.diff(G["vlist"])))
+    ref = oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], src)
+    with _graph(G) as g:
+        for budget in (1, 2, 3, 5, 100):
+            d, fr, st = g.sssp_budget(src, budget)
+            assert st.iterations <= budget
+            assert np.all(d >= ref)
+            d2, _ = g.host_bellman_ford(d, fr)
+            assert np.array_equal(d2, ref), budget
+            d3, st3 = g.sssp_migrate(src, budget)
+            assert np.array_equal(d3, ref)
