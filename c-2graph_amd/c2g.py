@@ -32,7 +32,8 @@ class C2GError(RuntimeError):
 class Stats(C.Structure):
     _fields_ = [("lane", C.c_int32), ("iterations", C.c_int32), ("host_iterations", C.c_int32),
                 ("reached", C.c_int32), ("frontier_total", C.c_int64), ("edges_relaxed", C.c_int64),
-                ("improvements", C.c_int64), ("reached_edges", C.c_int64), ("ms", C.c_double)]
+                ("improvements", C.c_int64), ("reached_edges", C.c_int64), ("csr_edges", C.c_int64),
+                ("items", C.c_int64), ("ms", C.c_double), ("kernel_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -46,7 +47,7 @@ class Opts(C.Structure):
 class Tune(C.Structure):
     _fields_ = [("block", C.c_int32), ("grid_per_cu", C.c_int32), ("chunk", C.c_int32),
                 ("batch_chunk", C.c_int32), ("iters_per_graph", C.c_int32), ("use_graph", C.c_int32),
-                ("delta", C.c_int32), ("reserved", C.c_int32 * 9)]
+                ("delta", C.c_int32), ("profile", C.c_int32), ("reserved", C.c_int32 * 8)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

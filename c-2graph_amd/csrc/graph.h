@@ -98,6 +98,7 @@ struct Graph {
 
 int device_upload(Graph* g, int device);
 void device_release(Graph* g);
+void batch_release(Graph* g);
 void default_tune(Graph* g);
 int ensure_sssp_ws(Graph* g);
 int ensure_batch_ws(Graph* g);

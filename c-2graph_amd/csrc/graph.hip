@@ -108,6 +108,7 @@ int device_upload(Graph* g, int device) {
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     g->device = device;
     g->num_cus = prop.multiProcessorCount;
+    g->on_device = false;
     HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     const int64_t V = g->V, E = g->E;
     HIP_TRY(hipMalloc(&g->d_row, sizeof(int32_t) * (V + 1)));
@@ -148,8 +149,9 @@ static void dfree(T*& p) {
 }
 
 void device_release(Graph* g) {
-    if (!g->on_device) return;
+    if (!g->stream && !g->d_row) return;
     (void)hipSetDevice(g->device);
+    batch_release(g);
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     if (g->sssp.exec) (void)hipGraphExecDestroy(g->sssp.exec);
     if (g->batch.exec) (void)hipGraphExecDestroy(g->batch.exec);

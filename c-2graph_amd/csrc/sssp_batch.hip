@@ -1,0 +1,585 @@
+// Batched multi-source SSSP on gfx950: 64 queries per group, one wave lane
+// per query.
+//
+// The reference runs one CalcuSSSP launch per query (src/sssp/base/Main.cu:
+// 59-76); here many sources of uneven reach share every launch.  Distances
+// are bit-identical to single-source runs: each lane is an independent
+// label-correcting SSSP with the same relaxation rule
+// (`cand = dist[u]+w; atomicMin; re-activate on improvement`,
+// src/sssp/base/Main.cu:217-221).
+//
+// HBM layout of a group g (queries 64g .. 64g+63):
+//   dist  [g][V][64] int32   vertex-major: relaxing edge (u,v) for all active
+//                            queries is one coalesced 256-byte row access
+//   mask  [g][V]     u64     lanes of v improved this round, not yet expanded
+//   hmask [g][V]     u64     lanes whose value at v came through a hidden edge
+// Per round (two kernels, captured in a hipGraph):
+//   relax_b  : each wave takes one item (u, e_begin, e_end, g|kind) with its
+//              lane mask; edges are loaded 64 at a time lane-parallel, then
+//              walked in groups of 8 (8 independent row loads in flight);
+//              improved lanes are OR-ed into mask[g][v] lane-parallel, and the
+//              0 -> nonzero transition appends (v,g) to the activation list
+//              (one atomicAdd per 64 edges).
+//   expand_b : drains each activated (v,g): takes mask[g][v] (clearing it),
+//              splits v's row into items of <= batch_chunk edges, and adds a
+//              hidden-list item for lanes in hmask (pruned graphs).
+// Vertices with neither out-edges nor hidden edges are never activated (bitmap
+// `nz`), which skips about half of all activations on R-MAT graphs.
+#include "graph.h"
+#include "wave.h"
+#include <cstring>
+
+namespace c2g {
+
+enum : int {
+    B_ACT = 0,     // activated (v,g) entries this round
+    B_ITEMS = 1,   // items for the next relax round
+    B_ITER = 2,    // relax rounds with work
+    B_DONE_R = 3,  // workgroups finished (relax)
+    B_DONE_E = 4,  // workgroups finished (expand)
+    B_FRONT = 5,   // sum over rounds of activated (v,g)
+    B_EDGES = 6,   // CSR edges streamed (per item, once for all lanes)
+    B_IMPR = 7,    // improving (query, edge) relaxations (U)
+    B_OVF = 8,     // capacity guard tripped (must stay 0)
+    B_LEDGES = 9,  // (query, edge) relaxation attempts
+    B_ITEMS_TOT = 10,  // items processed
+    B_N = 16
+};
+
+constexpr int kJ = 8;  // edges whose row loads are in flight together
+
+__global__ void batch_fill_k(int32_t* __restrict__ dist, int64_t n_dist, uint64_t* __restrict__ mask,
+                             uint64_t* __restrict__ hmask, int64_t n_mask) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int4* d4 = reinterpret_cast<int4*>(dist);
+    const int64_t n4 = n_dist / 4;
+    for (int64_t i = t; i < n4; i += stride) d4[i] = make_int4(INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX);
+    for (int64_t i = t; i < n_mask; i += stride) {
+        mask[i] = 0;
+        if (hmask) hmask[i] = 0;
+    }
+}
+
+// Seeds lane q of group q/64 at its source (one thread per query).
+__global__ void batch_seed_k(const int32_t* __restrict__ srcs, int n, const uint32_t* __restrict__ nz,
+                             int32_t* __restrict__ dist, uint64_t* __restrict__ mask,
+                             uint64_t* __restrict__ hmask, int2* __restrict__ acts,
+                             int64_t* __restrict__ ctrl, int64_t V) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q == 0)
+        for (int i = 0; i < B_N; ++i)
+            if (i != B_ACT) ctrl[i] = 0;
+    if (q >= n) return;
+    const int g = q >> 6, lane = q & 63;
+    const int32_t s = srcs[q];
+    dist[((int64_t)g * V + s) * 64 + lane] = 0;
+    if (hmask) atomicOr((unsigned long long*)&hmask[(int64_t)g * V + s], 1ull << lane);
+    if (!((nz[s >> 5] >> (s & 31)) & 1u)) return;
+    unsigned long long old = atomicOr((unsigned long long*)&mask[(int64_t)g * V + s], 1ull << lane);
+    if (old == 0) {
+        int64_t p = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ACT], 1ull);
+        acts[p] = make_int2(s, g);
+    }
+}
+
+template <bool PRUNED>
+__global__ void __launch_bounds__(256)
+relax_b_k(const uint32_t* __restrict__ nz, const int2* __restrict__ adj, const int2* __restrict__ hadj,
+          int32_t* __restrict__ dist, uint64_t* __restrict__ mask, uint64_t* __restrict__ hmask,
+          const int4* __restrict__ items, const uint64_t* __restrict__ imask,
+          int2* __restrict__ acts, int64_t* __restrict__ ctrl, int64_t V, int64_t actcap) {
+    const int64_t n = ctrl[B_ITEMS];
+    const int lane = lane_id();
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    int64_t impr = 0;
+
+    for (int64_t i = wave; i < n; i += nwaves) {
+        const int4 it = items[i];
+        const uint64_t m = imask[i];
+        const int u = it.x;
+        const int g = it.w & 0xffff;
+        const bool hid_item = PRUNED && (it.w >> 16);
+        int32_t* __restrict__ dG = dist + (int64_t)g * V * 64;
+        const bool act = (m >> lane) & 1ull;
+        const int du = act ? dG[(int64_t)u * 64 + lane] : INT32_MAX;
+        const int2* __restrict__ A = hid_item ? hadj : adj;
+        for (int e0 = it.y; e0 < it.z; e0 += 64) {
+            const int ne = min(64, it.z - e0);
+            int2 my = make_int2(0, 0);
+            if (lane < ne) my = A[e0 + lane];
+            uint64_t mybits = 0;
+            for (int j0 = 0; j0 < ne; j0 += kJ) {
+                int vj[kJ], cj[kJ], oj[kJ];
+#pragma unroll
+                for (int jj = 0; jj < kJ; ++jj) {
+                    const bool valid = j0 + jj < ne;
+                    const int j = valid ? j0 + jj : ne - 1;
+                    vj[jj] = __shfl(my.x, j, 64);
+                    const int w = __shfl(my.y, j, 64);
+                    cj[jj] = (valid && act && du <= INT32_MAX - w) ? du + w : INT32_MAX;
+                    oj[jj] = INT32_MAX;
+                    if (cj[jj] != INT32_MAX) oj[jj] = dG[(int64_t)vj[jj] * 64 + lane];
+                }
+#pragma unroll
+                for (int jj = 0; jj < kJ; ++jj) {
+                    bool imp = false;
+                    if (cj[jj] < oj[jj]) imp = cj[jj] < atomicMin(&dG[(int64_t)vj[jj] * 64 + lane], cj[jj]);
+                    const uint64_t b = __ballot(imp);
+                    impr += imp;
+                    if (lane == j0 + jj) mybits = b;
+                }
+            }
+            // Lane l now holds the improved-lane mask of edge l.
+            bool newv = false;
+            if (mybits) {
+                const int v = my.x;
+                if (PRUNED && hid_item)
+                    atomicOr((unsigned long long*)&hmask[(int64_t)g * V + v], (unsigned long long)mybits);
+                if ((nz[v >> 5] >> (v & 31)) & 1u) {
+                    unsigned long long o =
+                        atomicOr((unsigned long long*)&mask[(int64_t)g * V + v], (unsigned long long)mybits);
+                    newv = (o == 0);
+                }
+            }
+            const uint64_t nb = __ballot(newv);
+            if (nb) {
+                int64_t base = 0;
+                if (lane == 0) base = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ACT], (unsigned long long)__popcll(nb));
+                base = shfl64(base, 0);
+                if (newv) {
+                    const int64_t p = base + lanes_below(nb);
+                    if (p < actcap) acts[p] = make_int2(my.x, g);
+                }
+            }
+        }
+    }
+    impr = wave_sum64(impr);
+    if (lane == 0 && impr) atomicAdd((unsigned long long*)&ctrl[B_IMPR], (unsigned long long)impr);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[B_DONE_R], 1ull);
+        if (prev == (unsigned long long)gridDim.x - 1) {
+            __threadfence();
+            if (n > 0) ctrl[B_ITER] += 1;
+            ctrl[B_ITEMS_TOT] += n;
+            ctrl[B_ITEMS] = 0;
+            ctrl[B_DONE_R] = 0;
+            int64_t na = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ACT], 0ull);
+            if (na > actcap) { ctrl[B_OVF] = 1; ctrl[B_ACT] = 0; }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, uint64_t* __restrict__ mask,
+           const uint64_t* __restrict__ hmask, const int2* __restrict__ acts, int4* __restrict__ items,
+           uint64_t* __restrict__ imask, int64_t* __restrict__ ctrl, int64_t V, int chunk, int64_t itemcap) {
+    const int64_t n = ctrl[B_ACT];
+    const int lane = lane_id();
+    int64_t front = 0, edges = 0, ledges = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    // Uniform trip count per wave so the wave-wide scan below is legal.
+    const int64_t nn = (n + 63) & ~(int64_t)63;
+    for (int64_t i = t0; i < nn; i += stride) {
+        int v = 0, g = 0, r0 = 0, r1 = 0, h0 = 0, h1 = 0, nl = 0, nh = 0;
+        uint64_t m = 0, hm = 0;
+        if (i < n) {
+            const int2 a = acts[i];
+            v = a.x;
+            g = a.y;
+            uint64_t* mp = &mask[(int64_t)g * V + v];
+            m = *mp;
+            *mp = 0;
+            r0 = row[v];
+            r1 = row[v + 1];
+            nl = (r1 - r0 + chunk - 1) / chunk;
+            if (hrow) {
+                hm = m & hmask[(int64_t)g * V + v];
+                if (hm) {
+                    h0 = hrow[v];
+                    h1 = hrow[v + 1];
+                    nh = (h1 - h0 + chunk - 1) / chunk;
+                }
+            }
+            front += 1;
+            edges += (r1 - r0) + (h1 - h0);
+            ledges += (int64_t)__popcll(m) * (r1 - r0) + (int64_t)__popcll(hm) * (h1 - h0);
+        }
+        int tot;
+        const int off = wave_excl_scan(nl + nh, &tot);
+        if (tot > 0) {
+            int64_t base = 0;
+            if (lane == 0) base = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ITEMS], (unsigned long long)tot);
+            base = shfl64(base, 0);
+            if (base + tot <= itemcap) {
+                int64_t p = base + off;
+                for (int j = 0; j < nl; ++j, ++p) {
+                    const int b = r0 + j * chunk;
+                    items[p] = make_int4(v, b, min(b + chunk, r1), g);
+                    imask[p] = m;
+                }
+                for (int j = 0; j < nh; ++j, ++p) {
+                    const int b = h0 + j * chunk;
+                    items[p] = make_int4(v, b, min(b + chunk, h1), g | (1 << 16));
+                    imask[p] = hm;
+                }
+            } else if (lane == 0) {
+                atomicOr((unsigned long long*)&ctrl[B_OVF], 1ull);
+            }
+        }
+    }
+    front = wave_sum64(front);
+    edges = wave_sum64(edges);
+    ledges = wave_sum64(ledges);
+    if (lane == 0 && front) {
+        atomicAdd((unsigned long long*)&ctrl[B_FRONT], (unsigned long long)front);
+        atomicAdd((unsigned long long*)&ctrl[B_EDGES], (unsigned long long)edges);
+        atomicAdd((unsigned long long*)&ctrl[B_LEDGES], (unsigned long long)ledges);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[B_DONE_E], 1ull);
+        if (prev == (unsigned long long)gridDim.x - 1) {
+            __threadfence();
+            ctrl[B_ACT] = 0;
+            ctrl[B_DONE_E] = 0;
+            int64_t ni = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ITEMS], 0ull);
+            if (ni > itemcap) { ctrl[B_OVF] = 1; ctrl[B_ITEMS] = 0; }
+        }
+    }
+}
+
+// Per-lane checksums of one group: sum of finite distances, reached count,
+// and out-degree sum of reached vertices (m_s), for lanes [0, nlanes).
+__global__ void batch_checksum_k(const int32_t* __restrict__ row, const int32_t* __restrict__ dist,
+                                 int64_t V, int64_t* __restrict__ sums, int32_t* __restrict__ reached,
+                                 int64_t* __restrict__ medges) {
+    const int lane = lane_id();
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    int64_t s = 0, m = 0;
+    int c = 0;
+    for (int64_t v = wave; v < V; v += nwaves) {
+        const int d = dist[v * 64 + lane];
+        if (d != INT32_MAX) {
+            s += d;
+            c += 1;
+            m += row[v + 1] - row[v];
+        }
+    }
+    atomicAdd((unsigned long long*)&sums[lane], (unsigned long long)s);
+    atomicAdd(&reached[lane], c);
+    atomicAdd((unsigned long long*)&medges[lane], (unsigned long long)m);
+}
+
+// ------------------------------------------------------------------ host
+static int nz_bitmap(Graph* g, uint32_t** out) {
+    std::vector<uint32_t> bits(((size_t)g->V + 31) / 32 + 1, 0);
+    for (int32_t v = 0; v < g->V; ++v) {
+        bool has = g->vl[v + 1] > g->vl[v];
+        if (!g->hvl.empty()) has = has || g->hvl[v + 1] > g->hvl[v];
+        if (has) bits[v >> 5] |= 1u << (v & 31);
+    }
+    HIP_TRY(hipMalloc(out, sizeof(uint32_t) * bits.size()));
+    HIP_TRY(hipMemcpy(*out, bits.data(), sizeof(uint32_t) * bits.size(), hipMemcpyHostToDevice));
+    return C2G_OK;
+}
+
+struct BatchState {
+    int groups = 0;
+    int64_t actcap = 0, itemcap = 0;
+    uint32_t* nz = nullptr;
+    int2* acts = nullptr;
+    int4* items = nullptr;
+    uint64_t* imask = nullptr;
+    int32_t* srcs = nullptr;
+    int64_t* sums = nullptr;    // [64*groups]
+    int32_t* reach = nullptr;   // [64*groups]
+    int64_t* medges = nullptr;  // [64*groups]
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+static std::mutex g_bs_mu;
+static std::vector<std::pair<const Graph*, BatchState*>> g_bs;
+
+static BatchState* bstate(Graph* g) {
+    std::lock_guard<std::mutex> lk(g_bs_mu);
+    for (auto& p : g_bs)
+        if (p.first == g) return p.second;
+    auto* s = new BatchState;
+    g_bs.emplace_back(g, s);
+    return s;
+}
+
+void batch_release(Graph* g) {
+    std::lock_guard<std::mutex> lk(g_bs_mu);
+    for (size_t i = 0; i < g_bs.size(); ++i)
+        if (g_bs[i].first == g) {
+            BatchState* s = g_bs[i].second;
+            (void)hipFree(s->nz); (void)hipFree(s->acts); (void)hipFree(s->items);
+            (void)hipFree(s->imask); (void)hipFree(s->srcs); (void)hipFree(s->sums);
+            (void)hipFree(s->reach); (void)hipFree(s->medges);
+            for (auto& e : s->ev) if (e) (void)hipEventDestroy(e);
+            delete s;
+            g_bs.erase(g_bs.begin() + i);
+            return;
+        }
+}
+
+static int ensure_batch(Graph* g, int groups) {
+    BatchWs& w = g->batch;
+    BatchState* s = bstate(g);
+    HIP_TRY(hipSetDevice(g->device));
+    const int64_t V = g->V;
+    const int chunk = std::max(1, g->tune.batch_chunk);
+    if (!s->nz) {
+        int rc = nz_bitmap(g, &s->nz);
+        if (rc) return rc;
+        HIP_TRY(hipEventCreate(&s->ev[0]));
+        HIP_TRY(hipEventCreate(&s->ev[1]));
+        HIP_TRY(hipMalloc(&w.ctrl, sizeof(int64_t) * B_N));
+        HIP_TRY(hipMemset(w.ctrl, 0, sizeof(int64_t) * B_N));
+        HIP_TRY(hipHostMalloc((void**)&w.h_ctrl, sizeof(int64_t) * B_N, hipHostMallocDefault));
+    }
+    const int64_t needi = groups * (2 * V + (g->E + g->HE) / chunk) + 64;
+    if (groups > s->groups || s->itemcap < needi) {
+        if (w.exec) { (void)hipGraphExecDestroy(w.exec); w.exec = nullptr; }
+        const int G = std::max(groups, s->groups);
+        (void)hipFree(w.dist); (void)hipFree(w.mask[0]); (void)hipFree(w.hmask);
+        (void)hipFree(s->acts); (void)hipFree(s->items); (void)hipFree(s->imask); (void)hipFree(s->srcs);
+        (void)hipFree(s->sums); (void)hipFree(s->reach); (void)hipFree(s->medges);
+        w.dist = nullptr; w.mask[0] = nullptr; w.hmask = nullptr;
+        HIP_TRY(hipMalloc(&w.dist, sizeof(int32_t) * (size_t)G * V * 64 + 16));
+        HIP_TRY(hipMalloc(&w.mask[0], sizeof(uint64_t) * (size_t)G * V + 8));
+        if (g->d_hrow) HIP_TRY(hipMalloc(&w.hmask, sizeof(uint64_t) * (size_t)G * V + 8));
+        s->actcap = (int64_t)G * V + 64;
+        s->itemcap = G * (2 * V + (g->E + g->HE) / chunk) + 64;
+        HIP_TRY(hipMalloc(&s->acts, sizeof(int2) * s->actcap));
+        HIP_TRY(hipMalloc(&s->items, sizeof(int4) * s->itemcap));
+        HIP_TRY(hipMalloc(&s->imask, sizeof(uint64_t) * s->itemcap));
+        HIP_TRY(hipMalloc(&s->srcs, sizeof(int32_t) * 64 * G));
+        HIP_TRY(hipMalloc(&s->sums, sizeof(int64_t) * 64 * G));
+        HIP_TRY(hipMalloc(&s->reach, sizeof(int32_t) * 64 * G));
+        HIP_TRY(hipMalloc(&s->medges, sizeof(int64_t) * 64 * G));
+        s->groups = G;
+        w.groups = G;
+        w.exec_chunk = -1;
+    }
+    return C2G_OK;
+}
+
+static int relax_grid_b(const Graph* g) { return std::max(1, g->num_cus) * g->tune.grid_per_cu; }
+
+// One round = relax_b + expand_b.  When `ev` is non-null, events bracket the
+// relax kernel and its duration is added to *relax_ms (profiling runs).
+static int launch_round_b(Graph* g, double* relax_ms) {
+    BatchWs& w = g->batch;
+    BatchState* s = bstate(g);
+    const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
+    const int grid = relax_grid_b(g);
+    if (relax_ms) HIP_TRY(hipEventRecord(s->ev[0], g->stream));
+    if (pruned)
+        hipLaunchKernelGGL(relax_b_k<true>, dim3(grid), dim3(256), 0, g->stream, s->nz, g->d_adj,
+                           g->d_hadj, w.dist, w.mask[0], w.hmask, s->items, s->imask, s->acts, w.ctrl,
+                           (int64_t)g->V, s->actcap);
+    else
+        hipLaunchKernelGGL(relax_b_k<false>, dim3(grid), dim3(256), 0, g->stream, s->nz, g->d_adj,
+                           (const int2*)nullptr, w.dist, w.mask[0], (uint64_t*)nullptr, s->items,
+                           s->imask, s->acts, w.ctrl, (int64_t)g->V, s->actcap);
+    if (relax_ms) {
+        HIP_TRY(hipEventRecord(s->ev[1], g->stream));
+        HIP_TRY(hipEventSynchronize(s->ev[1]));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+        *relax_ms += ms;
+    }
+    hipLaunchKernelGGL(expand_b_k, dim3(grid), dim3(256), 0, g->stream, g->d_row,
+                       pruned ? g->d_hrow : (const int32_t*)nullptr, w.mask[0],
+                       pruned ? w.hmask : (const uint64_t*)nullptr, s->acts, s->items, s->imask, w.ctrl,
+                       (int64_t)g->V, g->tune.batch_chunk, s->itemcap);
+    HIP_TRY(hipGetLastError());
+    return C2G_OK;
+}
+
+static int ensure_exec_b(Graph* g) {
+    BatchWs& w = g->batch;
+    const int K = g->tune.iters_per_graph;
+    if (w.exec && w.exec_iters == K && w.exec_grid == relax_grid_b(g) && w.exec_chunk == g->tune.batch_chunk)
+        return C2G_OK;
+    if (w.exec) { (void)hipGraphExecDestroy(w.exec); w.exec = nullptr; }
+    hipGraph_t graph;
+    HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    int rc = C2G_OK;
+    for (int r = 0; r < K && rc == C2G_OK; ++r) rc = launch_round_b(g, nullptr);
+    hipError_t e = hipStreamEndCapture(g->stream, &graph);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(C2G_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&w.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(C2G_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    w.exec_iters = K;
+    w.exec_grid = relax_grid_b(g);
+    w.exec_chunk = g->tune.batch_chunk;
+    return C2G_OK;
+}
+
+// Runs one group of n <= 64 queries (lanes) in group slot 0 .. groups-1.
+// Groups are run together: srcs holds n_total sources, lane q of group q/64.
+static int batch_run(Graph* g, const int32_t* srcs, int32_t n, c2g_stats* st, double* relax_ms) {
+    BatchWs& w = g->batch;
+    BatchState* s = bstate(g);
+    const int groups = (n + 63) / 64;
+    const int64_t V = g->V;
+    const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
+    HIP_TRY(hipMemcpyAsync(s->srcs, srcs, sizeof(int32_t) * n, hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipMemsetAsync(w.ctrl, 0, sizeof(int64_t) * B_N, g->stream));
+    const int fill_grid = std::max(1, g->num_cus) * 16;
+    hipLaunchKernelGGL(batch_fill_k, dim3(fill_grid), dim3(256), 0, g->stream, w.dist,
+                       (int64_t)groups * V * 64, w.mask[0], pruned ? w.hmask : (uint64_t*)nullptr,
+                       (int64_t)groups * V);
+    hipLaunchKernelGGL(batch_seed_k, dim3((n + 255) / 256), dim3(256), 0, g->stream, s->srcs, n, s->nz,
+                       w.dist, w.mask[0], pruned ? w.hmask : (uint64_t*)nullptr, s->acts, w.ctrl, V);
+    // Turn the seeded activations into the first round's items.
+    hipLaunchKernelGGL(expand_b_k, dim3(relax_grid_b(g)), dim3(256), 0, g->stream, g->d_row,
+                       pruned ? g->d_hrow : (const int32_t*)nullptr, w.mask[0],
+                       pruned ? w.hmask : (const uint64_t*)nullptr, s->acts, s->items, s->imask, w.ctrl,
+                       V, g->tune.batch_chunk, s->itemcap);
+    HIP_TRY(hipGetLastError());
+    const bool use_graph = g->tune.use_graph && !relax_ms;
+    int rc;
+    if (use_graph && (rc = ensure_exec_b(g))) return rc;
+    for (;;) {
+        if (use_graph) {
+            HIP_TRY(hipGraphLaunch(w.exec, g->stream));
+        } else {
+            // Profiled runs check after every round so that no empty round
+            // is launched (launch count == rounds with work).
+            const int k = relax_ms ? 1 : g->tune.iters_per_graph;
+            for (int r = 0; r < k; ++r)
+                if ((rc = launch_round_b(g, relax_ms))) return rc;
+        }
+        HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, sizeof(int64_t) * B_N, hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        if (w.h_ctrl[B_OVF]) return fail(C2G_EDEVICE, "batched frontier capacity exceeded (internal)");
+        if (w.h_ctrl[B_ITEMS] == 0) break;
+    }
+    if (st) {
+        st->lane = C2G_LANE_GPU;
+        st->iterations = (int32_t)w.h_ctrl[B_ITER];
+        st->frontier_total = w.h_ctrl[B_FRONT];
+        st->edges_relaxed = w.h_ctrl[B_LEDGES];
+        st->improvements = w.h_ctrl[B_IMPR];
+        st->csr_edges = w.h_ctrl[B_EDGES];
+        st->items = w.h_ctrl[B_ITEMS_TOT];
+    }
+    return C2G_OK;
+}
+
+static int batch_checksums(Graph* g, int n, int64_t* sums, int32_t* reached, int64_t* medges_total) {
+    BatchWs& w = g->batch;
+    BatchState* s = bstate(g);
+    const int groups = (n + 63) / 64;
+    const int64_t V = g->V;
+    HIP_TRY(hipMemsetAsync(s->sums, 0, sizeof(int64_t) * 64 * groups, g->stream));
+    HIP_TRY(hipMemsetAsync(s->reach, 0, sizeof(int32_t) * 64 * groups, g->stream));
+    HIP_TRY(hipMemsetAsync(s->medges, 0, sizeof(int64_t) * 64 * groups, g->stream));
+    for (int gi = 0; gi < groups; ++gi)
+        hipLaunchKernelGGL(batch_checksum_k, dim3(std::max(1, g->num_cus) * 4), dim3(256), 0, g->stream,
+                           g->d_row, w.dist + (int64_t)gi * V * 64, V, s->sums + 64 * gi,
+                           s->reach + 64 * gi, s->medges + 64 * gi);
+    HIP_TRY(hipGetLastError());
+    std::vector<int64_t> hs(64 * groups), hm(64 * groups);
+    std::vector<int32_t> hr(64 * groups);
+    HIP_TRY(hipMemcpyAsync(hs.data(), s->sums, sizeof(int64_t) * 64 * groups, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipMemcpyAsync(hr.data(), s->reach, sizeof(int32_t) * 64 * groups, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipMemcpyAsync(hm.data(), s->medges, sizeof(int64_t) * 64 * groups, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    int64_t mt = 0;
+    for (int q = 0; q < n; ++q) {
+        if (sums) sums[q] = hs[q];
+        if (reached) reached[q] = hr[q];
+        mt += hm[q];
+    }
+    if (medges_total) *medges_total = mt;
+    return C2G_OK;
+}
+
+static int check_srcs(const Graph* g, const int32_t* srcs, int32_t n) {
+    if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");
+    if (!g->d_adj) return fail(C2G_ENOTSUP, "graph has no int weights (wlist) for SSSP");
+    if (n < 1) return fail(C2G_EINVAL, "need at least one source");
+    if (n > 64 * 1024) return fail(C2G_EINVAL, "at most 65536 sources per batch");
+    for (int32_t q = 0; q < n; ++q)
+        if (srcs[q] < 0 || srcs[q] >= g->V) return fail(C2G_EINVAL, "source %d out of range [0,%d)", srcs[q], g->V);
+    return C2G_OK;
+}
+
+int sssp_batch_group(Graph* g, const int32_t* srcs, int32_t n, int32_t* dist_out, int64_t* sums,
+                     int32_t* reached, c2g_stats* st) {
+    int rc = check_srcs(g, srcs, n);
+    if (rc) return rc;
+    const int groups = (n + 63) / 64;
+    if ((rc = ensure_batch(g, groups))) return rc;
+    const double t0 = now_ms();
+    double relax_ms = 0;
+    const bool prof = g->tune.profile != 0;
+    if ((rc = batch_run(g, srcs, n, st, prof ? &relax_ms : nullptr))) return rc;
+    const double t1 = now_ms();
+    int64_t medges = 0;
+    if ((rc = batch_checksums(g, n, sums, reached, &medges))) return rc;
+    if (st) {
+        st->ms = t1 - t0;
+        st->reached_edges = medges;
+        st->kernel_ms = relax_ms;
+        int32_t r = 0;
+        std::vector<int32_t> rr(n);
+        if (!reached) { (void)batch_checksums(g, n, nullptr, rr.data(), nullptr); reached = rr.data(); }
+        for (int q = 0; q < n; ++q) r += reached[q];
+        st->reached = r;
+    }
+    if (dist_out) {
+        // Device layout [g][V][64] -> caller layout [q][V].
+        const int64_t V = g->V;
+        std::vector<int32_t> blk((size_t)V * 64);
+        for (int gi = 0; gi < groups; ++gi) {
+            HIP_TRY(hipMemcpy(blk.data(), g->batch.dist + (int64_t)gi * V * 64, sizeof(int32_t) * V * 64,
+                              hipMemcpyDeviceToHost));
+            const int lanes = std::min(64, n - 64 * gi);
+            parallel_for(V, [&](int64_t lo, int64_t hi, int) {
+                for (int l = 0; l < lanes; ++l) {
+                    int32_t* out = dist_out + (int64_t)(64 * gi + l) * V;
+                    for (int64_t v = lo; v < hi; ++v) out[v] = blk[(size_t)v * 64 + l];
+                }
+            });
+        }
+    }
+    return C2G_OK;
+}
+
+}  // namespace c2g
+
+using namespace c2g;
+
+extern "C" {
+
+int c2g_sssp_batch(c2g_graph* h, const int32_t* srcs, int32_t n, int32_t* dist_out, c2g_stats* st) {
+    if (!h || !srcs || !dist_out) return fail(C2G_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(h->g.mu);
+    if (st) memset(st, 0, sizeof *st);
+    return sssp_batch_group(&h->g, srcs, n, dist_out, nullptr, nullptr, st);
+}
+
+int c2g_sssp_batch_resident(c2g_graph* h, const int32_t* srcs, int32_t n, int64_t* sums,
+                            int32_t* reached, c2g_stats* st) {
+    if (!h || !srcs) return fail(C2G_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(h->g.mu);
+    if (st) memset(st, 0, sizeof *st);
+    return sssp_batch_group(&h->g, srcs, n, nullptr, sums, reached, st);
+}
+
+}  // extern "C"

@@ -13,8 +13,6 @@ int c2g_prune_ppr_mem(int32_t, const int32_t*, const int32_t*, const float*, int
                       float**, int32_t**, int32_t**, float**) {
     return fail(C2G_ENOTSUP, "not implemented");
 }
-int c2g_sssp_batch(c2g_graph*, const int32_t*, int32_t, int32_t*, c2g_stats*) { return fail(C2G_ENOTSUP, "not implemented"); }
-int c2g_sssp_batch_resident(c2g_graph*, const int32_t*, int32_t, int64_t*, int32_t*, c2g_stats*) { return fail(C2G_ENOTSUP, "not implemented"); }
 int c2g_ppr(c2g_graph*, int32_t, float, float, float*, float*, c2g_stats*) { return fail(C2G_ENOTSUP, "not implemented"); }
 int c2g_ppr_batch(c2g_graph*, const int32_t*, int32_t, float, float, float*, float*, c2g_stats*) { return fail(C2G_ENOTSUP, "not implemented"); }
 int c2g_adsorption(c2g_graph*, const int32_t*, const int32_t*, int32_t, int32_t, float, float, float, int32_t, float*, c2g_stats*) { return fail(C2G_ENOTSUP, "not implemented"); }

@@ -83,7 +83,10 @@ typedef struct {
     int64_t edges_relaxed;   /* R: edge relaxations attempted                     */
     int64_t improvements;    /* U: relaxations that lowered a value               */
     int64_t reached_edges;   /* m_s = sum of outdeg over reached vertices (GTEPS) */
+    int64_t csr_edges;       /* batched: CSR edges streamed (once for all lanes)  */
+    int64_t items;           /* frontier work items processed                     */
     double  ms;              /* wall time of the call's device work               */
+    double  kernel_ms;       /* profile runs: summed duration of the relax kernel */
 } c2g_stats;
 
 /* Tunables picked by the autotuner (c2g_autotune) or set by hand. */
@@ -95,7 +98,9 @@ typedef struct {
     int32_t iters_per_graph;/* iterations captured per hipGraph replay         */
     int32_t use_graph;      /* 1 = replay captured iterations, 0 = plain launches */
     int32_t delta;          /* near/far split step for SSSP (0 = plain Bellman-Ford) */
-    int32_t reserved[9];
+    int32_t profile;        /* 1 = plain launches with HIP events around each relax
+                               kernel; the summed time lands in c2g_stats.kernel_ms */
+    int32_t reserved[8];
 } c2g_tune;
 
 /* ======================================================================= */

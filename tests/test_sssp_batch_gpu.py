@@ -1,0 +1,102 @@
+"""Batched multi-source SSSP (lane = query) on the GPU: every query's distances
+must be bit-identical to the oracle's Dijkstra and to the single-source path.
+"""
+import numpy as np
+import pytest
+
+import c2g
+import oracle
+from conftest import INT_MAX, check_sssp_certificate, random_graph, rmat_graph
+
+pytestmark = pytest.mark.gpu
+
+
+def _graph(G):
+    return c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], device=0)
+
+
+def test_batch_c1_against_dijkstra(gpu):
+    G = rmat_graph(16, 1, wmax=64)
+    deg = np.diff(G["vlist"])
+    rng = np.random.default_rng(7)
+    srcs = np.concatenate([[np.argmax(deg)], rng.choice(np.flatnonzero(deg > 0), 40),
+                           rng.choice(np.flatnonzero(deg == 1), 20), rng.choice(np.flatnonzero(deg == 0), 3)])
+    with _graph(G) as g:
+        d, st = g.sssp_batch(srcs)
+    assert st.lane == c2g.LANE_GPU
+    for q, s in enumerate(srcs):
+        ref = oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], int(s))
+        assert np.array_equal(d[q], ref), (q, s)
+
+
+def test_batch_multi_group_matches_single(gpu):
+    # 150 queries = 3 groups (64, 64, 22 lanes), duplicate sources included
+    G = rmat_graph(16, 1, wmax=64)
+    deg = np.diff(G["vlist"])
+    rng = np.random.default_rng(8)
+    srcs = rng.choice(np.flatnonzero(deg > 0), 150)
+    srcs[5] = srcs[70] = srcs[140]
+    with _graph(G) as g:
+        d, _ = g.sssp_batch(srcs)
+        sums, reached, _ = g.sssp_batch_resident(srcs)
+        sums_e, reached_e, _ = g.sssp_each_resident(srcs)
+        for q in range(0, 150, 7):
+            ds, _ = g.sssp(int(srcs[q]))
+            assert np.array_equal(d[q], ds), q
+    fin = d != INT_MAX
+    assert np.array_equal(sums, np.where(fin, d, 0).sum(axis=1))
+    assert np.array_equal(reached, fin.sum(axis=1))
+    assert np.array_equal(sums, sums_e) and np.array_equal(reached, reached_e)
+
+
+@pytest.mark.parametrize("chunk", [1, 5, 32, 64, 200])
+def test_batch_chunk_sizes(gpu, chunk):
+    G = random_graph(300, 3000, seed=chunk, wmax=20)
+    srcs = np.arange(0, 300, 3)
+    with _graph(G) as g:
+        g.set_tune(batch_chunk=chunk)
+        d, _ = g.sssp_batch(srcs)
+    for q, s in enumerate(srcs):
+        assert np.array_equal(d[q], oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], int(s))), (chunk, s)
+
+
+def test_batch_graph_replay_off(gpu):
+    G = random_graph(500, 4000, seed=2, wmax=9)
+    srcs = np.arange(64)
+    with _graph(G) as g:
+        g.set_tune(use_graph=0, iters_per_graph=2)
+        d0, _ = g.sssp_batch(srcs)
+        g.set_tune(use_graph=1, iters_per_graph=16)
+        d1, _ = g.sssp_batch(srcs)
+        g.set_tune(profile=1)
+        d2, st = g.sssp_batch(srcs)
+    assert np.array_equal(d0, d1) and np.array_equal(d0, d2)
+    assert st.kernel_ms > 0
+
+
+def test_batch_errors(gpu):
+    G = random_graph(20, 40, seed=1)
+    with _graph(G) as g:
+        with pytest.raises(c2g.C2GError) as e:
+            g.sssp_batch(np.array([1, 20]))
+        assert e.value.code == c2g.EINVAL
+
+
+@pytest.mark.slow
+def test_batch_scale20_certificate(gpu):
+    # uneven reach: half hubs, half out-degree-1 sources (the C4 recipe at scale 20)
+    G = rmat_graph(20, 4, wmax=1023)
+    deg = np.diff(G["vlist"])
+    rng = np.random.default_rng(4)
+    top = np.argsort(-deg)[: len(deg) // 100]
+    srcs = np.concatenate([rng.choice(top, 64, replace=False), rng.choice(np.flatnonzero(deg == 1), 64)])
+    with _graph(G) as g:
+        d, st = g.sssp_batch(srcs)
+        sums_e, reached_e, _ = g.sssp_each_resident(srcs)
+    fin = d != INT_MAX
+    assert np.array_equal(np.where(fin, d, 0).sum(axis=1), sums_e)
+    assert np.array_equal(fin.sum(axis=1), reached_e)
+    for q in (0, 1, 63, 64, 100, 127):
+        check_sssp_certificate(G["vlist"], G["elist"], G["wlist"], int(srcs[q]), d[q])
+    ref = oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], int(srcs[0]))
+    assert np.array_equal(d[0], ref)
