@@ -95,6 +95,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--skip-profile", action="store_true", help="skip the profiled roofline pass")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--single-profile", action="store_true", help="profile two single-source queries")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the profiled pass (for rocprofv3: it then sees exactly those launches)")
     args = ap.parse_args()
@@ -167,8 +168,22 @@ def main():
             "kernel": "relax_b_k<false>", "launches": sp.iterations,
             "avg_launch_us": round(sp.kernel_ms * 1e3 / max(1, sp.iterations), 2),
             "algo_bytes_per_launch": int(algo_bytes / max(1, sp.iterations)),
+            "counters": {"csr_edges": sp.csr_edges, "vertex_group_activations": sp.frontier_total,
+                         "query_edge_relaxations": sp.edges_relaxed, "improvements": sp.improvements,
+                         "items": sp.items, "rounds": sp.iterations},
         }
         log(f"profile: relax {sp.kernel_ms:.1f} ms over {sp.iterations} launches; {achieved:.0f} GB/s algorithmic")
+
+    if args.single_profile:
+        # single-source kernel, profiled: one hub and one out-degree-1 source
+        g.set_tune(profile=1)
+        for s in (int(srcs[0]), int(srcs[-1])):
+            d_, s1 = g.sssp(s)
+            log(f"single-source profile src {s}: rounds {s1.iterations} relax {s1.kernel_ms:.2f} ms "
+                f"wall {s1.ms:.2f} ms edges {s1.edges_relaxed} impr {s1.improvements} items {s1.items} "
+                f"frontier {s1.frontier_total} reached {s1.reached}")
+            extra.setdefault("single_profile", []).append(s1.as_dict())
+        g.set_tune(profile=0)
 
     if not args.skip_cpu:
         threads = min(16, os.cpu_count() or 1)

@@ -39,6 +39,7 @@ struct SsspWs {
     uint8_t* hid = nullptr;      // [V] 1 = distance was set through a hidden edge
     hipGraphExec_t exec = nullptr;
     int exec_iters = 0, exec_block = 0, exec_grid = 0, exec_chunk = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
 // Device workspace of batched SSSP (64 queries per group, lane = query).

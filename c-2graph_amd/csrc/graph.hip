@@ -114,7 +114,16 @@ int device_upload(Graph* g, int device) {
     HIP_TRY(hipMalloc(&g->d_row, sizeof(int32_t) * (V + 1)));
     HIP_TRY(hipMemcpy(g->d_row, g->vl.data(), sizeof(int32_t) * (V + 1), hipMemcpyHostToDevice));
     std::vector<int2> tmp;
-    auto pack = [&](const std::vector<int32_t>& e, const void* w, int64_t n, int2** dst) -> int {
+    // nz[v] = v has live or hidden out-edges.  SSSP edge entries carry it in
+    // bit 31 of the target id (vertex ids are < 2^31) so that kernels skip
+    // activating dead-end targets without another load.
+    std::vector<uint8_t> nz((size_t)V, 0);
+    for (int64_t v = 0; v < V; ++v) {
+        nz[v] = g->vl[v + 1] > g->vl[v];
+        if (!g->hvl.empty() && g->hvl[v + 1] > g->hvl[v]) nz[v] = 1;
+    }
+    auto pack = [&](const std::vector<int32_t>& e, const void* w, int64_t n, int2** dst,
+                    bool flag) -> int {
         HIP_TRY(hipMalloc(dst, sizeof(int2) * (size_t)std::max<int64_t>(n, 1)));
         const int64_t step = 1 << 24;  // pack and copy in 128 MiB slices
         tmp.resize((size_t)std::min<int64_t>(std::max<int64_t>(n, 1), step));
@@ -122,7 +131,10 @@ int device_upload(Graph* g, int device) {
             int64_t m = std::min<int64_t>(step, n - o);
             const int32_t* ww = (const int32_t*)w;
             parallel_for(m, [&](int64_t lo, int64_t hi, int) {
-                for (int64_t i = lo; i < hi; ++i) tmp[i] = make_int2(e[o + i], ww[o + i]);
+                for (int64_t i = lo; i < hi; ++i) {
+                    const int32_t t = e[o + i];
+                    tmp[i] = make_int2(flag && nz[t] ? (int32_t)((uint32_t)t | 0x80000000u) : t, ww[o + i]);
+                }
             });
             HIP_TRY(hipMemcpy(*dst + o, tmp.data(), sizeof(int2) * m, hipMemcpyHostToDevice));
         }
@@ -130,13 +142,13 @@ int device_upload(Graph* g, int device) {
     };
     int rc;
     // An edgeless graph still gets (1-element) edge arrays so every query runs.
-    if ((!g->wl.empty() || E == 0) && (rc = pack(g->el, g->wl.data(), E, &g->d_adj))) return rc;
-    if ((!g->fl.empty() || E == 0) && (rc = pack(g->el, g->fl.data(), E, &g->d_adjf))) return rc;
+    if ((!g->wl.empty() || E == 0) && (rc = pack(g->el, g->wl.data(), E, &g->d_adj, true))) return rc;
+    if ((!g->fl.empty() || E == 0) && (rc = pack(g->el, g->fl.data(), E, &g->d_adjf, false))) return rc;
     if (!g->hvl.empty()) {
         HIP_TRY(hipMalloc(&g->d_hrow, sizeof(int32_t) * (V + 1)));
         HIP_TRY(hipMemcpy(g->d_hrow, g->hvl.data(), sizeof(int32_t) * (V + 1), hipMemcpyHostToDevice));
-        if (!g->hwl.empty() && (rc = pack(g->hel, g->hwl.data(), g->HE, &g->d_hadj))) return rc;
-        if (!g->hwf.empty() && (rc = pack(g->hel, g->hwf.data(), g->HE, &g->d_hadjf))) return rc;
+        if (!g->hwl.empty() && (rc = pack(g->hel, g->hwl.data(), g->HE, &g->d_hadj, true))) return rc;
+        if (!g->hwf.empty() && (rc = pack(g->hel, g->hwf.data(), g->HE, &g->d_hadjf, false))) return rc;
     }
     (void)align_up;
     g->on_device = true;
@@ -158,6 +170,7 @@ void device_release(Graph* g) {
     dfree(g->sssp.dist); dfree(g->sssp.stamp); dfree(g->sssp.q[0]); dfree(g->sssp.q[1]);
     dfree(g->sssp.ctrl); dfree(g->sssp.hid);
     if (g->sssp.h_ctrl) (void)hipHostFree(g->sssp.h_ctrl);
+    for (auto& e : g->sssp.ev) if (e) (void)hipEventDestroy(e);
     dfree(g->batch.dist); dfree(g->batch.mask[0]); dfree(g->batch.mask[1]); dfree(g->batch.hmask);
     dfree(g->batch.q[0]); dfree(g->batch.q[1]); dfree(g->batch.ctrl); dfree(g->batch.sums);
     dfree(g->batch.reach);

@@ -39,32 +39,41 @@ namespace c2g {
 
 enum : int {
     C_CUR = 0,    // items in the current round's queue
-    C_NEXT = 1,   // items appended for the next round
-    C_ITER = 2,   // rounds completed
-    C_DONE = 3,   // workgroups finished in this launch
+    C_SPARE = 1,
+    C_ITER = 2,   // rounds with work
+    C_DONE = 3,   // workgroups finished (relax)
     C_FRONT = 4,  // sum of frontier vertices over rounds
     C_EDGES = 5,  // edges queued for relaxation (R)
     C_IMPR = 6,   // improving relaxations (U)
     C_OVF = 7,    // queue overflow guard tripped (must stay 0)
     C_HEDGES = 8, // hidden edges queued
+    C_DONE_E = 9, // workgroups finished (expand)
+    C_ITEMS = 10, // items processed
     C_N = 16
 };
 
-__global__ void sssp_init_k(int32_t* __restrict__ dist, int32_t* __restrict__ stamp,
-                            int32_t* __restrict__ hstamp, uint8_t* __restrict__ hid, int64_t V) {
+// Device edge entries carry a flag in bit 31 of the target id: set when the
+// target has out-edges (or hidden edges), i.e. when activating it can lead
+// anywhere.  Vertex ids are < 2^31, so the bit is otherwise unused.
+constexpr int kIdMask = 0x7fffffff;
+
+__global__ void sssp_init_k(int32_t* __restrict__ dist, uint32_t* __restrict__ bits,
+                            uint32_t* __restrict__ hbits, uint8_t* __restrict__ hid, int64_t V) {
+    const int64_t nw = (V + 31) >> 5;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < V;
          i += (int64_t)gridDim.x * blockDim.x) {
         dist[i] = INT32_MAX;
-        stamp[i] = -1;
-        if (hstamp) hstamp[i] = -1;
         if (hid) hid[i] = 0;
+        if (i < nw) {
+            bits[i] = 0;
+            if (hbits) hbits[i] = 0;
+        }
     }
 }
 
 // Queues the source: live items and (pruned graphs) hidden items.
 __global__ void sssp_seed_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
-                            int32_t* __restrict__ dist, int32_t* __restrict__ stamp,
-                            int32_t* __restrict__ hstamp, uint8_t* __restrict__ hid,
+                            int32_t* __restrict__ dist, uint8_t* __restrict__ hid,
                             int4* __restrict__ q, int64_t* __restrict__ ctrl, int32_t src,
                             int chunk) {
     const int r0 = row[src], r1 = row[src + 1];
@@ -82,8 +91,6 @@ __global__ void sssp_seed_k(const int32_t* __restrict__ row, const int32_t* __re
     }
     if (threadIdx.x == 0) {
         dist[src] = 0;
-        stamp[src] = 0;
-        if (hstamp) hstamp[src] = 0;
         if (hid) hid[src] = 1;  // the source's hidden list is always relaxed
         for (int i = 0; i < C_N; ++i) ctrl[i] = 0;
         ctrl[C_CUR] = nl + nh;
@@ -93,130 +100,168 @@ __global__ void sssp_seed_k(const int32_t* __restrict__ row, const int32_t* __re
     }
 }
 
+// Relaxation of one 64-edge step for lane k; returns the improved target or -1.
+template <bool PRUNED>
+__device__ __forceinline__ void relax_edge(const int2* __restrict__ adj, const int2* __restrict__ hadj,
+                                           int32_t* __restrict__ dist, uint32_t* __restrict__ bits,
+                                           uint32_t* __restrict__ hbits, uint8_t* __restrict__ hid,
+                                           int2 a, int d_u, int okind, int64_t& impr) {
+    const int v = a.x & kIdMask;
+    const int w = a.y;
+    if (d_u > INT32_MAX - w) return;
+    const int cand = d_u + w;
+    if (cand >= dist[v]) return;
+    if (cand >= atomicMin(&dist[v], cand)) return;
+    ++impr;
+    if (a.x < 0) atomicOr(&bits[v >> 5], 1u << (v & 31));
+    if (PRUNED) {
+        if (okind) hid[v] = 1;
+        if (okind || hid[v]) atomicOr(&hbits[v >> 5], 1u << (v & 31));
+    }
+}
+
+// One round of relaxation.  A wave takes 64 items, scans their edge counts
+// and walks the concatenated edges 128 at a time (two 64-edge steps whose
+// loads are in flight together); improved targets are marked in the
+// activation bitmap with fire-and-forget atomicOr (no queue atomics here).
 template <bool PRUNED>
 __global__ void __launch_bounds__(256)
-sssp_relax_k(const int32_t* __restrict__ row, const int2* __restrict__ adj,
-             const int32_t* __restrict__ hrow, const int2* __restrict__ hadj,
-             int32_t* __restrict__ dist, int32_t* __restrict__ stamp,
-             int32_t* __restrict__ hstamp, uint8_t* __restrict__ hid,
-             const int4* __restrict__ qin, int4* __restrict__ qout,
-             int64_t* __restrict__ ctrl, int chunk, int64_t qcap) {
+sssp_relax_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
+             int32_t* __restrict__ dist, uint32_t* __restrict__ bits,
+             uint32_t* __restrict__ hbits, uint8_t* __restrict__ hid,
+             const int4* __restrict__ q, int64_t* __restrict__ ctrl) {
     const int64_t n = ctrl[C_CUR];
-    const int next_it = (int)ctrl[C_ITER] + 1;
     const int lane = lane_id();
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    int64_t impr = 0, front = 0, qedges = 0, qhedges = 0;
+    int64_t impr = 0;
 
     for (int64_t base = wave * kWave; base < n; base += nwaves * kWave) {
         const int64_t i = base + lane;
-        int u = 0, eb = 0, cnt = 0, du = 0, kind = 0;
+        int eb = 0, cnt = 0, du = INT32_MAX, kind = 0;
         if (i < n) {
-            int4 it = qin[i];
-            u = it.x;
+            const int4 it = q[i];
             eb = it.y;
             cnt = it.z - it.y;
             kind = it.w;
-            du = dist[u];
+            du = dist[it.x];
         }
         int tot;
         const int pre = wave_excl_scan(cnt, &tot);
-        for (int s = 0; s < tot; s += kWave) {
-            const int k = s + lane;
-            const int o = wave_owner(pre, k);
-            const int e = __shfl(eb, o, 64) + (k - __shfl(pre, o, 64));
-            const int d_u = __shfl(du, o, 64);
-            const int okind = PRUNED ? __shfl(kind, o, 64) : 0;
-            bool claim_live = false, claim_hid = false;
-            int v = 0;
-            if (k < tot) {
-                const int2 a = (PRUNED && okind) ? hadj[e] : adj[e];
-                v = a.x;
-                const int w = a.y;
-                if (d_u <= INT32_MAX - w) {
-                    const int cand = d_u + w;
-                    if (cand < dist[v]) {
-                        const int old = atomicMin(&dist[v], cand);
-                        if (cand < old) {
-                            ++impr;
-                            claim_live = atomicExch(&stamp[v], next_it) != next_it;
-                            if (PRUNED) {
-                                if (okind) hid[v] = 1;
-                                if (okind || hid[v])
-                                    claim_hid = atomicExch(&hstamp[v], next_it) != next_it;
-                            }
-                        }
-                    }
-                }
-            }
-            // Append claimed vertices as chunked items (wave-aggregated).
-            if (__ballot(claim_live || claim_hid)) {
-                int r0 = 0, r1 = 0, h0 = 0, h1 = 0, nl = 0, nh = 0;
-                if (claim_live) {
-                    r0 = row[v];
-                    r1 = row[v + 1];
-                    nl = (r1 - r0 + chunk - 1) / chunk;
-                    front += 1;
-                    qedges += r1 - r0;
-                }
-                if (PRUNED && claim_hid) {
-                    h0 = hrow[v];
-                    h1 = hrow[v + 1];
-                    nh = (h1 - h0 + chunk - 1) / chunk;
-                    qhedges += h1 - h0;
-                }
-                int ntot;
-                const int off = wave_excl_scan(nl + nh, &ntot);
-                if (ntot > 0) {
-                    int64_t qb = 0;
-                    if (lane == 0)
-                        qb = (int64_t)atomicAdd((unsigned long long*)&ctrl[C_NEXT],
-                                                (unsigned long long)ntot);
-                    qb = shfl64(qb, 0);
-                    if (qb + ntot > qcap) {
-                        if (lane == 0) atomicOr((unsigned long long*)&ctrl[C_OVF], 1ull);
-                    } else {
-                        int4* dst = qout + qb + off;
-                        for (int j = 0; j < nl; ++j) {
-                            int b = r0 + j * chunk;
-                            dst[j] = make_int4(v, b, min(b + chunk, r1), 0);
-                        }
-                        for (int j = 0; j < nh; ++j) {
-                            int b = h0 + j * chunk;
-                            dst[nl + j] = make_int4(v, b, min(b + chunk, h1), 1);
-                        }
-                    }
-                }
-            }
+        for (int s = 0; s < tot; s += 2 * kWave) {
+            const int k0 = s + lane, k1 = s + kWave + lane;
+            const int o0 = wave_owner(pre, k0), o1 = wave_owner(pre, k1);
+            const int e0 = __shfl(eb, o0, 64) + (k0 - __shfl(pre, o0, 64));
+            const int e1 = __shfl(eb, o1, 64) + (k1 - __shfl(pre, o1, 64));
+            const int d0 = __shfl(du, o0, 64), d1 = __shfl(du, o1, 64);
+            const int kd0 = PRUNED ? __shfl(kind, o0, 64) : 0;
+            const int kd1 = PRUNED ? __shfl(kind, o1, 64) : 0;
+            int2 a0 = make_int2(0, 0), a1 = make_int2(0, 0);
+            if (k0 < tot) a0 = (PRUNED && kd0) ? hadj[e0] : adj[e0];
+            if (k1 < tot) a1 = (PRUNED && kd1) ? hadj[e1] : adj[e1];
+            if (k0 < tot) relax_edge<PRUNED>(adj, hadj, dist, bits, hbits, hid, a0, d0, kd0, impr);
+            if (k1 < tot) relax_edge<PRUNED>(adj, hadj, dist, bits, hbits, hid, a1, d1, kd1, impr);
         }
     }
-    // Statistics: one atomic per wave.
     impr = wave_sum64(impr);
-    front = wave_sum64(front);
-    qedges = wave_sum64(qedges);
-    if (PRUNED) qhedges = wave_sum64(qhedges);
-    if (lane == 0 && (impr | front | qedges | qhedges)) {
-        atomicAdd((unsigned long long*)&ctrl[C_IMPR], (unsigned long long)impr);
-        atomicAdd((unsigned long long*)&ctrl[C_FRONT], (unsigned long long)front);
-        atomicAdd((unsigned long long*)&ctrl[C_EDGES], (unsigned long long)qedges);
-        if (PRUNED) atomicAdd((unsigned long long*)&ctrl[C_HEDGES], (unsigned long long)qhedges);
-    }
-    // The last workgroup to finish turns the next queue into the current one.
+    if (lane == 0 && impr) atomicAdd((unsigned long long*)&ctrl[C_IMPR], (unsigned long long)impr);
+    // The last workgroup to finish retires the queue (expand refills it).
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
         unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[C_DONE], 1ull);
         if (prev == (unsigned long long)gridDim.x - 1) {
             __threadfence();
-            int64_t nn = (int64_t)atomicAdd((unsigned long long*)&ctrl[C_NEXT], 0ull);
-            if (n > 0) ctrl[C_ITER] = next_it;
+            if (n > 0) ctrl[C_ITER] += 1;
+            ctrl[C_ITEMS] += n;
+            ctrl[C_CUR] = 0;
+            ctrl[C_DONE] = 0;
+        }
+    }
+}
+
+// Compacts the activation bitmaps into the next round's items: each lane owns
+// one 32-vertex word, counts the items of its set bits (ceil(deg/chunk) per
+// live list, the same for hidden lists), the wave scans the counts and takes
+// its queue range with a single atomicAdd, and the words are cleared.
+template <bool PRUNED>
+__global__ void __launch_bounds__(256)
+sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
+              uint32_t* __restrict__ bits, uint32_t* __restrict__ hbits, int4* __restrict__ q,
+              int64_t* __restrict__ ctrl, int64_t V, int chunk, int64_t qcap) {
+    const int lane = lane_id();
+    const int64_t nw = (V + 31) >> 5;
+    const int64_t nwr = (nw + 63) & ~(int64_t)63;  // uniform trip count per wave
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t front = 0, edges = 0, hedges = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nwr; i += stride) {
+        uint32_t wl = 0, wh = 0;
+        if (i < nw) {
+            wl = bits[i];
+            if (wl) bits[i] = 0;
+            if (PRUNED) {
+                wh = hbits[i];
+                if (wh) hbits[i] = 0;
+            }
+        }
+        int cnt = 0;
+        for (uint32_t m = wl; m; m &= m - 1) {
+            const int v = (int)(i * 32 + __builtin_ctz(m));
+            const int d = row[v + 1] - row[v];
+            cnt += (d + chunk - 1) / chunk;
+            front += 1;
+            edges += d;
+        }
+        if (PRUNED)
+            for (uint32_t m = wh; m; m &= m - 1) {
+                const int v = (int)(i * 32 + __builtin_ctz(m));
+                const int d = hrow[v + 1] - hrow[v];
+                cnt += (d + chunk - 1) / chunk;
+                hedges += d;
+            }
+        int tot;
+        const int off = wave_excl_scan(cnt, &tot);
+        if (tot == 0) continue;
+        int64_t base = 0;
+        if (lane == 0) base = (int64_t)atomicAdd((unsigned long long*)&ctrl[C_CUR], (unsigned long long)tot);
+        base = shfl64(base, 0);
+        if (base + tot > qcap) {
+            if (lane == 0) atomicOr((unsigned long long*)&ctrl[C_OVF], 1ull);
+            continue;
+        }
+        int64_t p = base + off;
+        for (uint32_t m = wl; m; m &= m - 1) {
+            const int v = (int)(i * 32 + __builtin_ctz(m));
+            const int r0 = row[v], r1 = row[v + 1];
+            for (int b = r0; b < r1; b += chunk) q[p++] = make_int4(v, b, min(b + chunk, r1), 0);
+        }
+        if (PRUNED)
+            for (uint32_t m = wh; m; m &= m - 1) {
+                const int v = (int)(i * 32 + __builtin_ctz(m));
+                const int h0 = hrow[v], h1 = hrow[v + 1];
+                for (int b = h0; b < h1; b += chunk) q[p++] = make_int4(v, b, min(b + chunk, h1), 1);
+            }
+    }
+    front = wave_sum64(front);
+    edges = wave_sum64(edges);
+    if (PRUNED) hedges = wave_sum64(hedges);
+    if (lane == 0 && (front | hedges)) {
+        atomicAdd((unsigned long long*)&ctrl[C_FRONT], (unsigned long long)front);
+        atomicAdd((unsigned long long*)&ctrl[C_EDGES], (unsigned long long)edges);
+        if (PRUNED) atomicAdd((unsigned long long*)&ctrl[C_HEDGES], (unsigned long long)hedges);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[C_DONE_E], 1ull);
+        if (prev == (unsigned long long)gridDim.x - 1) {
+            __threadfence();
+            ctrl[C_DONE_E] = 0;
+            int64_t nn = (int64_t)atomicAdd((unsigned long long*)&ctrl[C_CUR], 0ull);
             if (nn > qcap) {  // items past the queue were dropped: stop, report
                 ctrl[C_OVF] = 1;
-                nn = 0;
+                ctrl[C_CUR] = 0;
             }
-            ctrl[C_CUR] = nn;
-            ctrl[C_NEXT] = 0;
-            ctrl[C_DONE] = 0;
         }
     }
 }
@@ -248,22 +293,25 @@ int ensure_sssp_ws(Graph* g) {
     SsspWs& w = g->sssp;
     const int64_t V = g->V;
     const int chunk = std::max(1, g->tune.chunk);
-    // Each vertex is claimed at most once per round for its live list and once
-    // for its hidden list, giving at most ceil(d/chunk) <= d/chunk + 1 items each.
+    // Each vertex is activated at most once per round for its live list and
+    // once for its hidden list: at most ceil(d/chunk) <= d/chunk + 1 items each.
     const int64_t need = 2 * V + (g->E + g->HE) / chunk + 64;
+    const int64_t nw = (V + 31) / 32 + 1;
     HIP_TRY(hipSetDevice(g->device));
     if (!w.dist) {
         HIP_TRY(hipMalloc(&w.dist, sizeof(int32_t) * std::max<int64_t>(V, 1)));
-        HIP_TRY(hipMalloc(&w.stamp, sizeof(int32_t) * std::max<int64_t>(V, 1) * 2));
+        // live and hidden activation bitmaps
+        HIP_TRY(hipMalloc(&w.stamp, sizeof(uint32_t) * nw * 2));
         HIP_TRY(hipMalloc(&w.ctrl, sizeof(int64_t) * C_N));
+        HIP_TRY(hipEventCreate(&w.ev[0]));
+        HIP_TRY(hipEventCreate(&w.ev[1]));
         HIP_TRY(hipHostMalloc((void**)&w.h_ctrl, sizeof(int64_t) * C_N, hipHostMallocDefault));
         if (g->d_hrow) HIP_TRY(hipMalloc(&w.hid, std::max<int64_t>(V, 1)));
     }
     if (w.qcap < need) {
         if (w.exec) { (void)hipGraphExecDestroy(w.exec); w.exec = nullptr; }
-        if (w.q[0]) { (void)hipFree(w.q[0]); (void)hipFree(w.q[1]); w.q[0] = w.q[1] = nullptr; }
+        if (w.q[0]) { (void)hipFree(w.q[0]); w.q[0] = nullptr; }
         HIP_TRY(hipMalloc(&w.q[0], sizeof(int4) * need));
-        HIP_TRY(hipMalloc(&w.q[1], sizeof(int4) * need));
         w.qcap = need;
     }
     return C2G_OK;
@@ -273,32 +321,47 @@ static int relax_grid(const Graph* g) {
     return std::max(1, g->num_cus) * g->tune.grid_per_cu;
 }
 
-// Launches rounds [0, k) of relax kernels starting with queue `par`.
-static int launch_rounds(Graph* g, int k, int par) {
+static bool is_pruned(const Graph* g) { return g->d_hrow != nullptr && g->d_hadj != nullptr; }
+
+// Launches k rounds (relax + expand).  With relax_ms non-null (profiling),
+// HIP events bracket each relax kernel and the time is accumulated.
+static int launch_rounds(Graph* g, int k, double* relax_ms = nullptr) {
     SsspWs& w = g->sssp;
-    const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
-    const int block = 256;  // relax kernels are compiled for 256 threads
+    const bool pruned = is_pruned(g);
     const int grid = relax_grid(g);
-    int32_t* hstamp = pruned ? w.stamp + g->V : nullptr;
+    const int64_t nw = (g->V + 31) / 32 + 1;
+    uint32_t* bits = (uint32_t*)w.stamp;
+    uint32_t* hbits = bits + nw;
+    const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (nw + 255) / 256));
     for (int r = 0; r < k; ++r) {
-        const int4* qin = w.q[(par + r) & 1];
-        int4* qout = w.q[(par + r + 1) & 1];
-        if (pruned)
-            hipLaunchKernelGGL(sssp_relax_k<true>, dim3(grid), dim3(block), 0, g->stream, g->d_row,
-                               g->d_adj, g->d_hrow, g->d_hadj, w.dist, w.stamp, hstamp, w.hid, qin,
-                               qout, w.ctrl, g->tune.chunk, w.qcap);
-        else
-            hipLaunchKernelGGL(sssp_relax_k<false>, dim3(grid), dim3(block), 0, g->stream, g->d_row,
-                               g->d_adj, (const int32_t*)nullptr, (const int2*)nullptr, w.dist,
-                               w.stamp, (int32_t*)nullptr, (uint8_t*)nullptr, qin, qout, w.ctrl,
-                               g->tune.chunk, w.qcap);
+        if (relax_ms) HIP_TRY(hipEventRecord(w.ev[0], g->stream));
+        if (pruned) {
+            hipLaunchKernelGGL(sssp_relax_k<true>, dim3(grid), dim3(256), 0, g->stream, g->d_adj,
+                               g->d_hadj, w.dist, bits, hbits, w.hid, w.q[0], w.ctrl);
+            if (relax_ms) HIP_TRY(hipEventRecord(w.ev[1], g->stream));
+            hipLaunchKernelGGL(sssp_expand_k<true>, dim3(egrid), dim3(256), 0, g->stream, g->d_row,
+                               g->d_hrow, bits, hbits, w.q[0], w.ctrl, (int64_t)g->V, g->tune.chunk, w.qcap);
+        } else {
+            hipLaunchKernelGGL(sssp_relax_k<false>, dim3(grid), dim3(256), 0, g->stream, g->d_adj,
+                               (const int2*)nullptr, w.dist, bits, (uint32_t*)nullptr, (uint8_t*)nullptr,
+                               w.q[0], w.ctrl);
+            if (relax_ms) HIP_TRY(hipEventRecord(w.ev[1], g->stream));
+            hipLaunchKernelGGL(sssp_expand_k<false>, dim3(egrid), dim3(256), 0, g->stream, g->d_row,
+                               (const int32_t*)nullptr, bits, (uint32_t*)nullptr, w.q[0], w.ctrl,
+                               (int64_t)g->V, g->tune.chunk, w.qcap);
+        }
+        if (relax_ms) {
+            HIP_TRY(hipEventSynchronize(w.ev[1]));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
+            *relax_ms += ms;
+        }
     }
     HIP_TRY(hipGetLastError());
     return C2G_OK;
 }
 
-// A hipGraph holding `iters_per_graph` rounds (even, so the queue parity is
-// the same before and after a replay).
+// A hipGraph holding `iters_per_graph` rounds.
 static int ensure_exec(Graph* g) {
     SsspWs& w = g->sssp;
     const int K = g->tune.iters_per_graph;
@@ -307,7 +370,7 @@ static int ensure_exec(Graph* g) {
     if (w.exec) { (void)hipGraphExecDestroy(w.exec); w.exec = nullptr; }
     hipGraph_t graph;
     HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
-    int rc = launch_rounds(g, K, 0);
+    int rc = launch_rounds(g, K);
     hipError_t e = hipStreamEndCapture(g->stream, &graph);
     if (rc) return rc;
     if (e != hipSuccess) return fail(C2G_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
@@ -327,6 +390,7 @@ static void fill_stats(c2g_stats* st, const int64_t* c, double ms, int lane) {
     st->frontier_total = c[C_FRONT];
     st->edges_relaxed = c[C_EDGES] + c[C_HEDGES];
     st->improvements = c[C_IMPR];
+    st->items = c[C_ITEMS];
     st->ms = ms;
 }
 
@@ -347,27 +411,29 @@ int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* 
     int rc = ensure_sssp_ws(g);
     if (rc) return rc;
     SsspWs& w = g->sssp;
-    const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
+    const bool pruned = is_pruned(g);
+    const int64_t nw = (g->V + 31) / 32 + 1;
+    uint32_t* bits = (uint32_t*)w.stamp;
     const double t0 = now_ms();
     const int tb = 256, gridv = (int)std::min<int64_t>((g->V + tb - 1) / tb, 65535);
-    hipLaunchKernelGGL(sssp_init_k, dim3(std::max(gridv, 1)), dim3(tb), 0, g->stream, w.dist,
-                       w.stamp, pruned ? w.stamp + g->V : (int32_t*)nullptr,
-                       pruned ? w.hid : (uint8_t*)nullptr, (int64_t)g->V);
+    hipLaunchKernelGGL(sssp_init_k, dim3(std::max(gridv, 1)), dim3(tb), 0, g->stream, w.dist, bits,
+                       pruned ? bits + nw : (uint32_t*)nullptr, pruned ? w.hid : (uint8_t*)nullptr,
+                       (int64_t)g->V);
     hipLaunchKernelGGL(sssp_seed_k, dim3(1), dim3(256), 0, g->stream, g->d_row,
-                       pruned ? g->d_hrow : (const int32_t*)nullptr, w.dist, w.stamp,
-                       pruned ? w.stamp + g->V : (int32_t*)nullptr,
+                       pruned ? g->d_hrow : (const int32_t*)nullptr, w.dist,
                        pruned ? w.hid : (uint8_t*)nullptr, w.q[0], w.ctrl, src, g->tune.chunk);
     HIP_TRY(hipGetLastError());
-    const bool use_graph = g->tune.use_graph && budget <= 0;
+    const bool prof = g->tune.profile != 0;
+    const bool use_graph = g->tune.use_graph && budget <= 0 && !prof;
     if (use_graph && (rc = ensure_exec(g))) return rc;
-    int par = 0, done_rounds = 0;
+    int done_rounds = 0;
+    double relax_ms = 0;
     for (;;) {
-        int k = g->tune.iters_per_graph;
+        int k = prof ? 1 : g->tune.iters_per_graph;
         if (budget > 0) k = std::min(k, budget - done_rounds);
         if (k <= 0) break;
         if (use_graph) HIP_TRY(hipGraphLaunch(w.exec, g->stream));
-        else if ((rc = launch_rounds(g, k, par))) return rc;
-        par = (par + k) & 1;
+        else if ((rc = launch_rounds(g, k, prof ? &relax_ms : nullptr))) return rc;
         done_rounds += k;
         HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, sizeof(int64_t) * C_N, hipMemcpyDeviceToHost,
                                g->stream));
@@ -384,7 +450,7 @@ int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* 
         // Remaining frontier (budgeted runs): distinct vertices of the current queue.
         int64_t n = c[C_CUR];
         std::vector<int4> items((size_t)n);
-        if (n) HIP_TRY(hipMemcpy(items.data(), w.q[par], sizeof(int4) * n, hipMemcpyDeviceToHost));
+        if (n) HIP_TRY(hipMemcpy(items.data(), w.q[0], sizeof(int4) * n, hipMemcpyDeviceToHost));
         int32_t m = 0;
         std::vector<uint8_t> seen((size_t)g->V, 0);
         for (auto& it : items)
@@ -394,6 +460,7 @@ int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* 
             HIP_TRY(hipMemcpy(hid_out, w.hid, g->V, hipMemcpyDeviceToHost));
     }
     fill_stats(st, c, t1 - t0, C2G_LANE_GPU);
+    if (st) st->kernel_ms = relax_ms;
     if (st && copy_out && dist_out) {
         int32_t r = 0;
         for (int32_t v = 0; v < g->V; ++v) r += dist_out[v] != INT32_MAX;
@@ -482,6 +549,8 @@ int c2g_sssp_each_resident(c2g_graph* h, const int32_t* srcs, int32_t n, int64_t
             st->frontier_total += s.frontier_total;
             st->edges_relaxed += s.edges_relaxed;
             st->improvements += s.improvements;
+            st->items += s.items;
+            st->kernel_ms += s.kernel_ms;
         }
     }
     if (rc == C2G_OK) {

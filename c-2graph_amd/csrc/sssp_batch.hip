@@ -17,14 +17,15 @@
 //   relax_b  : each wave takes one item (u, e_begin, e_end, g|kind) with its
 //              lane mask; edges are loaded 64 at a time lane-parallel, then
 //              walked in groups of 8 (8 independent row loads in flight);
-//              improved lanes are OR-ed into mask[g][v] lane-parallel, and the
-//              0 -> nonzero transition appends (v,g) to the activation list
-//              (one atomicAdd per 64 edges).
-//   expand_b : drains each activated (v,g): takes mask[g][v] (clearing it),
+//              improved lanes are OR-ed into mask[g][v] lane-parallel and
+//              (v,g) is marked in the group's activation bitmap.
+//   expand_b : scans the bitmaps; for each active (v,g) takes mask[g][v] (clearing it),
 //              splits v's row into items of <= batch_chunk edges, and adds a
 //              hidden-list item for lanes in hmask (pruned graphs).
-// Vertices with neither out-edges nor hidden edges are never activated (bitmap
-// `nz`), which skips about half of all activations on R-MAT graphs.
+// Vertices with neither out-edges nor hidden edges are never activated (flag
+// bit 31 of the edge target), which skips about half of all activations on
+// R-MAT graphs.  Activation is recorded in a per-group bitmap with
+// fire-and-forget atomics; expand_b compacts it (one atomicAdd per wave).
 #include "graph.h"
 #include "wave.h"
 #include <cstring>
@@ -32,7 +33,7 @@
 namespace c2g {
 
 enum : int {
-    B_ACT = 0,     // activated (v,g) entries this round
+    B_SPARE = 0,
     B_ITEMS = 1,   // items for the next relax round
     B_ITER = 2,    // relax rounds with work
     B_DONE_R = 3,  // workgroups finished (relax)
@@ -47,9 +48,11 @@ enum : int {
 };
 
 constexpr int kJ = 8;  // edges whose row loads are in flight together
+constexpr int kIdMaskB = 0x7fffffff;  // bit 31 of an edge target = target has edges
 
 __global__ void batch_fill_k(int32_t* __restrict__ dist, int64_t n_dist, uint64_t* __restrict__ mask,
-                             uint64_t* __restrict__ hmask, int64_t n_mask) {
+                             uint64_t* __restrict__ hmask, int64_t n_mask, uint32_t* __restrict__ abits,
+                             int64_t n_bits) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     int4* d4 = reinterpret_cast<int4*>(dist);
@@ -59,36 +62,34 @@ __global__ void batch_fill_k(int32_t* __restrict__ dist, int64_t n_dist, uint64_
         mask[i] = 0;
         if (hmask) hmask[i] = 0;
     }
+    for (int64_t i = t; i < n_bits; i += stride) abits[i] = 0;
 }
 
 // Seeds lane q of group q/64 at its source (one thread per query).
-__global__ void batch_seed_k(const int32_t* __restrict__ srcs, int n, const uint32_t* __restrict__ nz,
-                             int32_t* __restrict__ dist, uint64_t* __restrict__ mask,
-                             uint64_t* __restrict__ hmask, int2* __restrict__ acts,
-                             int64_t* __restrict__ ctrl, int64_t V) {
+__global__ void batch_seed_k(const int32_t* __restrict__ srcs, int n, const int32_t* __restrict__ row,
+                             const int32_t* __restrict__ hrow, int32_t* __restrict__ dist,
+                             uint64_t* __restrict__ mask, uint64_t* __restrict__ hmask,
+                             uint32_t* __restrict__ abits, int64_t* __restrict__ ctrl, int64_t V, int64_t nw) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q == 0)
-        for (int i = 0; i < B_N; ++i)
-            if (i != B_ACT) ctrl[i] = 0;
+        for (int i = 0; i < B_N; ++i) ctrl[i] = 0;
     if (q >= n) return;
     const int g = q >> 6, lane = q & 63;
     const int32_t s = srcs[q];
     dist[((int64_t)g * V + s) * 64 + lane] = 0;
     if (hmask) atomicOr((unsigned long long*)&hmask[(int64_t)g * V + s], 1ull << lane);
-    if (!((nz[s >> 5] >> (s & 31)) & 1u)) return;
-    unsigned long long old = atomicOr((unsigned long long*)&mask[(int64_t)g * V + s], 1ull << lane);
-    if (old == 0) {
-        int64_t p = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ACT], 1ull);
-        acts[p] = make_int2(s, g);
-    }
+    const bool has = row[s + 1] > row[s] || (hrow && hrow[s + 1] > hrow[s]);
+    if (!has) return;
+    atomicOr((unsigned long long*)&mask[(int64_t)g * V + s], 1ull << lane);
+    atomicOr(&abits[(int64_t)g * nw + (s >> 5)], 1u << (s & 31));
 }
 
 template <bool PRUNED>
 __global__ void __launch_bounds__(256)
-relax_b_k(const uint32_t* __restrict__ nz, const int2* __restrict__ adj, const int2* __restrict__ hadj,
+relax_b_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
           int32_t* __restrict__ dist, uint64_t* __restrict__ mask, uint64_t* __restrict__ hmask,
           const int4* __restrict__ items, const uint64_t* __restrict__ imask,
-          int2* __restrict__ acts, int64_t* __restrict__ ctrl, int64_t V, int64_t actcap) {
+          uint32_t* __restrict__ abits, int64_t* __restrict__ ctrl, int64_t V, int64_t nw) {
     const int64_t n = ctrl[B_ITEMS];
     const int lane = lane_id();
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -116,41 +117,34 @@ relax_b_k(const uint32_t* __restrict__ nz, const int2* __restrict__ adj, const i
                 for (int jj = 0; jj < kJ; ++jj) {
                     const bool valid = j0 + jj < ne;
                     const int j = valid ? j0 + jj : ne - 1;
-                    vj[jj] = __shfl(my.x, j, 64);
+                    vj[jj] = __shfl(my.x, j, 64) & kIdMaskB;
                     const int w = __shfl(my.y, j, 64);
                     cj[jj] = (valid && act && du <= INT32_MAX - w) ? du + w : INT32_MAX;
                     oj[jj] = INT32_MAX;
                     if (cj[jj] != INT32_MAX) oj[jj] = dG[(int64_t)vj[jj] * 64 + lane];
                 }
+                // Issue every atomic of the group before any result is used.
+#pragma unroll
+                for (int jj = 0; jj < kJ; ++jj)
+                    if (cj[jj] < oj[jj]) oj[jj] = atomicMin(&dG[(int64_t)vj[jj] * 64 + lane], cj[jj]);
+                    else cj[jj] = INT32_MAX;
 #pragma unroll
                 for (int jj = 0; jj < kJ; ++jj) {
-                    bool imp = false;
-                    if (cj[jj] < oj[jj]) imp = cj[jj] < atomicMin(&dG[(int64_t)vj[jj] * 64 + lane], cj[jj]);
+                    const bool imp = cj[jj] < oj[jj];
                     const uint64_t b = __ballot(imp);
                     impr += imp;
                     if (lane == j0 + jj) mybits = b;
                 }
             }
-            // Lane l now holds the improved-lane mask of edge l.
-            bool newv = false;
+            // Lane l now holds the improved-lane mask of edge l: record the
+            // lanes at v and mark (v,g) active (both fire-and-forget).
             if (mybits) {
-                const int v = my.x;
+                const int v = my.x & kIdMaskB;
                 if (PRUNED && hid_item)
                     atomicOr((unsigned long long*)&hmask[(int64_t)g * V + v], (unsigned long long)mybits);
-                if ((nz[v >> 5] >> (v & 31)) & 1u) {
-                    unsigned long long o =
-                        atomicOr((unsigned long long*)&mask[(int64_t)g * V + v], (unsigned long long)mybits);
-                    newv = (o == 0);
-                }
-            }
-            const uint64_t nb = __ballot(newv);
-            if (nb) {
-                int64_t base = 0;
-                if (lane == 0) base = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ACT], (unsigned long long)__popcll(nb));
-                base = shfl64(base, 0);
-                if (newv) {
-                    const int64_t p = base + lanes_below(nb);
-                    if (p < actcap) acts[p] = make_int2(my.x, g);
+                if (my.x < 0) {
+                    atomicOr((unsigned long long*)&mask[(int64_t)g * V + v], (unsigned long long)mybits);
+                    atomicOr(&abits[(int64_t)g * nw + (v >> 5)], 1u << (v & 31));
                 }
             }
         }
@@ -167,68 +161,81 @@ relax_b_k(const uint32_t* __restrict__ nz, const int2* __restrict__ adj, const i
             ctrl[B_ITEMS_TOT] += n;
             ctrl[B_ITEMS] = 0;
             ctrl[B_DONE_R] = 0;
-            int64_t na = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ACT], 0ull);
-            if (na > actcap) { ctrl[B_OVF] = 1; ctrl[B_ACT] = 0; }
         }
     }
 }
 
 __global__ void __launch_bounds__(256)
 expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, uint64_t* __restrict__ mask,
-           const uint64_t* __restrict__ hmask, const int2* __restrict__ acts, int4* __restrict__ items,
-           uint64_t* __restrict__ imask, int64_t* __restrict__ ctrl, int64_t V, int chunk, int64_t itemcap) {
-    const int64_t n = ctrl[B_ACT];
+           const uint64_t* __restrict__ hmask, uint32_t* __restrict__ abits, int4* __restrict__ items,
+           uint64_t* __restrict__ imask, int64_t* __restrict__ ctrl, int64_t V, int64_t nw, int groups,
+           int chunk, int64_t itemcap) {
     const int lane = lane_id();
     int64_t front = 0, edges = 0, ledges = 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    // Uniform trip count per wave so the wave-wide scan below is legal.
-    const int64_t nn = (n + 63) & ~(int64_t)63;
-    for (int64_t i = t0; i < nn; i += stride) {
-        int v = 0, g = 0, r0 = 0, r1 = 0, h0 = 0, h1 = 0, nl = 0, nh = 0;
-        uint64_t m = 0, hm = 0;
+    const int64_t n = nw * groups;
+    const int64_t nn = (n + 63) & ~(int64_t)63;  // uniform trip count per wave
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nn; i += stride) {
+        uint32_t word = 0;
+        int g = 0;
+        int64_t vb = 0;
         if (i < n) {
-            const int2 a = acts[i];
-            v = a.x;
-            g = a.y;
-            uint64_t* mp = &mask[(int64_t)g * V + v];
-            m = *mp;
-            *mp = 0;
-            r0 = row[v];
-            r1 = row[v + 1];
-            nl = (r1 - r0 + chunk - 1) / chunk;
+            word = abits[i];
+            if (word) abits[i] = 0;
+            g = (int)(i / nw);
+            vb = (i - (int64_t)g * nw) * 32;
+        }
+        const uint64_t* mg = mask + (int64_t)g * V;
+        int cnt = 0;
+        for (uint32_t b = word; b; b &= b - 1) {
+            const int v = (int)(vb + __builtin_ctz(b));
+            const uint64_t m = mg[v];
+            const int r0 = row[v], r1 = row[v + 1];
+            cnt += (r1 - r0 + chunk - 1) / chunk;
+            uint64_t hm = 0;
             if (hrow) {
                 hm = m & hmask[(int64_t)g * V + v];
                 if (hm) {
-                    h0 = hrow[v];
-                    h1 = hrow[v + 1];
-                    nh = (h1 - h0 + chunk - 1) / chunk;
+                    const int h0 = hrow[v], h1 = hrow[v + 1];
+                    cnt += (h1 - h0 + chunk - 1) / chunk;
+                    edges += h1 - h0;
+                    ledges += (int64_t)__popcll(hm) * (h1 - h0);
                 }
             }
             front += 1;
-            edges += (r1 - r0) + (h1 - h0);
-            ledges += (int64_t)__popcll(m) * (r1 - r0) + (int64_t)__popcll(hm) * (h1 - h0);
+            edges += r1 - r0;
+            ledges += (int64_t)__popcll(m) * (r1 - r0);
         }
         int tot;
-        const int off = wave_excl_scan(nl + nh, &tot);
-        if (tot > 0) {
-            int64_t base = 0;
-            if (lane == 0) base = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ITEMS], (unsigned long long)tot);
-            base = shfl64(base, 0);
-            if (base + tot <= itemcap) {
-                int64_t p = base + off;
-                for (int j = 0; j < nl; ++j, ++p) {
-                    const int b = r0 + j * chunk;
-                    items[p] = make_int4(v, b, min(b + chunk, r1), g);
-                    imask[p] = m;
+        const int off = wave_excl_scan(cnt, &tot);
+        if (tot == 0) continue;
+        int64_t base = 0;
+        if (lane == 0) base = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ITEMS], (unsigned long long)tot);
+        base = shfl64(base, 0);
+        if (base + tot > itemcap) {
+            if (lane == 0) atomicOr((unsigned long long*)&ctrl[B_OVF], 1ull);
+            continue;
+        }
+        int64_t p = base + off;
+        uint64_t* mw = mask + (int64_t)g * V;
+        for (uint32_t b = word; b; b &= b - 1) {
+            const int v = (int)(vb + __builtin_ctz(b));
+            const uint64_t m = mw[v];
+            mw[v] = 0;
+            const int r0 = row[v], r1 = row[v + 1];
+            for (int e = r0; e < r1; e += chunk, ++p) {
+                items[p] = make_int4(v, e, min(e + chunk, r1), g);
+                imask[p] = m;
+            }
+            if (hrow) {
+                const uint64_t hm = m & hmask[(int64_t)g * V + v];
+                if (hm) {
+                    const int h0 = hrow[v], h1 = hrow[v + 1];
+                    for (int e = h0; e < h1; e += chunk, ++p) {
+                        items[p] = make_int4(v, e, min(e + chunk, h1), g | (1 << 16));
+                        imask[p] = hm;
+                    }
                 }
-                for (int j = 0; j < nh; ++j, ++p) {
-                    const int b = h0 + j * chunk;
-                    items[p] = make_int4(v, b, min(b + chunk, h1), g | (1 << 16));
-                    imask[p] = hm;
-                }
-            } else if (lane == 0) {
-                atomicOr((unsigned long long*)&ctrl[B_OVF], 1ull);
             }
         }
     }
@@ -246,7 +253,6 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
         unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[B_DONE_E], 1ull);
         if (prev == (unsigned long long)gridDim.x - 1) {
             __threadfence();
-            ctrl[B_ACT] = 0;
             ctrl[B_DONE_E] = 0;
             int64_t ni = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ITEMS], 0ull);
             if (ni > itemcap) { ctrl[B_OVF] = 1; ctrl[B_ITEMS] = 0; }
@@ -278,23 +284,10 @@ __global__ void batch_checksum_k(const int32_t* __restrict__ row, const int32_t*
 }
 
 // ------------------------------------------------------------------ host
-static int nz_bitmap(Graph* g, uint32_t** out) {
-    std::vector<uint32_t> bits(((size_t)g->V + 31) / 32 + 1, 0);
-    for (int32_t v = 0; v < g->V; ++v) {
-        bool has = g->vl[v + 1] > g->vl[v];
-        if (!g->hvl.empty()) has = has || g->hvl[v + 1] > g->hvl[v];
-        if (has) bits[v >> 5] |= 1u << (v & 31);
-    }
-    HIP_TRY(hipMalloc(out, sizeof(uint32_t) * bits.size()));
-    HIP_TRY(hipMemcpy(*out, bits.data(), sizeof(uint32_t) * bits.size(), hipMemcpyHostToDevice));
-    return C2G_OK;
-}
-
 struct BatchState {
     int groups = 0;
-    int64_t actcap = 0, itemcap = 0;
-    uint32_t* nz = nullptr;
-    int2* acts = nullptr;
+    int64_t nw = 0, itemcap = 0;
+    uint32_t* abits = nullptr;  // [groups][nw] activation bitmaps
     int4* items = nullptr;
     uint64_t* imask = nullptr;
     int32_t* srcs = nullptr;
@@ -321,7 +314,7 @@ void batch_release(Graph* g) {
     for (size_t i = 0; i < g_bs.size(); ++i)
         if (g_bs[i].first == g) {
             BatchState* s = g_bs[i].second;
-            (void)hipFree(s->nz); (void)hipFree(s->acts); (void)hipFree(s->items);
+            (void)hipFree(s->abits); (void)hipFree(s->items);
             (void)hipFree(s->imask); (void)hipFree(s->srcs); (void)hipFree(s->sums);
             (void)hipFree(s->reach); (void)hipFree(s->medges);
             for (auto& e : s->ev) if (e) (void)hipEventDestroy(e);
@@ -337,29 +330,30 @@ static int ensure_batch(Graph* g, int groups) {
     HIP_TRY(hipSetDevice(g->device));
     const int64_t V = g->V;
     const int chunk = std::max(1, g->tune.batch_chunk);
-    if (!s->nz) {
-        int rc = nz_bitmap(g, &s->nz);
-        if (rc) return rc;
+    if (!s->ev[0]) {
         HIP_TRY(hipEventCreate(&s->ev[0]));
         HIP_TRY(hipEventCreate(&s->ev[1]));
         HIP_TRY(hipMalloc(&w.ctrl, sizeof(int64_t) * B_N));
         HIP_TRY(hipMemset(w.ctrl, 0, sizeof(int64_t) * B_N));
         HIP_TRY(hipHostMalloc((void**)&w.h_ctrl, sizeof(int64_t) * B_N, hipHostMallocDefault));
     }
+    // Each (v,g) is activated at most once per round: at most
+    // ceil(deg/chunk) + ceil(hdeg/chunk) <= 2 + (deg + hdeg)/chunk items.
     const int64_t needi = groups * (2 * V + (g->E + g->HE) / chunk) + 64;
     if (groups > s->groups || s->itemcap < needi) {
         if (w.exec) { (void)hipGraphExecDestroy(w.exec); w.exec = nullptr; }
         const int G = std::max(groups, s->groups);
         (void)hipFree(w.dist); (void)hipFree(w.mask[0]); (void)hipFree(w.hmask);
-        (void)hipFree(s->acts); (void)hipFree(s->items); (void)hipFree(s->imask); (void)hipFree(s->srcs);
+        (void)hipFree(s->abits); (void)hipFree(s->items); (void)hipFree(s->imask); (void)hipFree(s->srcs);
         (void)hipFree(s->sums); (void)hipFree(s->reach); (void)hipFree(s->medges);
         w.dist = nullptr; w.mask[0] = nullptr; w.hmask = nullptr;
+        s->nw = (V + 31) / 32 + 1;
         HIP_TRY(hipMalloc(&w.dist, sizeof(int32_t) * (size_t)G * V * 64 + 16));
         HIP_TRY(hipMalloc(&w.mask[0], sizeof(uint64_t) * (size_t)G * V + 8));
         if (g->d_hrow) HIP_TRY(hipMalloc(&w.hmask, sizeof(uint64_t) * (size_t)G * V + 8));
-        s->actcap = (int64_t)G * V + 64;
         s->itemcap = G * (2 * V + (g->E + g->HE) / chunk) + 64;
-        HIP_TRY(hipMalloc(&s->acts, sizeof(int2) * s->actcap));
+        HIP_TRY(hipMalloc(&s->abits, sizeof(uint32_t) * s->nw * G));
+        HIP_TRY(hipMemset(s->abits, 0, sizeof(uint32_t) * s->nw * G));
         HIP_TRY(hipMalloc(&s->items, sizeof(int4) * s->itemcap));
         HIP_TRY(hipMalloc(&s->imask, sizeof(uint64_t) * s->itemcap));
         HIP_TRY(hipMalloc(&s->srcs, sizeof(int32_t) * 64 * G));
@@ -375,22 +369,35 @@ static int ensure_batch(Graph* g, int groups) {
 
 static int relax_grid_b(const Graph* g) { return std::max(1, g->num_cus) * g->tune.grid_per_cu; }
 
-// One round = relax_b + expand_b.  When `ev` is non-null, events bracket the
-// relax kernel and its duration is added to *relax_ms (profiling runs).
-static int launch_round_b(Graph* g, double* relax_ms) {
+static int expand_b(Graph* g, int groups) {
+    BatchWs& w = g->batch;
+    BatchState* s = bstate(g);
+    const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
+    const int64_t words = s->nw * groups;
+    const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(relax_grid_b(g), (words + 255) / 256));
+    hipLaunchKernelGGL(expand_b_k, dim3(egrid), dim3(256), 0, g->stream, g->d_row,
+                       pruned ? g->d_hrow : (const int32_t*)nullptr, w.mask[0],
+                       pruned ? w.hmask : (const uint64_t*)nullptr, s->abits, s->items, s->imask, w.ctrl,
+                       (int64_t)g->V, s->nw, groups, g->tune.batch_chunk, s->itemcap);
+    return C2G_OK;
+}
+
+// One round = relax_b + expand_b.  When relax_ms is non-null, events bracket
+// the relax kernel and its duration is added to *relax_ms (profiling runs).
+static int launch_round_b(Graph* g, int groups, double* relax_ms) {
     BatchWs& w = g->batch;
     BatchState* s = bstate(g);
     const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
     const int grid = relax_grid_b(g);
     if (relax_ms) HIP_TRY(hipEventRecord(s->ev[0], g->stream));
     if (pruned)
-        hipLaunchKernelGGL(relax_b_k<true>, dim3(grid), dim3(256), 0, g->stream, s->nz, g->d_adj,
-                           g->d_hadj, w.dist, w.mask[0], w.hmask, s->items, s->imask, s->acts, w.ctrl,
-                           (int64_t)g->V, s->actcap);
+        hipLaunchKernelGGL(relax_b_k<true>, dim3(grid), dim3(256), 0, g->stream, g->d_adj, g->d_hadj,
+                           w.dist, w.mask[0], w.hmask, s->items, s->imask, s->abits, w.ctrl,
+                           (int64_t)g->V, s->nw);
     else
-        hipLaunchKernelGGL(relax_b_k<false>, dim3(grid), dim3(256), 0, g->stream, s->nz, g->d_adj,
+        hipLaunchKernelGGL(relax_b_k<false>, dim3(grid), dim3(256), 0, g->stream, g->d_adj,
                            (const int2*)nullptr, w.dist, w.mask[0], (uint64_t*)nullptr, s->items,
-                           s->imask, s->acts, w.ctrl, (int64_t)g->V, s->actcap);
+                           s->imask, s->abits, w.ctrl, (int64_t)g->V, s->nw);
     if (relax_ms) {
         HIP_TRY(hipEventRecord(s->ev[1], g->stream));
         HIP_TRY(hipEventSynchronize(s->ev[1]));
@@ -398,24 +405,22 @@ static int launch_round_b(Graph* g, double* relax_ms) {
         HIP_TRY(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
         *relax_ms += ms;
     }
-    hipLaunchKernelGGL(expand_b_k, dim3(grid), dim3(256), 0, g->stream, g->d_row,
-                       pruned ? g->d_hrow : (const int32_t*)nullptr, w.mask[0],
-                       pruned ? w.hmask : (const uint64_t*)nullptr, s->acts, s->items, s->imask, w.ctrl,
-                       (int64_t)g->V, g->tune.batch_chunk, s->itemcap);
+    expand_b(g, groups);
     HIP_TRY(hipGetLastError());
     return C2G_OK;
 }
 
-static int ensure_exec_b(Graph* g) {
+static int ensure_exec_b(Graph* g, int groups) {
     BatchWs& w = g->batch;
     const int K = g->tune.iters_per_graph;
-    if (w.exec && w.exec_iters == K && w.exec_grid == relax_grid_b(g) && w.exec_chunk == g->tune.batch_chunk)
+    if (w.exec && w.exec_iters == K && w.exec_grid == relax_grid_b(g) &&
+        w.exec_chunk == g->tune.batch_chunk && w.exec_block == groups)
         return C2G_OK;
     if (w.exec) { (void)hipGraphExecDestroy(w.exec); w.exec = nullptr; }
     hipGraph_t graph;
     HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
     int rc = C2G_OK;
-    for (int r = 0; r < K && rc == C2G_OK; ++r) rc = launch_round_b(g, nullptr);
+    for (int r = 0; r < K && rc == C2G_OK; ++r) rc = launch_round_b(g, groups, nullptr);
     hipError_t e = hipStreamEndCapture(g->stream, &graph);
     if (rc) return rc;
     if (e != hipSuccess) return fail(C2G_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
@@ -425,11 +430,11 @@ static int ensure_exec_b(Graph* g) {
     w.exec_iters = K;
     w.exec_grid = relax_grid_b(g);
     w.exec_chunk = g->tune.batch_chunk;
+    w.exec_block = groups;  // the captured expand scans this many groups
     return C2G_OK;
 }
 
-// Runs one group of n <= 64 queries (lanes) in group slot 0 .. groups-1.
-// Groups are run together: srcs holds n_total sources, lane q of group q/64.
+// Runs n queries: lane q of group q/64 starts at srcs[q].
 static int batch_run(Graph* g, const int32_t* srcs, int32_t n, c2g_stats* st, double* relax_ms) {
     BatchWs& w = g->batch;
     BatchState* s = bstate(g);
@@ -437,22 +442,19 @@ static int batch_run(Graph* g, const int32_t* srcs, int32_t n, c2g_stats* st, do
     const int64_t V = g->V;
     const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
     HIP_TRY(hipMemcpyAsync(s->srcs, srcs, sizeof(int32_t) * n, hipMemcpyHostToDevice, g->stream));
-    HIP_TRY(hipMemsetAsync(w.ctrl, 0, sizeof(int64_t) * B_N, g->stream));
     const int fill_grid = std::max(1, g->num_cus) * 16;
     hipLaunchKernelGGL(batch_fill_k, dim3(fill_grid), dim3(256), 0, g->stream, w.dist,
                        (int64_t)groups * V * 64, w.mask[0], pruned ? w.hmask : (uint64_t*)nullptr,
-                       (int64_t)groups * V);
-    hipLaunchKernelGGL(batch_seed_k, dim3((n + 255) / 256), dim3(256), 0, g->stream, s->srcs, n, s->nz,
-                       w.dist, w.mask[0], pruned ? w.hmask : (uint64_t*)nullptr, s->acts, w.ctrl, V);
+                       (int64_t)groups * V, s->abits, s->nw * groups);
+    hipLaunchKernelGGL(batch_seed_k, dim3((n + 255) / 256), dim3(256), 0, g->stream, s->srcs, n, g->d_row,
+                       pruned ? g->d_hrow : (const int32_t*)nullptr, w.dist, w.mask[0],
+                       pruned ? w.hmask : (uint64_t*)nullptr, s->abits, w.ctrl, V, s->nw);
     // Turn the seeded activations into the first round's items.
-    hipLaunchKernelGGL(expand_b_k, dim3(relax_grid_b(g)), dim3(256), 0, g->stream, g->d_row,
-                       pruned ? g->d_hrow : (const int32_t*)nullptr, w.mask[0],
-                       pruned ? w.hmask : (const uint64_t*)nullptr, s->acts, s->items, s->imask, w.ctrl,
-                       V, g->tune.batch_chunk, s->itemcap);
+    expand_b(g, groups);
     HIP_TRY(hipGetLastError());
     const bool use_graph = g->tune.use_graph && !relax_ms;
     int rc;
-    if (use_graph && (rc = ensure_exec_b(g))) return rc;
+    if (use_graph && (rc = ensure_exec_b(g, groups))) return rc;
     for (;;) {
         if (use_graph) {
             HIP_TRY(hipGraphLaunch(w.exec, g->stream));
@@ -461,7 +463,7 @@ static int batch_run(Graph* g, const int32_t* srcs, int32_t n, c2g_stats* st, do
             // is launched (launch count == rounds with work).
             const int k = relax_ms ? 1 : g->tune.iters_per_graph;
             for (int r = 0; r < k; ++r)
-                if ((rc = launch_round_b(g, relax_ms))) return rc;
+                if ((rc = launch_round_b(g, groups, relax_ms))) return rc;
         }
         HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, sizeof(int64_t) * B_N, hipMemcpyDeviceToHost, g->stream));
         HIP_TRY(hipStreamSynchronize(g->stream));
