@@ -3,16 +3,6 @@
 using namespace c2g;
 extern "C" {
 int c2g_convert_adjlist(const char*, const char*) { return fail(C2G_ENOTSUP, "not implemented"); }
-int c2g_prune_sssp(const char*, int64_t*) { return fail(C2G_ENOTSUP, "not implemented"); }
-int c2g_prune_ppr(const char*, int64_t*) { return fail(C2G_ENOTSUP, "not implemented"); }
-int c2g_prune_sssp_mem(int32_t, const int32_t*, const int32_t*, const int32_t*, int32_t**, int32_t**,
-                       int32_t**, int32_t**, int32_t**, int32_t**, int32_t**, int64_t*) {
-    return fail(C2G_ENOTSUP, "not implemented");
-}
-int c2g_prune_ppr_mem(int32_t, const int32_t*, const int32_t*, const float*, int32_t**, int32_t**,
-                      float**, int32_t**, int32_t**, float**) {
-    return fail(C2G_ENOTSUP, "not implemented");
-}
 int c2g_ppr(c2g_graph*, int32_t, float, float, float*, float*, c2g_stats*) { return fail(C2G_ENOTSUP, "not implemented"); }
 int c2g_ppr_batch(c2g_graph*, const int32_t*, int32_t, float, float, float*, float*, c2g_stats*) { return fail(C2G_ENOTSUP, "not implemented"); }
 int c2g_adsorption(c2g_graph*, const int32_t*, const int32_t*, int32_t, int32_t, float, float, float, int32_t, float*, c2g_stats*) { return fail(C2G_ENOTSUP, "not implemented"); }

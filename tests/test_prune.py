@@ -1,0 +1,131 @@
+"""purn-sssp / purn-ppr: byte-identical to the reference's own outputs
+(tests/golden/prune_*.npz, written by the reference binaries compiled from
+src/purn-sssp.cpp and src/purn-ppr.cpp), and the pruning guarantee: distances
+on the pruned graph with hidden-list seeding equal those on the full graph."""
+import glob
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import c2g
+import oracle
+from conftest import INT_MAX, ROOT, random_graph, rmat_graph
+
+GOLD = sorted(glob.glob(os.path.join(ROOT, "tests", "golden", "prune_*.npz")))
+REF = os.path.join(ROOT, "oracle", "_ref")
+
+
+def _compact(P):
+    """Per-vertex hidden lists -> the reference's prune-order layout."""
+    po, hv = P["porder"], P["hvlist"]
+    cnt = hv[po + 1] - hv[po]
+    chv = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int32)
+    che = np.concatenate([P["helist"][hv[p]:hv[p + 1]] for p in po] + [np.zeros(0, np.int32)])
+    chw = np.concatenate([P["hwlist"][hv[p]:hv[p + 1]] for p in po] + [np.zeros(0, np.int32)])
+    return chv, che.astype(np.int32), chw.astype(np.int32)
+
+
+@pytest.mark.parametrize("path", GOLD, ids=[os.path.basename(p) for p in GOLD])
+def test_sssp_prune_matches_reference_bytes(path):
+    z = np.load(path)
+    P = c2g.prune_sssp_mem(z["in_vlist"], z["in_elist"], z["in_wlist"])
+    for f in ("vlist", "elist", "wlist"):
+        assert P[f].tobytes() == z[f"sssp_{f}"].tobytes(), f
+    chv, che, chw = _compact(P)
+    assert chv.tobytes() == z["sssp_hvlist"].tobytes()
+    assert che.tobytes() == z["sssp_helist"].tobytes()
+    assert chw.tobytes() == z["sssp_hwlist"].tobytes()
+    # per-vertex layout has V+1 entries
+    assert len(P["hvlist"]) == len(z["in_vlist"])
+
+
+@pytest.mark.parametrize("path", GOLD, ids=[os.path.basename(p) for p in GOLD])
+def test_ppr_prune_matches_reference_bytes(path):
+    z = np.load(path)
+    P = c2g.prune_ppr_mem(z["in_vlist"], z["in_elist"], z["in_flist"])
+    for f in ("vlist", "elist", "wlist", "hvlist", "helist", "hwlist"):
+        assert P[f].tobytes() == z[f"ppr_{f}"].tobytes(), f
+
+
+def test_dataset_files_and_loader(tmp_path):
+    G = rmat_graph(12, 3, wmax=64, want_f=True)
+    ds = str(tmp_path / "ds")
+    c2g.write_dataset(ds, G["vlist"], G["elist"], G["wlist"], G["flist"])
+    st = c2g.prune_sssp(ds)
+    c2g.prune_ppr(ds)
+    assert st[0] == len(c2g.read_i32(ds + "/sssp/elist.bin"))
+    with c2g.Graph.open(ds, c2g.VARIANT_SSSP_PURN, no_device=True) as g:
+        assert g.V == G["V"] and g.HE == st[1]
+    with c2g.Graph.open(ds, c2g.VARIANT_PPR_PURN, no_device=True) as g:
+        assert g.V == G["V"]
+    # a prune-order (compacted) sssp/hvlist, as the reference writes it, is rejected
+    P = c2g.prune_sssp_mem(G["vlist"], G["elist"], G["wlist"])
+    _compact(P)[0].tofile(ds + "/sssp/hvlist.bin")
+    with pytest.raises(c2g.C2GError) as e:
+        c2g.Graph.open(ds, c2g.VARIANT_SSSP_PURN, no_device=True)
+    assert e.value.code == c2g.EFORMAT and "hvlist" in str(e.value)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "purn-sssp")), reason="reference binaries not built")
+@pytest.mark.parametrize("seed", [21, 22, 23, 24])
+def test_prune_matches_live_reference(tmp_path, seed):
+    G = random_graph(300, 700, seed=seed, wmax=20)
+    ds = str(tmp_path / "ds")
+    c2g.write_dataset(ds, G["vlist"], G["elist"], G["wlist"], G["flist"])
+    subprocess.run([os.path.join(REF, "purn-sssp"), ds], check=True, capture_output=True)
+    subprocess.run([os.path.join(REF, "purn-ppr"), ds], check=True, capture_output=True)
+    P = c2g.prune_sssp_mem(G["vlist"], G["elist"], G["wlist"])
+    for f in ("vlist", "elist", "wlist"):
+        assert P[f].tobytes() == np.fromfile(f"{ds}/sssp/{f}.bin", np.int32).tobytes()
+    Q = c2g.prune_ppr_mem(G["vlist"], G["elist"], G["flist"])
+    for f in ("vlist", "elist", "wlist", "hvlist", "helist", "hwlist"):
+        assert Q[f].tobytes() == open(f"{ds}/ppr/{f}.bin", "rb").read()
+
+
+def _pruned_graph(G, **kw):
+    P = c2g.prune_sssp_mem(G["vlist"], G["elist"], G["wlist"])
+    return P, c2g.Graph.from_csr(P["vlist"], P["elist"], P["wlist"], hvlist=P["hvlist"], helist=P["helist"],
+                                 hwlist=P["hwlist"], variant=c2g.VARIANT_SSSP_PURN, **kw)
+
+
+def test_pruning_guarantee_all_sources_host_lane():
+    # Every source of many sparse graphs: pruned + hidden seeding == full-graph Dijkstra.
+    for seed in range(60):
+        G = random_graph(40, 90, seed=seed, wmax=9)
+        P, g = _pruned_graph(G, no_device=True)
+        with g:
+            for s in range(G["V"]):
+                d, _ = g.host_sssp(s)
+                assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], s)), (seed, s)
+
+
+def test_one_level_seeding_is_not_enough():
+    # The reference seeds only H(s) (src/sssp/purn/Main.cu:72-84, 178-181).  On the
+    # counterexample a->s:1, s->v:1, v->x:1, a->v:5 that misses x from s; the
+    # transitive rule (relax H(x) when x was reached through a hidden edge) finds it.
+    z = np.load(os.path.join(ROOT, "tests", "golden", "prune_appendix.npz"))
+    G = dict(V=4, vlist=z["in_vlist"], elist=z["in_elist"], wlist=z["in_wlist"])
+    P, g = _pruned_graph(G, no_device=True)
+    with g:
+        s = 1
+        full = oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], s)
+        assert full.tolist() == [INT_MAX, 0, 1, 2]
+        d, _ = g.host_sssp(s)
+        assert np.array_equal(d, full)
+        # one-level seeding: dist from H(s), frontier {s} U H(s), no hidden flags
+        d1 = np.full(4, INT_MAX, np.int32)
+        d1[s] = 0
+        hv, he, hw = P["hvlist"], P["helist"], P["hwlist"]
+        d1[he[hv[s]:hv[s + 1]]] = hw[hv[s]:hv[s + 1]]
+        fr = np.concatenate([[s], he[hv[s]:hv[s + 1]]]).astype(np.int32)
+        d1, _ = g.host_bellman_ford(d1, fr, via_hidden=np.zeros(4, np.uint8))
+        assert d1[3] == INT_MAX  # x missed
+
+
+def test_prune_rejects_duplicate_targets():
+    vl = np.array([0, 2, 2], np.int32)
+    with pytest.raises(c2g.C2GError) as e:
+        c2g.prune_sssp_mem(vl, np.array([1, 1], np.int32), np.array([3, 4], np.int32))
+    assert e.value.code == c2g.EFORMAT
