@@ -100,6 +100,8 @@ struct Graph {
 int device_upload(Graph* g, int device);
 void device_release(Graph* g);
 void batch_release(Graph* g);
+void ppr_release(Graph* g);
+void ads_release(Graph* g);
 void default_tune(Graph* g);
 int ensure_sssp_ws(Graph* g);
 int ensure_batch_ws(Graph* g);

@@ -164,6 +164,8 @@ void device_release(Graph* g) {
     if (!g->stream && !g->d_row) return;
     (void)hipSetDevice(g->device);
     batch_release(g);
+    ppr_release(g);
+    ads_release(g);
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     if (g->sssp.exec) (void)hipGraphExecDestroy(g->sssp.exec);
     if (g->batch.exec) (void)hipGraphExecDestroy(g->batch.exec);

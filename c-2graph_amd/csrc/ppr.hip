@@ -1,0 +1,629 @@
+// Forward-push personalized PageRank on gfx950 (FP32 state).
+//
+// Replaces calcuatePPR<float> (src/ppr/base/Main.cu:121-388) and its pruned
+// variant (src/ppr/purn/Main.cu:78-101, 149-207).  Semantics kept: the source
+// starts with r=1 and p=alpha (Main.cu:146-150); each round every active u
+// sends (1-alpha)*r[u]*f(u,v) into a message buffer m[v] and its residual
+// drops to 0 (Main.cu:208-320); then every touched v adds m[v] into r[v] and
+// becomes active, crediting p[v] += alpha*r[v], when r[v] exceeds its
+// threshold (Main.cu:339-367).  Dangling mass is dropped (the redirect is
+// commented out in the reference, Main.cu:216-221).  Changed on purpose: the
+// threshold is eps*outdeg(v) as in the reference CPU lane
+// (src/ppr/w-base-perk/PPR.cpp:220) instead of one global rmax, alpha and eps
+// are arguments, and rounds run to convergence (no 500-round cap, Main.cu:184).
+//
+// Single source, one round = two kernels:
+//   ppr_push_k   : items (u, e_begin, e_end, bits of r[u]) walked 64 edges at
+//                  a time per wave (shuffle owner search as in SSSP); one
+//                  hardware FP32 atomicAdd into m[v] per edge.
+//   ppr_expand_k : scans m[] (V floats); a touched vertex adds its message,
+//                  activates above threshold, and its out-row is cut into
+//                  items carrying the residual being pushed (wave scan + one
+//                  atomicAdd per wave).
+// Pruned graphs (ppr/ variant): the first round's frontier is the source's
+// hidden and live neighbours with r = (1-alpha)*w (src/ppr/purn/Main.cu:78-101,
+// 193-198), activated unconditionally.
+//
+// Batched (64 queries per group, lane = query), [g][V][64] float arrays:
+//   pprb_push_k     : one item per wave; each edge is one 256-byte FP32
+//                     atomicAdd row over the active lanes, plus one touched bit.
+//   pprb_activate_k : one wave per touched vertex: per lane r += m, threshold,
+//                     credit p, snapshot the pushed residual; lanes -> mask.
+//   pprb_emit_k     : compacts touched vertices with a non-empty mask into items.
+#include "graph.h"
+#include "wave.h"
+#include <cstring>
+
+namespace c2g {
+
+enum : int {
+    P_CUR = 0, P_ITER = 1, P_DONE = 2, P_DONE_E = 3, P_FRONT = 4, P_EDGES = 5, P_ITEMS = 6,
+    P_OVF = 7, P_DONE_A = 8, P_N = 16
+};
+
+__global__ void ppr_init_k(float* __restrict__ p, float* __restrict__ r, float* __restrict__ m, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        p[i] = 0.0f;
+        r[i] = 0.0f;
+        m[i] = 0.0f;
+    }
+}
+
+// Seeds messages for the first round (activated unconditionally by the expand).
+__global__ void ppr_seed_k(const int32_t* __restrict__ row, const int2* __restrict__ adjf,
+                           const int32_t* __restrict__ hrow, const int2* __restrict__ hadjf,
+                           float* __restrict__ p, float* __restrict__ m, int64_t* __restrict__ ctrl,
+                           int32_t src, float alpha, int pruned) {
+    if (threadIdx.x == 0)
+        for (int i = 0; i < P_N; ++i) ctrl[i] = 0;
+    if (!pruned) {
+        if (threadIdx.x == 0) m[src] = 1.0f;
+        return;
+    }
+    // Pruned graph: p[s] = alpha, and every hidden then live neighbour h gets
+    // r[h] = (1-alpha)*w(s,h) (its p credit alpha*r[h] comes with activation).
+    if (threadIdx.x == 0) p[src] = alpha;
+    const int h0 = hrow[src], h1 = hrow[src + 1], r0 = row[src], r1 = row[src + 1];
+    for (int j = threadIdx.x; j < (h1 - h0) + (r1 - r0); j += blockDim.x) {
+        const int2 a = j < h1 - h0 ? hadjf[h0 + j] : adjf[r0 + j - (h1 - h0)];
+        atomicAdd(&m[a.x], (1.0f - alpha) * __int_as_float(a.y));
+    }
+}
+
+__global__ void __launch_bounds__(256)
+ppr_push_k(const int2* __restrict__ adjf, float* __restrict__ m, const int4* __restrict__ q,
+           int64_t* __restrict__ ctrl, float alpha) {
+    const int64_t n = ctrl[P_CUR];
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int lane = lane_id();
+    for (int64_t base = wave * kWave; base < n; base += nwaves * kWave) {
+        const int64_t i = base + lane;
+        int eb = 0, cnt = 0;
+        float msg = 0.0f;
+        if (i < n) {
+            const int4 it = q[i];
+            eb = it.y;
+            cnt = it.z - it.y;
+            msg = (1.0f - alpha) * __int_as_float(it.w);
+        }
+        int tot;
+        const int pre = wave_excl_scan(cnt, &tot);
+        for (int s = 0; s < tot; s += 2 * kWave) {
+            const int k0 = s + lane, k1 = s + kWave + lane;
+            const int o0 = wave_owner(pre, k0), o1 = wave_owner(pre, k1);
+            const int e0 = __shfl(eb, o0, 64) + (k0 - __shfl(pre, o0, 64));
+            const int e1 = __shfl(eb, o1, 64) + (k1 - __shfl(pre, o1, 64));
+            const float m0 = __shfl(msg, o0, 64), m1 = __shfl(msg, o1, 64);
+            int2 a0 = make_int2(0, 0), a1 = make_int2(0, 0);
+            if (k0 < tot) a0 = adjf[e0];
+            if (k1 < tot) a1 = adjf[e1];
+            if (k0 < tot) unsafeAtomicAdd(&m[a0.x], m0 * __int_as_float(a0.y));
+            if (k1 < tot) unsafeAtomicAdd(&m[a1.x], m1 * __int_as_float(a1.y));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[P_DONE], 1ull);
+        if (prev == (unsigned long long)gridDim.x - 1) {
+            __threadfence();
+            if (n > 0) ctrl[P_ITER] += 1;
+            ctrl[P_ITEMS] += n;
+            ctrl[P_CUR] = 0;
+            ctrl[P_DONE] = 0;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+ppr_expand_k(const int32_t* __restrict__ row, float* __restrict__ p, float* __restrict__ r,
+             float* __restrict__ m, int4* __restrict__ q, int64_t* __restrict__ ctrl, int64_t V,
+             float alpha, float eps, int chunk, int64_t qcap, int force) {
+    const int lane = lane_id();
+    const int64_t nn = (V + 63) & ~(int64_t)63;
+    int64_t front = 0, edges = 0;
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < nn; v += (int64_t)gridDim.x * blockDim.x) {
+        int cnt = 0, r0 = 0, r1 = 0;
+        float ru = 0.0f;
+        if (v < V) {
+            const float mv = m[v];
+            if (mv != 0.0f) {
+                m[v] = 0.0f;
+                const float rv = r[v] + mv;
+                r0 = row[v];
+                r1 = row[v + 1];
+                if (force || rv > (float)(r1 - r0) * eps) {
+                    p[v] += alpha * rv;
+                    r[v] = 0.0f;
+                    ru = rv;
+                    cnt = (r1 - r0 + chunk - 1) / chunk;
+                    front += 1;
+                    edges += r1 - r0;
+                } else {
+                    r[v] = rv;
+                }
+            }
+        }
+        int tot;
+        const int off = wave_excl_scan(cnt, &tot);
+        if (tot == 0) continue;
+        int64_t base = 0;
+        if (lane == 0) base = (int64_t)atomicAdd((unsigned long long*)&ctrl[P_CUR], (unsigned long long)tot);
+        base = shfl64(base, 0);
+        if (base + tot > qcap) {
+            if (lane == 0) atomicOr((unsigned long long*)&ctrl[P_OVF], 1ull);
+            continue;
+        }
+        int64_t pos = base + off;
+        for (int e = r0; e < r1 && cnt; e += chunk) q[pos++] = make_int4((int)v, e, min(e + chunk, r1), __float_as_int(ru));
+    }
+    front = wave_sum64(front);
+    edges = wave_sum64(edges);
+    if (lane == 0 && front) {
+        atomicAdd((unsigned long long*)&ctrl[P_FRONT], (unsigned long long)front);
+        atomicAdd((unsigned long long*)&ctrl[P_EDGES], (unsigned long long)edges);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[P_DONE_E], 1ull);
+        if (prev == (unsigned long long)gridDim.x - 1) {
+            __threadfence();
+            ctrl[P_DONE_E] = 0;
+            int64_t c = (int64_t)atomicAdd((unsigned long long*)&ctrl[P_CUR], 0ull);
+            if (c > qcap) { ctrl[P_OVF] = 1; ctrl[P_CUR] = 0; }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- batched
+__global__ void pprb_fill_k(float* __restrict__ a, int64_t n4, uint32_t* __restrict__ bits, int64_t nb) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    float4* a4 = reinterpret_cast<float4*>(a);
+    for (int64_t i = t; i < n4; i += stride) a4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int64_t i = t; i < nb; i += stride) bits[i] = 0;
+}
+
+// Lane q of group q/64 starts at srcs[q]: m = 1 (or the pruned one-hop seeds).
+__global__ void pprb_seed_k(const int32_t* __restrict__ srcs, int n, const int32_t* __restrict__ row,
+                            const int2* __restrict__ adjf, const int32_t* __restrict__ hrow,
+                            const int2* __restrict__ hadjf, float* __restrict__ P, float* __restrict__ M,
+                            uint32_t* __restrict__ tbits, int64_t* __restrict__ ctrl, int64_t V, int64_t nw,
+                            float alpha) {
+    const int q = blockIdx.x;
+    if (q == 0 && threadIdx.x == 0)
+        for (int i = 0; i < P_N; ++i) ctrl[i] = 0;
+    if (q >= n) return;
+    const int g = q >> 6, lane = q & 63;
+    const int s = srcs[q];
+    float* Mg = M + (int64_t)g * V * 64;
+    if (!hrow) {
+        if (threadIdx.x == 0) {
+            atomicAdd(&Mg[(int64_t)s * 64 + lane], 1.0f);
+            atomicOr(&tbits[(int64_t)g * nw + (s >> 5)], 1u << (s & 31));
+        }
+        return;
+    }
+    if (threadIdx.x == 0) P[((int64_t)g * V + s) * 64 + lane] = alpha;
+    const int h0 = hrow[s], h1 = hrow[s + 1], r0 = row[s], r1 = row[s + 1];
+    for (int j = threadIdx.x; j < (h1 - h0) + (r1 - r0); j += blockDim.x) {
+        const int2 a = j < h1 - h0 ? hadjf[h0 + j] : adjf[r0 + j - (h1 - h0)];
+        atomicAdd(&Mg[(int64_t)a.x * 64 + lane], (1.0f - alpha) * __int_as_float(a.y));
+        atomicOr(&tbits[(int64_t)g * nw + (a.x >> 5)], 1u << (a.x & 31));
+    }
+}
+
+__global__ void __launch_bounds__(256)
+pprb_push_k(const int2* __restrict__ adjf, float* __restrict__ M, const float* __restrict__ RU,
+            const int4* __restrict__ items, const uint64_t* __restrict__ imask, uint32_t* __restrict__ tbits,
+            int64_t* __restrict__ ctrl, int64_t V, int64_t nw, float alpha) {
+    const int64_t n = ctrl[P_CUR];
+    const int lane = lane_id();
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t i = wave; i < n; i += nwaves) {
+        const int4 it = items[i];
+        const uint64_t mk = imask[i];
+        const int g = it.w;
+        const bool act = (mk >> lane) & 1ull;
+        const float msg = act ? (1.0f - alpha) * RU[((int64_t)g * V + it.x) * 64 + lane] : 0.0f;
+        float* Mg = M + (int64_t)g * V * 64;
+        for (int e0 = it.y; e0 < it.z; e0 += 64) {
+            const int ne = min(64, it.z - e0);
+            int2 my = make_int2(0, 0);
+            if (lane < ne) my = adjf[e0 + lane];
+            if (lane < ne) atomicOr(&tbits[(int64_t)g * nw + (my.x >> 5)], 1u << (my.x & 31));
+            for (int j = 0; j < ne; ++j) {
+                const int v = __shfl(my.x, j, 64);
+                const float f = __int_as_float(__shfl(my.y, j, 64));
+                if (act) unsafeAtomicAdd(&Mg[(int64_t)v * 64 + lane], msg * f);
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[P_DONE], 1ull);
+        if (prev == (unsigned long long)gridDim.x - 1) {
+            __threadfence();
+            if (n > 0) ctrl[P_ITER] += 1;
+            ctrl[P_ITEMS] += n;
+            ctrl[P_CUR] = 0;
+            ctrl[P_DONE] = 0;
+        }
+    }
+}
+
+// One wave per touched vertex (walking the touched bitmap word by word).
+__global__ void __launch_bounds__(256)
+pprb_activate_k(const int32_t* __restrict__ row, float* __restrict__ P, float* __restrict__ R,
+                float* __restrict__ M, float* __restrict__ RU, uint64_t* __restrict__ mask,
+                const uint32_t* __restrict__ tbits, int64_t V, int64_t nw, int groups, float alpha, float eps,
+                int force) {
+    const int lane = lane_id();
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t words = nw * groups;
+    for (int64_t wi = wave; wi < words; wi += nwaves) {
+        uint32_t word = tbits[wi];
+        if (!word) continue;
+        const int g = (int)(wi / nw);
+        const int64_t vb = (wi - (int64_t)g * nw) * 32;
+        for (; word; word &= word - 1) {
+            const int64_t v = vb + __builtin_ctz(word);
+            const int64_t x = ((int64_t)g * V + v) * 64 + lane;
+            const float mv = M[x];
+            bool a = false;
+            if (mv != 0.0f) {
+                M[x] = 0.0f;
+                const float rv = R[x] + mv;
+                const float thr = (float)(row[v + 1] - row[v]) * eps;
+                if (force || rv > thr) {
+                    P[x] += alpha * rv;
+                    R[x] = 0.0f;
+                    RU[x] = rv;
+                    a = true;
+                } else {
+                    R[x] = rv;
+                }
+            }
+            const uint64_t b = __ballot(a);
+            if (lane == 0) mask[(int64_t)g * V + v] = b;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+pprb_emit_k(const int32_t* __restrict__ row, uint64_t* __restrict__ mask, uint32_t* __restrict__ tbits,
+            int4* __restrict__ items, uint64_t* __restrict__ imask, int64_t* __restrict__ ctrl, int64_t V,
+            int64_t nw, int groups, int chunk, int64_t itemcap) {
+    const int lane = lane_id();
+    int64_t front = 0, edges = 0;
+    const int64_t n = nw * groups;
+    const int64_t nn = (n + 63) & ~(int64_t)63;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nn; i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t word = 0;
+        int g = 0;
+        int64_t vb = 0;
+        if (i < n) {
+            word = tbits[i];
+            if (word) tbits[i] = 0;
+            g = (int)(i / nw);
+            vb = (i - (int64_t)g * nw) * 32;
+        }
+        int cnt = 0;
+        for (uint32_t b = word; b; b &= b - 1) {
+            const int64_t v = vb + __builtin_ctz(b);
+            if (mask[(int64_t)g * V + v]) {
+                const int d = row[v + 1] - row[v];
+                cnt += (d + chunk - 1) / chunk;
+                front += 1;
+                edges += d;
+            }
+        }
+        int tot;
+        const int off = wave_excl_scan(cnt, &tot);
+        int64_t base = 0;
+        if (tot > 0) {
+            if (lane == 0) base = (int64_t)atomicAdd((unsigned long long*)&ctrl[P_CUR], (unsigned long long)tot);
+            base = shfl64(base, 0);
+        }
+        const bool room = base + tot <= itemcap;
+        if (!room && lane == 0) atomicOr((unsigned long long*)&ctrl[P_OVF], 1ull);
+        int64_t pos = base + off;
+        for (uint32_t b = word; b; b &= b - 1) {
+            const int64_t v = vb + __builtin_ctz(b);
+            uint64_t* mp = &mask[(int64_t)g * V + v];
+            const uint64_t mk = *mp;
+            if (!mk) continue;
+            *mp = 0;
+            if (!room) continue;
+            const int r0 = row[v], r1 = row[v + 1];
+            for (int e = r0; e < r1; e += chunk, ++pos) {
+                items[pos] = make_int4((int)v, e, min(e + chunk, r1), g);
+                imask[pos] = mk;
+            }
+        }
+    }
+    front = wave_sum64(front);
+    edges = wave_sum64(edges);
+    if (lane == 0 && front) {
+        atomicAdd((unsigned long long*)&ctrl[P_FRONT], (unsigned long long)front);
+        atomicAdd((unsigned long long*)&ctrl[P_EDGES], (unsigned long long)edges);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[P_DONE_E], 1ull);
+        if (prev == (unsigned long long)gridDim.x - 1) {
+            __threadfence();
+            ctrl[P_DONE_E] = 0;
+            int64_t c = (int64_t)atomicAdd((unsigned long long*)&ctrl[P_CUR], 0ull);
+            if (c > itemcap) { ctrl[P_OVF] = 1; ctrl[P_CUR] = 0; }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ host
+struct PprState {
+    // single
+    float *p = nullptr, *r = nullptr, *m = nullptr;
+    int4* q = nullptr;
+    int64_t qcap = 0;
+    int64_t* ctrl = nullptr;
+    int64_t* h_ctrl = nullptr;
+    // batched
+    int groups = 0;
+    float *P = nullptr, *R = nullptr, *M = nullptr, *RU = nullptr;
+    uint64_t* mask = nullptr;
+    uint32_t* tbits = nullptr;
+    int64_t nw = 0;
+    int4* items = nullptr;
+    uint64_t* imask = nullptr;
+    int64_t itemcap = 0;
+    int32_t* srcs = nullptr;
+};
+
+static std::mutex g_ps_mu;
+static std::vector<std::pair<const Graph*, PprState*>> g_ps;
+
+static PprState* pstate(Graph* g) {
+    std::lock_guard<std::mutex> lk(g_ps_mu);
+    for (auto& p : g_ps)
+        if (p.first == g) return p.second;
+    auto* s = new PprState;
+    g_ps.emplace_back(g, s);
+    return s;
+}
+
+void ppr_release(Graph* g) {
+    std::lock_guard<std::mutex> lk(g_ps_mu);
+    for (size_t i = 0; i < g_ps.size(); ++i)
+        if (g_ps[i].first == g) {
+            PprState* s = g_ps[i].second;
+            for (void* p : {(void*)s->p, (void*)s->r, (void*)s->m, (void*)s->q, (void*)s->ctrl, (void*)s->P,
+                            (void*)s->R, (void*)s->M, (void*)s->RU, (void*)s->mask, (void*)s->tbits,
+                            (void*)s->items, (void*)s->imask, (void*)s->srcs})
+                if (p) (void)hipFree(p);
+            if (s->h_ctrl) (void)hipHostFree(s->h_ctrl);
+            delete s;
+            g_ps.erase(g_ps.begin() + i);
+            return;
+        }
+}
+
+static bool ppr_pruned(const Graph* g) { return g->d_hrow != nullptr && g->d_hadjf != nullptr; }
+
+static int ppr_check(Graph* g, float alpha, float eps) {
+    if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");
+    if (!g->d_adjf) return fail(C2G_ENOTSUP, "graph has no float weights (flist) for PPR");
+    if (!(alpha > 0.0f && alpha < 1.0f)) return fail(C2G_EINVAL, "alpha must be in (0,1)");
+    if (!(eps > 0.0f)) return fail(C2G_EINVAL, "eps must be > 0");
+    return C2G_OK;
+}
+
+static int ensure_ppr_single(Graph* g, PprState* s) {
+    HIP_TRY(hipSetDevice(g->device));
+    const int64_t V = std::max<int64_t>(g->V, 1);
+    const int chunk = std::max(1, g->tune.chunk);
+    const int64_t need = V + g->E / chunk + 64;
+    if (!s->p) {
+        HIP_TRY(hipMalloc(&s->p, sizeof(float) * V));
+        HIP_TRY(hipMalloc(&s->r, sizeof(float) * V));
+        HIP_TRY(hipMalloc(&s->m, sizeof(float) * V));
+        HIP_TRY(hipMalloc(&s->ctrl, sizeof(int64_t) * P_N));
+        HIP_TRY(hipHostMalloc((void**)&s->h_ctrl, sizeof(int64_t) * P_N, hipHostMallocDefault));
+    }
+    if (s->qcap < need) {
+        if (s->q) (void)hipFree(s->q);
+        HIP_TRY(hipMalloc(&s->q, sizeof(int4) * need));
+        s->qcap = need;
+    }
+    return C2G_OK;
+}
+
+static const int kPprMaxRounds = 100000;
+
+static int ppr_single(Graph* g, int32_t src, float alpha, float eps, float* p_out, float* r_out,
+                      c2g_stats* st) {
+    int rc = ppr_check(g, alpha, eps);
+    if (rc) return rc;
+    if (src < 0 || src >= g->V) return fail(C2G_EINVAL, "source %d out of range [0,%d)", src, g->V);
+    PprState* s = pstate(g);
+    if ((rc = ensure_ppr_single(g, s))) return rc;
+    const bool pruned = ppr_pruned(g);
+    const int64_t V = g->V;
+    const int grid = std::max(1, g->num_cus) * g->tune.grid_per_cu;
+    const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (V + 255) / 256));
+    const double t0 = now_ms();
+    hipLaunchKernelGGL(ppr_init_k, dim3(egrid), dim3(256), 0, g->stream, s->p, s->r, s->m, V);
+    hipLaunchKernelGGL(ppr_seed_k, dim3(1), dim3(256), 0, g->stream, g->d_row, g->d_adjf,
+                       pruned ? g->d_hrow : (const int32_t*)nullptr, pruned ? g->d_hadjf : (const int2*)nullptr,
+                       s->p, s->m, s->ctrl, src, alpha, pruned ? 1 : 0);
+    hipLaunchKernelGGL(ppr_expand_k, dim3(egrid), dim3(256), 0, g->stream, g->d_row, s->p, s->r, s->m, s->q,
+                       s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 1);
+    HIP_TRY(hipGetLastError());
+    int rounds = 0;
+    for (;;) {
+        const int K = g->tune.iters_per_graph;
+        for (int k = 0; k < K; ++k) {
+            hipLaunchKernelGGL(ppr_push_k, dim3(grid), dim3(256), 0, g->stream, g->d_adjf, s->m, s->q, s->ctrl,
+                               alpha);
+            hipLaunchKernelGGL(ppr_expand_k, dim3(egrid), dim3(256), 0, g->stream, g->d_row, s->p, s->r, s->m,
+                               s->q, s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 0);
+        }
+        HIP_TRY(hipGetLastError());
+        rounds += K;
+        HIP_TRY(hipMemcpyAsync(s->h_ctrl, s->ctrl, sizeof(int64_t) * P_N, hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        if (s->h_ctrl[P_OVF]) return fail(C2G_EDEVICE, "PPR frontier queue overflow (internal)");
+        if (s->h_ctrl[P_CUR] == 0) break;
+        if (rounds > kPprMaxRounds) return fail(C2G_ETIMEOUT, "PPR did not converge in %d rounds", kPprMaxRounds);
+    }
+    const double t1 = now_ms();
+    if (p_out) HIP_TRY(hipMemcpy(p_out, s->p, sizeof(float) * V, hipMemcpyDeviceToHost));
+    if (r_out) HIP_TRY(hipMemcpy(r_out, s->r, sizeof(float) * V, hipMemcpyDeviceToHost));
+    if (st) {
+        st->lane = C2G_LANE_GPU;
+        st->iterations = (int32_t)s->h_ctrl[P_ITER];
+        st->frontier_total = s->h_ctrl[P_FRONT];
+        st->edges_relaxed = s->h_ctrl[P_EDGES];
+        st->items = s->h_ctrl[P_ITEMS];
+        st->ms = t1 - t0;
+    }
+    return C2G_OK;
+}
+
+static int ensure_ppr_batch(Graph* g, PprState* s, int groups) {
+    HIP_TRY(hipSetDevice(g->device));
+    const int64_t V = std::max<int64_t>(g->V, 1);
+    const int chunk = std::max(1, g->tune.batch_chunk);
+    const int64_t needi = groups * (V + g->E / chunk) + 64;
+    if (!s->ctrl) {
+        HIP_TRY(hipMalloc(&s->ctrl, sizeof(int64_t) * P_N));
+        HIP_TRY(hipHostMalloc((void**)&s->h_ctrl, sizeof(int64_t) * P_N, hipHostMallocDefault));
+    }
+    if (groups > s->groups || s->itemcap < needi) {
+        const int G = std::max(groups, s->groups);
+        for (void** p : {(void**)&s->P, (void**)&s->R, (void**)&s->M, (void**)&s->RU, (void**)&s->mask,
+                         (void**)&s->tbits, (void**)&s->items, (void**)&s->imask, (void**)&s->srcs})
+            if (*p) { (void)hipFree(*p); *p = nullptr; }
+        const size_t n = (size_t)G * V * 64;
+        HIP_TRY(hipMalloc(&s->P, sizeof(float) * n));
+        HIP_TRY(hipMalloc(&s->R, sizeof(float) * n));
+        HIP_TRY(hipMalloc(&s->M, sizeof(float) * n));
+        HIP_TRY(hipMalloc(&s->RU, sizeof(float) * n));
+        HIP_TRY(hipMalloc(&s->mask, sizeof(uint64_t) * (size_t)G * V));
+        HIP_TRY(hipMemset(s->mask, 0, sizeof(uint64_t) * (size_t)G * V));
+        s->nw = (V + 31) / 32 + 1;
+        HIP_TRY(hipMalloc(&s->tbits, sizeof(uint32_t) * s->nw * G));
+        s->itemcap = G * (V + g->E / chunk) + 64;
+        HIP_TRY(hipMalloc(&s->items, sizeof(int4) * s->itemcap));
+        HIP_TRY(hipMalloc(&s->imask, sizeof(uint64_t) * s->itemcap));
+        HIP_TRY(hipMalloc(&s->srcs, sizeof(int32_t) * 64 * G));
+        s->groups = G;
+    }
+    return C2G_OK;
+}
+
+static int ppr_batch(Graph* g, const int32_t* srcs, int32_t n, float alpha, float eps, float* p_out,
+                     float* r_out, c2g_stats* st) {
+    int rc = ppr_check(g, alpha, eps);
+    if (rc) return rc;
+    if (n < 1 || n > 64 * 1024) return fail(C2G_EINVAL, "need 1..65536 sources");
+    for (int32_t q = 0; q < n; ++q)
+        if (srcs[q] < 0 || srcs[q] >= g->V) return fail(C2G_EINVAL, "source %d out of range [0,%d)", srcs[q], g->V);
+    PprState* s = pstate(g);
+    const int groups = (n + 63) / 64;
+    if ((rc = ensure_ppr_batch(g, s, groups))) return rc;
+    const bool pruned = ppr_pruned(g);
+    const int64_t V = g->V;
+    const int grid = std::max(1, g->num_cus) * g->tune.grid_per_cu;
+    const int64_t words = s->nw * groups;
+    const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (words + 255) / 256));
+    const double t0 = now_ms();
+    HIP_TRY(hipMemcpyAsync(s->srcs, srcs, sizeof(int32_t) * n, hipMemcpyHostToDevice, g->stream));
+    const int64_t n4 = (int64_t)groups * V * 64 / 4;
+    hipLaunchKernelGGL(pprb_fill_k, dim3(grid), dim3(256), 0, g->stream, s->P, n4, s->tbits, words);
+    hipLaunchKernelGGL(pprb_fill_k, dim3(grid), dim3(256), 0, g->stream, s->R, n4, s->tbits, (int64_t)0);
+    hipLaunchKernelGGL(pprb_fill_k, dim3(grid), dim3(256), 0, g->stream, s->M, n4, s->tbits, (int64_t)0);
+    hipLaunchKernelGGL(pprb_seed_k, dim3(n), dim3(256), 0, g->stream, s->srcs, n, g->d_row, g->d_adjf,
+                       pruned ? g->d_hrow : (const int32_t*)nullptr, pruned ? g->d_hadjf : (const int2*)nullptr,
+                       s->P, s->M, s->tbits, s->ctrl, V, s->nw, alpha);
+    auto threshold = [&](int force) {
+        hipLaunchKernelGGL(pprb_activate_k, dim3(grid), dim3(256), 0, g->stream, g->d_row, s->P, s->R, s->M,
+                           s->RU, s->mask, s->tbits, V, s->nw, groups, alpha, eps, force);
+        hipLaunchKernelGGL(pprb_emit_k, dim3(egrid), dim3(256), 0, g->stream, g->d_row, s->mask, s->tbits,
+                           s->items, s->imask, s->ctrl, V, s->nw, groups, g->tune.batch_chunk, s->itemcap);
+    };
+    threshold(1);
+    HIP_TRY(hipGetLastError());
+    int rounds = 0;
+    for (;;) {
+        const int K = g->tune.iters_per_graph;
+        for (int k = 0; k < K; ++k) {
+            hipLaunchKernelGGL(pprb_push_k, dim3(grid), dim3(256), 0, g->stream, g->d_adjf, s->M, s->RU, s->items,
+                               s->imask, s->tbits, s->ctrl, V, s->nw, alpha);
+            threshold(0);
+        }
+        HIP_TRY(hipGetLastError());
+        rounds += K;
+        HIP_TRY(hipMemcpyAsync(s->h_ctrl, s->ctrl, sizeof(int64_t) * P_N, hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        if (s->h_ctrl[P_OVF]) return fail(C2G_EDEVICE, "batched PPR capacity exceeded (internal)");
+        if (s->h_ctrl[P_CUR] == 0) break;
+        if (rounds > kPprMaxRounds) return fail(C2G_ETIMEOUT, "PPR did not converge in %d rounds", kPprMaxRounds);
+    }
+    const double t1 = now_ms();
+    // [g][V][64] -> caller [q][V]
+    std::vector<float> blk((size_t)V * 64);
+    for (int pass = 0; pass < 2; ++pass) {
+        float* out = pass ? r_out : p_out;
+        const float* dev = pass ? s->R : s->P;
+        if (!out) continue;
+        for (int gi = 0; gi < groups; ++gi) {
+            HIP_TRY(hipMemcpy(blk.data(), dev + (int64_t)gi * V * 64, sizeof(float) * V * 64, hipMemcpyDeviceToHost));
+            const int lanes = std::min(64, n - 64 * gi);
+            parallel_for(V, [&](int64_t lo, int64_t hi, int) {
+                for (int l = 0; l < lanes; ++l) {
+                    float* o = out + (int64_t)(64 * gi + l) * V;
+                    for (int64_t v = lo; v < hi; ++v) o[v] = blk[(size_t)v * 64 + l];
+                }
+            });
+        }
+    }
+    if (st) {
+        st->lane = C2G_LANE_GPU;
+        st->iterations = (int32_t)s->h_ctrl[P_ITER];
+        st->frontier_total = s->h_ctrl[P_FRONT];
+        st->edges_relaxed = s->h_ctrl[P_EDGES];
+        st->items = s->h_ctrl[P_ITEMS];
+        st->ms = t1 - t0;
+    }
+    return C2G_OK;
+}
+
+}  // namespace c2g
+
+using namespace c2g;
+
+extern "C" {
+
+int c2g_ppr(c2g_graph* h, int32_t src, float alpha, float eps, float* p_out, float* r_out, c2g_stats* st) {
+    if (!h || !p_out) return fail(C2G_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(h->g.mu);
+    if (st) memset(st, 0, sizeof *st);
+    return ppr_single(&h->g, src, alpha, eps, p_out, r_out, st);
+}
+
+int c2g_ppr_batch(c2g_graph* h, const int32_t* srcs, int32_t n, float alpha, float eps, float* p_out,
+                  float* r_out, c2g_stats* st) {
+    if (!h || !srcs || !p_out) return fail(C2G_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(h->g.mu);
+    if (st) memset(st, 0, sizeof *st);
+    return ppr_batch(&h->g, srcs, n, alpha, eps, p_out, r_out, st);
+}
+
+}  // extern "C"

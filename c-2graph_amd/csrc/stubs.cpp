@@ -3,9 +3,6 @@
 using namespace c2g;
 extern "C" {
 int c2g_convert_adjlist(const char*, const char*) { return fail(C2G_ENOTSUP, "not implemented"); }
-int c2g_ppr(c2g_graph*, int32_t, float, float, float*, float*, c2g_stats*) { return fail(C2G_ENOTSUP, "not implemented"); }
-int c2g_ppr_batch(c2g_graph*, const int32_t*, int32_t, float, float, float*, float*, c2g_stats*) { return fail(C2G_ENOTSUP, "not implemented"); }
-int c2g_adsorption(c2g_graph*, const int32_t*, const int32_t*, int32_t, int32_t, float, float, float, int32_t, float*, c2g_stats*) { return fail(C2G_ENOTSUP, "not implemented"); }
 int c2g_autotune(c2g_graph*, const int32_t*, int32_t, int32_t, c2g_tune*) { return fail(C2G_ENOTSUP, "not implemented"); }
 int c2g_server_start(c2g_graph*, const c2g_server_opts*, c2g_server**) { return fail(C2G_ENOTSUP, "not implemented"); }
 int64_t c2g_server_submit(c2g_server*, int32_t, int32_t) { return fail(C2G_ENOTSUP, "not implemented"); }
