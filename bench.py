@@ -82,6 +82,73 @@ def cpu_baseline(G, srcs, budget_s, threads):
     return len(done), wall
 
 
+def serve_phase(args):
+    """Served mixed stream (60% SSSP, 30% PPR, 10% adsorption) with Poisson arrivals."""
+    import c2g
+
+    t = time.time()
+    G = c2g.rmat(args.serve_scale, 16, 0.57, 0.19, 0.19, seed=5, wmax=255, want_f=True)
+    log(f"serve graph scale {args.serve_scale}: V={G['V']} E={G['E']} in {time.time() - t:.1f}s")
+    g = c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], G["flist"], device=0)
+    deg = np.diff(G["vlist"])
+    pool = np.flatnonzero(deg > 0)
+    rng = np.random.default_rng(5)
+    n = int(args.serve_qps * args.serve_seconds)
+    kinds = rng.choice([c2g.Q_SSSP, c2g.Q_PPR, c2g.Q_ADS], n, p=[0.6, 0.3, 0.1])
+    srcs = np.where(kinds == c2g.Q_ADS, rng.integers(0, 8, n), rng.choice(pool, n))
+    gaps = rng.exponential(1.0 / args.serve_qps, n)
+    lat = []
+    lanes = {}
+    with c2g.Server(g, max_batch=64, ads_labels=16, ads_iters=20, alpha=0.15, eps=1e-7) as s:
+        # warm the lanes (allocations, first kernel loads)
+        for k in (c2g.Q_SSSP, c2g.Q_PPR, c2g.Q_ADS):
+            s.poll(s.submit(k, int(pool[0]) if k != c2g.Q_ADS else 0), want_result=False)
+        import queue
+
+        pending = queue.Queue()
+
+        def poller():
+            while True:
+                item = pending.get()
+                if item is None:
+                    return
+                rid, k = item
+                info, _ = s.poll(rid, want_result=False)
+                lat.append((k, info["t_done_ms"] - info["t_submit_ms"], info["t_done_ms"]))
+                lanes[info["lane"]] = lanes.get(info["lane"], 0) + 1
+
+        th = threading.Thread(target=poller)
+        th.start()
+        t0 = time.perf_counter()
+        nxt = t0
+        for i in range(n):
+            nxt += gaps[i]
+            d = nxt - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            pending.put((s.submit(int(kinds[i]), int(srcs[i])), int(kinds[i])))
+            if i % max(1, n // 10) == 0:
+                log(f"serve: submitted {i}/{n}, completed {len(lat)}")
+        pending.put(None)
+        th.join()
+        t1 = time.perf_counter()
+    g.close()
+    L = np.array([x[1] for x in lat])
+    per = {}
+    for k, name in ((c2g.Q_SSSP, "sssp"), (c2g.Q_PPR, "ppr"), (c2g.Q_ADS, "adsorption")):
+        v = np.array([x[1] for x in lat if x[0] == k])
+        if len(v):
+            per[name] = {"n": int(len(v)), "p50_ms": round(float(np.percentile(v, 50)), 2),
+                         "p99_ms": round(float(np.percentile(v, 99)), 2)}
+    return {
+        "workload": f"R-MAT scale {args.serve_scale} seed 5, int w U[1,255] + row-normalised f; Poisson arrivals; "
+                    "60% SSSP / 30% PPR (alpha 0.15, eps 1e-7*deg) / 10% adsorption (16 labels on 1%, 20 it)",
+        "offered_qps": args.serve_qps, "achieved_qps": round(n / (t1 - t0), 2), "n": n,
+        "p50_ms": round(float(np.percentile(L, 50)), 2), "p99_ms": round(float(np.percentile(L, 99)), 2),
+        "per_kind": per, "lanes": {("gpu" if k == 0 else "host"): v for k, v in lanes.items()},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +163,11 @@ def main():
     ap.add_argument("--skip-profile", action="store_true", help="skip the profiled roofline pass")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--single-profile", action="store_true", help="profile two single-source queries")
+    ap.add_argument("--skip-serve", action="store_true", help="skip the served mixed stream")
+    ap.add_argument("--serve-only", action="store_true", help="only the served mixed stream")
+    ap.add_argument("--serve-qps", type=float, default=40.0)
+    ap.add_argument("--serve-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-scale", type=int, default=23)
     ap.add_argument("--profile-only", action="store_true",
                     help="only the profiled pass (for rocprofv3: it then sees exactly those launches)")
     args = ap.parse_args()
@@ -110,6 +182,10 @@ def main():
     if not c2g.device_available():
         print("bench.py: no gfx950 device visible", file=sys.stderr)
         sys.exit(1)
+
+    if args.serve_only:
+        print(json.dumps({"serving": serve_phase(args)}), flush=True)
+        return
 
     t = time.time()
     G = c2g.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=args.seed, wmax=args.wmax)
@@ -194,6 +270,11 @@ def main():
                       f"restated from src/sssp/base-perk/SSSP.cpp:88-119, one query per thread, {wall:.1f}s wall",
         }
         log(f"cpu baseline: {n_done} queries in {wall:.1f}s on {threads} threads")
+
+    if not args.skip_serve:
+        g.close()
+        extra["serving"] = serve_phase(args)
+        log(f"serving: {extra['serving']}")
 
     out = {
         "metric": METRIC, "value": round(qps, 3), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,

@@ -93,6 +93,8 @@ def lib():
                                C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_int64)]
         L.c2g_free.argtypes = [C.c_void_p]
         L.c2g_server_submit.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        L.c2g_server_stop.argtypes = [C.c_void_p]
+        L.c2g_server_start.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]
         L.c2g_server_poll.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.POINTER(ResultInfo)]
         _lib = L
     return _lib
@@ -386,3 +388,56 @@ class Graph:
         t = Tune()
         check(lib().c2g_autotune(self._h, _ptr(srcs, C.c_int32), len(srcs), int(batched), C.byref(t)))
         return t
+
+
+class Server:
+    """Continuous-batching query server over one graph (c2g_server_*)."""
+
+    def __init__(self, graph, max_batch=64, max_inflight=0, hbm_reserve=0, fake_free_hbm=0, host_lane=False,
+                 device_failed=False, alpha=0.15, eps=1e-7, ads_labels=16, ads_iters=20, p_inj=0.25, p_cont=0.70,
+                 p_abnd=0.05):
+        self.graph = graph
+        self.L = ads_labels
+        o = ServerOpts()
+        o.max_batch, o.max_inflight, o.hbm_reserve, o.fake_free_hbm = max_batch, max_inflight, hbm_reserve, fake_free_hbm
+        o.host_lane, o.device_failed, o.alpha, o.eps = int(host_lane), int(device_failed), alpha, eps
+        o.ads_labels, o.ads_iters, o.p_inj, o.p_cont, o.p_abnd = ads_labels, ads_iters, p_inj, p_cont, p_abnd
+        self._s = C.c_void_p()
+        check(lib().c2g_server_start(graph._h, C.byref(o), C.byref(self._s)))
+
+    def submit(self, kind, src):
+        return check(lib().c2g_server_submit(self._s, int(kind), int(src)))
+
+    def poll(self, rid, wait_ms=-1, want_result=True):
+        """Returns (info dict, result array) or None if not done within wait_ms."""
+        info = ResultInfo()
+        out = None
+        V = self.graph.V
+        if want_result:
+            out = np.empty(V * self.L, np.float32)  # large enough for every kind
+        rc = lib().c2g_server_poll(self._s, rid, int(wait_ms),
+                                   out.ctypes.data_as(C.c_void_p) if out is not None else None, C.byref(info))
+        check(rc)
+        if rc == 0:
+            return None
+        d = {k: getattr(info, k) for k, _ in ResultInfo._fields_ if k != "pad"}
+        res = None
+        if out is not None:
+            if info.kind == Q_SSSP:
+                res = out[:V].view(np.int32).copy()
+            elif info.kind == Q_PPR:
+                res = out[:V].copy()
+            else:
+                res = out[: V * self.L].reshape(V, self.L).copy()
+        return d, res
+
+    def stop(self):
+        if self._s:
+            lib().c2g_server_stop(self._s)
+            self._s = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.stop()

@@ -151,13 +151,14 @@ static int ads_prepare(Graph* g, AdsState* s, int L, int nseed) {
 }
 
 template <int G>
-static void launch_spmm(Graph* g, AdsState* s, const float* Yt, float* Yn, const float* Yinj, int L, float pc) {
+static void launch_spmm(Graph* g, hipStream_t strm, AdsState* s, const float* Yt, float* Yn, const float* Yinj,
+                        int L, float pc) {
     const int grid = std::max(1, g->num_cus) * g->tune.grid_per_cu;
-    hipLaunchKernelGGL(ads_spmm_k<G>, dim3(grid), dim3(256), 0, g->stream, g->d_adjf, s->items, s->nitems, Yt, Yn,
+    hipLaunchKernelGGL(ads_spmm_k<G>, dim3(grid), dim3(256), 0, strm, g->d_adjf, s->items, s->nitems, Yt, Yn,
                        Yinj, L, pc);
 }
 
-static int ads_run(Graph* g, const int32_t* sv, const int32_t* sl, int32_t nseed, int32_t L, float p_inj,
+int ads_run(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t* sl, int32_t nseed, int32_t L, float p_inj,
                    float p_cont, float p_abnd, int32_t iters, float* y_out, c2g_stats* st) {
     if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");
     if (!g->d_adjf) return fail(C2G_ENOTSUP, "graph has no float weights (flist) for adsorption");
@@ -176,8 +177,8 @@ static int ads_run(Graph* g, const int32_t* sv, const int32_t* sl, int32_t nseed
     const double t0 = now_ms();
     float* Yinj = s->Y[2];
     // Yinj = p_inj * Y0 ; Y_0 = Y0
-    hipLaunchKernelGGL(ads_init_k, dim3(grid), dim3(256), 0, g->stream, Yinj, n);
-    hipLaunchKernelGGL(ads_init_k, dim3(grid), dim3(256), 0, g->stream, s->Y[0], n);
+    hipLaunchKernelGGL(ads_init_k, dim3(grid), dim3(256), 0, strm, Yinj, n);
+    hipLaunchKernelGGL(ads_init_k, dim3(grid), dim3(256), 0, strm, s->Y[0], n);
     if (nseed > 0) {
         // duplicate (v,l) seeds count once: Y0 is an indicator
         std::vector<std::pair<int32_t, int32_t>> u;
@@ -188,12 +189,12 @@ static int ads_run(Graph* g, const int32_t* sv, const int32_t* sl, int32_t nseed
         std::vector<int32_t> hv(u.size()), hl(u.size());
         for (size_t i = 0; i < u.size(); ++i) { hv[i] = u[i].first; hl[i] = u[i].second; }
         const int m = (int)u.size();
-        HIP_TRY(hipMemcpyAsync(s->seeds, hv.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, g->stream));
-        HIP_TRY(hipMemcpyAsync(s->seeds + nseed, hl.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, g->stream));
-        HIP_TRY(hipStreamSynchronize(g->stream));  // hv/hl are pageable host vectors
-        hipLaunchKernelGGL(ads_seed_k, dim3((m + 255) / 256), dim3(256), 0, g->stream, Yinj, s->seeds,
+        HIP_TRY(hipMemcpyAsync(s->seeds, hv.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, strm));
+        HIP_TRY(hipMemcpyAsync(s->seeds + nseed, hl.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, strm));
+        HIP_TRY(hipStreamSynchronize(strm));  // hv/hl are pageable host vectors
+        hipLaunchKernelGGL(ads_seed_k, dim3((m + 255) / 256), dim3(256), 0, strm, Yinj, s->seeds,
                            s->seeds + nseed, m, L, p_inj);
-        hipLaunchKernelGGL(ads_seed_k, dim3((m + 255) / 256), dim3(256), 0, g->stream, s->Y[0], s->seeds,
+        hipLaunchKernelGGL(ads_seed_k, dim3((m + 255) / 256), dim3(256), 0, strm, s->Y[0], s->seeds,
                            s->seeds + nseed, m, L, 1.0f);
     }
     int cur = 0;
@@ -201,14 +202,14 @@ static int ads_run(Graph* g, const int32_t* sv, const int32_t* sl, int32_t nseed
         float* Yt = s->Y[cur];
         float* Yn = s->Y[cur ^ 1];
         // split rows accumulate atomically into p_inj*Y0
-        HIP_TRY(hipMemcpyAsync(Yn, Yinj, sizeof(float) * n, hipMemcpyDeviceToDevice, g->stream));
-        if (L <= 16) launch_spmm<16>(g, s, Yt, Yn, Yinj, L, p_cont);
-        else if (L <= 32) launch_spmm<32>(g, s, Yt, Yn, Yinj, L, p_cont);
-        else launch_spmm<64>(g, s, Yt, Yn, Yinj, L, p_cont);
+        HIP_TRY(hipMemcpyAsync(Yn, Yinj, sizeof(float) * n, hipMemcpyDeviceToDevice, strm));
+        if (L <= 16) launch_spmm<16>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
+        else if (L <= 32) launch_spmm<32>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
+        else launch_spmm<64>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
         cur ^= 1;
     }
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(g->stream));
+    HIP_TRY(hipStreamSynchronize(strm));
     const double t1 = now_ms();
     if (y_out) HIP_TRY(hipMemcpy(y_out, s->Y[cur], sizeof(float) * n, hipMemcpyDeviceToHost));
     if (st) {
@@ -231,5 +232,5 @@ extern "C" int c2g_adsorption(c2g_graph* h, const int32_t* seed_v, const int32_t
     if (!h || (nseed > 0 && (!seed_v || !seed_label)) || nseed < 0) return fail(C2G_EINVAL, "bad arguments");
     std::lock_guard<std::mutex> lk(h->g.mu);
     if (st) memset(st, 0, sizeof *st);
-    return ads_run(&h->g, seed_v, seed_label, nseed, L, p_inj, p_cont, p_abnd, iters, y_out, st);
+    return ads_run(&h->g, h->g.stream, seed_v, seed_label, nseed, L, p_inj, p_cont, p_abnd, iters, y_out, st);
 }

@@ -102,6 +102,7 @@ void device_release(Graph* g);
 void batch_release(Graph* g);
 void ppr_release(Graph* g);
 void ads_release(Graph* g);
+void serve_release(Graph* g);
 void default_tune(Graph* g);
 int ensure_sssp_ws(Graph* g);
 int ensure_batch_ws(Graph* g);
@@ -117,6 +118,19 @@ int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* 
              int32_t* count_out, uint8_t* hid_out, c2g_stats* st, bool copy_out);
 int sssp_batch_group(Graph* g, const int32_t* srcs, int32_t n, int32_t* dist_out,
                      int64_t* sums, int32_t* reached, c2g_stats* st);
+int ppr_single(Graph* g, hipStream_t strm, int32_t src, float alpha, float eps, float* p_out,
+               float* r_out, c2g_stats* st);
+int ads_run(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t* sl, int32_t nseed, int32_t L,
+            float p_inj, float p_cont, float p_abnd, int32_t iters, float* y_out, c2g_stats* st);
+int host_ads(const Graph* g, const int32_t* sv, const int32_t* sl, int32_t nseed, int32_t L, float p_inj,
+             float p_cont, int32_t iters, float* y_out);
+
+// Server lane of the batched SSSP kernels: one 64-lane group, queries join
+// free lanes between rounds and leave when their lane has no pending work.
+int bserve_init(Graph* g, hipStream_t strm);
+int bserve_admit(Graph* g, hipStream_t strm, int lane, int32_t src);
+int bserve_step(Graph* g, hipStream_t strm, uint64_t* lanes_with_work);
+int bserve_extract(Graph* g, hipStream_t strm, const int* lanes, int n, int32_t* const* outs);
 
 }  // namespace c2g
 

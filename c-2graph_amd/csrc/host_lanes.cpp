@@ -113,6 +113,34 @@ static int host_push(const Graph* g, float alpha, float eps, float* p, float* r,
     return C2G_OK;
 }
 
+// Adsorption on host cores (server fallback lane): the Jacobi recurrence of
+// the device kernel, FP32, rows in parallel.
+int host_ads(const Graph* g, const int32_t* sv, const int32_t* sl, int32_t nseed, int32_t L, float p_inj,
+             float p_cont, int32_t iters, float* y_out) {
+    if (g->fl.empty() && g->E > 0) return fail(C2G_ENOTSUP, "graph has no float weights (flist)");
+    const int64_t n = (int64_t)g->V * L;
+    std::vector<float> y0((size_t)n, 0.0f), yt((size_t)n), yn((size_t)n);
+    for (int32_t i = 0; i < nseed; ++i) y0[(size_t)sv[i] * L + sl[i]] = 1.0f;
+    yt = y0;
+    for (int32_t t = 0; t < iters; ++t) {
+        parallel_for(g->V, [&](int64_t lo, int64_t hi, int) {
+            std::vector<float> acc((size_t)L);
+            for (int64_t v = lo; v < hi; ++v) {
+                std::fill(acc.begin(), acc.end(), 0.0f);
+                for (int32_t e = g->vl[v]; e < g->vl[v + 1]; ++e) {
+                    const float f = g->fl[e];
+                    const float* y = &yt[(size_t)g->el[e] * L];
+                    for (int32_t l = 0; l < L; ++l) acc[l] += f * y[l];
+                }
+                for (int32_t l = 0; l < L; ++l) yn[(size_t)v * L + l] = p_inj * y0[(size_t)v * L + l] + p_cont * acc[l];
+            }
+        });
+        yt.swap(yn);
+    }
+    memcpy(y_out, yt.data(), sizeof(float) * (size_t)n);
+    return C2G_OK;
+}
+
 }  // namespace c2g
 
 using namespace c2g;

@@ -445,9 +445,21 @@ static int ensure_ppr_single(Graph* g, PprState* s) {
 }
 
 static const int kPprMaxRounds = 100000;
+static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, float eps, int32_t budget,
+                         int32_t handoff_min, float* p_out, float* r_out, std::vector<int32_t>* frontier,
+                         std::vector<float>* fr_ru, c2g_stats* st);
 
-static int ppr_single(Graph* g, int32_t src, float alpha, float eps, float* p_out, float* r_out,
+int ppr_single(Graph* g, hipStream_t strm, int32_t src, float alpha, float eps, float* p_out, float* r_out,
                       c2g_stats* st) {
+    return ppr_single_ex(g, strm, src, alpha, eps, 0, 0, p_out, r_out, nullptr, nullptr, st);
+}
+
+// budget > 0: after `budget` rounds, stop as soon as the frontier holds at
+// most handoff_min vertices; the remaining frontier is returned with r[u]
+// restored to the residual it was about to push (p already credited).
+static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, float eps, int32_t budget,
+                         int32_t handoff_min, float* p_out, float* r_out, std::vector<int32_t>* frontier,
+                         std::vector<float>* fr_ru, c2g_stats* st) {
     int rc = ppr_check(g, alpha, eps);
     if (rc) return rc;
     if (src < 0 || src >= g->V) return fail(C2G_EINVAL, "source %d out of range [0,%d)", src, g->V);
@@ -458,29 +470,52 @@ static int ppr_single(Graph* g, int32_t src, float alpha, float eps, float* p_ou
     const int grid = std::max(1, g->num_cus) * g->tune.grid_per_cu;
     const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (V + 255) / 256));
     const double t0 = now_ms();
-    hipLaunchKernelGGL(ppr_init_k, dim3(egrid), dim3(256), 0, g->stream, s->p, s->r, s->m, V);
-    hipLaunchKernelGGL(ppr_seed_k, dim3(1), dim3(256), 0, g->stream, g->d_row, g->d_adjf,
+    hipLaunchKernelGGL(ppr_init_k, dim3(egrid), dim3(256), 0, strm, s->p, s->r, s->m, V);
+    hipLaunchKernelGGL(ppr_seed_k, dim3(1), dim3(256), 0, strm, g->d_row, g->d_adjf,
                        pruned ? g->d_hrow : (const int32_t*)nullptr, pruned ? g->d_hadjf : (const int2*)nullptr,
                        s->p, s->m, s->ctrl, src, alpha, pruned ? 1 : 0);
-    hipLaunchKernelGGL(ppr_expand_k, dim3(egrid), dim3(256), 0, g->stream, g->d_row, s->p, s->r, s->m, s->q,
+    hipLaunchKernelGGL(ppr_expand_k, dim3(egrid), dim3(256), 0, strm, g->d_row, s->p, s->r, s->m, s->q,
                        s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 1);
     HIP_TRY(hipGetLastError());
     int rounds = 0;
+    int64_t front_prev = 1;
     for (;;) {
-        const int K = g->tune.iters_per_graph;
+        const int K = budget > 0 ? 1 : g->tune.iters_per_graph;
         for (int k = 0; k < K; ++k) {
-            hipLaunchKernelGGL(ppr_push_k, dim3(grid), dim3(256), 0, g->stream, g->d_adjf, s->m, s->q, s->ctrl,
+            hipLaunchKernelGGL(ppr_push_k, dim3(grid), dim3(256), 0, strm, g->d_adjf, s->m, s->q, s->ctrl,
                                alpha);
-            hipLaunchKernelGGL(ppr_expand_k, dim3(egrid), dim3(256), 0, g->stream, g->d_row, s->p, s->r, s->m,
+            hipLaunchKernelGGL(ppr_expand_k, dim3(egrid), dim3(256), 0, strm, g->d_row, s->p, s->r, s->m,
                                s->q, s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 0);
         }
         HIP_TRY(hipGetLastError());
         rounds += K;
-        HIP_TRY(hipMemcpyAsync(s->h_ctrl, s->ctrl, sizeof(int64_t) * P_N, hipMemcpyDeviceToHost, g->stream));
-        HIP_TRY(hipStreamSynchronize(g->stream));
+        HIP_TRY(hipMemcpyAsync(s->h_ctrl, s->ctrl, sizeof(int64_t) * P_N, hipMemcpyDeviceToHost, strm));
+        HIP_TRY(hipStreamSynchronize(strm));
         if (s->h_ctrl[P_OVF]) return fail(C2G_EDEVICE, "PPR frontier queue overflow (internal)");
         if (s->h_ctrl[P_CUR] == 0) break;
+        if (budget > 0) {
+            const int64_t fsize = s->h_ctrl[P_FRONT] - front_prev;  // vertices activated this round
+            front_prev = s->h_ctrl[P_FRONT];
+            if (rounds >= budget && fsize <= handoff_min) break;
+        }
         if (rounds > kPprMaxRounds) return fail(C2G_ETIMEOUT, "PPR did not converge in %d rounds", kPprMaxRounds);
+    }
+    if (frontier) {
+        // Hand over the queued frontier: distinct u with the residual it carries.
+        const int64_t n = s->h_ctrl[P_CUR];
+        std::vector<int4> items((size_t)n);
+        if (n) HIP_TRY(hipMemcpy(items.data(), s->q, sizeof(int4) * n, hipMemcpyDeviceToHost));
+        std::vector<uint8_t> seen((size_t)V, 0);
+        frontier->clear();
+        fr_ru->clear();
+        for (const auto& it : items)
+            if (!seen[it.x]) {
+                seen[it.x] = 1;
+                frontier->push_back(it.x);
+                float f;
+                memcpy(&f, &it.w, sizeof f);
+                fr_ru->push_back(f);
+            }
     }
     const double t1 = now_ms();
     if (p_out) HIP_TRY(hipMemcpy(p_out, s->p, sizeof(float) * V, hipMemcpyDeviceToHost));
@@ -615,7 +650,37 @@ int c2g_ppr(c2g_graph* h, int32_t src, float alpha, float eps, float* p_out, flo
     if (!h || !p_out) return fail(C2G_EINVAL, "null argument");
     std::lock_guard<std::mutex> lk(h->g.mu);
     if (st) memset(st, 0, sizeof *st);
-    return ppr_single(&h->g, src, alpha, eps, p_out, r_out, st);
+    return ppr_single(&h->g, h->g.stream, src, alpha, eps, p_out, r_out, st);
+}
+
+int c2g_ppr_migrate(c2g_graph* h, int32_t src, float alpha, float eps, int32_t budget, int32_t handoff_min,
+                    float* p_out, float* r_out, c2g_stats* st) {
+    if (!h || !p_out) return fail(C2G_EINVAL, "null argument");
+    if (budget < 1) return fail(C2G_EINVAL, "budget must be >= 1");
+    std::lock_guard<std::mutex> lk(h->g.mu);
+    Graph* g = &h->g;
+    if (st) memset(st, 0, sizeof *st);
+    std::vector<float> r((size_t)g->V);
+    std::vector<int32_t> fr;
+    std::vector<float> ru;
+    c2g_stats gs{};
+    int rc = ppr_single_ex(g, g->stream, src, alpha, eps, budget, std::max(0, handoff_min), p_out, r.data(), &fr,
+                           &ru, &gs);
+    if (rc) return rc;
+    for (size_t i = 0; i < fr.size(); ++i) r[fr[i]] = ru[i];
+    c2g_stats hs{};
+    if (!fr.empty()) {
+        rc = c2g_host_forward_push(h, alpha, eps, p_out, r.data(), fr.data(), (int32_t)fr.size(), &hs);
+        if (rc) return rc;
+    }
+    if (r_out) memcpy(r_out, r.data(), sizeof(float) * g->V);
+    if (st) {
+        *st = gs;
+        st->lane = fr.empty() ? C2G_LANE_GPU : C2G_LANE_MIXED;
+        st->host_iterations = hs.host_iterations;
+        st->edges_relaxed += hs.edges_relaxed;
+    }
+    return C2G_OK;
 }
 
 int c2g_ppr_batch(c2g_graph* h, const int32_t* srcs, int32_t n, float alpha, float eps, float* p_out,

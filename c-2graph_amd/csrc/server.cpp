@@ -1,0 +1,370 @@
+// Query server: continuous batching of mixed SSSP / PPR / adsorption queries
+// over one resident graph.
+//
+// The reference's collaboration loop (src/sssp/purn-perk/Main.cu:87-246) runs
+// two GPU streams and one pthread CPU worker fed through a pinned ring
+// (src/sssp/purn-perk/SSSP.cpp:18-84); it serves the same query repeatedly and
+// moves work by queue length.  Here the same roles become lanes of a server:
+//   * SSSP lane (stream A): one 64-lane group of the batched SSSP kernels.  A
+//     query is admitted into a free lane between rounds (its seed joins the
+//     next round's frontier) and retires as soon as its lane has no queued
+//     work; its distance column is copied out and reset.  Requests that
+//     arrive while others run therefore overlap with them on the device.
+//   * aux lane (stream B): PPR and adsorption queries, one at a time, running
+//     concurrently with the SSSP rounds on the device.
+//   * host lane (optional): the reference CPU worker's algorithms
+//     (Bellman-Ford / forward push / adsorption Jacobi) on host threads, used
+//     when the device is marked failed or a request cannot fit in HBM.
+// Admission: every request reserves its device result buffer from a budget =
+// free HBM at start (or opts.fake_free_hbm) minus opts.hbm_reserve; requests
+// wait in FIFO order until the budget and a lane free up.  Submission beyond
+// max_inflight outstanding requests is refused with C2G_EBUSY.
+#include "graph.h"
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <unordered_map>
+
+namespace c2g {
+
+struct Request {
+    int64_t id = 0;
+    int kind = 0;
+    int32_t src = 0;
+    int status = 0;
+    int lane = C2G_LANE_GPU;
+    bool done = false;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    int64_t bytes = 0;
+    std::vector<char> result;
+};
+
+}  // namespace c2g
+
+struct c2g_server {
+    c2g_graph* h = nullptr;
+    c2g::Graph* g = nullptr;
+    c2g_server_opts o{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<int64_t> q_sssp, q_aux, q_host;
+    std::unordered_map<int64_t, c2g::Request> reqs;
+    int64_t next_id = 0;
+    int64_t inflight = 0;
+    int64_t budget = 0, used = 0;
+    bool stop = false;
+    std::thread t_sssp, t_aux;
+    std::vector<std::thread> t_host;
+    hipStream_t s_sssp = nullptr, s_aux = nullptr;
+    double t0 = 0;
+    int error = 0;
+};
+
+namespace c2g {
+
+static int64_t result_bytes(const c2g_server* s, int kind) {
+    const int64_t V = s->g->V;
+    if (kind == C2G_Q_SSSP) return 4 * V;
+    if (kind == C2G_Q_PPR) return 4 * V;
+    return 4 * V * s->o.ads_labels;
+}
+
+// Seed set `k` of adsorption queries: labels on a seeded 1% of vertices.
+static void ads_seeds(const Graph* g, int32_t k, int L, std::vector<int32_t>& sv, std::vector<int32_t>& sl) {
+    const int64_t n = std::max<int64_t>(1, g->V / 100);
+    Stream rs((uint64_t)k, 11);
+    sv.resize(n);
+    sl.resize(n);
+    for (int64_t i = 0; i < n; ++i) {
+        uint64_t h = rs((uint64_t)i);
+        sv[i] = (int32_t)((h >> 32) % (uint64_t)g->V);
+        sl[i] = (int32_t)((uint32_t)h % (uint32_t)L);
+    }
+}
+
+static void finish(c2g_server* s, Request& r, int status, int lane) {
+    r.status = status;
+    r.lane = lane;
+    r.t_done = now_ms() - s->t0;
+    r.done = true;
+}
+
+static void sssp_worker(c2g_server* s) {
+    Graph* g = s->g;
+    (void)hipSetDevice(g->device);
+    int rc = bserve_init(g, s->s_sssp);
+    int64_t lane_req[64];
+    for (auto& x : lane_req) x = -1;
+    while (true) {
+        std::vector<std::pair<int, int32_t>> admit;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            bool busy = false;
+            for (auto x : lane_req) busy |= x >= 0;
+            if (!busy)
+                s->cv_work.wait(lk, [&] { return s->stop || !s->q_sssp.empty(); });
+            if (s->stop) break;
+            for (int l = 0; l < std::min(64, std::max(1, s->o.max_batch)) && !s->q_sssp.empty(); ++l) {
+                if (lane_req[l] >= 0) continue;
+                Request& r = s->reqs[s->q_sssp.front()];
+                if (s->used + r.bytes > s->budget) break;  // wait for memory
+                s->q_sssp.pop_front();
+                s->used += r.bytes;
+                r.t_admit = now_ms() - s->t0;
+                lane_req[l] = r.id;
+                admit.emplace_back(l, r.src);
+            }
+        }
+        {
+            bool busy = false;
+            for (auto x : lane_req) busy |= x >= 0;
+            if (!busy) {  // head request waits for HBM budget held by the aux lane
+                std::unique_lock<std::mutex> lk(s->mu);
+                s->cv_work.wait_for(lk, std::chrono::milliseconds(1));
+                continue;
+            }
+        }
+        for (auto& a : admit)
+            if (rc == C2G_OK) rc = bserve_admit(g, s->s_sssp, a.first, a.second);
+        uint64_t work = 0;
+        if (rc == C2G_OK) rc = bserve_step(g, s->s_sssp, &work);
+        std::vector<int> lanes;
+        std::vector<int32_t*> outs;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (int l = 0; l < 64; ++l)
+                if (lane_req[l] >= 0 && (rc != C2G_OK || !((work >> l) & 1ull))) {
+                    Request& r = s->reqs[lane_req[l]];
+                    r.result.resize(sizeof(int32_t) * g->V);
+                    lanes.push_back(l);
+                    outs.push_back((int32_t*)r.result.data());
+                }
+        }
+        if (!lanes.empty() && rc == C2G_OK) rc = bserve_extract(g, s->s_sssp, lanes.data(), (int)lanes.size(), outs.data());
+        if (!lanes.empty()) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (int l : lanes) {
+                Request& r = s->reqs[lane_req[l]];
+                s->used -= r.bytes;
+                finish(s, r, rc, C2G_LANE_GPU);
+                lane_req[l] = -1;
+            }
+            s->cv_done.notify_all();
+        }
+        if (rc != C2G_OK) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->error = rc;
+        }
+    }
+}
+
+static void aux_worker(c2g_server* s) {
+    Graph* g = s->g;
+    (void)hipSetDevice(g->device);
+    std::vector<int32_t> sv, sl;
+    while (true) {
+        int64_t id;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [&] {
+                if (s->stop) return true;
+                if (s->q_aux.empty()) return false;
+                return s->used + s->reqs[s->q_aux.front()].bytes <= s->budget;
+            });
+            if (s->stop) break;
+            id = s->q_aux.front();
+            s->q_aux.pop_front();
+            Request& r = s->reqs[id];
+            s->used += r.bytes;
+            r.t_admit = now_ms() - s->t0;
+            r.result.resize(r.bytes);
+        }
+        Request* r;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            r = &s->reqs[id];
+        }
+        int rc;
+        if (r->kind == C2G_Q_PPR) {
+            rc = ppr_single(g, s->s_aux, r->src, s->o.alpha, s->o.eps, (float*)r->result.data(), nullptr, nullptr);
+        } else {
+            ads_seeds(g, r->src, s->o.ads_labels, sv, sl);
+            rc = ads_run(g, s->s_aux, sv.data(), sl.data(), (int32_t)sv.size(), s->o.ads_labels, s->o.p_inj,
+                         s->o.p_cont, s->o.p_abnd, s->o.ads_iters, (float*)r->result.data(), nullptr);
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->used -= r->bytes;
+        finish(s, *r, rc, C2G_LANE_GPU);
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();  // memory freed: the SSSP lane may admit
+    }
+}
+
+static void host_worker(c2g_server* s) {
+    Graph* g = s->g;
+    std::vector<int32_t> sv, sl;
+    while (true) {
+        int64_t id;
+        Request* r;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [&] { return s->stop || !s->q_host.empty(); });
+            if (s->stop) break;
+            id = s->q_host.front();
+            s->q_host.pop_front();
+            r = &s->reqs[id];
+            r->t_admit = now_ms() - s->t0;
+            r->result.resize(r->bytes);
+        }
+        int rc = C2G_OK;
+        if (r->kind == C2G_Q_SSSP) {
+            int32_t* d = (int32_t*)r->result.data();
+            for (int32_t v = 0; v < g->V; ++v) d[v] = INT32_MAX;
+            d[r->src] = 0;
+            std::vector<uint8_t> hid;
+            if (!g->hvl.empty()) { hid.assign((size_t)g->V, 0); hid[r->src] = 1; }
+            rc = host_bf(g, d, &r->src, 1, hid.empty() ? nullptr : hid.data(), nullptr);
+        } else if (r->kind == C2G_Q_PPR) {
+            rc = c2g_host_ppr(s->h, r->src, s->o.alpha, s->o.eps, (float*)r->result.data(), nullptr, nullptr);
+        } else {
+            ads_seeds(g, r->src, s->o.ads_labels, sv, sl);
+            rc = host_ads(g, sv.data(), sl.data(), (int32_t)sv.size(), s->o.ads_labels, s->o.p_inj, s->o.p_cont,
+                          s->o.ads_iters, (float*)r->result.data());
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        finish(s, *r, rc, C2G_LANE_HOST);
+        s->cv_done.notify_all();
+    }
+}
+
+}  // namespace c2g
+
+using namespace c2g;
+
+extern "C" {
+
+int c2g_server_start(c2g_graph* h, const c2g_server_opts* o, c2g_server** out) {
+    if (!h || !out) return fail(C2G_EINVAL, "null argument");
+    Graph* g = &h->g;
+    auto* s = new c2g_server;
+    s->h = h;
+    s->g = g;
+    if (o) s->o = *o;
+    if (s->o.max_batch <= 0 || s->o.max_batch > 64) s->o.max_batch = 64;
+    if (s->o.max_inflight <= 0) s->o.max_inflight = 1 << 20;
+    if (s->o.alpha <= 0) s->o.alpha = 0.15f;
+    if (s->o.eps <= 0) s->o.eps = 1e-7f;
+    if (s->o.ads_labels <= 0) s->o.ads_labels = 16;
+    if (s->o.ads_iters <= 0) s->o.ads_iters = 20;
+    if (s->o.p_inj <= 0 && s->o.p_cont <= 0) { s->o.p_inj = 0.25f; s->o.p_cont = 0.70f; s->o.p_abnd = 0.05f; }
+    const bool device_ok = g->on_device && !s->o.device_failed;
+    if (!device_ok && !s->o.host_lane) {
+        delete s;
+        return fail(C2G_EDEVICE, "server: no usable device and the host lane is disabled");
+    }
+    if (device_ok) {
+        int rc;
+        if (hipSetDevice(g->device) != hipSuccess) { delete s; return fail(C2G_EDEVICE, "hipSetDevice failed"); }
+        size_t fr = 0, tot = 0;
+        if (s->o.fake_free_hbm > 0) fr = (size_t)s->o.fake_free_hbm;
+        else if (hipMemGetInfo(&fr, &tot) != hipSuccess) { delete s; return fail(C2G_EDEVICE, "hipMemGetInfo failed"); }
+        s->budget = (int64_t)fr - s->o.hbm_reserve;
+        if (hipStreamCreateWithFlags(&s->s_sssp, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&s->s_aux, hipStreamNonBlocking) != hipSuccess) {
+            delete s;
+            return fail(C2G_EDEVICE, "stream creation failed");
+        }
+        if (!g->d_adj && !g->d_adjf) { delete s; return fail(C2G_ENOTSUP, "graph has neither weight kind"); }
+        (void)rc;
+    }
+    s->t0 = now_ms();
+    if (device_ok) {
+        if (g->d_adj) s->t_sssp = std::thread(sssp_worker, s);
+        if (g->d_adjf) s->t_aux = std::thread(aux_worker, s);
+    }
+    if (s->o.host_lane)
+        for (int i = 0; i < 2; ++i) s->t_host.emplace_back(host_worker, s);
+    *out = s;
+    return C2G_OK;
+}
+
+int64_t c2g_server_submit(c2g_server* s, int32_t kind, int32_t src) {
+    if (!s) return fail(C2G_EINVAL, "null server");
+    if (kind < C2G_Q_SSSP || kind > C2G_Q_ADS) return fail(C2G_EINVAL, "unknown query kind %d", kind);
+    Graph* g = s->g;
+    if (kind != C2G_Q_ADS && (src < 0 || src >= g->V)) return fail(C2G_EINVAL, "source %d out of range", src);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return fail(C2G_EINVAL, "server stopped");
+    if (s->inflight >= s->o.max_inflight) return fail(C2G_EBUSY, "server: %lld requests in flight", (long long)s->inflight);
+    Request r;
+    r.id = s->next_id++;
+    r.kind = kind;
+    r.src = src;
+    r.bytes = result_bytes(s, kind);
+    r.t_submit = now_ms() - s->t0;
+    const bool device_ok = g->on_device && !s->o.device_failed;
+    const bool has_dev_lane = device_ok && (kind == C2G_Q_SSSP ? g->d_adj != nullptr : g->d_adjf != nullptr);
+    const bool fits = r.bytes <= s->budget;
+    if (!has_dev_lane || !fits) {
+        if (!s->o.host_lane)
+            return fail(C2G_EBUSY, "server: request needs %lld bytes of HBM, budget %lld", (long long)r.bytes,
+                        (long long)s->budget);
+        s->q_host.push_back(r.id);
+    } else if (kind == C2G_Q_SSSP) {
+        s->q_sssp.push_back(r.id);
+    } else {
+        s->q_aux.push_back(r.id);
+    }
+    s->reqs.emplace(r.id, std::move(r));
+    s->inflight++;
+    s->cv_work.notify_all();
+    return s->next_id - 1;
+}
+
+int c2g_server_poll(c2g_server* s, int64_t id, int32_t wait_ms, void* out, c2g_result_info* info) {
+    if (!s) return fail(C2G_EINVAL, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(C2G_EINVAL, "unknown request id %lld", (long long)id);
+    auto ready = [&] { return s->reqs[id].done; };
+    if (!ready()) {
+        if (wait_ms == 0) return 0;
+        if (wait_ms < 0) s->cv_done.wait(lk, ready);
+        else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), ready)) return 0;
+    }
+    Request& r = s->reqs[id];
+    if (info) {
+        info->id = r.id;
+        info->kind = r.kind;
+        info->status = r.status;
+        info->lane = r.lane;
+        info->t_submit_ms = r.t_submit;
+        info->t_admit_ms = r.t_admit;
+        info->t_done_ms = r.t_done;
+    }
+    if (out && r.status == C2G_OK) memcpy(out, r.result.data(), r.result.size());
+    const int st = r.status;
+    s->reqs.erase(id);
+    s->inflight--;
+    if (st < 0) return fail(st, "request %lld failed", (long long)id);
+    return 1;
+}
+
+int c2g_server_stop(c2g_server* s) {
+    if (!s) return C2G_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    s->cv_done.notify_all();
+    if (s->t_sssp.joinable()) s->t_sssp.join();
+    if (s->t_aux.joinable()) s->t_aux.join();
+    for (auto& t : s->t_host) t.join();
+    if (s->s_sssp) { (void)hipStreamSynchronize(s->s_sssp); (void)hipStreamDestroy(s->s_sssp); }
+    if (s->s_aux) { (void)hipStreamSynchronize(s->s_aux); (void)hipStreamDestroy(s->s_aux); }
+    delete s;
+    return C2G_OK;
+}
+
+}  // extern "C"

@@ -44,6 +44,7 @@ enum : int {
     B_OVF = 8,     // capacity guard tripped (must stay 0)
     B_LEDGES = 9,  // (query, edge) relaxation attempts
     B_ITEMS_TOT = 10,  // items processed
+    B_LANEOR = 11,     // server lane: OR of the lane masks queued for the next round
     B_N = 16
 };
 
@@ -161,6 +162,7 @@ relax_b_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
             ctrl[B_ITEMS_TOT] += n;
             ctrl[B_ITEMS] = 0;
             ctrl[B_DONE_R] = 0;
+            ctrl[B_LANEOR] = 0;
         }
     }
 }
@@ -169,9 +171,10 @@ __global__ void __launch_bounds__(256)
 expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, uint64_t* __restrict__ mask,
            const uint64_t* __restrict__ hmask, uint32_t* __restrict__ abits, int4* __restrict__ items,
            uint64_t* __restrict__ imask, int64_t* __restrict__ ctrl, int64_t V, int64_t nw, int groups,
-           int chunk, int64_t itemcap) {
+           int chunk, int64_t itemcap, int lane_or) {
     const int lane = lane_id();
     int64_t front = 0, edges = 0, ledges = 0;
+    uint64_t lor = 0;  // lanes with queued work (server lane, one group)
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t n = nw * groups;
     const int64_t nn = (n + 63) & ~(int64_t)63;  // uniform trip count per wave
@@ -192,12 +195,14 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
             const uint64_t m = mg[v];
             const int r0 = row[v], r1 = row[v + 1];
             cnt += (r1 - r0 + chunk - 1) / chunk;
+            if (r1 > r0) lor |= m;
             uint64_t hm = 0;
             if (hrow) {
                 hm = m & hmask[(int64_t)g * V + v];
                 if (hm) {
                     const int h0 = hrow[v], h1 = hrow[v + 1];
                     cnt += (h1 - h0 + chunk - 1) / chunk;
+                    if (h1 > h0) lor |= hm;
                     edges += h1 - h0;
                     ledges += (int64_t)__popcll(hm) * (h1 - h0);
                 }
@@ -246,6 +251,11 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
         atomicAdd((unsigned long long*)&ctrl[B_FRONT], (unsigned long long)front);
         atomicAdd((unsigned long long*)&ctrl[B_EDGES], (unsigned long long)edges);
         atomicAdd((unsigned long long*)&ctrl[B_LEDGES], (unsigned long long)ledges);
+    }
+    if (lane_or) {
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) lor |= (uint64_t)shfl64((int64_t)lor, lane ^ d);
+        if (lane == 0 && lor) atomicOr((unsigned long long*)&ctrl[B_LANEOR], (unsigned long long)lor);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -369,43 +379,45 @@ static int ensure_batch(Graph* g, int groups) {
 
 static int relax_grid_b(const Graph* g) { return std::max(1, g->num_cus) * g->tune.grid_per_cu; }
 
-static int expand_b(Graph* g, int groups) {
+static int expand_b(Graph* g, int groups, hipStream_t strm, int lane_or = 0) {
     BatchWs& w = g->batch;
     BatchState* s = bstate(g);
     const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
     const int64_t words = s->nw * groups;
     const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(relax_grid_b(g), (words + 255) / 256));
-    hipLaunchKernelGGL(expand_b_k, dim3(egrid), dim3(256), 0, g->stream, g->d_row,
+    hipLaunchKernelGGL(expand_b_k, dim3(egrid), dim3(256), 0, strm, g->d_row,
                        pruned ? g->d_hrow : (const int32_t*)nullptr, w.mask[0],
                        pruned ? w.hmask : (const uint64_t*)nullptr, s->abits, s->items, s->imask, w.ctrl,
-                       (int64_t)g->V, s->nw, groups, g->tune.batch_chunk, s->itemcap);
+                       (int64_t)g->V, s->nw, groups, g->tune.batch_chunk, s->itemcap, lane_or);
     return C2G_OK;
 }
 
 // One round = relax_b + expand_b.  When relax_ms is non-null, events bracket
 // the relax kernel and its duration is added to *relax_ms (profiling runs).
-static int launch_round_b(Graph* g, int groups, double* relax_ms) {
+static int launch_round_b(Graph* g, int groups, double* relax_ms, hipStream_t strm = nullptr,
+                          int lane_or = 0) {
+    if (!strm) strm = g->stream;
     BatchWs& w = g->batch;
     BatchState* s = bstate(g);
     const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
     const int grid = relax_grid_b(g);
-    if (relax_ms) HIP_TRY(hipEventRecord(s->ev[0], g->stream));
+    if (relax_ms) HIP_TRY(hipEventRecord(s->ev[0], strm));
     if (pruned)
-        hipLaunchKernelGGL(relax_b_k<true>, dim3(grid), dim3(256), 0, g->stream, g->d_adj, g->d_hadj,
+        hipLaunchKernelGGL(relax_b_k<true>, dim3(grid), dim3(256), 0, strm, g->d_adj, g->d_hadj,
                            w.dist, w.mask[0], w.hmask, s->items, s->imask, s->abits, w.ctrl,
                            (int64_t)g->V, s->nw);
     else
-        hipLaunchKernelGGL(relax_b_k<false>, dim3(grid), dim3(256), 0, g->stream, g->d_adj,
+        hipLaunchKernelGGL(relax_b_k<false>, dim3(grid), dim3(256), 0, strm, g->d_adj,
                            (const int2*)nullptr, w.dist, w.mask[0], (uint64_t*)nullptr, s->items,
                            s->imask, s->abits, w.ctrl, (int64_t)g->V, s->nw);
     if (relax_ms) {
-        HIP_TRY(hipEventRecord(s->ev[1], g->stream));
+        HIP_TRY(hipEventRecord(s->ev[1], strm));
         HIP_TRY(hipEventSynchronize(s->ev[1]));
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
         *relax_ms += ms;
     }
-    expand_b(g, groups);
+    expand_b(g, groups, strm, lane_or);
     HIP_TRY(hipGetLastError());
     return C2G_OK;
 }
@@ -450,7 +462,7 @@ static int batch_run(Graph* g, const int32_t* srcs, int32_t n, c2g_stats* st, do
                        pruned ? g->d_hrow : (const int32_t*)nullptr, w.dist, w.mask[0],
                        pruned ? w.hmask : (uint64_t*)nullptr, s->abits, w.ctrl, V, s->nw);
     // Turn the seeded activations into the first round's items.
-    expand_b(g, groups);
+    expand_b(g, groups, g->stream);
     HIP_TRY(hipGetLastError());
     const bool use_graph = g->tune.use_graph && !relax_ms;
     int rc;
@@ -559,6 +571,130 @@ int sssp_batch_group(Graph* g, const int32_t* srcs, int32_t n, int32_t* dist_out
                 }
             });
         }
+    }
+    return C2G_OK;
+}
+
+
+// ------------------------------------------------------------ server lane
+// A query joins lane `lane` of group 0: its distance column was reset to
+// INT32_MAX when the lane was last freed (or at init).
+__global__ void bserve_admit_k(const int32_t* __restrict__ row, int32_t* __restrict__ dist,
+                               uint64_t* __restrict__ mask, uint32_t* __restrict__ abits, int lane, int32_t s) {
+    dist[(int64_t)s * 64 + lane] = 0;
+    if (row[s + 1] > row[s]) {
+        atomicOr((unsigned long long*)&mask[s], 1ull << lane);
+        atomicOr(&abits[s >> 5], 1u << (s & 31));
+    }
+}
+
+// Copies the distance columns of `n` lanes into out[k][V] and resets them.
+__global__ void bserve_extract_k(int32_t* __restrict__ dist, int64_t V, uint64_t lanes,
+                                 const int8_t* __restrict__ slot, int32_t* __restrict__ out) {
+    const int lane = lane_id();
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const bool mine = (lanes >> lane) & 1ull;
+    const int k = slot[lane];
+    for (int64_t v = wave; v < V; v += nwaves) {
+        if (!mine) continue;
+        const int64_t x = v * 64 + lane;
+        out[(int64_t)k * V + v] = dist[x];
+        dist[x] = INT32_MAX;
+    }
+}
+
+struct ServeState {
+    int32_t* out = nullptr;   // [8][V] device staging
+    int32_t* h_out = nullptr; // pinned mirror
+    int8_t* slot = nullptr;
+};
+static std::mutex g_ss_mu;
+static std::vector<std::pair<const Graph*, ServeState*>> g_ss;
+static ServeState* sstate(Graph* g) {
+    std::lock_guard<std::mutex> lk(g_ss_mu);
+    for (auto& p : g_ss)
+        if (p.first == g) return p.second;
+    auto* s = new ServeState;
+    g_ss.emplace_back(g, s);
+    return s;
+}
+void serve_release(Graph* g) {
+    std::lock_guard<std::mutex> lk(g_ss_mu);
+    for (size_t i = 0; i < g_ss.size(); ++i)
+        if (g_ss[i].first == g) {
+            ServeState* s = g_ss[i].second;
+            if (s->out) (void)hipFree(s->out);
+            if (s->slot) (void)hipFree(s->slot);
+            if (s->h_out) (void)hipHostFree(s->h_out);
+            delete s;
+            g_ss.erase(g_ss.begin() + i);
+            return;
+        }
+}
+constexpr int kServeStage = 8;  // lanes copied out per transfer
+
+int bserve_init(Graph* g, hipStream_t strm) {
+    if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");
+    if (!g->d_adj) return fail(C2G_ENOTSUP, "graph has no int weights (wlist) for SSSP");
+    int rc = ensure_batch(g, 1);
+    if (rc) return rc;
+    ServeState* ss = sstate(g);
+    if (!ss->out) {
+        HIP_TRY(hipMalloc(&ss->out, sizeof(int32_t) * (size_t)kServeStage * std::max(g->V, 1)));
+        HIP_TRY(hipHostMalloc((void**)&ss->h_out, sizeof(int32_t) * (size_t)kServeStage * std::max(g->V, 1),
+                              hipHostMallocDefault));
+        HIP_TRY(hipMalloc(&ss->slot, 64));
+    }
+    BatchWs& w = g->batch;
+    BatchState* s = bstate(g);
+    const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
+    HIP_TRY(hipMemsetAsync(w.ctrl, 0, sizeof(int64_t) * B_N, strm));
+    hipLaunchKernelGGL(batch_fill_k, dim3(std::max(1, g->num_cus) * 16), dim3(256), 0, strm, w.dist,
+                       (int64_t)g->V * 64, w.mask[0], pruned ? w.hmask : (uint64_t*)nullptr, (int64_t)g->V,
+                       s->abits, s->nw);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(strm));
+    return C2G_OK;
+}
+
+int bserve_admit(Graph* g, hipStream_t strm, int lane, int32_t src) {
+    if (lane < 0 || lane > 63 || src < 0 || src >= g->V) return fail(C2G_EINVAL, "bad lane/source");
+    BatchState* s = bstate(g);
+    hipLaunchKernelGGL(bserve_admit_k, dim3(1), dim3(1), 0, strm, g->d_row, g->batch.dist, g->batch.mask[0],
+                       s->abits, lane, src);
+    HIP_TRY(hipGetLastError());
+    return C2G_OK;
+}
+
+int bserve_step(Graph* g, hipStream_t strm, uint64_t* lanes_with_work) {
+    int rc = launch_round_b(g, 1, nullptr, strm, 1);
+    if (rc) return rc;
+    BatchWs& w = g->batch;
+    HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, sizeof(int64_t) * B_N, hipMemcpyDeviceToHost, strm));
+    HIP_TRY(hipStreamSynchronize(strm));
+    if (w.h_ctrl[B_OVF]) return fail(C2G_EDEVICE, "server lane capacity exceeded (internal)");
+    *lanes_with_work = (uint64_t)w.h_ctrl[B_LANEOR];
+    return C2G_OK;
+}
+
+int bserve_extract(Graph* g, hipStream_t strm, const int* lanes, int n, int32_t* const* outs) {
+    ServeState* ss = sstate(g);
+    const int64_t V = g->V;
+    for (int b = 0; b < n; b += kServeStage) {
+        const int m = std::min(kServeStage, n - b);
+        int8_t slot[64];
+        memset(slot, 0, sizeof slot);
+        uint64_t lm = 0;
+        for (int k = 0; k < m; ++k) { lm |= 1ull << lanes[b + k]; slot[lanes[b + k]] = (int8_t)k; }
+        HIP_TRY(hipMemcpyAsync(ss->slot, slot, 64, hipMemcpyHostToDevice, strm));
+        hipLaunchKernelGGL(bserve_extract_k, dim3(std::max(1, g->num_cus) * 8), dim3(256), 0, strm,
+                           g->batch.dist, V, lm, ss->slot, ss->out);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(ss->h_out, ss->out, sizeof(int32_t) * V * m, hipMemcpyDeviceToHost, strm));
+        HIP_TRY(hipStreamSynchronize(strm));
+        for (int k = 0; k < m; ++k)
+            if (outs[b + k]) memcpy(outs[b + k], ss->h_out + (int64_t)k * V, sizeof(int32_t) * V);
     }
     return C2G_OK;
 }

@@ -272,6 +272,12 @@ int c2g_host_ppr(const c2g_graph* g, int32_t src, float alpha, float eps,
  * r[v] > eps*outdeg(v).  FP32 state.  p_out, r_out: float[V] (r nullable). */
 int c2g_ppr(c2g_graph* g, int32_t src, float alpha, float eps, float* p_out,
             float* r_out, c2g_stats* st);
+/* ppr-base-M / ppr-purn-M (src/ppr/w-purn-perk/Main.cu:119-192, 379): GPU
+ * rounds continue while (round < budget) || (|frontier| > handoff_min); the
+ * host forward push (src/ppr/w-base-perk/PPR.cpp:178-236) then finishes from
+ * the handed-over (p, r, frontier).  handoff_min = 10000 in the reference. */
+int c2g_ppr_migrate(c2g_graph* g, int32_t src, float alpha, float eps, int32_t budget,
+                    int32_t handoff_min, float* p_out, float* r_out, c2g_stats* st);
 /* n sources in one launch sequence; p_out/r_out are n*V (query-major). */
 int c2g_ppr_batch(c2g_graph* g, const int32_t* srcs, int32_t n, float alpha,
                   float eps, float* p_out, float* r_out, c2g_stats* st);

@@ -1,0 +1,100 @@
+"""Command-line tools keep the reference's contracts (positional dataset_path,
+source, migration / iter; dataset directory layout)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import c2g
+import oracle
+from conftest import INT_MAX, ROOT
+
+BIN = os.path.join(ROOT, "c-2graph_amd", "bin")
+
+
+def run(*args, check=True):
+    p = subprocess.run([os.path.join(BIN, args[0])] + [str(a) for a in args[1:]], capture_output=True, text=True,
+                       timeout=300)
+    if check and p.returncode != 0:
+        raise AssertionError(f"{args}: rc={p.returncode}\n{p.stdout}\n{p.stderr}")
+    return p
+
+
+@pytest.fixture
+def ds(tmp_path):
+    d = str(tmp_path / "ds")
+    run("rmat-gen", d, 12, "--seed", 3, "--wmax", 64)
+    return d
+
+
+def test_rmat_gen_and_prune_tools(ds):
+    vl = c2g.read_i32(ds + "/origin/vlist.bin")
+    G = c2g.rmat(12, 16, seed=3, wmax=64, want_f=True)
+    assert np.array_equal(vl, G["vlist"])
+    assert np.array_equal(c2g.read_f32(ds + "/origin/flist.bin"), G["flist"])
+    out = run("purn-sssp", ds).stdout
+    assert "the ratio is" in out
+    run("purn-ppr", ds)
+    P = c2g.prune_sssp_mem(G["vlist"], G["elist"], G["wlist"])
+    assert np.array_equal(c2g.read_i32(ds + "/sssp/elist.bin"), P["elist"])
+    assert np.array_equal(c2g.read_i32(ds + "/sssp/hvlist.bin"), P["hvlist"])
+    assert np.array_equal(c2g.read_i32(ds + "/sssp/porder.bin"), P["porder"])
+
+
+def test_convert2binary_tool(tmp_path):
+    txt = tmp_path / "g.txt"
+    txt.write_text("0 1:2:2:0\n2 0\nbad\n5 3:1\n")
+    d = str(tmp_path / "c")
+    run("convert2binary", str(txt), d)
+    assert c2g.read_i32(d + "/origin/vlist.bin").tolist() == [0, 2, 2, 3, 3, 3, 5]
+    assert c2g.read_i32(d + "/origin/elist.bin").tolist() == [1, 2, 0, 1, 3]
+
+
+def test_usage_errors():
+    assert run("sssp-base", check=False).returncode == 2
+    assert run("purn-sssp", check=False).returncode == 2
+    p = run("sssp-base", "/nonexistent", 0, check=False)
+    assert p.returncode == 1 and "not exist" in p.stderr
+
+
+@pytest.mark.gpu
+def test_sssp_tools(gpu, ds, tmp_path):
+    G = c2g.rmat(12, 16, seed=3, wmax=64)
+    deg = np.diff(G["vlist"])
+    run("purn-sssp", ds)
+    for tool in ("sssp-base", "sssp-purn"):
+        out = str(tmp_path / f"{tool}.bin")
+        p = run(tool, ds, 0, "--reps", 2, "--out", out)
+        assert "gpu runtime:" in p.stdout and "源顶点source = " in p.stdout
+        s = int(p.stdout.split("源顶点source = ")[1].split()[0])
+        # skip rule of sssp/base/Main.cu:61-64, on the graph the tool loads
+        dg = deg if tool == "sssp-base" else np.diff(c2g.read_i32(ds + "/sssp/vlist.bin"))
+        assert dg[s] >= 1 and np.all(dg[:s] < 1)
+        d = c2g.read_i32(out)
+        assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], s))
+        first = [int(l.split("\t")[1]) for l in p.stdout.splitlines() if l.count("\t") == 1 and l.split("\t")[0].isdigit()]
+        assert first[:10] == d[:10].tolist()
+    out = str(tmp_path / "m.bin")
+    p = run("sssp-base-M", ds, 0, 2, "--reps", 1, "--out", out)
+    s = int(p.stdout.split("源顶点source = ")[1].split()[0])
+    indeg = np.bincount(G["elist"], minlength=G["V"])
+    assert deg[s] >= 10 and indeg[s] >= 10
+    assert np.array_equal(c2g.read_i32(out), oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], s))
+
+
+@pytest.mark.gpu
+def test_ppr_tools(gpu, ds, tmp_path):
+    G = c2g.rmat(12, 16, seed=3, wmax=64, want_f=True)
+    run("purn-ppr", ds)
+    out = str(tmp_path / "p.bin")
+    p = run("ppr-base", ds, 7, "--reps", 1, "--out", out)
+    assert p.stdout.count("pagerank") == 11
+    po, _, _, _ = oracle.ppr_push(G["vlist"], G["elist"], G["flist"], 7, alpha=0.15, eps=1e-7)
+    assert np.abs(c2g.read_f32(out).astype(np.float64) - po).sum() <= 2e-5
+    p = run("ppr-base-M", ds, 7, 2, "--reps", 1, "--handoff", 0, "--out", out)
+    assert np.abs(c2g.read_f32(out).astype(np.float64) - po).sum() <= 2e-5
+    run("ppr-purn", ds, 7, "--reps", 1)
+    run("ppr-purn-M", ds, 7, 2, "--reps", 1)
+    p = run("adsorption", ds, "--labels", 4, "--iters", 3)
+    assert "gpu runtime" in p.stdout

@@ -1,0 +1,93 @@
+"""Query server on the GPU: a mixed SSSP / PPR / adsorption stream with
+continuous batching.  Each result must equal the same query run alone:
+SSSP bit-exact (and equal to Dijkstra); PPR within L1 2e-5 of the standalone
+GPU push (FP32 atomics in arbitrary order); adsorption within max-abs 2e-6 of
+the host lane's FP32 Jacobi."""
+import time
+
+import numpy as np
+import pytest
+
+import c2g
+import oracle
+from conftest import rmat_graph
+
+pytestmark = pytest.mark.gpu
+
+
+def _G():
+    return rmat_graph(14, 9, wmax=255, want_f=True)
+
+
+def _graph(G):
+    return c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], G["flist"], device=0)
+
+
+def test_mixed_stream_matches_alone(gpu):
+    G = _G()
+    rng = np.random.default_rng(5)
+    deg = np.diff(G["vlist"])
+    kinds = rng.choice([c2g.Q_SSSP] * 6 + [c2g.Q_PPR] * 3 + [c2g.Q_ADS], 80)
+    srcs = [int(rng.choice(np.flatnonzero(deg > 0))) if k != c2g.Q_ADS else int(rng.integers(0, 4)) for k in kinds]
+    with _graph(G) as g:
+        with c2g.Server(g, ads_labels=16, ads_iters=20) as s:
+            ids = []
+            for k, v in zip(kinds, srcs):
+                ids.append(s.submit(k, v))
+                time.sleep(rng.exponential(0.002))  # Poisson arrivals while others run
+            res = [s.poll(i) for i in ids]
+        ads_alone = {}
+        with c2g.Server(g, ads_labels=16, ads_iters=20, device_failed=True, host_lane=True) as h:
+            for v in sorted(set(v for k, v in zip(kinds, srcs) if k == c2g.Q_ADS)):
+                ads_alone[v] = h.poll(h.submit(c2g.Q_ADS, v))[1]
+        for (info, r), k, v in zip(res, kinds, srcs):
+            assert info["status"] == 0 and info["lane"] == c2g.LANE_GPU
+            if k == c2g.Q_SSSP:
+                d, _ = g.sssp(v)
+                assert np.array_equal(r, d)
+                assert np.array_equal(r, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], v))
+            elif k == c2g.Q_PPR:
+                p, _, _ = g.ppr(v, alpha=0.15, eps=1e-7)
+                assert np.abs(r.astype(np.float64) - p).sum() <= 2e-5
+            else:
+                assert np.abs(r - ads_alone[v]).max() <= 2e-6
+    lat = [i["t_done_ms"] - i["t_submit_ms"] for i, _ in res]
+    assert min(lat) >= 0
+
+
+def test_memory_admission_and_fallback(gpu):
+    G = _G()
+    V = G["V"]
+    with _graph(G) as g:
+        # room for two SSSP results at a time: everything still completes (queued)
+        with c2g.Server(g, fake_free_hbm=2 * 4 * V + 16) as s:
+            ids = [s.submit(c2g.Q_SSSP, v) for v in range(0, 400, 40)]
+            for i, v in zip(ids, range(0, 400, 40)):
+                info, d = s.poll(i)
+                assert info["lane"] == c2g.LANE_GPU
+                assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], v))
+        # no room at all: refused without the host lane, host lane with it
+        with c2g.Server(g, fake_free_hbm=1024) as s:
+            with pytest.raises(c2g.C2GError) as e:
+                s.submit(c2g.Q_SSSP, 0)
+            assert e.value.code == c2g.EBUSY
+        with c2g.Server(g, fake_free_hbm=1024, host_lane=True) as s:
+            info, d = s.poll(s.submit(c2g.Q_SSSP, 7))
+            assert info["lane"] == c2g.LANE_HOST
+            assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], 7))
+        # device marked failed: the host lane takes everything
+        with c2g.Server(g, device_failed=True, host_lane=True) as s:
+            info, p = s.poll(s.submit(c2g.Q_PPR, 3))
+            assert info["lane"] == c2g.LANE_HOST
+
+
+def test_lane_reuse_many_queries(gpu):
+    # more queries than lanes: lanes are freed and reused, columns reset
+    G = _G()
+    deg = np.diff(G["vlist"])
+    srcs = np.flatnonzero(deg > 0)[:150]
+    with _graph(G) as g, c2g.Server(g, max_batch=16) as s:
+        ids = [s.submit(c2g.Q_SSSP, int(v)) for v in srcs]
+        for i, v in zip(ids, srcs):
+            info, d = s.poll(i)
+            assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], int(v))), v
