@@ -150,6 +150,7 @@ static void sssp_worker(c2g_server* s) {
                 lane_req[l] = -1;
             }
             s->cv_done.notify_all();
+            s->cv_work.notify_all();  // memory freed: the aux lane may admit
         }
         if (rc != C2G_OK) {
             std::lock_guard<std::mutex> lk(s->mu);
