@@ -164,6 +164,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--single-profile", action="store_true", help="profile two single-source queries")
     ap.add_argument("--skip-serve", action="store_true", help="skip the served mixed stream")
+    ap.add_argument("--copy-calib", action="store_true", help="only a 1 GiB device copy (counter calibration)")
     ap.add_argument("--serve-only", action="store_true", help="only the served mixed stream")
     ap.add_argument("--serve-qps", type=float, default=40.0)
     ap.add_argument("--serve-seconds", type=float, default=15.0)
@@ -183,6 +184,13 @@ def main():
         print("bench.py: no gfx950 device visible", file=sys.stderr)
         sys.exit(1)
 
+    if args.copy_calib:
+        import ctypes as C
+        gb = C.c_double()
+        c2g.check(c2g.lib().c2g_diag_copy(1 << 30, 5, C.byref(gb)))
+        print(json.dumps({"copy_bytes": 1 << 30, "reps": 5, "GBps": round(gb.value, 1)}), flush=True)
+        return
+
     if args.serve_only:
         print(json.dumps({"serving": serve_phase(args)}), flush=True)
         return
@@ -197,7 +205,7 @@ def main():
 
     if args.profile_only:
         args.warmup = args.steps = 0
-        args.skip_each = args.skip_cpu = True
+        args.skip_each = args.skip_cpu = args.skip_serve = True
         args.skip_profile = False
     for i in range(args.warmup):
         sums_b, reach_b, st = g.sssp_batch_resident(srcs)

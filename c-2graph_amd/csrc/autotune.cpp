@@ -1,0 +1,76 @@
+// Autotuner: picks launch shape, work-item chunk size and rounds per hipGraph
+// replay for the SSSP kernels by timing them on sample sources.
+//
+// The reference compiles in one shape (20 workgroups x 512 threads,
+// core/Util.cuh:31-35).  Here the candidates are timed on the caller's sample
+// and the fastest is stored in the handle.  Every setting computes the same
+// distances (the tunables change scheduling only), so tuning never changes
+// results.  Search: coordinate descent over grid_per_cu, chunk (single-source)
+// or batch_chunk (batched) and iters_per_graph, two passes, best of two runs
+// per candidate.
+#include "graph.h"
+#include <cstring>
+
+using namespace c2g;
+
+namespace {
+
+double time_once(Graph* g, const int32_t* srcs, int32_t n, bool batched, int* rc) {
+    const double t0 = now_ms();
+    if (batched) {
+        *rc = sssp_batch_group(g, srcs, n, nullptr, nullptr, nullptr, nullptr);
+    } else {
+        for (int32_t i = 0; i < n && *rc == C2G_OK; ++i)
+            *rc = sssp_run(g, srcs[i], 0, nullptr, nullptr, nullptr, nullptr, nullptr, false);
+    }
+    return now_ms() - t0;
+}
+
+double measure(Graph* g, const int32_t* srcs, int32_t n, bool batched, int* rc) {
+    double best = 1e30;
+    for (int r = 0; r < 2 && *rc == C2G_OK; ++r) best = std::min(best, time_once(g, srcs, n, batched, rc));
+    return best;
+}
+
+}  // namespace
+
+extern "C" int c2g_autotune(c2g_graph* h, const int32_t* srcs, int32_t n, int32_t batched, c2g_tune* best_out) {
+    if (!h || !srcs || n < 1) return fail(C2G_EINVAL, "c2g_autotune: need at least one sample source");
+    std::lock_guard<std::mutex> lk(h->g.mu);
+    Graph* g = &h->g;
+    if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");
+    for (int32_t i = 0; i < n; ++i)
+        if (srcs[i] < 0 || srcs[i] >= g->V) return fail(C2G_EINVAL, "source %d out of range", srcs[i]);
+    const bool b = batched != 0;
+    const int32_t m = b ? std::min<int32_t>(n, 64 * 4) : std::min<int32_t>(n, 16);
+    c2g_tune best = g->tune;
+    best.profile = 0;
+    g->tune = best;
+    int rc = C2G_OK;
+    (void)time_once(g, srcs, m, b, &rc);  // warm allocations and code objects
+    double tbest = measure(g, srcs, m, b, &rc);
+    const std::vector<int> grids = {2, 4, 8, 16};
+    const std::vector<int> chunks = b ? std::vector<int>{16, 32, 64, 128} : std::vector<int>{16, 32, 64, 128, 256};
+    const std::vector<int> iters = {4, 8, 16};
+    for (int pass = 0; pass < 2 && rc == C2G_OK; ++pass) {
+        for (int which = 0; which < 3 && rc == C2G_OK; ++which) {
+            const std::vector<int>& vals = which == 0 ? grids : which == 1 ? chunks : iters;
+            for (int v : vals) {
+                c2g_tune t = best;
+                if (which == 0) t.grid_per_cu = v;
+                else if (which == 1) (b ? t.batch_chunk : t.chunk) = v;
+                else t.iters_per_graph = v;
+                if (!memcmp(&t, &best, sizeof t)) continue;
+                g->tune = t;
+                const double tt = measure(g, srcs, m, b, &rc);
+                if (rc) break;
+                if (tt < tbest * 0.98) { tbest = tt; best = t; }
+            }
+            g->tune = best;
+        }
+    }
+    g->tune = best;
+    if (rc) return rc;
+    if (best_out) *best_out = best;
+    return C2G_OK;
+}

@@ -207,7 +207,42 @@ static int finish_open(c2g_graph* h, const c2g_opts* o, c2g_graph** out) {
 
 using namespace c2g;
 
+__global__ void diag_copy_k(const int4* __restrict__ a, int4* __restrict__ b, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
 extern "C" {
+
+int c2g_diag_copy(int64_t bytes, int32_t reps, double* gbps) {
+    if (bytes < 16 || reps < 1) return fail(C2G_EINVAL, "bytes >= 16 and reps >= 1");
+    int4 *a = nullptr, *b = nullptr;
+    const int64_t n = bytes / 16;
+    HIP_TRY(hipMalloc(&a, 16 * n));
+    HIP_TRY(hipMalloc(&b, 16 * n));
+    HIP_TRY(hipMemset(a, 1, 16 * n));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    hipDeviceProp_t p;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipGetDeviceProperties(&p, dev));
+    const int grid = p.multiProcessorCount * 8;
+    hipLaunchKernelGGL(diag_copy_k, dim3(grid), dim3(256), 0, 0, a, b, n);  // warm
+    HIP_TRY(hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(diag_copy_k, dim3(grid), dim3(256), 0, 0, a, b, n);
+    HIP_TRY(hipEventRecord(e1, 0));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    if (gbps) *gbps = 2.0 * 16 * n * reps / (ms * 1e-3) / 1e9;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return C2G_OK;
+}
 
 int c2g_device_available(void) {
     int n = 0;

@@ -306,6 +306,11 @@ int c2g_adsorption(c2g_graph* g, const int32_t* seed_v, const int32_t* seed_labe
 int c2g_autotune(c2g_graph* g, const int32_t* srcs, int32_t n, int32_t batched,
                  c2g_tune* best_out);
 
+/* Diagnostic (no reference counterpart): a plain device copy of `bytes`
+ * bytes repeated `reps` times, used to calibrate HBM byte counters; returns
+ * the copy rate in GB/s (read + write bytes over time). */
+int c2g_diag_copy(int64_t bytes, int32_t reps, double* gbps);
+
 /* ======================================================================= */
 /* Serving loop (continuous batching)                                        */
 /* ======================================================================= */
