@@ -15,6 +15,15 @@ using namespace c2g;
 
 namespace {
 
+// Captured launch sequences hold tunables as kernel arguments.
+void set_tune_locked(Graph* g, const c2g_tune& t) {
+    if (memcmp(&g->tune, &t, sizeof t) != 0) {
+        if (g->sssp.exec) { (void)hipGraphExecDestroy(g->sssp.exec); g->sssp.exec = nullptr; }
+        if (g->batch.exec) { (void)hipGraphExecDestroy(g->batch.exec); g->batch.exec = nullptr; }
+    }
+    g->tune = t;
+}
+
 double time_once(Graph* g, const int32_t* srcs, int32_t n, bool batched, int* rc) {
     const double t0 = now_ms();
     if (batched) {
@@ -45,7 +54,7 @@ extern "C" int c2g_autotune(c2g_graph* h, const int32_t* srcs, int32_t n, int32_
     const int32_t m = b ? std::min<int32_t>(n, 64 * 4) : std::min<int32_t>(n, 16);
     c2g_tune best = g->tune;
     best.profile = 0;
-    g->tune = best;
+    set_tune_locked(g, best);
     int rc = C2G_OK;
     (void)time_once(g, srcs, m, b, &rc);  // warm allocations and code objects
     double tbest = measure(g, srcs, m, b, &rc);
@@ -61,15 +70,15 @@ extern "C" int c2g_autotune(c2g_graph* h, const int32_t* srcs, int32_t n, int32_
                 else if (which == 1) (b ? t.batch_chunk : t.chunk) = v;
                 else t.iters_per_graph = v;
                 if (!memcmp(&t, &best, sizeof t)) continue;
-                g->tune = t;
+                set_tune_locked(g, t);
                 const double tt = measure(g, srcs, m, b, &rc);
                 if (rc) break;
                 if (tt < tbest * 0.98) { tbest = tt; best = t; }
             }
-            g->tune = best;
+            set_tune_locked(g, best);
         }
     }
-    g->tune = best;
+    set_tune_locked(g, best);
     if (rc) return rc;
     if (best_out) *best_out = best;
     return C2G_OK;

@@ -327,6 +327,11 @@ int c2g_set_tune(c2g_graph* h, const c2g_tune* t) {
         return fail(C2G_EINVAL, "iters_per_graph must be even, in [2,256]");
     if (t->delta < 0) return fail(C2G_EINVAL, "delta >= 0");
     std::lock_guard<std::mutex> lk(h->g.mu);
+    if (memcmp(&h->g.tune, t, sizeof *t) != 0) {
+        // Captured launch sequences hold the old values as kernel arguments.
+        if (h->g.sssp.exec) { (void)hipGraphExecDestroy(h->g.sssp.exec); h->g.sssp.exec = nullptr; }
+        if (h->g.batch.exec) { (void)hipGraphExecDestroy(h->g.batch.exec); h->g.batch.exec = nullptr; }
+    }
     h->g.tune = *t;
     return C2G_OK;
 }

@@ -49,6 +49,10 @@ enum : int {
     C_HEDGES = 8, // hidden edges queued
     C_DONE_E = 9, // workgroups finished (expand)
     C_ITEMS = 10, // items processed
+    C_THR = 11,   // delta-stepping: current bucket bound (INT32_MAX = plain rounds)
+    C_FMIN = 12,  // smallest deferred distance this round
+    C_NEAR = 13,  // vertices expanded this round
+    C_FAR = 14,   // 1 while deferred vertices remain
     C_N = 16
 };
 
@@ -75,7 +79,7 @@ __global__ void sssp_init_k(int32_t* __restrict__ dist, uint32_t* __restrict__ b
 __global__ void sssp_seed_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
                             int32_t* __restrict__ dist, uint8_t* __restrict__ hid,
                             int4* __restrict__ q, int64_t* __restrict__ ctrl, int32_t src,
-                            int chunk) {
+                            int chunk, int delta) {
     const int r0 = row[src], r1 = row[src + 1];
     const int nl = (r1 - r0 + chunk - 1) / chunk;
     int h0 = 0, h1 = 0, nh = 0;
@@ -93,6 +97,8 @@ __global__ void sssp_seed_k(const int32_t* __restrict__ row, const int32_t* __re
         dist[src] = 0;
         if (hid) hid[src] = 1;  // the source's hidden list is always relaxed
         for (int i = 0; i < C_N; ++i) ctrl[i] = 0;
+        ctrl[C_THR] = delta > 0 ? delta : INT32_MAX;
+        ctrl[C_FMIN] = INT32_MAX;
         ctrl[C_CUR] = nl + nh;
         ctrl[C_FRONT] = 1;
         ctrl[C_EDGES] = r1 - r0;
@@ -183,29 +189,46 @@ sssp_relax_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
 // Compacts the activation bitmaps into the next round's items: each lane owns
 // one 32-vertex word, counts the items of its set bits (ceil(deg/chunk) per
 // live list, the same for hidden lists), the wave scans the counts and takes
-// its queue range with a single atomicAdd, and the words are cleared.
+// its queue range with a single atomicAdd.  Delta-stepping (delta > 0): only
+// vertices with dist < ctrl[C_THR] are expanded; the others keep their bit
+// for a later round, and when a round expands nothing the last workgroup
+// moves the bound to the bucket of the smallest deferred distance.
 template <bool PRUNED>
 __global__ void __launch_bounds__(256)
 sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
               uint32_t* __restrict__ bits, uint32_t* __restrict__ hbits, int4* __restrict__ q,
-              int64_t* __restrict__ ctrl, int64_t V, int chunk, int64_t qcap) {
+              int64_t* __restrict__ ctrl, int64_t V, int chunk, int64_t qcap,
+              const int32_t* __restrict__ dist, int delta) {
     const int lane = lane_id();
     const int64_t nw = (V + 31) >> 5;
     const int64_t nwr = (nw + 63) & ~(int64_t)63;  // uniform trip count per wave
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    int64_t front = 0, edges = 0, hedges = 0;
+    const int32_t T = (int32_t)ctrl[C_THR];
+    int64_t front = 0, edges = 0, hedges = 0, near_n = 0;
+    int32_t fmin = INT32_MAX;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nwr; i += stride) {
         uint32_t wl = 0, wh = 0;
         if (i < nw) {
             wl = bits[i];
-            if (wl) bits[i] = 0;
-            if (PRUNED) {
-                wh = hbits[i];
-                if (wh) hbits[i] = 0;
+            if (PRUNED) wh = hbits[i];
+        }
+        // Split by distance: near bits are expanded now, far bits stay.
+        uint32_t nl = wl, nh = wh;
+        if (delta > 0 && (wl | wh)) {
+            nl = nh = 0;
+            for (uint32_t m = wl | wh; m; m &= m - 1) {
+                const int b = __builtin_ctz(m);
+                const int32_t d = dist[i * 32 + b];
+                if (d < T) { nl |= wl & (1u << b); nh |= wh & (1u << b); }
+                else fmin = min(fmin, d);
             }
         }
+        if (i < nw) {
+            if (wl != (wl & ~nl)) bits[i] = wl & ~nl;
+            if (PRUNED && wh != (wh & ~nh)) hbits[i] = wh & ~nh;
+        }
         int cnt = 0;
-        for (uint32_t m = wl; m; m &= m - 1) {
+        for (uint32_t m = nl; m; m &= m - 1) {
             const int v = (int)(i * 32 + __builtin_ctz(m));
             const int d = row[v + 1] - row[v];
             cnt += (d + chunk - 1) / chunk;
@@ -213,12 +236,13 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
             edges += d;
         }
         if (PRUNED)
-            for (uint32_t m = wh; m; m &= m - 1) {
+            for (uint32_t m = nh; m; m &= m - 1) {
                 const int v = (int)(i * 32 + __builtin_ctz(m));
                 const int d = hrow[v + 1] - hrow[v];
                 cnt += (d + chunk - 1) / chunk;
                 hedges += d;
             }
+        near_n += __popc(nl | nh);
         int tot;
         const int off = wave_excl_scan(cnt, &tot);
         if (tot == 0) continue;
@@ -230,13 +254,13 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
             continue;
         }
         int64_t p = base + off;
-        for (uint32_t m = wl; m; m &= m - 1) {
+        for (uint32_t m = nl; m; m &= m - 1) {
             const int v = (int)(i * 32 + __builtin_ctz(m));
             const int r0 = row[v], r1 = row[v + 1];
             for (int b = r0; b < r1; b += chunk) q[p++] = make_int4(v, b, min(b + chunk, r1), 0);
         }
         if (PRUNED)
-            for (uint32_t m = wh; m; m &= m - 1) {
+            for (uint32_t m = nh; m; m &= m - 1) {
                 const int v = (int)(i * 32 + __builtin_ctz(m));
                 const int h0 = hrow[v], h1 = hrow[v + 1];
                 for (int b = h0; b < h1; b += chunk) q[p++] = make_int4(v, b, min(b + chunk, h1), 1);
@@ -245,10 +269,17 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
     front = wave_sum64(front);
     edges = wave_sum64(edges);
     if (PRUNED) hedges = wave_sum64(hedges);
-    if (lane == 0 && (front | hedges)) {
-        atomicAdd((unsigned long long*)&ctrl[C_FRONT], (unsigned long long)front);
-        atomicAdd((unsigned long long*)&ctrl[C_EDGES], (unsigned long long)edges);
-        if (PRUNED) atomicAdd((unsigned long long*)&ctrl[C_HEDGES], (unsigned long long)hedges);
+    near_n = wave_sum64(near_n);
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) fmin = min(fmin, __shfl_xor(fmin, d, 64));
+    if (lane == 0) {
+        if (front | hedges) {
+            atomicAdd((unsigned long long*)&ctrl[C_FRONT], (unsigned long long)front);
+            atomicAdd((unsigned long long*)&ctrl[C_EDGES], (unsigned long long)edges);
+            if (PRUNED) atomicAdd((unsigned long long*)&ctrl[C_HEDGES], (unsigned long long)hedges);
+        }
+        if (near_n) atomicAdd((unsigned long long*)&ctrl[C_NEAR], (unsigned long long)near_n);
+        if (fmin != INT32_MAX) atomicMin((long long*)&ctrl[C_FMIN], (long long)fmin);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -257,6 +288,13 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
         if (prev == (unsigned long long)gridDim.x - 1) {
             __threadfence();
             ctrl[C_DONE_E] = 0;
+            const int64_t fm = (int64_t)atomicAdd((unsigned long long*)&ctrl[C_FMIN], 0ull);
+            const int64_t nearc = (int64_t)atomicAdd((unsigned long long*)&ctrl[C_NEAR], 0ull);
+            ctrl[C_FAR] = fm != INT32_MAX ? 1 : 0;
+            if (delta > 0 && nearc == 0 && fm != INT32_MAX)
+                ctrl[C_THR] = min((int64_t)INT32_MAX, (fm / delta + 1) * delta);
+            ctrl[C_FMIN] = INT32_MAX;
+            ctrl[C_NEAR] = 0;
             int64_t nn = (int64_t)atomicAdd((unsigned long long*)&ctrl[C_CUR], 0ull);
             if (nn > qcap) {  // items past the queue were dropped: stop, report
                 ctrl[C_OVF] = 1;
@@ -340,7 +378,8 @@ static int launch_rounds(Graph* g, int k, double* relax_ms = nullptr) {
                                g->d_hadj, w.dist, bits, hbits, w.hid, w.q[0], w.ctrl);
             if (relax_ms) HIP_TRY(hipEventRecord(w.ev[1], g->stream));
             hipLaunchKernelGGL(sssp_expand_k<true>, dim3(egrid), dim3(256), 0, g->stream, g->d_row,
-                               g->d_hrow, bits, hbits, w.q[0], w.ctrl, (int64_t)g->V, g->tune.chunk, w.qcap);
+                               g->d_hrow, bits, hbits, w.q[0], w.ctrl, (int64_t)g->V, g->tune.chunk, w.qcap,
+                               (const int32_t*)w.dist, g->tune.delta);
         } else {
             hipLaunchKernelGGL(sssp_relax_k<false>, dim3(grid), dim3(256), 0, g->stream, g->d_adj,
                                (const int2*)nullptr, w.dist, bits, (uint32_t*)nullptr, (uint8_t*)nullptr,
@@ -348,7 +387,7 @@ static int launch_rounds(Graph* g, int k, double* relax_ms = nullptr) {
             if (relax_ms) HIP_TRY(hipEventRecord(w.ev[1], g->stream));
             hipLaunchKernelGGL(sssp_expand_k<false>, dim3(egrid), dim3(256), 0, g->stream, g->d_row,
                                (const int32_t*)nullptr, bits, (uint32_t*)nullptr, w.q[0], w.ctrl,
-                               (int64_t)g->V, g->tune.chunk, w.qcap);
+                               (int64_t)g->V, g->tune.chunk, w.qcap, (const int32_t*)w.dist, g->tune.delta);
         }
         if (relax_ms) {
             HIP_TRY(hipEventSynchronize(w.ev[1]));
@@ -421,7 +460,7 @@ int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* 
                        (int64_t)g->V);
     hipLaunchKernelGGL(sssp_seed_k, dim3(1), dim3(256), 0, g->stream, g->d_row,
                        pruned ? g->d_hrow : (const int32_t*)nullptr, w.dist,
-                       pruned ? w.hid : (uint8_t*)nullptr, w.q[0], w.ctrl, src, g->tune.chunk);
+                       pruned ? w.hid : (uint8_t*)nullptr, w.q[0], w.ctrl, src, g->tune.chunk, g->tune.delta);
     HIP_TRY(hipGetLastError());
     const bool prof = g->tune.profile != 0;
     const bool use_graph = g->tune.use_graph && budget <= 0 && !prof;
@@ -439,7 +478,7 @@ int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* 
                                g->stream));
         HIP_TRY(hipStreamSynchronize(g->stream));
         if (w.h_ctrl[C_OVF]) return fail(C2G_EDEVICE, "frontier queue overflow (internal)");
-        if (w.h_ctrl[C_CUR] == 0) break;
+        if (w.h_ctrl[C_CUR] == 0 && w.h_ctrl[C_FAR] == 0) break;
     }
     const double t1 = now_ms();
     int64_t c[C_N];
@@ -455,6 +494,16 @@ int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* 
         std::vector<uint8_t> seen((size_t)g->V, 0);
         for (auto& it : items)
             if (!seen[it.x]) { seen[it.x] = 1; if (frontier_out) frontier_out[m] = it.x; ++m; }
+        // vertices deferred by delta-stepping are still pending
+        const int64_t nwd = (g->V + 31) / 32;
+        std::vector<uint32_t> bw((size_t)nwd * (pruned ? 2 : 1));
+        HIP_TRY(hipMemcpy(bw.data(), bits, sizeof(uint32_t) * nwd, hipMemcpyDeviceToHost));
+        if (pruned) HIP_TRY(hipMemcpy(bw.data() + nwd, bits + nw, sizeof(uint32_t) * nwd, hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < (int64_t)bw.size(); ++k)
+            for (uint32_t x = bw[k]; x; x &= x - 1) {
+                const int32_t v = (int32_t)((k % nwd) * 32 + __builtin_ctz(x));
+                if (!seen[v]) { seen[v] = 1; if (frontier_out) frontier_out[m] = v; ++m; }
+            }
         *count_out = m;
         if (hid_out && pruned)
             HIP_TRY(hipMemcpy(hid_out, w.hid, g->V, hipMemcpyDeviceToHost));

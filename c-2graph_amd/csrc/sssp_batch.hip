@@ -44,7 +44,8 @@ enum : int {
     B_OVF = 8,     // capacity guard tripped (must stay 0)
     B_LEDGES = 9,  // (query, edge) relaxation attempts
     B_ITEMS_TOT = 10,  // items processed
-    B_LANEOR = 11,     // server lane: OR of the lane masks queued for the next round
+    B_LANEOR = 11,     // server lane: lanes of group 0 with pending work after expand
+    B_FAR = 12,        // 1 while some lane has deferred (far) work
     B_N = 16
 };
 
@@ -162,78 +163,112 @@ relax_b_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
             ctrl[B_ITEMS_TOT] += n;
             ctrl[B_ITEMS] = 0;
             ctrl[B_DONE_R] = 0;
-            ctrl[B_LANEOR] = 0;
         }
     }
 }
 
+// Expand one round.  For each active (v,g): take the pending lanes m of v.
+// With delta > 0 the lanes are split by v's current distance against each
+// lane's bucket bound thr[g][q] (delta-stepping): near lanes (dist < thr) are
+// expanded into items, far lanes stay pending in mask/abits for a later round.
+// Per-lane bookkeeping of the round goes to lor[2g] (lanes with near work),
+// lor[2g+1] (lanes with far work) and fmin[g][q] (smallest far distance); the
+// last workgroup moves every lane that has far work but no near work to the
+// bucket of its smallest far distance.  With delta == 0 every pending lane is
+// near (plain Bellman-Ford rounds).
 __global__ void __launch_bounds__(256)
 expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, uint64_t* __restrict__ mask,
            const uint64_t* __restrict__ hmask, uint32_t* __restrict__ abits, int4* __restrict__ items,
            uint64_t* __restrict__ imask, int64_t* __restrict__ ctrl, int64_t V, int64_t nw, int groups,
-           int chunk, int64_t itemcap, int lane_or) {
+           int chunk, int64_t itemcap, const int32_t* __restrict__ dist, int32_t* __restrict__ thr,
+           int32_t* __restrict__ fmin, uint64_t* __restrict__ lor, int delta) {
+    extern __shared__ int32_t lds_fmin[];  // [groups * 64] when delta > 0
     const int lane = lane_id();
+    if (delta > 0) {
+        for (int t = threadIdx.x; t < groups * 64; t += blockDim.x) lds_fmin[t] = INT32_MAX;
+        __syncthreads();
+    }
     int64_t front = 0, edges = 0, ledges = 0;
-    uint64_t lor = 0;  // lanes with queued work (server lane, one group)
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t n = nw * groups;
-    const int64_t nn = (n + 63) & ~(int64_t)63;  // uniform trip count per wave
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nn; i += stride) {
-        uint32_t word = 0;
-        int g = 0;
-        int64_t vb = 0;
-        if (i < n) {
-            word = abits[i];
-            if (word) abits[i] = 0;
-            g = (int)(i / nw);
-            vb = (i - (int64_t)g * nw) * 32;
-        }
+    const int64_t n = nw * groups;  // nw is a multiple of 64: a wave never straddles groups
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t word = abits[i];
+        const int g = (int)(i / nw);
+        const int64_t vb = (i - (int64_t)g * nw) * 32;
         const uint64_t* mg = mask + (int64_t)g * V;
+        const int32_t* dG = dist + (int64_t)g * V * 64;
+        const int32_t* tg = thr + g * 64;
+        uint64_t nor = 0, forr = 0;
+        uint32_t keep = 0;
+        // Near lanes of v (and far lanes, recorded once in pass 1).
+        auto split = [&](int v, uint64_t m, bool record) -> uint64_t {
+            if (delta <= 0) return m;
+            uint64_t near = 0;
+            for (int c = 0; c < 16; ++c) {
+                const uint64_t mc = (m >> (4 * c)) & 0xFull;
+                if (!mc) continue;
+                const int4 d4 = *reinterpret_cast<const int4*>(dG + (int64_t)v * 64 + 4 * c);
+                const int dv[4] = {d4.x, d4.y, d4.z, d4.w};
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (!((mc >> k) & 1ull)) continue;
+                    const int q = 4 * c + k;
+                    if (dv[k] < tg[q]) near |= 1ull << q;
+                    else if (record) atomicMin(&lds_fmin[g * 64 + q], dv[k]);
+                }
+            }
+            return near;
+        };
         int cnt = 0;
         for (uint32_t b = word; b; b &= b - 1) {
             const int v = (int)(vb + __builtin_ctz(b));
             const uint64_t m = mg[v];
+            const uint64_t near = split(v, m, true);
+            const uint64_t far = m & ~near;
+            forr |= far;
+            if (far) keep |= b & (0u - b);
+            if (!near) continue;
+            nor |= near;
             const int r0 = row[v], r1 = row[v + 1];
             cnt += (r1 - r0 + chunk - 1) / chunk;
-            if (r1 > r0) lor |= m;
             uint64_t hm = 0;
             if (hrow) {
-                hm = m & hmask[(int64_t)g * V + v];
+                hm = near & hmask[(int64_t)g * V + v];
                 if (hm) {
                     const int h0 = hrow[v], h1 = hrow[v + 1];
                     cnt += (h1 - h0 + chunk - 1) / chunk;
-                    if (h1 > h0) lor |= hm;
                     edges += h1 - h0;
                     ledges += (int64_t)__popcll(hm) * (h1 - h0);
                 }
             }
             front += 1;
             edges += r1 - r0;
-            ledges += (int64_t)__popcll(m) * (r1 - r0);
+            ledges += (int64_t)__popcll(near) * (r1 - r0);
         }
         int tot;
         const int off = wave_excl_scan(cnt, &tot);
-        if (tot == 0) continue;
         int64_t base = 0;
-        if (lane == 0) base = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ITEMS], (unsigned long long)tot);
-        base = shfl64(base, 0);
-        if (base + tot > itemcap) {
-            if (lane == 0) atomicOr((unsigned long long*)&ctrl[B_OVF], 1ull);
-            continue;
+        if (tot > 0) {
+            if (lane == 0) base = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ITEMS], (unsigned long long)tot);
+            base = shfl64(base, 0);
         }
+        const bool room = base + tot <= itemcap;
+        if (!room && lane == 0) atomicOr((unsigned long long*)&ctrl[B_OVF], 1ull);
         int64_t p = base + off;
         uint64_t* mw = mask + (int64_t)g * V;
         for (uint32_t b = word; b; b &= b - 1) {
             const int v = (int)(vb + __builtin_ctz(b));
             const uint64_t m = mw[v];
-            mw[v] = 0;
+            const uint64_t near = split(v, m, false);
+            mw[v] = m & ~near;
+            if (!near || !room) continue;
             const int r0 = row[v], r1 = row[v + 1];
             for (int e = r0; e < r1; e += chunk, ++p) {
                 items[p] = make_int4(v, e, min(e + chunk, r1), g);
-                imask[p] = m;
+                imask[p] = near;
             }
             if (hrow) {
-                const uint64_t hm = m & hmask[(int64_t)g * V + v];
+                const uint64_t hm = near & hmask[(int64_t)g * V + v];
                 if (hm) {
                     const int h0 = hrow[v], h1 = hrow[v + 1];
                     for (int e = h0; e < h1; e += chunk, ++p) {
@@ -242,6 +277,17 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
                     }
                 }
             }
+        }
+        if (word != keep) abits[i] = keep;
+        // Lane bookkeeping of this group (the wave's 64 words share g).
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) {
+            nor |= (uint64_t)shfl64((int64_t)nor, lane ^ d);
+            forr |= (uint64_t)shfl64((int64_t)forr, lane ^ d);
+        }
+        if (lane == 0) {
+            if (nor) atomicOr((unsigned long long*)&lor[2 * g], (unsigned long long)nor);
+            if (forr) atomicOr((unsigned long long*)&lor[2 * g + 1], (unsigned long long)forr);
         }
     }
     front = wave_sum64(front);
@@ -252,22 +298,55 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
         atomicAdd((unsigned long long*)&ctrl[B_EDGES], (unsigned long long)edges);
         atomicAdd((unsigned long long*)&ctrl[B_LEDGES], (unsigned long long)ledges);
     }
-    if (lane_or) {
-#pragma unroll
-        for (int d = 32; d > 0; d >>= 1) lor |= (uint64_t)shfl64((int64_t)lor, lane ^ d);
-        if (lane == 0 && lor) atomicOr((unsigned long long*)&ctrl[B_LANEOR], (unsigned long long)lor);
+    if (delta > 0) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < groups * 64; t += blockDim.x)
+            if (lds_fmin[t] != INT32_MAX) atomicMin(&fmin[t], lds_fmin[t]);
     }
+    // The last workgroup closes the round: bucket moves and lane summaries.
+    __shared__ int is_last;
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
         unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[B_DONE_E], 1ull);
-        if (prev == (unsigned long long)gridDim.x - 1) {
-            __threadfence();
-            ctrl[B_DONE_E] = 0;
-            int64_t ni = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ITEMS], 0ull);
-            if (ni > itemcap) { ctrl[B_OVF] = 1; ctrl[B_ITEMS] = 0; }
-        }
+        is_last = prev == (unsigned long long)gridDim.x - 1;
     }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    for (int t = threadIdx.x; t < groups * 64; t += blockDim.x) {
+        const int gg = t >> 6, q = t & 63;
+        const uint64_t nb = (uint64_t)atomicOr((unsigned long long*)&lor[2 * gg], 0ull);
+        const uint64_t fb = (uint64_t)atomicOr((unsigned long long*)&lor[2 * gg + 1], 0ull);
+        const int32_t fm = atomicMin(&fmin[t], INT32_MAX);
+        if (delta > 0 && !((nb >> q) & 1ull) && ((fb >> q) & 1ull))
+            thr[t] = (int32_t)min((int64_t)INT32_MAX, ((int64_t)fm / delta + 1) * delta);
+        fmin[t] = INT32_MAX;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t any_far = 0;
+        for (int gg = 0; gg < groups; ++gg)
+            any_far |= (uint64_t)atomicOr((unsigned long long*)&lor[2 * gg + 1], 0ull);
+        ctrl[B_FAR] = any_far ? 1 : 0;
+        // group 0: lanes with any pending work (the server lane)
+        ctrl[B_LANEOR] = (int64_t)((uint64_t)atomicOr((unsigned long long*)&lor[0], 0ull) |
+                                   (uint64_t)atomicOr((unsigned long long*)&lor[1], 0ull));
+        for (int gg = 0; gg < groups; ++gg) lor[2 * gg] = lor[2 * gg + 1] = 0;
+        ctrl[B_DONE_E] = 0;
+        int64_t ni = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ITEMS], 0ull);
+        if (ni > itemcap) { ctrl[B_OVF] = 1; ctrl[B_ITEMS] = 0; }
+    }
+}
+
+__global__ void batch_thr_init_k(int32_t* __restrict__ thr, int32_t* __restrict__ fmin,
+                                 uint64_t* __restrict__ lor, int n, int delta) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        thr[t] = delta > 0 ? delta : INT32_MAX;
+        fmin[t] = INT32_MAX;
+    }
+    if (t < 2 * (n / 64)) lor[t] = 0;
 }
 
 // Per-lane checksums of one group: sum of finite distances, reached count,
@@ -297,7 +376,10 @@ __global__ void batch_checksum_k(const int32_t* __restrict__ row, const int32_t*
 struct BatchState {
     int groups = 0;
     int64_t nw = 0, itemcap = 0;
-    uint32_t* abits = nullptr;  // [groups][nw] activation bitmaps
+    uint32_t* abits = nullptr;  // [groups][nw] activation bitmaps (nw padded to 64 words)
+    int32_t* thr = nullptr;     // [groups][64] bucket bound per lane (delta-stepping)
+    int32_t* fmin = nullptr;    // [groups][64] smallest deferred distance this round
+    uint64_t* lor = nullptr;    // [groups][2] lanes with near / far work this round
     int4* items = nullptr;
     uint64_t* imask = nullptr;
     int32_t* srcs = nullptr;
@@ -325,6 +407,7 @@ void batch_release(Graph* g) {
         if (g_bs[i].first == g) {
             BatchState* s = g_bs[i].second;
             (void)hipFree(s->abits); (void)hipFree(s->items);
+            (void)hipFree(s->thr); (void)hipFree(s->fmin); (void)hipFree(s->lor);
             (void)hipFree(s->imask); (void)hipFree(s->srcs); (void)hipFree(s->sums);
             (void)hipFree(s->reach); (void)hipFree(s->medges);
             for (auto& e : s->ev) if (e) (void)hipEventDestroy(e);
@@ -356,8 +439,12 @@ static int ensure_batch(Graph* g, int groups) {
         (void)hipFree(w.dist); (void)hipFree(w.mask[0]); (void)hipFree(w.hmask);
         (void)hipFree(s->abits); (void)hipFree(s->items); (void)hipFree(s->imask); (void)hipFree(s->srcs);
         (void)hipFree(s->sums); (void)hipFree(s->reach); (void)hipFree(s->medges);
+        (void)hipFree(s->thr); (void)hipFree(s->fmin); (void)hipFree(s->lor);
         w.dist = nullptr; w.mask[0] = nullptr; w.hmask = nullptr;
-        s->nw = (V + 31) / 32 + 1;
+        s->nw = (((V + 31) / 32 + 1) + 63) & ~(int64_t)63;  // whole waves per group
+        HIP_TRY(hipMalloc(&s->thr, sizeof(int32_t) * 64 * G));
+        HIP_TRY(hipMalloc(&s->fmin, sizeof(int32_t) * 64 * G));
+        HIP_TRY(hipMalloc(&s->lor, sizeof(uint64_t) * 2 * G));
         HIP_TRY(hipMalloc(&w.dist, sizeof(int32_t) * (size_t)G * V * 64 + 16));
         HIP_TRY(hipMalloc(&w.mask[0], sizeof(uint64_t) * (size_t)G * V + 8));
         if (g->d_hrow) HIP_TRY(hipMalloc(&w.hmask, sizeof(uint64_t) * (size_t)G * V + 8));
@@ -379,23 +466,25 @@ static int ensure_batch(Graph* g, int groups) {
 
 static int relax_grid_b(const Graph* g) { return std::max(1, g->num_cus) * g->tune.grid_per_cu; }
 
-static int expand_b(Graph* g, int groups, hipStream_t strm, int lane_or = 0) {
+static int expand_b(Graph* g, int groups, hipStream_t strm) {
     BatchWs& w = g->batch;
     BatchState* s = bstate(g);
     const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
     const int64_t words = s->nw * groups;
     const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(relax_grid_b(g), (words + 255) / 256));
-    hipLaunchKernelGGL(expand_b_k, dim3(egrid), dim3(256), 0, strm, g->d_row,
+    const int delta = g->tune.delta;
+    const size_t lds = delta > 0 ? sizeof(int32_t) * 64 * groups : 0;
+    hipLaunchKernelGGL(expand_b_k, dim3(egrid), dim3(256), lds, strm, g->d_row,
                        pruned ? g->d_hrow : (const int32_t*)nullptr, w.mask[0],
                        pruned ? w.hmask : (const uint64_t*)nullptr, s->abits, s->items, s->imask, w.ctrl,
-                       (int64_t)g->V, s->nw, groups, g->tune.batch_chunk, s->itemcap, lane_or);
+                       (int64_t)g->V, s->nw, groups, g->tune.batch_chunk, s->itemcap, w.dist, s->thr, s->fmin,
+                       s->lor, delta);
     return C2G_OK;
 }
 
 // One round = relax_b + expand_b.  When relax_ms is non-null, events bracket
 // the relax kernel and its duration is added to *relax_ms (profiling runs).
-static int launch_round_b(Graph* g, int groups, double* relax_ms, hipStream_t strm = nullptr,
-                          int lane_or = 0) {
+static int launch_round_b(Graph* g, int groups, double* relax_ms, hipStream_t strm = nullptr) {
     if (!strm) strm = g->stream;
     BatchWs& w = g->batch;
     BatchState* s = bstate(g);
@@ -417,7 +506,7 @@ static int launch_round_b(Graph* g, int groups, double* relax_ms, hipStream_t st
         HIP_TRY(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
         *relax_ms += ms;
     }
-    expand_b(g, groups, strm, lane_or);
+    expand_b(g, groups, strm);
     HIP_TRY(hipGetLastError());
     return C2G_OK;
 }
@@ -458,6 +547,8 @@ static int batch_run(Graph* g, const int32_t* srcs, int32_t n, c2g_stats* st, do
     hipLaunchKernelGGL(batch_fill_k, dim3(fill_grid), dim3(256), 0, g->stream, w.dist,
                        (int64_t)groups * V * 64, w.mask[0], pruned ? w.hmask : (uint64_t*)nullptr,
                        (int64_t)groups * V, s->abits, s->nw * groups);
+    hipLaunchKernelGGL(batch_thr_init_k, dim3((64 * groups + 255) / 256), dim3(256), 0, g->stream, s->thr, s->fmin,
+                       s->lor, 64 * groups, g->tune.delta);
     hipLaunchKernelGGL(batch_seed_k, dim3((n + 255) / 256), dim3(256), 0, g->stream, s->srcs, n, g->d_row,
                        pruned ? g->d_hrow : (const int32_t*)nullptr, w.dist, w.mask[0],
                        pruned ? w.hmask : (uint64_t*)nullptr, s->abits, w.ctrl, V, s->nw);
@@ -480,7 +571,7 @@ static int batch_run(Graph* g, const int32_t* srcs, int32_t n, c2g_stats* st, do
         HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, sizeof(int64_t) * B_N, hipMemcpyDeviceToHost, g->stream));
         HIP_TRY(hipStreamSynchronize(g->stream));
         if (w.h_ctrl[B_OVF]) return fail(C2G_EDEVICE, "batched frontier capacity exceeded (internal)");
-        if (w.h_ctrl[B_ITEMS] == 0) break;
+        if (w.h_ctrl[B_ITEMS] == 0 && w.h_ctrl[B_FAR] == 0) break;
     }
     if (st) {
         st->lane = C2G_LANE_GPU;
@@ -528,6 +619,7 @@ static int check_srcs(const Graph* g, const int32_t* srcs, int32_t n) {
     if (!g->d_adj) return fail(C2G_ENOTSUP, "graph has no int weights (wlist) for SSSP");
     if (n < 1) return fail(C2G_EINVAL, "need at least one source");
     if (n > 64 * 1024) return fail(C2G_EINVAL, "at most 65536 sources per batch");
+    if (g->tune.delta > 0 && n > 64 * 256) return fail(C2G_EINVAL, "at most 16384 sources per batch with delta > 0");
     for (int32_t q = 0; q < n; ++q)
         if (srcs[q] < 0 || srcs[q] >= g->V) return fail(C2G_EINVAL, "source %d out of range [0,%d)", srcs[q], g->V);
     return C2G_OK;
@@ -580,8 +672,10 @@ int sssp_batch_group(Graph* g, const int32_t* srcs, int32_t n, int32_t* dist_out
 // A query joins lane `lane` of group 0: its distance column was reset to
 // INT32_MAX when the lane was last freed (or at init).
 __global__ void bserve_admit_k(const int32_t* __restrict__ row, int32_t* __restrict__ dist,
-                               uint64_t* __restrict__ mask, uint32_t* __restrict__ abits, int lane, int32_t s) {
+                               uint64_t* __restrict__ mask, uint32_t* __restrict__ abits, int32_t* __restrict__ thr,
+                               int lane, int32_t s, int delta) {
     dist[(int64_t)s * 64 + lane] = 0;
+    thr[lane] = delta > 0 ? delta : INT32_MAX;
     if (row[s + 1] > row[s]) {
         atomicOr((unsigned long long*)&mask[s], 1ull << lane);
         atomicOr(&abits[s >> 5], 1u << (s & 31));
@@ -650,6 +744,7 @@ int bserve_init(Graph* g, hipStream_t strm) {
     BatchState* s = bstate(g);
     const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
     HIP_TRY(hipMemsetAsync(w.ctrl, 0, sizeof(int64_t) * B_N, strm));
+    hipLaunchKernelGGL(batch_thr_init_k, dim3(1), dim3(64), 0, strm, s->thr, s->fmin, s->lor, 64, g->tune.delta);
     hipLaunchKernelGGL(batch_fill_k, dim3(std::max(1, g->num_cus) * 16), dim3(256), 0, strm, w.dist,
                        (int64_t)g->V * 64, w.mask[0], pruned ? w.hmask : (uint64_t*)nullptr, (int64_t)g->V,
                        s->abits, s->nw);
@@ -662,13 +757,13 @@ int bserve_admit(Graph* g, hipStream_t strm, int lane, int32_t src) {
     if (lane < 0 || lane > 63 || src < 0 || src >= g->V) return fail(C2G_EINVAL, "bad lane/source");
     BatchState* s = bstate(g);
     hipLaunchKernelGGL(bserve_admit_k, dim3(1), dim3(1), 0, strm, g->d_row, g->batch.dist, g->batch.mask[0],
-                       s->abits, lane, src);
+                       s->abits, s->thr, lane, src, g->tune.delta);
     HIP_TRY(hipGetLastError());
     return C2G_OK;
 }
 
 int bserve_step(Graph* g, hipStream_t strm, uint64_t* lanes_with_work) {
-    int rc = launch_round_b(g, 1, nullptr, strm, 1);
+    int rc = launch_round_b(g, 1, nullptr, strm);
     if (rc) return rc;
     BatchWs& w = g->batch;
     HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, sizeof(int64_t) * B_N, hipMemcpyDeviceToHost, strm));

@@ -81,13 +81,16 @@ def test_memory_admission_and_fallback(gpu):
             assert info["lane"] == c2g.LANE_HOST
 
 
-def test_lane_reuse_many_queries(gpu):
-    # more queries than lanes: lanes are freed and reused, columns reset
+@pytest.mark.parametrize("delta", [0, 100])
+def test_lane_reuse_many_queries(gpu, delta):
+    # more queries than lanes: lanes are freed and reused, columns and bucket bounds reset
     G = _G()
     deg = np.diff(G["vlist"])
     srcs = np.flatnonzero(deg > 0)[:150]
-    with _graph(G) as g, c2g.Server(g, max_batch=16) as s:
-        ids = [s.submit(c2g.Q_SSSP, int(v)) for v in srcs]
-        for i, v in zip(ids, srcs):
-            info, d = s.poll(i)
-            assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], int(v))), v
+    with _graph(G) as g:
+        g.set_tune(delta=delta)
+        with c2g.Server(g, max_batch=16) as s:
+            ids = [s.submit(c2g.Q_SSSP, int(v)) for v in srcs]
+            for i, v in zip(ids, srcs):
+                info, d = s.poll(i)
+                assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], int(v))), v

@@ -15,13 +15,15 @@ def _graph(G):
     return c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], device=0)
 
 
-def test_batch_c1_against_dijkstra(gpu):
+@pytest.mark.parametrize("delta", [0, 32])
+def test_batch_c1_against_dijkstra(gpu, delta):
     G = rmat_graph(16, 1, wmax=64)
     deg = np.diff(G["vlist"])
     rng = np.random.default_rng(7)
     srcs = np.concatenate([[np.argmax(deg)], rng.choice(np.flatnonzero(deg > 0), 40),
                            rng.choice(np.flatnonzero(deg == 1), 20), rng.choice(np.flatnonzero(deg == 0), 3)])
     with _graph(G) as g:
+        g.set_tune(delta=delta)
         d, st = g.sssp_batch(srcs)
     assert st.lane == c2g.LANE_GPU
     for q, s in enumerate(srcs):
@@ -49,12 +51,12 @@ def test_batch_multi_group_matches_single(gpu):
     assert np.array_equal(sums, sums_e) and np.array_equal(reached, reached_e)
 
 
-@pytest.mark.parametrize("chunk", [1, 5, 32, 64, 200])
-def test_batch_chunk_sizes(gpu, chunk):
+@pytest.mark.parametrize("chunk,delta", [(1, 0), (5, 7), (32, 0), (64, 40), (200, 1)])
+def test_batch_chunk_sizes(gpu, chunk, delta):
     G = random_graph(300, 3000, seed=chunk, wmax=20)
     srcs = np.arange(0, 300, 3)
     with _graph(G) as g:
-        g.set_tune(batch_chunk=chunk)
+        g.set_tune(batch_chunk=chunk, delta=delta)
         d, _ = g.sssp_batch(srcs)
     for q, s in enumerate(srcs):
         assert np.array_equal(d[q], oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], int(s))), (chunk, s)
@@ -83,7 +85,8 @@ def test_batch_errors(gpu):
 
 
 @pytest.mark.slow
-def test_batch_scale20_certificate(gpu):
+@pytest.mark.parametrize("delta", [0, 256])
+def test_batch_scale20_certificate(gpu, delta):
     # uneven reach: half hubs, half out-degree-1 sources (the C4 recipe at scale 20)
     G = rmat_graph(20, 4, wmax=1023)
     deg = np.diff(G["vlist"])
@@ -91,6 +94,7 @@ def test_batch_scale20_certificate(gpu):
     top = np.argsort(-deg)[: len(deg) // 100]
     srcs = np.concatenate([rng.choice(top, 64, replace=False), rng.choice(np.flatnonzero(deg == 1), 64)])
     with _graph(G) as g:
+        g.set_tune(delta=delta)
         d, st = g.sssp_batch(srcs)
         sums_e, reached_e, _ = g.sssp_each_resident(srcs)
     fin = d != INT_MAX
@@ -100,3 +104,16 @@ def test_batch_scale20_certificate(gpu):
         check_sssp_certificate(G["vlist"], G["elist"], G["wlist"], int(srcs[q]), d[q])
     ref = oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], int(srcs[0]))
     assert np.array_equal(d[0], ref)
+
+
+def test_retune_between_runs_takes_effect(gpu):
+    # captured launch sequences must follow set_tune (they hold kernel arguments)
+    G = rmat_graph(16, 1, wmax=64)
+    srcs = np.flatnonzero(np.diff(G["vlist"]) > 0)[:64].astype(np.int32)
+    with _graph(G) as g:
+        _, _, s0 = g.sssp_batch_resident(srcs)
+        g.set_tune(delta=8)
+        sums, reached, s1 = g.sssp_batch_resident(srcs)
+        d, st = g.sssp(int(srcs[0]))
+    assert s1.iterations > s0.iterations  # narrow buckets need more rounds
+    assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], int(srcs[0])))

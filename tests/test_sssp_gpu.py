@@ -18,11 +18,13 @@ def _graph(G):
     return c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], device=0)
 
 
-def test_c1_max_degree_source(gpu):
+@pytest.mark.parametrize("delta", [0, 16, 64])
+def test_c1_max_degree_source(gpu, delta):
     # Workload C1: R-MAT scale 16, ef 16, seed 1, weights U[1,64], max out-degree source.
     G = rmat_graph(16, 1, wmax=64)
     src = int(np.argmax(np.diff(G["vlist"])))
     with _graph(G) as g:
+        g.set_tune(delta=delta)
         d, st = g.sssp(src)
     ref = oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], src)
     assert np.array_equal(d, ref)
@@ -71,18 +73,21 @@ def test_hub_rows_split_across_chunks(gpu):
     w = (np.arange(1, n) % 7 + 1).astype(np.int32)
     with c2g.Graph.from_csr(vl, el, w, device=0) as g:
         for chunk in (1, 7, 64, 4096):
-            g.set_tune(chunk=chunk)
-            d, _ = g.sssp(0)
-            assert d[0] == 0 and np.array_equal(d[1:], w), chunk
+            for delta in (0, 3):
+                g.set_tune(chunk=chunk, delta=delta)
+                d, _ = g.sssp(0)
+                assert d[0] == 0 and np.array_equal(d[1:], w), (chunk, delta)
 
 
 @pytest.mark.slow
-def test_scale20_against_dijkstra(gpu):
+@pytest.mark.parametrize("delta", [0, 300])
+def test_scale20_against_dijkstra(gpu, delta):
     G = rmat_graph(20, 4, wmax=1023)
     deg = np.diff(G["vlist"])
     rng = np.random.default_rng(5)
     srcs = [int(np.argmax(deg))] + [int(x) for x in rng.choice(np.flatnonzero(deg > 0), 3)]
     with _graph(G) as g:
+        g.set_tune(delta=delta)
         for s in srcs:
             d, _ = g.sssp(s)
             assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], s)), s
@@ -101,12 +106,15 @@ def test_scale22_certificate(gpu):
             check_sssp_certificate(G["vlist"], G["elist"], G["wlist"], int(s), d)
 
 
-def test_budget_and_migration(gpu):
-    # sssp-purn-M semantics: GPU head for `budget` rounds, host Bellman-Ford tail
+@pytest.mark.parametrize("delta", [0, 20])
+def test_budget_and_migration(gpu, delta):
+    # sssp-purn-M semantics: GPU head for `budget` rounds, host Bellman-Ford tail;
+    # with delta-stepping the deferred vertices are part of the handed-over frontier
     G = rmat_graph(16, 1, wmax=64)
     src = int(np.argmax(np.diff(G["vlist"])))
     ref = oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], src)
     with _graph(G) as g:
+        g.set_tune(delta=delta)
         for budget in (1, 2, 3, 5, 100):
             d, fr, st = g.sssp_budget(src, budget)
             assert st.iterations <= budget

@@ -17,12 +17,14 @@ def _pruned(G):
     return P, g
 
 
-def test_all_sources_sparse(gpu):
+@pytest.mark.parametrize("delta", [0, 4])
+def test_all_sources_sparse(gpu, delta):
     for seed in range(8):
         G = random_graph(60, 130, seed=seed, wmax=9)
         P, g = _pruned(G)
         assert len(P["helist"]) > 0
         with g:
+            g.set_tune(delta=delta)
             srcs = np.arange(G["V"], dtype=np.int32)
             db, _ = g.sssp_batch(srcs)
             for s in range(G["V"]):
