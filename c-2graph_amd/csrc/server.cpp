@@ -10,8 +10,8 @@
 //     next round's frontier) and retires as soon as its lane has no queued
 //     work; its distance column is copied out and reset.  Requests that
 //     arrive while others run therefore overlap with them on the device.
-//   * aux lane (stream B): PPR and adsorption queries, one at a time, running
-//     concurrently with the SSSP rounds on the device.
+//   * PPR lane (stream B) and adsorption lane (stream C): one query at a time
+//     each, running concurrently with the SSSP rounds on the device.
 //   * host lane (optional): the reference CPU worker's algorithms
 //     (Bellman-Ford / forward push / adsorption Jacobi) on host threads, used
 //     when the device is marked failed or a request cannot fit in HBM.
@@ -47,15 +47,15 @@ struct c2g_server {
     c2g_server_opts o{};
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
-    std::deque<int64_t> q_sssp, q_aux, q_host;
+    std::deque<int64_t> q_sssp, q_aux, q_ads, q_host;
     std::unordered_map<int64_t, c2g::Request> reqs;
     int64_t next_id = 0;
     int64_t inflight = 0;
     int64_t budget = 0, used = 0;
     bool stop = false;
-    std::thread t_sssp, t_aux;
+    std::thread t_sssp, t_aux, t_ads;
     std::vector<std::thread> t_host;
-    hipStream_t s_sssp = nullptr, s_aux = nullptr;
+    hipStream_t s_sssp = nullptr, s_aux = nullptr, s_ads = nullptr;
     double t0 = 0;
     int error = 0;
 };
@@ -159,9 +159,13 @@ static void sssp_worker(c2g_server* s) {
     }
 }
 
-static void aux_worker(c2g_server* s) {
+// One device lane per kind (PPR on one stream, adsorption on another), so a
+// short PPR query never waits behind a long adsorption run.
+static void aux_worker(c2g_server* s, bool ads) {
     Graph* g = s->g;
     (void)hipSetDevice(g->device);
+    std::deque<int64_t>& q = ads ? s->q_ads : s->q_aux;
+    hipStream_t strm = ads ? s->s_ads : s->s_aux;
     std::vector<int32_t> sv, sl;
     while (true) {
         int64_t id;
@@ -169,12 +173,12 @@ static void aux_worker(c2g_server* s) {
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv_work.wait(lk, [&] {
                 if (s->stop) return true;
-                if (s->q_aux.empty()) return false;
-                return s->used + s->reqs[s->q_aux.front()].bytes <= s->budget;
+                if (q.empty()) return false;
+                return s->used + s->reqs[q.front()].bytes <= s->budget;
             });
             if (s->stop) break;
-            id = s->q_aux.front();
-            s->q_aux.pop_front();
+            id = q.front();
+            q.pop_front();
             Request& r = s->reqs[id];
             s->used += r.bytes;
             r.t_admit = now_ms() - s->t0;
@@ -187,10 +191,10 @@ static void aux_worker(c2g_server* s) {
         }
         int rc;
         if (r->kind == C2G_Q_PPR) {
-            rc = ppr_single(g, s->s_aux, r->src, s->o.alpha, s->o.eps, (float*)r->result.data(), nullptr, nullptr);
+            rc = ppr_single(g, strm, r->src, s->o.alpha, s->o.eps, (float*)r->result.data(), nullptr, nullptr);
         } else {
             ads_seeds(g, r->src, s->o.ads_labels, sv, sl);
-            rc = ads_run(g, s->s_aux, sv.data(), sl.data(), (int32_t)sv.size(), s->o.ads_labels, s->o.p_inj,
+            rc = ads_run(g, strm, sv.data(), sl.data(), (int32_t)sv.size(), s->o.ads_labels, s->o.p_inj,
                          s->o.p_cont, s->o.p_abnd, s->o.ads_iters, (float*)r->result.data(), nullptr);
         }
         std::lock_guard<std::mutex> lk(s->mu);
@@ -271,7 +275,8 @@ int c2g_server_start(c2g_graph* h, const c2g_server_opts* o, c2g_server** out) {
         else if (hipMemGetInfo(&fr, &tot) != hipSuccess) { delete s; return fail(C2G_EDEVICE, "hipMemGetInfo failed"); }
         s->budget = (int64_t)fr - s->o.hbm_reserve;
         if (hipStreamCreateWithFlags(&s->s_sssp, hipStreamNonBlocking) != hipSuccess ||
-            hipStreamCreateWithFlags(&s->s_aux, hipStreamNonBlocking) != hipSuccess) {
+            hipStreamCreateWithFlags(&s->s_aux, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&s->s_ads, hipStreamNonBlocking) != hipSuccess) {
             delete s;
             return fail(C2G_EDEVICE, "stream creation failed");
         }
@@ -281,7 +286,10 @@ int c2g_server_start(c2g_graph* h, const c2g_server_opts* o, c2g_server** out) {
     s->t0 = now_ms();
     if (device_ok) {
         if (g->d_adj) s->t_sssp = std::thread(sssp_worker, s);
-        if (g->d_adjf) s->t_aux = std::thread(aux_worker, s);
+        if (g->d_adjf) {
+            s->t_aux = std::thread(aux_worker, s, false);
+            s->t_ads = std::thread(aux_worker, s, true);
+        }
     }
     if (s->o.host_lane)
         for (int i = 0; i < 2; ++i) s->t_host.emplace_back(host_worker, s);
@@ -313,8 +321,10 @@ int64_t c2g_server_submit(c2g_server* s, int32_t kind, int32_t src) {
         s->q_host.push_back(r.id);
     } else if (kind == C2G_Q_SSSP) {
         s->q_sssp.push_back(r.id);
-    } else {
+    } else if (kind == C2G_Q_PPR) {
         s->q_aux.push_back(r.id);
+    } else {
+        s->q_ads.push_back(r.id);
     }
     s->reqs.emplace(r.id, std::move(r));
     s->inflight++;
@@ -361,9 +371,11 @@ int c2g_server_stop(c2g_server* s) {
     s->cv_done.notify_all();
     if (s->t_sssp.joinable()) s->t_sssp.join();
     if (s->t_aux.joinable()) s->t_aux.join();
+    if (s->t_ads.joinable()) s->t_ads.join();
     for (auto& t : s->t_host) t.join();
     if (s->s_sssp) { (void)hipStreamSynchronize(s->s_sssp); (void)hipStreamDestroy(s->s_sssp); }
     if (s->s_aux) { (void)hipStreamSynchronize(s->s_aux); (void)hipStreamDestroy(s->s_aux); }
+    if (s->s_ads) { (void)hipStreamSynchronize(s->s_ads); (void)hipStreamDestroy(s->s_ads); }
     delete s;
     return C2G_OK;
 }
