@@ -56,20 +56,16 @@ ads_spmm_k(const int2* __restrict__ adjf, const int4* __restrict__ items, int64_
         const int4 it = items[i];
         float acc = 0.0f;
         int e = it.y;
-        // 4 gathers in flight per lane group
-        for (; e + 4 <= it.z; e += 4) {
-            const int2 a0 = adjf[e], a1 = adjf[e + 1], a2 = adjf[e + 2], a3 = adjf[e + 3];
-            float y0 = 0, y1 = 0, y2 = 0, y3 = 0;
-            if (l < L) {
-                y0 = Yt[(int64_t)a0.x * L + l];
-                y1 = Yt[(int64_t)a1.x * L + l];
-                y2 = Yt[(int64_t)a2.x * L + l];
-                y3 = Yt[(int64_t)a3.x * L + l];
-            }
-            acc += __int_as_float(a0.y) * y0;
-            acc += __int_as_float(a1.y) * y1;
-            acc += __int_as_float(a2.y) * y2;
-            acc += __int_as_float(a3.y) * y3;
+        // 8 row gathers in flight per lane group
+        for (; e + 8 <= it.z; e += 8) {
+            int2 a[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) a[k] = adjf[e + k];
+            float y[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) y[k] = l < L ? Yt[(int64_t)a[k].x * L + l] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) acc += __int_as_float(a[k].y) * y[k];
         }
         for (; e < it.z; ++e) {
             const int2 a = adjf[e];
