@@ -31,8 +31,8 @@ namespace c2g {
 // Device workspace of single-query SSSP.
 struct SsspWs {
     int32_t* dist = nullptr;     // [V]
-    int32_t* stamp = nullptr;    // [V] iteration for which v is queued
-    int4* q[2] = {nullptr, nullptr};  // frontier items (u, e_begin, e_end, flags)
+    int32_t* stamp = nullptr;    // activation bitmaps: live [nw], hidden [nw]
+    int4* q[1] = {nullptr};      // frontier items (u, e_begin, e_end, kind)
     int64_t qcap = 0;
     int64_t* ctrl = nullptr;     // control block, see sssp.hip
     int64_t* h_ctrl = nullptr;   // pinned mirror
@@ -42,31 +42,17 @@ struct SsspWs {
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
-// Device workspace of batched SSSP (64 queries per group, lane = query).
+// Device workspace of batched SSSP (64 queries per group, lane = query);
+// the per-round queues and bitmaps live in sssp_batch.hip.
 struct BatchWs {
-    int32_t* dist = nullptr;     // [V][64]
-    uint64_t* mask[2] = {nullptr, nullptr};  // [V] active lane masks cur/next
-    uint64_t* hmask = nullptr;   // [V] lanes whose value came through a hidden edge
-    int4* q[2] = {nullptr, nullptr};
-    int64_t qcap = 0;
+    int32_t* dist = nullptr;     // [group][V][64]
+    uint64_t* mask[1] = {nullptr};  // [group][V] lanes pending expansion
+    uint64_t* hmask = nullptr;   // [group][V] lanes whose value came through a hidden edge
     int64_t* ctrl = nullptr;
     int64_t* h_ctrl = nullptr;
-    int64_t* sums = nullptr;     // [64] checksum per lane
-    int32_t* reach = nullptr;    // [64]
     hipGraphExec_t exec = nullptr;
     int exec_iters = 0, exec_block = 0, exec_grid = 0, exec_chunk = 0;
     int groups = 0;              // number of groups allocated
-};
-
-struct PprWs {
-    float* p = nullptr;
-    float* r = nullptr;
-    float* m = nullptr;
-    int32_t* stamp = nullptr;
-    int32_t* q[2] = {nullptr, nullptr};
-    int64_t* ctrl = nullptr;
-    int64_t* h_ctrl = nullptr;
-    int nq = 0;                  // queries allocated (batched: vertex-major [V][nq])
 };
 
 struct Graph {
@@ -89,7 +75,6 @@ struct Graph {
     int2* d_hadjf = nullptr;
     SsspWs sssp;
     BatchWs batch;
-    PprWs ppr;
     c2g_tune tune{};
     std::mutex mu;  // serialises device use of this handle (server vs caller)
 
@@ -105,9 +90,6 @@ void ads_release(Graph* g);
 void serve_release(Graph* g);
 void default_tune(Graph* g);
 int ensure_sssp_ws(Graph* g);
-int ensure_batch_ws(Graph* g);
-int ensure_ppr_ws(Graph* g, int nq);
-int64_t dev_read_i32(const std::string&, std::vector<int32_t>&);
 
 // Host-side helpers shared by lanes
 int host_bf(const Graph* g, int32_t* dist, const int32_t* frontier, int32_t count,

@@ -170,17 +170,12 @@ void device_release(Graph* g) {
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     if (g->sssp.exec) (void)hipGraphExecDestroy(g->sssp.exec);
     if (g->batch.exec) (void)hipGraphExecDestroy(g->batch.exec);
-    dfree(g->sssp.dist); dfree(g->sssp.stamp); dfree(g->sssp.q[0]); dfree(g->sssp.q[1]);
+    dfree(g->sssp.dist); dfree(g->sssp.stamp); dfree(g->sssp.q[0]);
     dfree(g->sssp.ctrl); dfree(g->sssp.hid);
     if (g->sssp.h_ctrl) (void)hipHostFree(g->sssp.h_ctrl);
     for (auto& e : g->sssp.ev) if (e) (void)hipEventDestroy(e);
-    dfree(g->batch.dist); dfree(g->batch.mask[0]); dfree(g->batch.mask[1]); dfree(g->batch.hmask);
-    dfree(g->batch.q[0]); dfree(g->batch.q[1]); dfree(g->batch.ctrl); dfree(g->batch.sums);
-    dfree(g->batch.reach);
+    dfree(g->batch.dist); dfree(g->batch.mask[0]); dfree(g->batch.hmask); dfree(g->batch.ctrl);
     if (g->batch.h_ctrl) (void)hipHostFree(g->batch.h_ctrl);
-    dfree(g->ppr.p); dfree(g->ppr.r); dfree(g->ppr.m); dfree(g->ppr.stamp);
-    dfree(g->ppr.q[0]); dfree(g->ppr.q[1]); dfree(g->ppr.ctrl);
-    if (g->ppr.h_ctrl) (void)hipHostFree(g->ppr.h_ctrl);
     dfree(g->d_row); dfree(g->d_adj); dfree(g->d_adjf); dfree(g->d_hrow); dfree(g->d_hadj);
     dfree(g->d_hadjf);
     if (g->stream) (void)hipStreamDestroy(g->stream);
