@@ -5,9 +5,9 @@
 // core/Util.cuh:31-35).  Here the candidates are timed on the caller's sample
 // and the fastest is stored in the handle.  Every setting computes the same
 // distances (the tunables change scheduling only), so tuning never changes
-// results.  Search: coordinate descent over grid_per_cu, chunk (single-source)
-// or batch_chunk (batched) and iters_per_graph, two passes, best of two runs
-// per candidate.
+// results.  Search: coordinate descent over grid_per_cu_single and chunk
+// (single-source) or grid_per_cu and batch_chunk (batched), then
+// iters_per_graph; two passes, best of two runs per candidate.
 #include "graph.h"
 #include <cstring>
 
@@ -66,7 +66,7 @@ extern "C" int c2g_autotune(c2g_graph* h, const int32_t* srcs, int32_t n, int32_
             const std::vector<int>& vals = which == 0 ? grids : which == 1 ? chunks : iters;
             for (int v : vals) {
                 c2g_tune t = best;
-                if (which == 0) t.grid_per_cu = v;
+                if (which == 0) (b ? t.grid_per_cu : t.grid_per_cu_single) = v;
                 else if (which == 1) (b ? t.batch_chunk : t.chunk) = v;
                 else t.iters_per_graph = v;
                 if (!memcmp(&t, &best, sizeof t)) continue;

@@ -102,6 +102,10 @@ int sssp_batch_group(Graph* g, const int32_t* srcs, int32_t n, int32_t* dist_out
                      int64_t* sums, int32_t* reached, c2g_stats* st);
 int ppr_single(Graph* g, hipStream_t strm, int32_t src, float alpha, float eps, float* p_out,
                float* r_out, c2g_stats* st);
+// Batched PPR (64 queries per lane group) on `strm`; query q's p (and r)
+// land in p_outs[q] (r_outs[q]); null rows or a null r_outs are skipped.
+int ppr_batch_to(Graph* g, hipStream_t strm, const int32_t* srcs, int32_t n, float alpha, float eps,
+                 float* const* p_outs, float* const* r_outs, c2g_stats* st);
 int ads_run(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t* sl, int32_t nseed, int32_t L,
             float p_inj, float p_cont, float p_abnd, int32_t iters, float* y_out, c2g_stats* st);
 int host_ads(const Graph* g, const int32_t* sv, const int32_t* sl, int32_t nseed, int32_t L, float p_inj,

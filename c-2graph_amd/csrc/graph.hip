@@ -56,8 +56,9 @@ void default_tune(Graph* g) {
     memset(&t, 0, sizeof t);
     t.block = 256;
     t.grid_per_cu = 8;
-    t.chunk = 64;
-    t.batch_chunk = 32;
+    t.grid_per_cu_single = 4;
+    t.chunk = 32;
+    t.batch_chunk = 64;
     t.iters_per_graph = 8;
     t.use_graph = 1;
     t.delta = 0;
@@ -316,6 +317,8 @@ int c2g_set_tune(c2g_graph* h, const c2g_tune* t) {
     if (t->block != 64 && t->block != 128 && t->block != 256 && t->block != 512 && t->block != 1024)
         return fail(C2G_EINVAL, "block must be one of 64..1024 (power of two)");
     if (t->grid_per_cu < 1 || t->grid_per_cu > 64) return fail(C2G_EINVAL, "grid_per_cu in [1,64]");
+    if (t->grid_per_cu_single < 0 || t->grid_per_cu_single > 64)
+        return fail(C2G_EINVAL, "grid_per_cu_single in [0,64]");
     if (t->chunk < 1 || t->chunk > 4096) return fail(C2G_EINVAL, "chunk in [1,4096]");
     if (t->batch_chunk < 1 || t->batch_chunk > 4096) return fail(C2G_EINVAL, "batch_chunk in [1,4096]");
     if (t->iters_per_graph < 2 || t->iters_per_graph > 256 || (t->iters_per_graph & 1))

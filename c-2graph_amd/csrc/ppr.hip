@@ -366,6 +366,32 @@ pprb_emit_k(const int32_t* __restrict__ row, uint64_t* __restrict__ mask, uint32
     }
 }
 
+constexpr int kPprStage = 8;  // query columns moved per extraction transfer
+static_assert(64 % kPprStage == 0, "extraction chunks must not span lane groups");
+
+// out[k][v] = A[v][l0 + k] for k < m <= kPprStage: 64-vertex tiles staged
+// through LDS so both the column read (32-byte segments) and the row writes
+// are contiguous.
+__global__ void __launch_bounds__(256)
+pprb_extract_k(const float* __restrict__ A, int64_t V, int l0, int m, float* __restrict__ out) {
+    __shared__ float tile[64][kPprStage + 1];
+    const int t = threadIdx.x;
+    for (int64_t v0 = (int64_t)blockIdx.x * 64; v0 < V; v0 += (int64_t)gridDim.x * 64) {
+        for (int i = t; i < 64 * m; i += 256) {
+            const int r = i / m, c = i - r * m;
+            const int64_t v = v0 + r;
+            tile[r][c] = v < V ? A[v * 64 + l0 + c] : 0.0f;
+        }
+        __syncthreads();
+        for (int i = t; i < 64 * m; i += 256) {
+            const int k = i >> 6, r = i & 63;
+            const int64_t v = v0 + r;
+            if (v < V) out[(int64_t)k * V + v] = tile[r][k];
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------ host
 struct PprState {
     // single
@@ -384,6 +410,8 @@ struct PprState {
     uint64_t* imask = nullptr;
     int64_t itemcap = 0;
     int32_t* srcs = nullptr;
+    float* stage = nullptr;    // [kPprStage][V] device staging for extraction
+    float* h_stage = nullptr;  // pinned mirror
 };
 
 static std::mutex g_ps_mu;
@@ -405,9 +433,10 @@ void ppr_release(Graph* g) {
             PprState* s = g_ps[i].second;
             for (void* p : {(void*)s->p, (void*)s->r, (void*)s->m, (void*)s->q, (void*)s->ctrl, (void*)s->P,
                             (void*)s->R, (void*)s->M, (void*)s->RU, (void*)s->mask, (void*)s->tbits,
-                            (void*)s->items, (void*)s->imask, (void*)s->srcs})
+                            (void*)s->items, (void*)s->imask, (void*)s->srcs, (void*)s->stage})
                 if (p) (void)hipFree(p);
             if (s->h_ctrl) (void)hipHostFree(s->h_ctrl);
+            if (s->h_stage) (void)hipHostFree(s->h_stage);
             delete s;
             g_ps.erase(g_ps.begin() + i);
             return;
@@ -563,8 +592,10 @@ static int ensure_ppr_batch(Graph* g, PprState* s, int groups) {
     return C2G_OK;
 }
 
-static int ppr_batch(Graph* g, const int32_t* srcs, int32_t n, float alpha, float eps, float* p_out,
-                     float* r_out, c2g_stats* st) {
+// Runs n queries (64 lanes per group) to convergence on `strm`; P and R stay
+// in HBM as [group][V][64] until the next batched call.
+static int pprb_run(Graph* g, hipStream_t strm, const int32_t* srcs, int32_t n, float alpha, float eps,
+                    c2g_stats* st) {
     int rc = ppr_check(g, alpha, eps);
     if (rc) return rc;
     if (n < 1 || n > 64 * 1024) return fail(C2G_EINVAL, "need 1..65536 sources");
@@ -579,18 +610,18 @@ static int ppr_batch(Graph* g, const int32_t* srcs, int32_t n, float alpha, floa
     const int64_t words = s->nw * groups;
     const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (words + 255) / 256));
     const double t0 = now_ms();
-    HIP_TRY(hipMemcpyAsync(s->srcs, srcs, sizeof(int32_t) * n, hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipMemcpyAsync(s->srcs, srcs, sizeof(int32_t) * n, hipMemcpyHostToDevice, strm));
     const int64_t n4 = (int64_t)groups * V * 64 / 4;
-    hipLaunchKernelGGL(pprb_fill_k, dim3(grid), dim3(256), 0, g->stream, s->P, n4, s->tbits, words);
-    hipLaunchKernelGGL(pprb_fill_k, dim3(grid), dim3(256), 0, g->stream, s->R, n4, s->tbits, (int64_t)0);
-    hipLaunchKernelGGL(pprb_fill_k, dim3(grid), dim3(256), 0, g->stream, s->M, n4, s->tbits, (int64_t)0);
-    hipLaunchKernelGGL(pprb_seed_k, dim3(n), dim3(256), 0, g->stream, s->srcs, n, g->d_row, g->d_adjf,
+    hipLaunchKernelGGL(pprb_fill_k, dim3(grid), dim3(256), 0, strm, s->P, n4, s->tbits, words);
+    hipLaunchKernelGGL(pprb_fill_k, dim3(grid), dim3(256), 0, strm, s->R, n4, s->tbits, (int64_t)0);
+    hipLaunchKernelGGL(pprb_fill_k, dim3(grid), dim3(256), 0, strm, s->M, n4, s->tbits, (int64_t)0);
+    hipLaunchKernelGGL(pprb_seed_k, dim3(n), dim3(256), 0, strm, s->srcs, n, g->d_row, g->d_adjf,
                        pruned ? g->d_hrow : (const int32_t*)nullptr, pruned ? g->d_hadjf : (const int2*)nullptr,
                        s->P, s->M, s->tbits, s->ctrl, V, s->nw, alpha);
     auto threshold = [&](int force) {
-        hipLaunchKernelGGL(pprb_activate_k, dim3(grid), dim3(256), 0, g->stream, g->d_row, s->P, s->R, s->M,
+        hipLaunchKernelGGL(pprb_activate_k, dim3(grid), dim3(256), 0, strm, g->d_row, s->P, s->R, s->M,
                            s->RU, s->mask, s->tbits, V, s->nw, groups, alpha, eps, force);
-        hipLaunchKernelGGL(pprb_emit_k, dim3(egrid), dim3(256), 0, g->stream, g->d_row, s->mask, s->tbits,
+        hipLaunchKernelGGL(pprb_emit_k, dim3(egrid), dim3(256), 0, strm, g->d_row, s->mask, s->tbits,
                            s->items, s->imask, s->ctrl, V, s->nw, groups, g->tune.batch_chunk, s->itemcap);
     };
     threshold(1);
@@ -599,35 +630,17 @@ static int ppr_batch(Graph* g, const int32_t* srcs, int32_t n, float alpha, floa
     for (;;) {
         const int K = g->tune.iters_per_graph;
         for (int k = 0; k < K; ++k) {
-            hipLaunchKernelGGL(pprb_push_k, dim3(grid), dim3(256), 0, g->stream, g->d_adjf, s->M, s->RU, s->items,
+            hipLaunchKernelGGL(pprb_push_k, dim3(grid), dim3(256), 0, strm, g->d_adjf, s->M, s->RU, s->items,
                                s->imask, s->tbits, s->ctrl, V, s->nw, alpha);
             threshold(0);
         }
         HIP_TRY(hipGetLastError());
         rounds += K;
-        HIP_TRY(hipMemcpyAsync(s->h_ctrl, s->ctrl, sizeof(int64_t) * P_N, hipMemcpyDeviceToHost, g->stream));
-        HIP_TRY(hipStreamSynchronize(g->stream));
+        HIP_TRY(hipMemcpyAsync(s->h_ctrl, s->ctrl, sizeof(int64_t) * P_N, hipMemcpyDeviceToHost, strm));
+        HIP_TRY(hipStreamSynchronize(strm));
         if (s->h_ctrl[P_OVF]) return fail(C2G_EDEVICE, "batched PPR capacity exceeded (internal)");
         if (s->h_ctrl[P_CUR] == 0) break;
         if (rounds > kPprMaxRounds) return fail(C2G_ETIMEOUT, "PPR did not converge in %d rounds", kPprMaxRounds);
-    }
-    const double t1 = now_ms();
-    // [g][V][64] -> caller [q][V]
-    std::vector<float> blk((size_t)V * 64);
-    for (int pass = 0; pass < 2; ++pass) {
-        float* out = pass ? r_out : p_out;
-        const float* dev = pass ? s->R : s->P;
-        if (!out) continue;
-        for (int gi = 0; gi < groups; ++gi) {
-            HIP_TRY(hipMemcpy(blk.data(), dev + (int64_t)gi * V * 64, sizeof(float) * V * 64, hipMemcpyDeviceToHost));
-            const int lanes = std::min(64, n - 64 * gi);
-            parallel_for(V, [&](int64_t lo, int64_t hi, int) {
-                for (int l = 0; l < lanes; ++l) {
-                    float* o = out + (int64_t)(64 * gi + l) * V;
-                    for (int64_t v = lo; v < hi; ++v) o[v] = blk[(size_t)v * 64 + l];
-                }
-            });
-        }
     }
     if (st) {
         st->lane = C2G_LANE_GPU;
@@ -635,9 +648,52 @@ static int ppr_batch(Graph* g, const int32_t* srcs, int32_t n, float alpha, floa
         st->frontier_total = s->h_ctrl[P_FRONT];
         st->edges_relaxed = s->h_ctrl[P_EDGES];
         st->items = s->h_ctrl[P_ITEMS];
-        st->ms = t1 - t0;
+        st->ms = now_ms() - t0;
     }
     return C2G_OK;
+}
+
+// Copies columns of the [group][V][64] result arrays out to per-query host
+// rows: a tile transpose on the device into [kPprStage][V], one pinned
+// transfer, then a parallel host copy into the callers' buffers.
+static int pprb_extract(Graph* g, hipStream_t strm, int32_t n, float* const* p_outs, float* const* r_outs) {
+    PprState* s = pstate(g);
+    const int64_t V = g->V;
+    if (!s->stage) {
+        HIP_TRY(hipMalloc(&s->stage, sizeof(float) * (size_t)kPprStage * std::max<int64_t>(V, 1)));
+        HIP_TRY(hipHostMalloc((void**)&s->h_stage, sizeof(float) * (size_t)kPprStage * std::max<int64_t>(V, 1),
+                              hipHostMallocDefault));
+    }
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((V + 63) / 64, (int64_t)std::max(1, g->num_cus) * 16));
+    for (int pass = 0; pass < 2; ++pass) {
+        float* const* outs = pass ? r_outs : p_outs;
+        const float* dev = pass ? s->R : s->P;
+        if (!outs) continue;
+        for (int32_t q0 = 0; q0 < n; q0 += kPprStage) {
+            const int gi = q0 / 64, l0 = q0 % 64;  // a chunk never spans two groups
+            const int m = std::min<int32_t>(kPprStage, n - q0);
+            hipLaunchKernelGGL(pprb_extract_k, dim3(grid), dim3(256), 0, strm, dev + (int64_t)gi * V * 64, V, l0, m,
+                               s->stage);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(s->h_stage, s->stage, sizeof(float) * V * m, hipMemcpyDeviceToHost, strm));
+            HIP_TRY(hipStreamSynchronize(strm));
+            parallel_for((int64_t)m * V, [&](int64_t lo, int64_t hi, int) {
+                while (lo < hi) {
+                    const int64_t k = lo / V, v0 = lo - k * V, len = std::min(hi - lo, V - v0);
+                    if (outs[q0 + k]) memcpy(outs[q0 + k] + v0, s->h_stage + k * V + v0, sizeof(float) * len);
+                    lo += len;
+                }
+            });
+        }
+    }
+    return C2G_OK;
+}
+
+int ppr_batch_to(Graph* g, hipStream_t strm, const int32_t* srcs, int32_t n, float alpha, float eps,
+                 float* const* p_outs, float* const* r_outs, c2g_stats* st) {
+    int rc = pprb_run(g, strm, srcs, n, alpha, eps, st);
+    if (rc) return rc;
+    return pprb_extract(g, strm, n, p_outs, r_outs);
 }
 
 }  // namespace c2g
@@ -688,7 +744,14 @@ int c2g_ppr_batch(c2g_graph* h, const int32_t* srcs, int32_t n, float alpha, flo
     if (!h || !srcs || !p_out) return fail(C2G_EINVAL, "null argument");
     std::lock_guard<std::mutex> lk(h->g.mu);
     if (st) memset(st, 0, sizeof *st);
-    return ppr_batch(&h->g, srcs, n, alpha, eps, p_out, r_out, st);
+    if (n < 1 || n > 64 * 1024) return fail(C2G_EINVAL, "need 1..65536 sources");
+    const int64_t V = h->g.V;
+    std::vector<float*> po((size_t)n), ro(r_out ? (size_t)n : 0);
+    for (int32_t q = 0; q < n; ++q) {
+        po[q] = p_out + (int64_t)q * V;
+        if (r_out) ro[q] = r_out + (int64_t)q * V;
+    }
+    return ppr_batch_to(&h->g, h->g.stream, srcs, n, alpha, eps, po.data(), r_out ? ro.data() : nullptr, st);
 }
 
 }  // extern "C"

@@ -10,8 +10,10 @@
 //     next round's frontier) and retires as soon as its lane has no queued
 //     work; its distance column is copied out and reset.  Requests that
 //     arrive while others run therefore overlap with them on the device.
-//   * PPR lane (stream B) and adsorption lane (stream C): one query at a time
-//     each, running concurrently with the SSSP rounds on the device.
+//   * PPR lane (stream B): the PPR requests queued when the lane frees up
+//     (up to 64) run as one batched push, lane = query.
+//   * adsorption lane (stream C): one query at a time.  Both run concurrently
+//     with the SSSP rounds on the device.
 //   * host lane (optional): the reference CPU worker's algorithms
 //     (Bellman-Ford / forward push / adsorption Jacobi) on host threads, used
 //     when the device is marked failed or a request cannot fit in HBM.
@@ -166,9 +168,14 @@ static void aux_worker(c2g_server* s, bool ads) {
     (void)hipSetDevice(g->device);
     std::deque<int64_t>& q = ads ? s->q_ads : s->q_aux;
     hipStream_t strm = ads ? s->s_ads : s->s_aux;
-    std::vector<int32_t> sv, sl;
+    // PPR requests queued together run as one batched pass (lane = query);
+    // adsorption runs one query at a time.
+    const size_t cap = ads ? 1 : (size_t)std::min(64, std::max(1, s->o.max_batch));
+    std::vector<int32_t> sv, sl, srcs;
+    std::vector<int64_t> ids;
+    std::vector<float*> outs;
     while (true) {
-        int64_t id;
+        ids.clear();
         {
             std::unique_lock<std::mutex> lk(s->mu);
             s->cv_work.wait(lk, [&] {
@@ -177,29 +184,44 @@ static void aux_worker(c2g_server* s, bool ads) {
                 return s->used + s->reqs[q.front()].bytes <= s->budget;
             });
             if (s->stop) break;
-            id = q.front();
-            q.pop_front();
-            Request& r = s->reqs[id];
-            s->used += r.bytes;
-            r.t_admit = now_ms() - s->t0;
-            r.result.resize(r.bytes);
+            const double t_admit = now_ms() - s->t0;
+            while (!q.empty() && ids.size() < cap && s->used + s->reqs[q.front()].bytes <= s->budget) {
+                Request& r = s->reqs[q.front()];
+                q.pop_front();
+                s->used += r.bytes;
+                r.t_admit = t_admit;
+                r.result.resize(r.bytes);
+                ids.push_back(r.id);
+            }
         }
-        Request* r;
+        std::vector<Request*> rs;
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            r = &s->reqs[id];
+            for (int64_t id : ids) rs.push_back(&s->reqs[id]);
         }
         int rc;
-        if (r->kind == C2G_Q_PPR) {
-            rc = ppr_single(g, strm, r->src, s->o.alpha, s->o.eps, (float*)r->result.data(), nullptr, nullptr);
-        } else {
+        if (ads) {
+            Request* r = rs[0];
             ads_seeds(g, r->src, s->o.ads_labels, sv, sl);
             rc = ads_run(g, strm, sv.data(), sl.data(), (int32_t)sv.size(), s->o.ads_labels, s->o.p_inj,
                          s->o.p_cont, s->o.p_abnd, s->o.ads_iters, (float*)r->result.data(), nullptr);
+        } else if (rs.size() == 1) {
+            rc = ppr_single(g, strm, rs[0]->src, s->o.alpha, s->o.eps, (float*)rs[0]->result.data(), nullptr, nullptr);
+        } else {
+            srcs.clear();
+            outs.clear();
+            for (Request* r : rs) {
+                srcs.push_back(r->src);
+                outs.push_back((float*)r->result.data());
+            }
+            rc = ppr_batch_to(g, strm, srcs.data(), (int32_t)srcs.size(), s->o.alpha, s->o.eps, outs.data(),
+                              nullptr, nullptr);
         }
         std::lock_guard<std::mutex> lk(s->mu);
-        s->used -= r->bytes;
-        finish(s, *r, rc, C2G_LANE_GPU);
+        for (Request* r : rs) {
+            s->used -= r->bytes;
+            finish(s, *r, rc, C2G_LANE_GPU);
+        }
         s->cv_done.notify_all();
         s->cv_work.notify_all();  // memory freed: the SSSP lane may admit
     }

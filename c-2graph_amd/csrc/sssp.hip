@@ -356,7 +356,8 @@ int ensure_sssp_ws(Graph* g) {
 }
 
 static int relax_grid(const Graph* g) {
-    return std::max(1, g->num_cus) * g->tune.grid_per_cu;
+    const int per_cu = g->tune.grid_per_cu_single > 0 ? g->tune.grid_per_cu_single : g->tune.grid_per_cu;
+    return std::max(1, g->num_cus) * per_cu;
 }
 
 static bool is_pruned(const Graph* g) { return g->d_hrow != nullptr && g->d_hadj != nullptr; }

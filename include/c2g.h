@@ -100,7 +100,9 @@ typedef struct {
     int32_t delta;          /* near/far split step for SSSP (0 = plain Bellman-Ford) */
     int32_t profile;        /* 1 = plain launches with HIP events around each relax
                                kernel; the summed time lands in c2g_stats.kernel_ms */
-    int32_t reserved[8];
+    int32_t grid_per_cu_single; /* workgroups per CU for the single-source SSSP
+                               kernels (0 = grid_per_cu)                      */
+    int32_t reserved[7];
 } c2g_tune;
 
 /* ======================================================================= */

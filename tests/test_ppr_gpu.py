@@ -90,6 +90,19 @@ def test_batched_equals_single(gpu):
         assert abs(l1 - R[q].astype(np.float64).sum()) <= 1e-5
 
 
+def test_batched_ragged_sizes(gpu):
+    # V not a multiple of the 64-vertex extraction tile; 14 queries = one full
+    # and one partial 8-column transfer; r columns come out too
+    G = random_graph(301, 2500, seed=6)
+    srcs = np.arange(0, 301, 23, dtype=np.int32)
+    with _g(G) as g:
+        P, R, _ = g.ppr_batch(srcs, alpha=ALPHA, eps=1e-7)
+        for q, s in enumerate(srcs):
+            p, r, _ = g.ppr(int(s), alpha=ALPHA, eps=1e-7)
+            assert np.abs(P[q].astype(np.float64) - p).sum() <= 2e-5, q
+            assert np.abs(R[q].astype(np.float64) - r).sum() <= 2e-5, q
+
+
 def test_pruned_graph_seeding(gpu):
     # ppr/ variant: the first frontier is H(s) U N'(s) with r = (1-alpha) w; the
     # oracle runs the same push from that seeded state on the pruned graph.

@@ -94,3 +94,21 @@ def test_lane_reuse_many_queries(gpu, delta):
             for i, v in zip(ids, srcs):
                 info, d = s.poll(i)
                 assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], int(v))), v
+
+
+def test_ppr_burst_is_batched(gpu):
+    # a burst of PPR requests queues behind the first and runs as one batched
+    # pass; every result must match the query run alone
+    G = _G()
+    deg = np.diff(G["vlist"])
+    srcs = np.flatnonzero(deg > 0)[::97][:40]
+    with _graph(G) as g:
+        with c2g.Server(g) as s:
+            ids = [s.submit(c2g.Q_PPR, int(v)) for v in srcs]
+            res = [s.poll(i) for i in ids]
+        for (info, p), v in zip(res, srcs):
+            assert info["status"] == 0 and info["lane"] == c2g.LANE_GPU
+            ref, _, _ = g.ppr(int(v), alpha=0.15, eps=1e-7)
+            assert np.abs(p.astype(np.float64) - ref).sum() <= 2e-5, v
+    admits = [i["t_admit_ms"] for i, _ in res]
+    assert len(set(admits)) < len(admits)  # a batch shares one admission time
