@@ -106,7 +106,7 @@ __global__ void sssp_seed_k(const int32_t* __restrict__ row, const int32_t* __re
     }
 }
 
-// Relaxation of one 64-edge step for lane k; returns the improved target or -1.
+// Relaxation of one edge for the calling lane.
 template <bool PRUNED>
 __device__ __forceinline__ void relax_edge(const int2* __restrict__ adj, const int2* __restrict__ hadj,
                                            int32_t* __restrict__ dist, uint32_t* __restrict__ bits,
@@ -117,7 +117,9 @@ __device__ __forceinline__ void relax_edge(const int2* __restrict__ adj, const i
     if (d_u > INT32_MAX - w) return;
     const int cand = d_u + w;
     if (cand >= dist[v]) return;
-    if (cand >= atomicMin(&dist[v], cand)) return;
+    // No wait for the old value: a lower dist[v] written since the read above
+    // comes from this round, whose writer marked the same activation bit.
+    atomicMin(&dist[v], cand);
     ++impr;
     if (a.x < 0) atomicOr(&bits[v >> 5], 1u << (v & 31));
     if (PRUNED) {

@@ -125,11 +125,14 @@ relax_b_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
                     oj[jj] = INT32_MAX;
                     if (cj[jj] != INT32_MAX) oj[jj] = dG[(int64_t)vj[jj] * 64 + lane];
                 }
-                // Issue every atomic of the group before any result is used.
+                // Fire-and-forget atomicMin: a lane whose candidate beats the
+                // value it read marks (v, lane) active.  If another relaxation
+                // lowered dist[v][lane] below the candidate since the read, it
+                // was made in this round and marked the same bit, so waiting
+                // for the old value would change nothing but the latency.
 #pragma unroll
                 for (int jj = 0; jj < kJ; ++jj)
-                    if (cj[jj] < oj[jj]) oj[jj] = atomicMin(&dG[(int64_t)vj[jj] * 64 + lane], cj[jj]);
-                    else cj[jj] = INT32_MAX;
+                    if (cj[jj] < oj[jj]) atomicMin(&dG[(int64_t)vj[jj] * 64 + lane], cj[jj]);
 #pragma unroll
                 for (int jj = 0; jj < kJ; ++jj) {
                     const bool imp = cj[jj] < oj[jj];
