@@ -82,6 +82,66 @@ def cpu_baseline(G, srcs, budget_s, threads):
     return len(done), wall
 
 
+def aux_phase(args):
+    """Forward-push PPR on R-MAT scale 22 symmetrised (seed 3; alpha 0.15,
+    eps 1e-7*outdeg): 16 single-source queries, then 64 sources in one batched
+    pass.  Adsorption on R-MAT scale 23 (seed 5): 16 labels seeded on 1% of
+    the vertices, 20 iterations.  Inputs resident in HBM; times exclude the
+    result copies to the host except where noted."""
+    import c2g
+
+    out = {}
+    G3 = c2g.rmat(22, 16, 0.57, 0.19, 0.19, seed=3, wmax=0, symmetric=True, want_f=True)
+    g3 = c2g.Graph.from_csr(G3["vlist"], G3["elist"], flist=G3["flist"], device=0)
+    deg = np.diff(G3["vlist"])
+    rng = np.random.default_rng(3)
+    ps = rng.choice(np.flatnonzero(deg > 0), 64).astype(np.int32)
+    g3.ppr(int(ps[0]))
+    dev_ms, pushes, rounds, t = 0.0, 0, 0, time.perf_counter()
+    for s_ in ps[:16]:
+        _, r, st = g3.ppr(int(s_))
+        dev_ms += st.ms
+        pushes += st.edges_relaxed
+        rounds += st.iterations
+    wall = time.perf_counter() - t
+    out["ppr_single"] = {
+        "workload": f"R-MAT scale 22 symmetrised seed 3 (V={G3['V']}, E={G3['E']}), alpha 0.15, eps 1e-7*outdeg",
+        "queries": 16, "ms_per_query_wall": round(wall / 16 * 1e3, 2), "ms_per_query_device": round(dev_ms / 16, 2),
+        "rounds_per_query": round(rounds / 16, 1), "edge_pushes_per_query": pushes // 16,
+        "residual_mass_last": round(float(r.astype(np.float64).sum()), 4)}
+    g3.ppr_batch(ps)
+    t = time.perf_counter()
+    P, R, stb = g3.ppr_batch(ps)
+    wall = time.perf_counter() - t
+    out["ppr_batch64"] = {
+        "queries": 64, "ms_device": round(stb.ms, 1), "queries_per_s_device": round(64 / (stb.ms * 1e-3), 1),
+        "ms_wall_with_p_r_to_host": round(wall * 1e3, 1), "rounds": stb.iterations,
+        "vertex_group_activations": stb.frontier_total, "csr_edges": stb.edges_relaxed}
+    g3.close()
+    del G3, P, R
+
+    G5 = c2g.rmat(23, 16, 0.57, 0.19, 0.19, seed=5, wmax=255, want_f=True)
+    g5 = c2g.Graph.from_csr(G5["vlist"], G5["elist"], G5["wlist"], G5["flist"], device=0)
+    rng = np.random.default_rng(5)
+    V, E = G5["V"], G5["E"]
+    sv = rng.choice(V, V // 100, replace=False).astype(np.int32)
+    sl = rng.integers(0, 16, len(sv)).astype(np.int32)
+    g5.adsorption(sv, sl)
+    t = time.perf_counter()
+    Y, sa = g5.adsorption(sv, sl)
+    wall = time.perf_counter() - t
+    per_it = sa.ms / 20
+    algo = 8 * E + 4 * V + 64 * E + 128 * V
+    out["adsorption"] = {
+        "workload": f"R-MAT scale 23 seed 5 (V={V}, E={E}), 16 labels on 1% of vertices, 20 iterations, fp32",
+        "ms_device": round(sa.ms, 1), "ms_wall_with_Y_to_host": round(wall * 1e3, 1),
+        "ms_per_iteration": round(per_it, 3), "algo_bytes_per_iteration": algo,
+        "achieved_GBps": round(algo / (per_it * 1e-3) / 1e9, 1),
+        "gather_free_lower_bound_GBps": round((8 * E + 4 * V + 128 * V) / (per_it * 1e-3) / 1e9, 1)}
+    g5.close()
+    return out
+
+
 def serve_phase(args):
     """Served mixed stream (60% SSSP, 30% PPR, 10% adsorption) with Poisson arrivals."""
     import c2g
@@ -169,6 +229,8 @@ def main():
     ap.add_argument("--serve-qps", type=float, default=40.0)
     ap.add_argument("--serve-seconds", type=float, default=15.0)
     ap.add_argument("--serve-scale", type=int, default=23)
+    ap.add_argument("--aux", action="store_true", help="also time PPR (scale 22) and adsorption (scale 23)")
+    ap.add_argument("--aux-only", action="store_true", help="only the PPR and adsorption timings")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the profiled pass (for rocprofv3: it then sees exactly those launches)")
     args = ap.parse_args()
@@ -193,6 +255,9 @@ def main():
 
     if args.serve_only:
         print(json.dumps({"serving": serve_phase(args)}), flush=True)
+        return
+    if args.aux_only:
+        print(json.dumps(aux_phase(args)), flush=True)
         return
 
     t = time.time()
@@ -278,6 +343,10 @@ def main():
                       f"restated from src/sssp/base-perk/SSSP.cpp:88-119, one query per thread, {wall:.1f}s wall",
         }
         log(f"cpu baseline: {n_done} queries in {wall:.1f}s on {threads} threads")
+
+    if args.aux:
+        extra["aux"] = aux_phase(args)
+        log(f"aux: {extra['aux']}")
 
     if not args.skip_serve:
         g.close()

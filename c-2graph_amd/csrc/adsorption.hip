@@ -31,6 +31,17 @@ __global__ void ads_init_k(float* __restrict__ Y, int64_t n) {
         Y[i] = 0.0f;
 }
 
+// Rows split across items start from the injected image (their items add
+// into it atomically); whole rows are written outright by their item.
+__global__ void ads_split_init_k(float* __restrict__ Yn, const float* __restrict__ Yinj,
+                                 const int32_t* __restrict__ splitv, int64_t nsplit, int L) {
+    const int64_t n = nsplit * L;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t x = (int64_t)splitv[i / L] * L + i % L;
+        Yn[x] = Yinj[x];
+    }
+}
+
 __global__ void ads_seed_k(float* __restrict__ Y, const int32_t* __restrict__ sv, const int32_t* __restrict__ sl,
                            int nseed, int L, float val) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -86,7 +97,9 @@ struct AdsState {
     int64_t cap = 0;
     int32_t* seeds = nullptr;
     int64_t seedcap = 0;
-    std::vector<int32_t> whole;  // host: vertex ids written whole (not split)
+    int32_t* splitv = nullptr;  // vertices whose row is split across items
+    int64_t nsplit = 0;
+    HostStage out;              // result copies to the host
 };
 
 static std::mutex g_as_mu;
@@ -109,6 +122,8 @@ void ads_release(Graph* g) {
             if (s->items) (void)hipFree(s->items);
             for (auto* y : s->Y) if (y) (void)hipFree(y);
             if (s->seeds) (void)hipFree(s->seeds);
+            if (s->splitv) (void)hipFree(s->splitv);
+            s->out.release();
             delete s;
             g_as.erase(g_as.begin() + i);
             return;
@@ -119,18 +134,24 @@ static int ads_prepare(Graph* g, AdsState* s, int L, int nseed) {
     HIP_TRY(hipSetDevice(g->device));
     if (!s->items) {
         std::vector<int4> it;
+        std::vector<int32_t> split;
         it.reserve((size_t)g->V + g->E / kAdsChunk + 1);
         for (int32_t v = 0; v < g->V; ++v) {
             const int r0 = g->vl[v], r1 = g->vl[v + 1];
             if (r1 - r0 <= kAdsChunk) {
                 it.push_back(make_int4(v, r0, r1, 0));  // whole row (also deg 0: writes p_inj*Y0)
             } else {
+                split.push_back(v);
                 for (int e = r0; e < r1; e += kAdsChunk) it.push_back(make_int4(v, e, std::min(e + kAdsChunk, r1), 1));
             }
         }
         s->nitems = (int64_t)it.size();
         HIP_TRY(hipMalloc(&s->items, sizeof(int4) * std::max<int64_t>(s->nitems, 1)));
         HIP_TRY(hipMemcpy(s->items, it.data(), sizeof(int4) * s->nitems, hipMemcpyHostToDevice));
+        s->nsplit = (int64_t)split.size();
+        HIP_TRY(hipMalloc(&s->splitv, sizeof(int32_t) * std::max<int64_t>(s->nsplit, 1)));
+        if (s->nsplit)
+            HIP_TRY(hipMemcpy(s->splitv, split.data(), sizeof(int32_t) * s->nsplit, hipMemcpyHostToDevice));
     }
     const int64_t need = (int64_t)std::max(g->V, 1) * L;
     if (s->cap < need) {
@@ -198,7 +219,9 @@ int ads_run(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t* sl, in
         float* Yt = s->Y[cur];
         float* Yn = s->Y[cur ^ 1];
         // split rows accumulate atomically into p_inj*Y0
-        HIP_TRY(hipMemcpyAsync(Yn, Yinj, sizeof(float) * n, hipMemcpyDeviceToDevice, strm));
+        if (s->nsplit)
+            hipLaunchKernelGGL(ads_split_init_k, dim3((int)std::min<int64_t>(grid, (s->nsplit * L + 255) / 256)),
+                               dim3(256), 0, strm, Yn, Yinj, s->splitv, s->nsplit, L);
         if (L <= 16) launch_spmm<16>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
         else if (L <= 32) launch_spmm<32>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
         else launch_spmm<64>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
@@ -207,7 +230,7 @@ int ads_run(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t* sl, in
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(strm));
     const double t1 = now_ms();
-    if (y_out) HIP_TRY(hipMemcpy(y_out, s->Y[cur], sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (y_out && (rc = s->out.copy_out(y_out, s->Y[cur], sizeof(float) * n, strm))) return rc;
     if (st) {
         st->lane = C2G_LANE_GPU;
         st->iterations = iters;

@@ -28,6 +28,32 @@ namespace c2g {
                                hipGetErrorString(_e), __FILE__, __LINE__);             \
     } while (0)
 
+// Device-to-host copies into pageable memory through two pinned chunks: the
+// DMA of one chunk overlaps the parallel host copy of the previous one.
+// One instance per lane or state (not shared between threads).
+struct HostStage {
+    char* buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int copy_out(void* dst, const void* src, size_t bytes, hipStream_t strm);
+    void release();
+};
+
+struct Graph;
+
+// Per-query rows out of a lane-major result array [group][V][64] of 32-bit
+// words (query q = lane q%64 of group q/64): a device tile transpose of
+// kColStage columns at a time into [kColStage][V], one pinned transfer, and a
+// parallel host copy into outs[q] (null rows are skipped).  One instance per
+// lane or state (not shared between threads).
+constexpr int kColStage = 8;
+struct ColStage {
+    uint32_t* dev = nullptr;   // [kColStage][V] device staging
+    uint32_t* host = nullptr;  // pinned mirror
+    int64_t cap = 0;           // words per staging buffer
+    int copy_out(Graph* g, hipStream_t strm, const uint32_t* A, int32_t n, void* const* outs);
+    void release();
+};
+
 // Device workspace of single-query SSSP.
 struct SsspWs {
     int32_t* dist = nullptr;     // [V]
@@ -40,6 +66,7 @@ struct SsspWs {
     hipGraphExec_t exec = nullptr;
     int exec_iters = 0, exec_block = 0, exec_grid = 0, exec_chunk = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    HostStage out;               // distance copies to the host
 };
 
 // Device workspace of batched SSSP (64 queries per group, lane = query);

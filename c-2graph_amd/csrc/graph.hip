@@ -174,6 +174,7 @@ void device_release(Graph* g) {
     dfree(g->sssp.dist); dfree(g->sssp.stamp); dfree(g->sssp.q[0]);
     dfree(g->sssp.ctrl); dfree(g->sssp.hid);
     if (g->sssp.h_ctrl) (void)hipHostFree(g->sssp.h_ctrl);
+    g->sssp.out.release();
     for (auto& e : g->sssp.ev) if (e) (void)hipEventDestroy(e);
     dfree(g->batch.dist); dfree(g->batch.mask[0]); dfree(g->batch.hmask); dfree(g->batch.ctrl);
     if (g->batch.h_ctrl) (void)hipHostFree(g->batch.h_ctrl);
@@ -197,6 +198,111 @@ static int finish_open(c2g_graph* h, const c2g_opts* o, c2g_graph** out) {
     }
     *out = h;
     return C2G_OK;
+}
+
+constexpr size_t kStageChunk = (size_t)32 << 20;
+static_assert(64 % kColStage == 0, "column chunks must not span lane groups");
+
+// out[k][v] = A[v][l0 + k] for k < m <= kColStage: 64-vertex tiles staged
+// through LDS so both the column read (32-byte segments) and the row writes
+// are contiguous.
+__global__ void __launch_bounds__(256)
+extract_cols_k(const uint32_t* __restrict__ A, int64_t V, int l0, int m, uint32_t* __restrict__ out) {
+    __shared__ uint32_t tile[64][kColStage + 1];
+    const int t = threadIdx.x;
+    for (int64_t v0 = (int64_t)blockIdx.x * 64; v0 < V; v0 += (int64_t)gridDim.x * 64) {
+        for (int i = t; i < 64 * m; i += 256) {
+            const int r = i / m, c = i - r * m;
+            const int64_t v = v0 + r;
+            tile[r][c] = v < V ? A[v * 64 + l0 + c] : 0u;
+        }
+        __syncthreads();
+        for (int i = t; i < 64 * m; i += 256) {
+            const int k = i >> 6, r = i & 63;
+            const int64_t v = v0 + r;
+            if (v < V) out[(int64_t)k * V + v] = tile[r][k];
+        }
+        __syncthreads();
+    }
+}
+
+int ColStage::copy_out(Graph* g, hipStream_t strm, const uint32_t* A, int32_t n, void* const* outs) {
+    const int64_t V = g->V;
+    if (V == 0 || n <= 0) return C2G_OK;
+    if (cap < kColStage * V) {
+        release();
+        HIP_TRY(hipMalloc(&dev, sizeof(uint32_t) * kColStage * V));
+        HIP_TRY(hipHostMalloc((void**)&host, sizeof(uint32_t) * kColStage * V, hipHostMallocDefault));
+        cap = kColStage * V;
+    }
+    const int grid = (int)std::min<int64_t>((V + 63) / 64, (int64_t)std::max(1, g->num_cus) * 16);
+    for (int32_t q0 = 0; q0 < n; q0 += kColStage) {
+        const int gi = q0 / 64, l0 = q0 % 64;  // a chunk never spans two groups
+        const int m = std::min<int32_t>(kColStage, n - q0);
+        bool any = false;
+        for (int k = 0; k < m; ++k) any |= outs[q0 + k] != nullptr;
+        if (!any) continue;
+        hipLaunchKernelGGL(extract_cols_k, dim3(grid), dim3(256), 0, strm, A + (int64_t)gi * V * 64, V, l0, m, dev);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(host, dev, sizeof(uint32_t) * V * m, hipMemcpyDeviceToHost, strm));
+        HIP_TRY(hipStreamSynchronize(strm));
+        parallel_for((int64_t)m * V, [&](int64_t lo, int64_t hi, int) {
+            while (lo < hi) {
+                const int64_t k = lo / V, v0 = lo - k * V, len = std::min(hi - lo, V - v0);
+                if (outs[q0 + k]) memcpy((uint32_t*)outs[q0 + k] + v0, host + k * V + v0, sizeof(uint32_t) * len);
+                lo += len;
+            }
+        }, 0, (int64_t)1 << 18);
+    }
+    return C2G_OK;
+}
+
+void ColStage::release() {
+    if (dev) (void)hipFree(dev);
+    if (host) (void)hipHostFree(host);
+    dev = nullptr;
+    host = nullptr;
+    cap = 0;
+}
+
+
+int HostStage::copy_out(void* dst, const void* src, size_t bytes, hipStream_t strm) {
+    if (bytes == 0) return C2G_OK;
+    if (!buf[0]) {
+        for (int k = 0; k < 2; ++k) {
+            HIP_TRY(hipHostMalloc((void**)&buf[k], kStageChunk, hipHostMallocDefault));
+            HIP_TRY(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
+        }
+    }
+    const size_t nchunks = (bytes + kStageChunk - 1) / kStageChunk;
+    auto host_copy = [&](size_t c) {
+        const size_t off = c * kStageChunk, len = std::min(kStageChunk, bytes - off);
+        const char* from = buf[c & 1];
+        char* to = (char*)dst + off;
+        parallel_for((int64_t)len, [&](int64_t lo, int64_t hi, int) { memcpy(to + lo, from + lo, hi - lo); },
+                     0, (int64_t)1 << 20);
+    };
+    for (size_t c = 0; c < nchunks; ++c) {
+        const size_t off = c * kStageChunk, len = std::min(kStageChunk, bytes - off);
+        HIP_TRY(hipMemcpyAsync(buf[c & 1], (const char*)src + off, len, hipMemcpyDeviceToHost, strm));
+        HIP_TRY(hipEventRecord(ev[c & 1], strm));
+        if (c > 0) {
+            HIP_TRY(hipEventSynchronize(ev[(c - 1) & 1]));
+            host_copy(c - 1);
+        }
+    }
+    HIP_TRY(hipEventSynchronize(ev[(nchunks - 1) & 1]));
+    host_copy(nchunks - 1);
+    return C2G_OK;
+}
+
+void HostStage::release() {
+    for (int k = 0; k < 2; ++k) {
+        if (buf[k]) (void)hipHostFree(buf[k]);
+        if (ev[k]) (void)hipEventDestroy(ev[k]);
+        buf[k] = nullptr;
+        ev[k] = nullptr;
+    }
 }
 
 }  // namespace c2g

@@ -366,32 +366,6 @@ pprb_emit_k(const int32_t* __restrict__ row, uint64_t* __restrict__ mask, uint32
     }
 }
 
-constexpr int kPprStage = 8;  // query columns moved per extraction transfer
-static_assert(64 % kPprStage == 0, "extraction chunks must not span lane groups");
-
-// out[k][v] = A[v][l0 + k] for k < m <= kPprStage: 64-vertex tiles staged
-// through LDS so both the column read (32-byte segments) and the row writes
-// are contiguous.
-__global__ void __launch_bounds__(256)
-pprb_extract_k(const float* __restrict__ A, int64_t V, int l0, int m, float* __restrict__ out) {
-    __shared__ float tile[64][kPprStage + 1];
-    const int t = threadIdx.x;
-    for (int64_t v0 = (int64_t)blockIdx.x * 64; v0 < V; v0 += (int64_t)gridDim.x * 64) {
-        for (int i = t; i < 64 * m; i += 256) {
-            const int r = i / m, c = i - r * m;
-            const int64_t v = v0 + r;
-            tile[r][c] = v < V ? A[v * 64 + l0 + c] : 0.0f;
-        }
-        __syncthreads();
-        for (int i = t; i < 64 * m; i += 256) {
-            const int k = i >> 6, r = i & 63;
-            const int64_t v = v0 + r;
-            if (v < V) out[(int64_t)k * V + v] = tile[r][k];
-        }
-        __syncthreads();
-    }
-}
-
 // ------------------------------------------------------------------ host
 struct PprState {
     // single
@@ -410,8 +384,8 @@ struct PprState {
     uint64_t* imask = nullptr;
     int64_t itemcap = 0;
     int32_t* srcs = nullptr;
-    float* stage = nullptr;    // [kPprStage][V] device staging for extraction
-    float* h_stage = nullptr;  // pinned mirror
+    ColStage cols;             // per-query rows out of P and R
+    HostStage out;             // single-query p / r copies to the host
 };
 
 static std::mutex g_ps_mu;
@@ -433,10 +407,11 @@ void ppr_release(Graph* g) {
             PprState* s = g_ps[i].second;
             for (void* p : {(void*)s->p, (void*)s->r, (void*)s->m, (void*)s->q, (void*)s->ctrl, (void*)s->P,
                             (void*)s->R, (void*)s->M, (void*)s->RU, (void*)s->mask, (void*)s->tbits,
-                            (void*)s->items, (void*)s->imask, (void*)s->srcs, (void*)s->stage})
+                            (void*)s->items, (void*)s->imask, (void*)s->srcs})
                 if (p) (void)hipFree(p);
             if (s->h_ctrl) (void)hipHostFree(s->h_ctrl);
-            if (s->h_stage) (void)hipHostFree(s->h_stage);
+            s->cols.release();
+            s->out.release();
             delete s;
             g_ps.erase(g_ps.begin() + i);
             return;
@@ -547,8 +522,8 @@ static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, f
             }
     }
     const double t1 = now_ms();
-    if (p_out) HIP_TRY(hipMemcpy(p_out, s->p, sizeof(float) * V, hipMemcpyDeviceToHost));
-    if (r_out) HIP_TRY(hipMemcpy(r_out, s->r, sizeof(float) * V, hipMemcpyDeviceToHost));
+    if (p_out && (rc = s->out.copy_out(p_out, s->p, sizeof(float) * V, strm))) return rc;
+    if (r_out && (rc = s->out.copy_out(r_out, s->r, sizeof(float) * V, strm))) return rc;
     if (st) {
         st->lane = C2G_LANE_GPU;
         st->iterations = (int32_t)s->h_ctrl[P_ITER];
@@ -653,47 +628,14 @@ static int pprb_run(Graph* g, hipStream_t strm, const int32_t* srcs, int32_t n, 
     return C2G_OK;
 }
 
-// Copies columns of the [group][V][64] result arrays out to per-query host
-// rows: a tile transpose on the device into [kPprStage][V], one pinned
-// transfer, then a parallel host copy into the callers' buffers.
-static int pprb_extract(Graph* g, hipStream_t strm, int32_t n, float* const* p_outs, float* const* r_outs) {
-    PprState* s = pstate(g);
-    const int64_t V = g->V;
-    if (!s->stage) {
-        HIP_TRY(hipMalloc(&s->stage, sizeof(float) * (size_t)kPprStage * std::max<int64_t>(V, 1)));
-        HIP_TRY(hipHostMalloc((void**)&s->h_stage, sizeof(float) * (size_t)kPprStage * std::max<int64_t>(V, 1),
-                              hipHostMallocDefault));
-    }
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((V + 63) / 64, (int64_t)std::max(1, g->num_cus) * 16));
-    for (int pass = 0; pass < 2; ++pass) {
-        float* const* outs = pass ? r_outs : p_outs;
-        const float* dev = pass ? s->R : s->P;
-        if (!outs) continue;
-        for (int32_t q0 = 0; q0 < n; q0 += kPprStage) {
-            const int gi = q0 / 64, l0 = q0 % 64;  // a chunk never spans two groups
-            const int m = std::min<int32_t>(kPprStage, n - q0);
-            hipLaunchKernelGGL(pprb_extract_k, dim3(grid), dim3(256), 0, strm, dev + (int64_t)gi * V * 64, V, l0, m,
-                               s->stage);
-            HIP_TRY(hipGetLastError());
-            HIP_TRY(hipMemcpyAsync(s->h_stage, s->stage, sizeof(float) * V * m, hipMemcpyDeviceToHost, strm));
-            HIP_TRY(hipStreamSynchronize(strm));
-            parallel_for((int64_t)m * V, [&](int64_t lo, int64_t hi, int) {
-                while (lo < hi) {
-                    const int64_t k = lo / V, v0 = lo - k * V, len = std::min(hi - lo, V - v0);
-                    if (outs[q0 + k]) memcpy(outs[q0 + k] + v0, s->h_stage + k * V + v0, sizeof(float) * len);
-                    lo += len;
-                }
-            });
-        }
-    }
-    return C2G_OK;
-}
-
 int ppr_batch_to(Graph* g, hipStream_t strm, const int32_t* srcs, int32_t n, float alpha, float eps,
                  float* const* p_outs, float* const* r_outs, c2g_stats* st) {
     int rc = pprb_run(g, strm, srcs, n, alpha, eps, st);
     if (rc) return rc;
-    return pprb_extract(g, strm, n, p_outs, r_outs);
+    PprState* s = pstate(g);
+    if (p_outs && (rc = s->cols.copy_out(g, strm, (const uint32_t*)s->P, n, (void* const*)p_outs))) return rc;
+    if (r_outs && (rc = s->cols.copy_out(g, strm, (const uint32_t*)s->R, n, (void* const*)r_outs))) return rc;
+    return C2G_OK;
 }
 
 }  // namespace c2g

@@ -486,8 +486,10 @@ int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* 
     const double t1 = now_ms();
     int64_t c[C_N];
     memcpy(c, w.h_ctrl, sizeof c);
-    if (copy_out && dist_out)
-        HIP_TRY(hipMemcpy(dist_out, w.dist, sizeof(int32_t) * g->V, hipMemcpyDeviceToHost));
+    if (copy_out && dist_out) {
+        const int rc = w.out.copy_out(dist_out, w.dist, sizeof(int32_t) * g->V, g->stream);
+        if (rc) return rc;
+    }
     if (count_out) {
         // Remaining frontier (budgeted runs): distinct vertices of the current queue.
         int64_t n = c[C_CUR];

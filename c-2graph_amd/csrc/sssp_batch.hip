@@ -390,6 +390,7 @@ struct BatchState {
     int32_t* reach = nullptr;   // [64*groups]
     int64_t* medges = nullptr;  // [64*groups]
     hipEvent_t ev[2] = {nullptr, nullptr};
+    ColStage cols;              // per-query distance rows to the host
 };
 
 static std::mutex g_bs_mu;
@@ -414,6 +415,7 @@ void batch_release(Graph* g) {
             (void)hipFree(s->imask); (void)hipFree(s->srcs); (void)hipFree(s->sums);
             (void)hipFree(s->reach); (void)hipFree(s->medges);
             for (auto& e : s->ev) if (e) (void)hipEventDestroy(e);
+            s->cols.release();
             delete s;
             g_bs.erase(g_bs.begin() + i);
             return;
@@ -653,19 +655,10 @@ int sssp_batch_group(Graph* g, const int32_t* srcs, int32_t n, int32_t* dist_out
     }
     if (dist_out) {
         // Device layout [g][V][64] -> caller layout [q][V].
-        const int64_t V = g->V;
-        std::vector<int32_t> blk((size_t)V * 64);
-        for (int gi = 0; gi < groups; ++gi) {
-            HIP_TRY(hipMemcpy(blk.data(), g->batch.dist + (int64_t)gi * V * 64, sizeof(int32_t) * V * 64,
-                              hipMemcpyDeviceToHost));
-            const int lanes = std::min(64, n - 64 * gi);
-            parallel_for(V, [&](int64_t lo, int64_t hi, int) {
-                for (int l = 0; l < lanes; ++l) {
-                    int32_t* out = dist_out + (int64_t)(64 * gi + l) * V;
-                    for (int64_t v = lo; v < hi; ++v) out[v] = blk[(size_t)v * 64 + l];
-                }
-            });
-        }
+        std::vector<void*> outs((size_t)n);
+        for (int32_t q = 0; q < n; ++q) outs[q] = dist_out + (int64_t)q * g->V;
+        if ((rc = bstate(g)->cols.copy_out(g, g->stream, (const uint32_t*)g->batch.dist, n, outs.data())))
+            return rc;
     }
     return C2G_OK;
 }
@@ -791,8 +784,13 @@ int bserve_extract(Graph* g, hipStream_t strm, const int* lanes, int n, int32_t*
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(ss->h_out, ss->out, sizeof(int32_t) * V * m, hipMemcpyDeviceToHost, strm));
         HIP_TRY(hipStreamSynchronize(strm));
-        for (int k = 0; k < m; ++k)
-            if (outs[b + k]) memcpy(outs[b + k], ss->h_out + (int64_t)k * V, sizeof(int32_t) * V);
+        parallel_for((int64_t)m * V, [&](int64_t lo, int64_t hi, int) {
+            while (lo < hi) {
+                const int64_t k = lo / V, v0 = lo - k * V, len = std::min(hi - lo, V - v0);
+                if (outs[b + k]) memcpy(outs[b + k] + v0, ss->h_out + k * V + v0, sizeof(int32_t) * len);
+                lo += len;
+            }
+        }, 0, (int64_t)1 << 18);
     }
     return C2G_OK;
 }
