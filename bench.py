@@ -242,6 +242,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=15.0)
     ap.add_argument("--serve-scale", type=int, default=23)
     ap.add_argument("--aux", action="store_true", help="also time PPR (scale 22) and adsorption (scale 23)")
+    ap.add_argument("--pcie", action="store_true",
+                    help="also time one batched step with every distance vector copied to host memory")
     ap.add_argument("--aux-only", action="store_true", help="only the PPR and adsorption timings")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the profiled pass (for rocprofv3: it then sees exactly those launches)")
@@ -316,6 +318,19 @@ def main():
         if not match:
             bad = np.flatnonzero((sums_e != sums_b) | (reach_e != reach_b))
             log(f"MISMATCH at queries {bad[:10].tolist()}")
+
+    if args.pcie and times:
+        t0 = time.perf_counter()
+        D, st_p = g.sssp_batch(srcs)
+        tp = time.perf_counter() - t0
+        fin = D != np.iinfo(np.int32).max
+        match = bool(np.array_equal(np.where(fin, D, 0).sum(axis=1, dtype=np.int64), sums_b)
+                     and np.array_equal(fin.sum(axis=1), reach_b))
+        extra["pcie_inclusive"] = {
+            "queries_per_s": round(args.queries / tp, 3), "ms": round(tp * 1e3, 1),
+            "device_ms": round(st_p.ms, 1), "bytes_to_host": int(D.nbytes), "checksums_equal": match}
+        log(f"pcie-inclusive step: {tp * 1e3:.1f} ms ({D.nbytes / 1e9:.1f} GB to host), checksums equal: {match}")
+        del D, fin
 
     if not args.skip_profile:
         g.set_tune(profile=1)

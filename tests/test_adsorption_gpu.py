@@ -61,3 +61,14 @@ def test_zero_iterations_and_errors(gpu):
             g.adsorption(sv, sl, L=2, p_inj=0.5, p_cont=0.5, p_abnd=0.5)
         with pytest.raises(c2g.C2GError):
             g.adsorption(np.array([100], np.int32), np.array([0], np.int32), L=2)
+
+
+def test_multi_chunk_result_copy(gpu):
+    # 524,288 x 40 labels x 4 B = 80 MiB of Y: the result leaves the device in
+    # two pinned 32 MiB chunks and a partial third that reuses the first buffer
+    G = rmat_graph(19, 6, wmax=0, want_f=True)
+    sv, sl = _seeds(G["V"], 40, 0.01, 6)
+    with c2g.Graph.from_csr(G["vlist"], G["elist"], flist=G["flist"], device=0) as g:
+        Y, _ = g.adsorption(sv, sl, L=40, iters=4)
+    assert 2 * (32 << 20) < Y.nbytes < 3 * (32 << 20)
+    _check(Y, oracle.adsorption(G["vlist"], G["elist"], G["flist"], sv, sl, L=40, iters=4))

@@ -123,3 +123,26 @@ def test_budget_and_migration(gpu, delta):
             assert np.array_equal(d2, ref), budget
             d3, st3 = g.sssp_migrate(src, budget)
             assert np.array_equal(d3, ref)
+
+
+@pytest.mark.slow
+def test_multi_chunk_distance_copy(gpu):
+    # binary tree on V = 17,000,000 vertices (i -> 2i+1, 2i+2, weight 1 + c % 7):
+    # 68 MB of distances leave the device in two pinned 32 MiB chunks and a
+    # partial third that reuses the first buffer; every distance has a closed form
+    V = 17_000_000
+    nchild = np.clip(V - 1 - 2 * np.arange(V, dtype=np.int64), 0, 2)
+    vl = np.concatenate([[0], np.cumsum(nchild)]).astype(np.int32)
+    el = np.arange(1, V, dtype=np.int32)
+    wl = (1 + el % 7).astype(np.int32)
+    want = np.zeros(V, np.int64)
+    lo = 1
+    while lo < V:  # level by level: parents come before children
+        hi = min(V, 2 * lo + 1)
+        c = np.arange(lo, hi)
+        want[c] = want[(c - 1) // 2] + 1 + c % 7
+        lo = hi
+    with c2g.Graph.from_csr(vl, el, wl, device=0) as g:
+        d, st = g.sssp(0)
+    assert 2 * (32 << 20) < d.nbytes < 3 * (32 << 20)
+    assert np.array_equal(d, want.astype(np.int32))
