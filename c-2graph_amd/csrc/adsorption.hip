@@ -90,6 +90,71 @@ ads_spmm_k(const int2* __restrict__ adjf, const int4* __restrict__ items, int64_
     }
 }
 
+// The same product with 4 labels per lane (float4 rows, L % 4 == 0): a row
+// of L labels takes G = L/4 lanes (rounded up to a power of two), so one wave
+// gathers 64/G rows per instruction, 16 bytes per lane.  Items are sorted by
+// edge count, so the rows of a wave run for about the same number of steps.
+template <int G>
+__global__ void __launch_bounds__(256)
+ads_spmm4_k(const int2* __restrict__ adjf, const int4* __restrict__ items, int64_t nitems,
+            const float4* __restrict__ Yt, float4* __restrict__ Yn, const float4* __restrict__ Yinj, int L4,
+            float p_cont) {
+    constexpr int PER = 64 / G;
+    const int lane = lane_id();
+    const int sub = lane / G, l = lane % G;
+    const bool on = l < L4;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t base = wave * PER; base < nitems; base += nwaves * PER) {
+        const int64_t i = base + sub;
+        if (i >= nitems) continue;
+        const int4 it = items[i];
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        int e = it.y;
+        for (; e + 8 <= it.z; e += 8) {
+            int2 a[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) a[k] = adjf[e + k];
+            float4 y[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) y[k] = on ? Yt[(int64_t)a[k].x * L4 + l] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const float f = __int_as_float(a[k].y);
+                acc.x += f * y[k].x;
+                acc.y += f * y[k].y;
+                acc.z += f * y[k].z;
+                acc.w += f * y[k].w;
+            }
+        }
+        for (; e < it.z; ++e) {
+            const int2 a = adjf[e];
+            if (on) {
+                const float f = __int_as_float(a.y);
+                const float4 y = Yt[(int64_t)a.x * L4 + l];
+                acc.x += f * y.x;
+                acc.y += f * y.y;
+                acc.z += f * y.z;
+                acc.w += f * y.w;
+            }
+        }
+        if (on) {
+            const int64_t x = (int64_t)it.x * L4 + l;
+            if (it.w) {
+                float* yn = reinterpret_cast<float*>(&Yn[x]);
+                unsafeAtomicAdd(&yn[0], p_cont * acc.x);
+                unsafeAtomicAdd(&yn[1], p_cont * acc.y);
+                unsafeAtomicAdd(&yn[2], p_cont * acc.z);
+                unsafeAtomicAdd(&yn[3], p_cont * acc.w);
+            } else {
+                const float4 b = Yinj[x];
+                Yn[x] = make_float4(b.x + p_cont * acc.x, b.y + p_cont * acc.y, b.z + p_cont * acc.z,
+                                    b.w + p_cont * acc.w);
+            }
+        }
+    }
+}
+
 struct AdsState {
     int4* items = nullptr;
     int64_t nitems = 0;
@@ -145,6 +210,9 @@ static int ads_prepare(Graph* g, AdsState* s, int L, int nseed) {
                 for (int e = r0; e < r1; e += kAdsChunk) it.push_back(make_int4(v, e, std::min(e + kAdsChunk, r1), 1));
             }
         }
+        // Degree binning: longest items first, so each wave's rows take about
+        // the same number of steps (stable: equal lengths keep vertex order).
+        std::stable_sort(it.begin(), it.end(), [](const int4& a, const int4& b) { return a.z - a.y > b.z - b.y; });
         s->nitems = (int64_t)it.size();
         HIP_TRY(hipMalloc(&s->items, sizeof(int4) * std::max<int64_t>(s->nitems, 1)));
         HIP_TRY(hipMemcpy(s->items, it.data(), sizeof(int4) * s->nitems, hipMemcpyHostToDevice));
@@ -173,6 +241,14 @@ static void launch_spmm(Graph* g, hipStream_t strm, AdsState* s, const float* Yt
     const int grid = std::max(1, g->num_cus) * g->tune.grid_per_cu;
     hipLaunchKernelGGL(ads_spmm_k<G>, dim3(grid), dim3(256), 0, strm, g->d_adjf, s->items, s->nitems, Yt, Yn,
                        Yinj, L, pc);
+}
+
+template <int G>
+static void launch_spmm4(Graph* g, hipStream_t strm, AdsState* s, const float* Yt, float* Yn, const float* Yinj,
+                         int L, float pc) {
+    const int grid = std::max(1, g->num_cus) * g->tune.grid_per_cu;
+    hipLaunchKernelGGL(ads_spmm4_k<G>, dim3(grid), dim3(256), 0, strm, g->d_adjf, s->items, s->nitems,
+                       (const float4*)Yt, (float4*)Yn, (const float4*)Yinj, L / 4, pc);
 }
 
 int ads_run(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t* sl, int32_t nseed, int32_t L, float p_inj,
@@ -222,7 +298,13 @@ int ads_run(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t* sl, in
         if (s->nsplit)
             hipLaunchKernelGGL(ads_split_init_k, dim3((int)std::min<int64_t>(grid, (s->nsplit * L + 255) / 256)),
                                dim3(256), 0, strm, Yn, Yinj, s->splitv, s->nsplit, L);
-        if (L <= 16) launch_spmm<16>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
+        if (L % 4 == 0) {  // 16-byte rows per lane
+            if (L <= 4) launch_spmm4<1>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
+            else if (L <= 8) launch_spmm4<2>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
+            else if (L <= 16) launch_spmm4<4>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
+            else if (L <= 32) launch_spmm4<8>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
+            else launch_spmm4<16>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
+        } else if (L <= 16) launch_spmm<16>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
         else if (L <= 32) launch_spmm<32>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
         else launch_spmm<64>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
         cur ^= 1;
