@@ -48,11 +48,14 @@ def pick_sources(vlist, n, seed):
 
 
 def cpu_baseline(G, srcs, budget_s, threads):
-    """Restated reference CPU lane, P independent query threads, bounded sample."""
+    """Restated reference CPU lane (frontier Bellman-Ford), over a bounded
+    sample: P independent query threads (queries/s and GTEPS), then one query
+    alone on one thread (the reference's lane is a single pthread)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
     vl, el, wl = G["vlist"], G["elist"], G["wlist"]
+    deg = np.diff(vl).astype(np.int64)
     oracle.lib()
     # interleave hub and out-degree-1 sources so the sample has both kinds
     half = len(srcs) // 2
@@ -70,8 +73,9 @@ def cpu_baseline(G, srcs, budget_s, threads):
                 s = order[nxt[0]]
                 nxt[0] += 1
             d, rounds, rel = oracle.bellman_ford(vl, el, wl, int(s))
+            m_s = int(deg[d != np.iinfo(np.int32).max].sum())
             with lock:
-                done.append((int(s), rel))
+                done.append((int(s), rel, m_s))
 
     ts = [threading.Thread(target=worker) for _ in range(threads)]
     for t in ts:
@@ -79,7 +83,49 @@ def cpu_baseline(G, srcs, budget_s, threads):
     for t in ts:
         t.join()
     wall = time.time() - t0
-    return len(done), wall
+    t1 = time.time()
+    oracle.bellman_ford(vl, el, wl, int(srcs[0]))
+    single = time.time() - t1
+    return {"n": len(done), "wall": wall, "gteps": sum(x[2] for x in done) / wall / 1e9,
+            "single_s": single, "single_src": int(srcs[0])}
+
+
+def cpu_baseline_ppr(G, srcs, budget_s, threads, alpha, eps):
+    """Restated reference CPU push lane (forward_push, src/ppr/w-base-perk/
+    PPR.cpp:178-236, as restated in oracle/): P query threads over a bounded
+    sample, then one query alone on one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    oracle.lib()
+    vl, el, fl = G["vlist"], G["elist"], G["flist"]
+    lock = threading.Lock()
+    nxt, done = [0], [0]
+    t0 = time.time()
+
+    def worker():
+        while True:
+            with lock:
+                if nxt[0] >= len(srcs) or time.time() - t0 > budget_s:
+                    return
+                s = srcs[nxt[0]]
+                nxt[0] += 1
+            oracle.ppr_push(vl, el, fl, int(s), alpha=alpha, eps=eps)
+            with lock:
+                done[0] += 1
+
+    ts = [threading.Thread(target=worker) for _ in range(threads)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    wall = time.time() - t0
+    t1 = time.time()
+    oracle.ppr_push(vl, el, fl, int(srcs[0]), alpha=alpha, eps=eps)
+    return {"value": round(done[0] / wall, 3), "unit": "queries/s", "cores": threads, "kind": "port",
+            "single_thread_ms_per_query": round((time.time() - t1) * 1e3, 1),
+            "sample": f"{done[0]} of the {len(srcs)} batched sources in {wall:.1f}s; forward push restated from "
+                      "src/ppr/w-base-perk/PPR.cpp:178-236 (threshold eps*outdeg), one query per thread"}
 
 
 def measured_traffic(kernel, args):
@@ -130,7 +176,11 @@ def aux_phase(args):
         "ms_wall_with_p_r_to_host": round(wall * 1e3, 1), "rounds": stb.iterations,
         "vertex_group_activations": stb.frontier_total, "csr_edges": stb.edges_relaxed}
     g3.close()
-    del G3, P, R
+    del P, R
+    if not args.skip_cpu:
+        out["ppr_cpu_baseline"] = cpu_baseline_ppr(G3, ps, min(args.cpu_seconds, 10.0), min(16, os.cpu_count() or 1),
+                                                   0.15, 1e-7)
+    del G3
 
     G5 = c2g.rmat(23, 16, 0.57, 0.19, 0.19, seed=5, wmax=255, want_f=True)
     g5 = c2g.Graph.from_csr(G5["vlist"], G5["elist"], G5["wlist"], G5["flist"], device=0)
@@ -271,6 +321,7 @@ def main():
         print(json.dumps({"serving": serve_phase(args)}), flush=True)
         return
     if args.aux_only:
+        args.skip_cpu = True  # kernels only (the profiled runs)
         print(json.dumps(aux_phase(args)), flush=True)
         return
 
@@ -364,13 +415,15 @@ def main():
 
     if not args.skip_cpu:
         threads = min(16, os.cpu_count() or 1)
-        n_done, wall = cpu_baseline(G, srcs, args.cpu_seconds, threads)
+        cb = cpu_baseline(G, srcs, args.cpu_seconds, threads)
         extra["cpu_baseline"] = {
-            "value": round(n_done / wall, 4), "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": f"{n_done} of the 256 sources (hub/out-degree-1 interleaved), frontier Bellman-Ford "
-                      f"restated from src/sssp/base-perk/SSSP.cpp:88-119, one query per thread, {wall:.1f}s wall",
+            "value": round(cb["n"] / cb["wall"], 4), "unit": "queries/s", "cores": threads, "kind": "port",
+            "gteps": round(cb["gteps"], 4), "single_thread_s_per_query": round(cb["single_s"], 2),
+            "sample": f"{cb['n']} of the 256 sources (hub/out-degree-1 interleaved), frontier Bellman-Ford "
+                      f"restated from src/sssp/base-perk/SSSP.cpp:88-119, one query per thread, "
+                      f"{cb['wall']:.1f}s wall; then source {cb['single_src']} alone on one thread",
         }
-        log(f"cpu baseline: {n_done} queries in {wall:.1f}s on {threads} threads")
+        log(f"cpu baseline: {cb}")
 
     if args.aux:
         extra["aux"] = aux_phase(args)
