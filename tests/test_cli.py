@@ -81,6 +81,14 @@ def test_sssp_tools(gpu, ds, tmp_path):
     indeg = np.bincount(G["elist"], minlength=G["V"])
     assert deg[s] >= 10 and indeg[s] >= 10
     assert np.array_equal(c2g.read_i32(out), oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], s))
+    # sssp-purn-M: pruned graph, GPU budget of 2 rounds then the host lane;
+    # source rule of sssp/purn-perk/Main.cu (10 <= |hidden list| <= 1000)
+    hv = np.diff(c2g.read_i32(ds + "/sssp/hvlist.bin"))
+    assert np.any((hv >= 10) & (hv <= 1000))
+    p = run("sssp-purn-M", ds, 0, 2, "--reps", 1, "--out", out)
+    s = int(p.stdout.split("源顶点source = ")[1].split()[0])
+    assert 10 <= hv[s] <= 1000 and np.all((hv[:s] < 10) | (hv[:s] > 1000))
+    assert np.array_equal(c2g.read_i32(out), oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], s))
 
 
 @pytest.mark.gpu
