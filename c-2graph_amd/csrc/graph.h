@@ -145,6 +145,15 @@ int bserve_admit(Graph* g, hipStream_t strm, int lane, int32_t src);
 int bserve_step(Graph* g, hipStream_t strm, uint64_t* lanes_with_work);
 int bserve_extract(Graph* g, hipStream_t strm, const int* lanes, int n, int32_t* const* outs);
 
+// Server lane of the single-query kernels run side by side: `slots` queries
+// (at most 64), query-major state, items tagged with the slot.  Used while
+// few queries are in flight, when the 64-lane rows above would be mostly idle.
+int qserve_init(Graph* g, hipStream_t strm, int slots);
+int qserve_admit(Graph* g, hipStream_t strm, int slot, int32_t src);
+int qserve_step(Graph* g, hipStream_t strm, uint64_t* slots_with_work);
+int qserve_extract(Graph* g, hipStream_t strm, int slot, int32_t* out);
+void qserve_release(Graph* g);
+
 }  // namespace c2g
 
 struct c2g_graph {

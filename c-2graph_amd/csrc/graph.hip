@@ -168,6 +168,7 @@ void device_release(Graph* g) {
     ppr_release(g);
     ads_release(g);
     serve_release(g);
+    qserve_release(g);
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     if (g->sssp.exec) (void)hipGraphExecDestroy(g->sssp.exec);
     if (g->batch.exec) (void)hipGraphExecDestroy(g->batch.exec);

@@ -5,10 +5,11 @@
 // two GPU streams and one pthread CPU worker fed through a pinned ring
 // (src/sssp/purn-perk/SSSP.cpp:18-84); it serves the same query repeatedly and
 // moves work by queue length.  Here the same roles become lanes of a server:
-//   * SSSP lane (stream A): one 64-lane group of the batched SSSP kernels.  A
-//     query is admitted into a free lane between rounds (its seed joins the
-//     next round's frontier) and retires as soon as its lane has no queued
-//     work; its distance column is copied out and reset.  Requests that
+//   * SSSP lane (stream A): a pool of query-major slots (the single-query
+//     kernels run side by side) and one 64-lane group of the batched kernels
+//     for the overflow.  A query is admitted into a free slot or lane between
+//     rounds (its seed joins the next round's work) and retires as soon as
+//     it has no queued work; its distances are copied out.  Requests that
 //     arrive while others run therefore overlap with them on the device.
 //   * PPR lane (stream B): the PPR requests queued when the lane frees up
 //     (up to 64) run as one batched push, lane = query.
@@ -91,60 +92,104 @@ static void finish(c2g_server* s, Request& r, int status, int lane) {
     r.done = true;
 }
 
+// SSSP lane.  Two continuously batched pools share one stream: the slot pool
+// (query-major, the single-query kernels side by side) takes requests while
+// it has a free slot; the 64-lane group (lane-major rows) takes the overflow.
+// With few queries in flight a 256-byte lane row would carry mostly idle
+// lanes, so light load stays in the slots; heavy load fills the lanes, where
+// one row read serves many queries.
 static void sssp_worker(c2g_server* s) {
     Graph* g = s->g;
     (void)hipSetDevice(g->device);
-    int rc = bserve_init(g, s->s_sssp);
-    int64_t lane_req[64];
-    for (auto& x : lane_req) x = -1;
+    const int nslots = s->o.sssp_slots < 0 ? 0 : std::min(64, s->o.sssp_slots == 0 ? 16 : s->o.sssp_slots);
+    const int nlanes = std::min(64, std::max(1, s->o.max_batch));
+    int rc = nslots > 0 ? qserve_init(g, s->s_sssp, nslots) : C2G_OK;
+    bool lanes_ready = false;
+    int64_t slot_req[64], lane_req[64];
+    for (int i = 0; i < 64; ++i) slot_req[i] = lane_req[i] = -1;
+    auto busy = [&](const int64_t* a, int n) {
+        for (int i = 0; i < n; ++i)
+            if (a[i] >= 0) return true;
+        return false;
+    };
     while (true) {
-        std::vector<std::pair<int, int32_t>> admit;
+        std::vector<std::pair<int, int32_t>> admit_s, admit_l;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            bool busy = false;
-            for (auto x : lane_req) busy |= x >= 0;
-            if (!busy)
+            if (!busy(slot_req, nslots) && !busy(lane_req, nlanes))
                 s->cv_work.wait(lk, [&] { return s->stop || !s->q_sssp.empty(); });
             if (s->stop) break;
-            for (int l = 0; l < std::min(64, std::max(1, s->o.max_batch)) && !s->q_sssp.empty(); ++l) {
-                if (lane_req[l] >= 0) continue;
+            int si = 0, li = 0;
+            while (!s->q_sssp.empty()) {
+                while (si < nslots && slot_req[si] >= 0) ++si;
+                while (li < nlanes && lane_req[li] >= 0) ++li;
+                if (si >= nslots && li >= nlanes) break;
                 Request& r = s->reqs[s->q_sssp.front()];
                 if (s->used + r.bytes > s->budget) break;  // wait for memory
                 s->q_sssp.pop_front();
                 s->used += r.bytes;
                 r.t_admit = now_ms() - s->t0;
-                lane_req[l] = r.id;
-                admit.emplace_back(l, r.src);
+                if (si < nslots) {
+                    slot_req[si] = r.id;
+                    admit_s.emplace_back(si, r.src);
+                } else {
+                    lane_req[li] = r.id;
+                    admit_l.emplace_back(li, r.src);
+                }
             }
         }
-        {
-            bool busy = false;
-            for (auto x : lane_req) busy |= x >= 0;
-            if (!busy) {  // head request waits for HBM budget held by the aux lane
-                std::unique_lock<std::mutex> lk(s->mu);
-                s->cv_work.wait_for(lk, std::chrono::milliseconds(1));
-                continue;
-            }
+        const bool sbusy = busy(slot_req, nslots), lbusy = busy(lane_req, nlanes);
+        if (!sbusy && !lbusy) {  // head request waits for HBM budget held by another lane
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait_for(lk, std::chrono::milliseconds(1));
+            continue;
         }
-        for (auto& a : admit)
+        if (!admit_l.empty() && !lanes_ready && rc == C2G_OK) {
+            rc = bserve_init(g, s->s_sssp);
+            lanes_ready = rc == C2G_OK;
+        }
+        for (auto& a : admit_s)
+            if (rc == C2G_OK) rc = qserve_admit(g, s->s_sssp, a.first, a.second);
+        for (auto& a : admit_l)
             if (rc == C2G_OK) rc = bserve_admit(g, s->s_sssp, a.first, a.second);
-        uint64_t work = 0;
-        if (rc == C2G_OK) rc = bserve_step(g, s->s_sssp, &work);
-        std::vector<int> lanes;
+        uint64_t swork = 0, lwork = 0;
+        if (sbusy && rc == C2G_OK) rc = qserve_step(g, s->s_sssp, &swork);
+        if (lbusy && rc == C2G_OK) rc = bserve_step(g, s->s_sssp, &lwork);
+        // Retire queries with no queued work (all of them after an error).
+        std::vector<int> done_s, lanes;
         std::vector<int32_t*> outs;
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            for (int l = 0; l < 64; ++l)
-                if (lane_req[l] >= 0 && (rc != C2G_OK || !((work >> l) & 1ull))) {
+            for (int i = 0; i < nslots; ++i)
+                if (slot_req[i] >= 0 && (rc != C2G_OK || !((swork >> i) & 1ull))) {
+                    s->reqs[slot_req[i]].result.resize(sizeof(int32_t) * g->V);
+                    done_s.push_back(i);
+                }
+            for (int l = 0; l < nlanes; ++l)
+                if (lane_req[l] >= 0 && (rc != C2G_OK || !((lwork >> l) & 1ull))) {
                     Request& r = s->reqs[lane_req[l]];
                     r.result.resize(sizeof(int32_t) * g->V);
                     lanes.push_back(l);
                     outs.push_back((int32_t*)r.result.data());
                 }
         }
+        for (int i : done_s) {
+            Request* r;
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                r = &s->reqs[slot_req[i]];
+            }
+            if (rc == C2G_OK) rc = qserve_extract(g, s->s_sssp, i, (int32_t*)r->result.data());
+        }
         if (!lanes.empty() && rc == C2G_OK) rc = bserve_extract(g, s->s_sssp, lanes.data(), (int)lanes.size(), outs.data());
-        if (!lanes.empty()) {
+        if (!done_s.empty() || !lanes.empty()) {
             std::lock_guard<std::mutex> lk(s->mu);
+            for (int i : done_s) {
+                Request& r = s->reqs[slot_req[i]];
+                s->used -= r.bytes;
+                finish(s, r, rc, C2G_LANE_GPU);
+                slot_req[i] = -1;
+            }
             for (int l : lanes) {
                 Request& r = s->reqs[lane_req[l]];
                 s->used -= r.bytes;
@@ -152,7 +197,7 @@ static void sssp_worker(c2g_server* s) {
                 lane_req[l] = -1;
             }
             s->cv_done.notify_all();
-            s->cv_work.notify_all();  // memory freed: the aux lane may admit
+            s->cv_work.notify_all();  // memory freed: other lanes may admit
         }
         if (rc != C2G_OK) {
             std::lock_guard<std::mutex> lk(s->mu);

@@ -53,6 +53,7 @@ enum : int {
     C_FMIN = 12,  // smallest deferred distance this round
     C_NEAR = 13,  // vertices expanded this round
     C_FAR = 14,   // 1 while deferred vertices remain
+    C_SLOTS = 15, // slot pools: bit s set when slot s has items queued for the next round
     C_N = 16
 };
 
@@ -132,12 +133,15 @@ __device__ __forceinline__ void relax_edge(const int2* __restrict__ adj, const i
 // and walks the concatenated edges 128 at a time (two 64-edge steps whose
 // loads are in flight together); improved targets are marked in the
 // activation bitmap with fire-and-forget atomicOr (no queue atomics here).
+// Items are int4(u, e_begin, e_end, kind | slot << 1): a single query is
+// slot 0; a slot pool runs several queries side by side, slot s owning
+// dist[s*V..], hid[s*V..] and the bitmap words [s*nwp..] of each map.
 template <bool PRUNED>
 __global__ void __launch_bounds__(256)
 sssp_relax_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
              int32_t* __restrict__ dist, uint32_t* __restrict__ bits,
              uint32_t* __restrict__ hbits, uint8_t* __restrict__ hid,
-             const int4* __restrict__ q, int64_t* __restrict__ ctrl) {
+             const int4* __restrict__ q, int64_t* __restrict__ ctrl, int64_t V, int64_t nwp) {
     const int64_t n = ctrl[C_CUR];
     const int lane = lane_id();
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -146,13 +150,13 @@ sssp_relax_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
 
     for (int64_t base = wave * kWave; base < n; base += nwaves * kWave) {
         const int64_t i = base + lane;
-        int eb = 0, cnt = 0, du = INT32_MAX, kind = 0;
+        int eb = 0, cnt = 0, du = INT32_MAX, kw = 0;
         if (i < n) {
             const int4 it = q[i];
             eb = it.y;
             cnt = it.z - it.y;
-            kind = it.w;
-            du = dist[it.x];
+            kw = it.w;
+            du = dist[(int64_t)(kw >> 1) * V + it.x];
         }
         int tot;
         const int pre = wave_excl_scan(cnt, &tot);
@@ -162,13 +166,19 @@ sssp_relax_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
             const int e0 = __shfl(eb, o0, 64) + (k0 - __shfl(pre, o0, 64));
             const int e1 = __shfl(eb, o1, 64) + (k1 - __shfl(pre, o1, 64));
             const int d0 = __shfl(du, o0, 64), d1 = __shfl(du, o1, 64);
-            const int kd0 = PRUNED ? __shfl(kind, o0, 64) : 0;
-            const int kd1 = PRUNED ? __shfl(kind, o1, 64) : 0;
+            const int w0 = __shfl(kw, o0, 64), w1 = __shfl(kw, o1, 64);
+            const int kd0 = PRUNED ? (w0 & 1) : 0;
+            const int kd1 = PRUNED ? (w1 & 1) : 0;
             int2 a0 = make_int2(0, 0), a1 = make_int2(0, 0);
             if (k0 < tot) a0 = (PRUNED && kd0) ? hadj[e0] : adj[e0];
             if (k1 < tot) a1 = (PRUNED && kd1) ? hadj[e1] : adj[e1];
-            if (k0 < tot) relax_edge<PRUNED>(adj, hadj, dist, bits, hbits, hid, a0, d0, kd0, impr);
-            if (k1 < tot) relax_edge<PRUNED>(adj, hadj, dist, bits, hbits, hid, a1, d1, kd1, impr);
+            const int64_t s0 = w0 >> 1, s1 = w1 >> 1;
+            if (k0 < tot)
+                relax_edge<PRUNED>(adj, hadj, dist + s0 * V, bits + s0 * nwp, PRUNED ? hbits + s0 * nwp : hbits,
+                                   PRUNED ? hid + s0 * V : hid, a0, d0, kd0, impr);
+            if (k1 < tot)
+                relax_edge<PRUNED>(adj, hadj, dist + s1 * V, bits + s1 * nwp, PRUNED ? hbits + s1 * nwp : hbits,
+                                   PRUNED ? hid + s1 * V : hid, a1, d1, kd1, impr);
         }
     }
     impr = wave_sum64(impr);
@@ -184,6 +194,7 @@ sssp_relax_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
             ctrl[C_ITEMS] += n;
             ctrl[C_CUR] = 0;
             ctrl[C_DONE] = 0;
+            ctrl[C_SLOTS] = 0;
         }
     }
 }
@@ -200,17 +211,19 @@ __global__ void __launch_bounds__(256)
 sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
               uint32_t* __restrict__ bits, uint32_t* __restrict__ hbits, int4* __restrict__ q,
               int64_t* __restrict__ ctrl, int64_t V, int chunk, int64_t qcap,
-              const int32_t* __restrict__ dist, int delta) {
+              const int32_t* __restrict__ dist, int delta, int slots, int64_t nwp) {
     const int lane = lane_id();
     const int64_t nw = (V + 31) >> 5;
-    const int64_t nwr = (nw + 63) & ~(int64_t)63;  // uniform trip count per wave
+    const int64_t nwr = (int64_t)slots * nwp;  // nwp is a multiple of 64: a wave stays in one slot
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int32_t T = (int32_t)ctrl[C_THR];
     int64_t front = 0, edges = 0, hedges = 0, near_n = 0;
     int32_t fmin = INT32_MAX;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nwr; i += stride) {
+        const int slot = (int)(i / nwp);
+        const int64_t wi = i - (int64_t)slot * nwp;
         uint32_t wl = 0, wh = 0;
-        if (i < nw) {
+        if (wi < nw) {
             wl = bits[i];
             if (PRUNED) wh = hbits[i];
         }
@@ -220,18 +233,19 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
             nl = nh = 0;
             for (uint32_t m = wl | wh; m; m &= m - 1) {
                 const int b = __builtin_ctz(m);
-                const int32_t d = dist[i * 32 + b];
+                const int32_t d = dist[(int64_t)slot * V + wi * 32 + b];
                 if (d < T) { nl |= wl & (1u << b); nh |= wh & (1u << b); }
                 else fmin = min(fmin, d);
             }
         }
-        if (i < nw) {
+        if (wi < nw) {
             if (wl != (wl & ~nl)) bits[i] = wl & ~nl;
             if (PRUNED && wh != (wh & ~nh)) hbits[i] = wh & ~nh;
         }
+        const int tag = slot << 1;
         int cnt = 0;
         for (uint32_t m = nl; m; m &= m - 1) {
-            const int v = (int)(i * 32 + __builtin_ctz(m));
+            const int v = (int)(wi * 32 + __builtin_ctz(m));
             const int d = row[v + 1] - row[v];
             cnt += (d + chunk - 1) / chunk;
             front += 1;
@@ -239,7 +253,7 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
         }
         if (PRUNED)
             for (uint32_t m = nh; m; m &= m - 1) {
-                const int v = (int)(i * 32 + __builtin_ctz(m));
+                const int v = (int)(wi * 32 + __builtin_ctz(m));
                 const int d = hrow[v + 1] - hrow[v];
                 cnt += (d + chunk - 1) / chunk;
                 hedges += d;
@@ -249,7 +263,10 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
         const int off = wave_excl_scan(cnt, &tot);
         if (tot == 0) continue;
         int64_t base = 0;
-        if (lane == 0) base = (int64_t)atomicAdd((unsigned long long*)&ctrl[C_CUR], (unsigned long long)tot);
+        if (lane == 0) {
+            base = (int64_t)atomicAdd((unsigned long long*)&ctrl[C_CUR], (unsigned long long)tot);
+            atomicOr((unsigned long long*)&ctrl[C_SLOTS], 1ull << slot);
+        }
         base = shfl64(base, 0);
         if (base + tot > qcap) {
             if (lane == 0) atomicOr((unsigned long long*)&ctrl[C_OVF], 1ull);
@@ -257,15 +274,15 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
         }
         int64_t p = base + off;
         for (uint32_t m = nl; m; m &= m - 1) {
-            const int v = (int)(i * 32 + __builtin_ctz(m));
+            const int v = (int)(wi * 32 + __builtin_ctz(m));
             const int r0 = row[v], r1 = row[v + 1];
-            for (int b = r0; b < r1; b += chunk) q[p++] = make_int4(v, b, min(b + chunk, r1), 0);
+            for (int b = r0; b < r1; b += chunk) q[p++] = make_int4(v, b, min(b + chunk, r1), tag);
         }
         if (PRUNED)
             for (uint32_t m = nh; m; m &= m - 1) {
-                const int v = (int)(i * 32 + __builtin_ctz(m));
+                const int v = (int)(wi * 32 + __builtin_ctz(m));
                 const int h0 = hrow[v], h1 = hrow[v + 1];
-                for (int b = h0; b < h1; b += chunk) q[p++] = make_int4(v, b, min(b + chunk, h1), 1);
+                for (int b = h0; b < h1; b += chunk) q[p++] = make_int4(v, b, min(b + chunk, h1), tag | 1);
             }
     }
     front = wave_sum64(front);
@@ -329,6 +346,9 @@ __global__ void dist_checksum_k(const int32_t* __restrict__ row, const int32_t* 
 }
 
 // ------------------------------------------------------------------ host
+// Words per activation bitmap of one slot, padded to whole waves.
+static int64_t bitmap_words(int64_t V) { return (((V + 31) / 32 + 1) + 63) & ~(int64_t)63; }
+
 int ensure_sssp_ws(Graph* g) {
     SsspWs& w = g->sssp;
     const int64_t V = g->V;
@@ -336,12 +356,12 @@ int ensure_sssp_ws(Graph* g) {
     // Each vertex is activated at most once per round for its live list and
     // once for its hidden list: at most ceil(d/chunk) <= d/chunk + 1 items each.
     const int64_t need = 2 * V + (g->E + g->HE) / chunk + 64;
-    const int64_t nw = (V + 31) / 32 + 1;
+    const int64_t nwp = bitmap_words(V);
     HIP_TRY(hipSetDevice(g->device));
     if (!w.dist) {
         HIP_TRY(hipMalloc(&w.dist, sizeof(int32_t) * std::max<int64_t>(V, 1)));
         // live and hidden activation bitmaps
-        HIP_TRY(hipMalloc(&w.stamp, sizeof(uint32_t) * nw * 2));
+        HIP_TRY(hipMalloc(&w.stamp, sizeof(uint32_t) * nwp * 2));
         HIP_TRY(hipMalloc(&w.ctrl, sizeof(int64_t) * C_N));
         HIP_TRY(hipEventCreate(&w.ev[0]));
         HIP_TRY(hipEventCreate(&w.ev[1]));
@@ -370,27 +390,27 @@ static int launch_rounds(Graph* g, int k, double* relax_ms = nullptr) {
     SsspWs& w = g->sssp;
     const bool pruned = is_pruned(g);
     const int grid = relax_grid(g);
-    const int64_t nw = (g->V + 31) / 32 + 1;
+    const int64_t nwp = bitmap_words(g->V);
     uint32_t* bits = (uint32_t*)w.stamp;
-    uint32_t* hbits = bits + nw;
-    const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (nw + 255) / 256));
+    uint32_t* hbits = bits + nwp;
+    const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (nwp + 255) / 256));
     for (int r = 0; r < k; ++r) {
         if (relax_ms) HIP_TRY(hipEventRecord(w.ev[0], g->stream));
         if (pruned) {
             hipLaunchKernelGGL(sssp_relax_k<true>, dim3(grid), dim3(256), 0, g->stream, g->d_adj,
-                               g->d_hadj, w.dist, bits, hbits, w.hid, w.q[0], w.ctrl);
+                               g->d_hadj, w.dist, bits, hbits, w.hid, w.q[0], w.ctrl, (int64_t)g->V, nwp);
             if (relax_ms) HIP_TRY(hipEventRecord(w.ev[1], g->stream));
             hipLaunchKernelGGL(sssp_expand_k<true>, dim3(egrid), dim3(256), 0, g->stream, g->d_row,
                                g->d_hrow, bits, hbits, w.q[0], w.ctrl, (int64_t)g->V, g->tune.chunk, w.qcap,
-                               (const int32_t*)w.dist, g->tune.delta);
+                               (const int32_t*)w.dist, g->tune.delta, 1, nwp);
         } else {
             hipLaunchKernelGGL(sssp_relax_k<false>, dim3(grid), dim3(256), 0, g->stream, g->d_adj,
                                (const int2*)nullptr, w.dist, bits, (uint32_t*)nullptr, (uint8_t*)nullptr,
-                               w.q[0], w.ctrl);
+                               w.q[0], w.ctrl, (int64_t)g->V, nwp);
             if (relax_ms) HIP_TRY(hipEventRecord(w.ev[1], g->stream));
             hipLaunchKernelGGL(sssp_expand_k<false>, dim3(egrid), dim3(256), 0, g->stream, g->d_row,
                                (const int32_t*)nullptr, bits, (uint32_t*)nullptr, w.q[0], w.ctrl,
-                               (int64_t)g->V, g->tune.chunk, w.qcap, (const int32_t*)w.dist, g->tune.delta);
+                               (int64_t)g->V, g->tune.chunk, w.qcap, (const int32_t*)w.dist, g->tune.delta, 1, nwp);
         }
         if (relax_ms) {
             HIP_TRY(hipEventSynchronize(w.ev[1]));
@@ -454,12 +474,12 @@ int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* 
     if (rc) return rc;
     SsspWs& w = g->sssp;
     const bool pruned = is_pruned(g);
-    const int64_t nw = (g->V + 31) / 32 + 1;
+    const int64_t nwp = bitmap_words(g->V);
     uint32_t* bits = (uint32_t*)w.stamp;
     const double t0 = now_ms();
     const int tb = 256, gridv = (int)std::min<int64_t>((g->V + tb - 1) / tb, 65535);
     hipLaunchKernelGGL(sssp_init_k, dim3(std::max(gridv, 1)), dim3(tb), 0, g->stream, w.dist, bits,
-                       pruned ? bits + nw : (uint32_t*)nullptr, pruned ? w.hid : (uint8_t*)nullptr,
+                       pruned ? bits + nwp : (uint32_t*)nullptr, pruned ? w.hid : (uint8_t*)nullptr,
                        (int64_t)g->V);
     hipLaunchKernelGGL(sssp_seed_k, dim3(1), dim3(256), 0, g->stream, g->d_row,
                        pruned ? g->d_hrow : (const int32_t*)nullptr, w.dist,
@@ -503,7 +523,7 @@ int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* 
         const int64_t nwd = (g->V + 31) / 32;
         std::vector<uint32_t> bw((size_t)nwd * (pruned ? 2 : 1));
         HIP_TRY(hipMemcpy(bw.data(), bits, sizeof(uint32_t) * nwd, hipMemcpyDeviceToHost));
-        if (pruned) HIP_TRY(hipMemcpy(bw.data() + nwd, bits + nw, sizeof(uint32_t) * nwd, hipMemcpyDeviceToHost));
+        if (pruned) HIP_TRY(hipMemcpy(bw.data() + nwd, bits + nwp, sizeof(uint32_t) * nwd, hipMemcpyDeviceToHost));
         for (int64_t k = 0; k < (int64_t)bw.size(); ++k)
             for (uint32_t x = bw[k]; x; x &= x - 1) {
                 const int32_t v = (int32_t)((k % nwd) * 32 + __builtin_ctz(x));
@@ -522,6 +542,165 @@ int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* 
         st->reached_edges = reached_edges(g, dist_out);
     }
     return C2G_OK;
+}
+
+// ------------------------------------------------------------ slot pool
+// Clears slot s before a query joins it.
+__global__ void slot_reset_k(int32_t* __restrict__ dist, uint8_t* __restrict__ hid, uint32_t* __restrict__ bits,
+                             uint32_t* __restrict__ hbits, int64_t V, int64_t nw) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < V; i += (int64_t)gridDim.x * blockDim.x) {
+        dist[i] = INT32_MAX;
+        if (hid) hid[i] = 0;
+        if (i < nw) {
+            bits[i] = 0;
+            if (hbits) hbits[i] = 0;
+        }
+    }
+}
+
+// Queues the source's items (live list, and hidden list on pruned graphs) at
+// the end of the pending queue, tagged with the slot.
+__global__ void slot_seed_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
+                            int32_t* __restrict__ dist, uint8_t* __restrict__ hid, int4* __restrict__ q,
+                            int64_t* __restrict__ ctrl, int64_t qcap, int slot, int32_t src, int chunk) {
+    __shared__ int64_t base;
+    const int r0 = row[src], r1 = row[src + 1];
+    const int nl = (r1 - r0 + chunk - 1) / chunk;
+    int h0 = 0, h1 = 0, nh = 0;
+    if (hrow) { h0 = hrow[src]; h1 = hrow[src + 1]; nh = (h1 - h0 + chunk - 1) / chunk; }
+    if (threadIdx.x == 0) {
+        dist[src] = 0;
+        if (hid) hid[src] = 1;  // the source's hidden list is always relaxed
+        base = (int64_t)atomicAdd((unsigned long long*)&ctrl[C_CUR], (unsigned long long)(nl + nh));
+        if (base + nl + nh > qcap) ctrl[C_OVF] = 1;
+    }
+    __syncthreads();
+    if (base + nl + nh > qcap) return;
+    const int tag = slot << 1;
+    for (int j = threadIdx.x; j < nl + nh; j += blockDim.x) {
+        if (j < nl) {
+            const int b = r0 + j * chunk;
+            q[base + j] = make_int4(src, b, min(b + chunk, r1), tag);
+        } else {
+            const int b = h0 + (j - nl) * chunk;
+            q[base + j] = make_int4(src, b, min(b + chunk, h1), tag | 1);
+        }
+    }
+}
+
+struct SlotPool {
+    int slots = 0;
+    int64_t nwp = 0, qcap = 0;
+    int32_t* dist = nullptr;   // [slots][V]
+    uint32_t* bits = nullptr;  // live [slots][nwp], then hidden [slots][nwp]
+    uint8_t* hid = nullptr;    // [slots][V] (pruned graphs)
+    int4* q = nullptr;
+    int64_t* ctrl = nullptr;
+    int64_t* h_ctrl = nullptr;
+    HostStage out;
+};
+static std::mutex g_sp_mu;
+static std::vector<std::pair<const Graph*, SlotPool*>> g_sp;
+
+static SlotPool* spool(Graph* g) {
+    std::lock_guard<std::mutex> lk(g_sp_mu);
+    for (auto& p : g_sp)
+        if (p.first == g) return p.second;
+    auto* s = new SlotPool;
+    g_sp.emplace_back(g, s);
+    return s;
+}
+
+void qserve_release(Graph* g) {
+    std::lock_guard<std::mutex> lk(g_sp_mu);
+    for (size_t i = 0; i < g_sp.size(); ++i)
+        if (g_sp[i].first == g) {
+            SlotPool* s = g_sp[i].second;
+            for (void* p : {(void*)s->dist, (void*)s->bits, (void*)s->hid, (void*)s->q, (void*)s->ctrl})
+                if (p) (void)hipFree(p);
+            if (s->h_ctrl) (void)hipHostFree(s->h_ctrl);
+            s->out.release();
+            delete s;
+            g_sp.erase(g_sp.begin() + i);
+            return;
+        }
+}
+
+int qserve_init(Graph* g, hipStream_t strm, int slots) {
+    if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");
+    if (!g->d_adj) return fail(C2G_ENOTSUP, "graph has no int weights (wlist) for SSSP");
+    if (slots < 1 || slots > 64) return fail(C2G_EINVAL, "slot pool size must be in [1,64]");
+    SlotPool* s = spool(g);
+    if (s->slots) return s->slots == slots ? C2G_OK : fail(C2G_EINVAL, "slot pool already sized %d", s->slots);
+    HIP_TRY(hipSetDevice(g->device));
+    const int64_t V = std::max<int64_t>(g->V, 1);
+    const int chunk = std::max(1, g->tune.chunk);
+    s->nwp = bitmap_words(V);
+    // per slot and round: each vertex at most once per list, ceil(d/chunk) <= d/chunk + 1 items
+    s->qcap = (int64_t)slots * (2 * V + (g->E + g->HE) / chunk) + 64;
+    HIP_TRY(hipMalloc(&s->dist, sizeof(int32_t) * slots * V));
+    HIP_TRY(hipMalloc(&s->bits, sizeof(uint32_t) * 2 * slots * s->nwp));
+    HIP_TRY(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * 2 * slots * s->nwp, strm));
+    if (g->d_hrow) HIP_TRY(hipMalloc(&s->hid, (size_t)slots * V));
+    HIP_TRY(hipMalloc(&s->q, sizeof(int4) * s->qcap));
+    HIP_TRY(hipMalloc(&s->ctrl, sizeof(int64_t) * C_N));
+    HIP_TRY(hipMemsetAsync(s->ctrl, 0, sizeof(int64_t) * C_N, strm));
+    HIP_TRY(hipHostMalloc((void**)&s->h_ctrl, sizeof(int64_t) * C_N, hipHostMallocDefault));
+    HIP_TRY(hipStreamSynchronize(strm));
+    s->slots = slots;
+    return C2G_OK;
+}
+
+int qserve_admit(Graph* g, hipStream_t strm, int slot, int32_t src) {
+    SlotPool* s = spool(g);
+    if (slot < 0 || slot >= s->slots || src < 0 || src >= g->V) return fail(C2G_EINVAL, "bad slot/source");
+    const bool pruned = is_pruned(g);
+    const int64_t V = g->V;
+    const int grid = (int)std::min<int64_t>((V + 255) / 256, (int64_t)std::max(1, g->num_cus) * 8);
+    hipLaunchKernelGGL(slot_reset_k, dim3(std::max(grid, 1)), dim3(256), 0, strm, s->dist + slot * V,
+                       pruned ? s->hid + slot * V : (uint8_t*)nullptr, s->bits + slot * s->nwp,
+                       pruned ? s->bits + (s->slots + slot) * s->nwp : (uint32_t*)nullptr, V, (V + 31) / 32);
+    hipLaunchKernelGGL(slot_seed_k, dim3(1), dim3(256), 0, strm, g->d_row,
+                       pruned ? g->d_hrow : (const int32_t*)nullptr, s->dist + slot * V,
+                       pruned ? s->hid + slot * V : (uint8_t*)nullptr, s->q, s->ctrl, s->qcap, slot, src,
+                       std::max(1, g->tune.chunk));
+    HIP_TRY(hipGetLastError());
+    return C2G_OK;
+}
+
+int qserve_step(Graph* g, hipStream_t strm, uint64_t* slots_with_work) {
+    SlotPool* s = spool(g);
+    const bool pruned = is_pruned(g);
+    const int64_t V = g->V;
+    const int grid = std::max(1, g->num_cus) * g->tune.grid_per_cu;
+    const int64_t words = (int64_t)s->slots * s->nwp;
+    const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (words + 255) / 256));
+    uint32_t* hb = s->bits + (int64_t)s->slots * s->nwp;
+    if (pruned) {
+        hipLaunchKernelGGL(sssp_relax_k<true>, dim3(grid), dim3(256), 0, strm, g->d_adj, g->d_hadj, s->dist,
+                           s->bits, hb, s->hid, s->q, s->ctrl, V, s->nwp);
+        hipLaunchKernelGGL(sssp_expand_k<true>, dim3(egrid), dim3(256), 0, strm, g->d_row, g->d_hrow, s->bits, hb,
+                           s->q, s->ctrl, V, std::max(1, g->tune.chunk), s->qcap, (const int32_t*)s->dist, 0,
+                           s->slots, s->nwp);
+    } else {
+        hipLaunchKernelGGL(sssp_relax_k<false>, dim3(grid), dim3(256), 0, strm, g->d_adj, (const int2*)nullptr,
+                           s->dist, s->bits, (uint32_t*)nullptr, (uint8_t*)nullptr, s->q, s->ctrl, V, s->nwp);
+        hipLaunchKernelGGL(sssp_expand_k<false>, dim3(egrid), dim3(256), 0, strm, g->d_row, (const int32_t*)nullptr,
+                           s->bits, (uint32_t*)nullptr, s->q, s->ctrl, V, std::max(1, g->tune.chunk), s->qcap,
+                           (const int32_t*)s->dist, 0, s->slots, s->nwp);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(s->h_ctrl, s->ctrl, sizeof(int64_t) * C_N, hipMemcpyDeviceToHost, strm));
+    HIP_TRY(hipStreamSynchronize(strm));
+    if (s->h_ctrl[C_OVF]) return fail(C2G_EDEVICE, "server slot pool capacity exceeded (internal)");
+    *slots_with_work = (uint64_t)s->h_ctrl[C_SLOTS];
+    return C2G_OK;
+}
+
+int qserve_extract(Graph* g, hipStream_t strm, int slot, int32_t* out) {
+    SlotPool* s = spool(g);
+    if (slot < 0 || slot >= s->slots) return fail(C2G_EINVAL, "bad slot");
+    return s->out.copy_out(out, s->dist + (int64_t)slot * g->V, sizeof(int32_t) * g->V, strm);
 }
 
 }  // namespace c2g

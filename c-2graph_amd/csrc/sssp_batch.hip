@@ -667,26 +667,32 @@ int sssp_batch_group(Graph* g, const int32_t* srcs, int32_t n, int32_t* dist_out
 // ------------------------------------------------------------ server lane
 // A query joins lane `lane` of group 0: its distance column was reset to
 // INT32_MAX when the lane was last freed (or at init).
-__global__ void bserve_admit_k(const int32_t* __restrict__ row, int32_t* __restrict__ dist,
-                               uint64_t* __restrict__ mask, uint32_t* __restrict__ abits, int32_t* __restrict__ thr,
-                               int lane, int32_t s, int delta) {
+__global__ void bserve_admit_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
+                               int32_t* __restrict__ dist, uint64_t* __restrict__ mask, uint64_t* __restrict__ hmask,
+                               uint32_t* __restrict__ abits, int32_t* __restrict__ thr, int lane, int32_t s,
+                               int delta) {
     dist[(int64_t)s * 64 + lane] = 0;
     thr[lane] = delta > 0 ? delta : INT32_MAX;
-    if (row[s + 1] > row[s]) {
+    // the source's hidden list is always relaxed (pruned graphs)
+    if (hmask) atomicOr((unsigned long long*)&hmask[s], 1ull << lane);
+    if (row[s + 1] > row[s] || (hrow && hrow[s + 1] > hrow[s])) {
         atomicOr((unsigned long long*)&mask[s], 1ull << lane);
         atomicOr(&abits[s >> 5], 1u << (s & 31));
     }
 }
 
-// Copies the distance columns of `n` lanes into out[k][V] and resets them.
-__global__ void bserve_extract_k(int32_t* __restrict__ dist, int64_t V, uint64_t lanes,
-                                 const int8_t* __restrict__ slot, int32_t* __restrict__ out) {
+// Copies the distance columns of `n` lanes into out[k][V] and resets them
+// (and, on pruned graphs, the lanes' hidden-edge marks).
+__global__ void bserve_extract_k(int32_t* __restrict__ dist, uint64_t* __restrict__ hmask, int64_t V,
+                                 uint64_t lanes, const int8_t* __restrict__ slot, int32_t* __restrict__ out) {
     const int lane = lane_id();
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     const bool mine = (lanes >> lane) & 1ull;
     const int k = slot[lane];
     for (int64_t v = wave; v < V; v += nwaves) {
+        if (hmask && lane == 0 && (hmask[v] & lanes))
+            atomicAnd((unsigned long long*)&hmask[v], ~(unsigned long long)lanes);
         if (!mine) continue;
         const int64_t x = v * 64 + lane;
         out[(int64_t)k * V + v] = dist[x];
@@ -752,8 +758,10 @@ int bserve_init(Graph* g, hipStream_t strm) {
 int bserve_admit(Graph* g, hipStream_t strm, int lane, int32_t src) {
     if (lane < 0 || lane > 63 || src < 0 || src >= g->V) return fail(C2G_EINVAL, "bad lane/source");
     BatchState* s = bstate(g);
-    hipLaunchKernelGGL(bserve_admit_k, dim3(1), dim3(1), 0, strm, g->d_row, g->batch.dist, g->batch.mask[0],
-                       s->abits, s->thr, lane, src, g->tune.delta);
+    const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
+    hipLaunchKernelGGL(bserve_admit_k, dim3(1), dim3(1), 0, strm, g->d_row,
+                       pruned ? g->d_hrow : (const int32_t*)nullptr, g->batch.dist, g->batch.mask[0],
+                       pruned ? g->batch.hmask : (uint64_t*)nullptr, s->abits, s->thr, lane, src, g->tune.delta);
     HIP_TRY(hipGetLastError());
     return C2G_OK;
 }
@@ -779,8 +787,9 @@ int bserve_extract(Graph* g, hipStream_t strm, const int* lanes, int n, int32_t*
         uint64_t lm = 0;
         for (int k = 0; k < m; ++k) { lm |= 1ull << lanes[b + k]; slot[lanes[b + k]] = (int8_t)k; }
         HIP_TRY(hipMemcpyAsync(ss->slot, slot, 64, hipMemcpyHostToDevice, strm));
+        const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
         hipLaunchKernelGGL(bserve_extract_k, dim3(std::max(1, g->num_cus) * 8), dim3(256), 0, strm,
-                           g->batch.dist, V, lm, ss->slot, ss->out);
+                           g->batch.dist, pruned ? g->batch.hmask : (uint64_t*)nullptr, V, lm, ss->slot, ss->out);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(ss->h_out, ss->out, sizeof(int32_t) * V * m, hipMemcpyDeviceToHost, strm));
         HIP_TRY(hipStreamSynchronize(strm));

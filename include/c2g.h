@@ -326,7 +326,10 @@ typedef struct {
     float   alpha, eps;       /* PPR parameters                                */
     int32_t ads_labels, ads_iters;
     float   p_inj, p_cont, p_abnd;
-    int32_t reserved[8];
+    int32_t sssp_slots;       /* SSSP queries run side by side in query-major
+                                 slots before the 64-lane group takes the
+                                 overflow: 0 = 16, <0 = lane group only, <=64 */
+    int32_t reserved[7];
 } c2g_server_opts;
 
 typedef struct {

@@ -112,3 +112,35 @@ def test_ppr_burst_is_batched(gpu):
             assert np.abs(p.astype(np.float64) - ref).sum() <= 2e-5, v
     admits = [i["t_admit_ms"] for i, _ in res]
     assert len(set(admits)) < len(admits)  # a batch shares one admission time
+
+
+@pytest.mark.parametrize("slots,lanes", [(64, 64), (-1, 64), (4, 8), (1, 1)])
+def test_sssp_pools(gpu, slots, lanes):
+    # slot pool only, lane group only, and a burst that overflows a small slot
+    # pool into the lane group; every pool's results equal Dijkstra
+    G = _G()
+    srcs = np.flatnonzero(np.diff(G["vlist"]) > 0)[::37][:120]
+    with _graph(G) as g:
+        with c2g.Server(g, max_batch=lanes, sssp_slots=slots) as s:
+            ids = [s.submit(c2g.Q_SSSP, int(v)) for v in srcs]
+            for i, v in zip(ids, srcs):
+                info, d = s.poll(i)
+                assert info["status"] == 0
+                assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], int(v))), v
+
+
+def test_pruned_graph_server(gpu):
+    # pruned graph (hidden lists relaxed per slot / per lane): results equal
+    # Dijkstra on the full graph, through both SSSP pools
+    from conftest import random_graph
+    for seed in range(3):
+        G = random_graph(300, 700, seed=seed, wmax=9)
+        P = c2g.prune_sssp_mem(G["vlist"], G["elist"], G["wlist"])
+        assert len(P["helist"]) > 0
+        with c2g.Graph.from_csr(P["vlist"], P["elist"], P["wlist"], hvlist=P["hvlist"], helist=P["helist"],
+                                hwlist=P["hwlist"], variant=c2g.VARIANT_SSSP_PURN, device=0) as g:
+            with c2g.Server(g, max_batch=16, sssp_slots=8) as s:
+                ids = [s.submit(c2g.Q_SSSP, v) for v in range(300)]
+                for v, i in enumerate(ids):
+                    info, d = s.poll(i)
+                    assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], v)), (seed, v)
