@@ -11,21 +11,24 @@
 // uses 1999999999), no software grid barrier (Main.cu:143-147, unsafe without
 // co-residency), no O(V) isactive rescan per round (Main.cu:313-340).
 //
-// Layout of one round (one kernel launch, `sssp_relax`):
-//   * The frontier is a queue of work items int4(u, e_begin, e_end, kind),
-//     kind 0 = live CSR edges, 1 = hidden-list edges.  A vertex of out-degree
-//     d gives ceil(d/chunk) items, so hubs are split across waves when they
-//     are queued (the reference's block/warp/thread tiers, Main.cu:197-293,
-//     become one balanced path).
-//   * A wave takes 64 items, scans their edge counts (wave64 shuffle scan),
-//     and walks the concatenated edge list 64 edges at a time; lane k finds
-//     its owner item with a 6-step shuffle search.  Edge (dst,w) pairs are one
-//     8-byte load from the interleaved `adj` array.
-//   * An improving lane claims v for the next round by exchanging v's stamp
-//     to the next round number (dedup without clearing any array); claimed
-//     vertices are appended with one atomicAdd per wave.
-//   * The last workgroup to finish publishes the next round's item count, so
-//     rounds are plain back-to-back launches (captured in a hipGraph).
+// One round is two launches:
+//   * `sssp_relax_k` walks the round's work items int4(u, e_begin, e_end,
+//     kind | slot << 1) (kind 1 = hidden-list edges).  A vertex of out-degree
+//     d gives ceil(d/chunk) items, so hubs are split across waves (the
+//     reference's block/warp/thread tiers, Main.cu:197-293, become one
+//     balanced path).  A wave takes 64 items, scans their edge counts (wave64
+//     shuffle scan) and walks the concatenated edge list; lane k finds its
+//     owner item with a 6-step shuffle search.  Edge (dst,w) pairs are one
+//     8-byte load from the interleaved `adj` array.  An improving lane issues
+//     atomicMin without waiting for the old value and marks v in the
+//     activation bitmap with a fire-and-forget atomicOr.
+//   * `sssp_expand_k` compacts the bitmap into the next round's items (wave
+//     ballot + scan, one queue atomic per wave) and clears it.
+//   * The last workgroup of each kernel publishes the counters, so rounds are
+//     plain back-to-back launches (captured in a hipGraph); there is no grid
+//     barrier.
+// A slot pool runs several queries side by side with the same kernels: slot s
+// owns dist[s*V..], its own bitmap words and hidden flags (server lane).
 //
 // Pruned graphs: a vertex's hidden list is relaxed when the vertex is the
 // source or when its distance was ever lowered through a hidden edge
@@ -44,7 +47,7 @@ enum : int {
     C_DONE = 3,   // workgroups finished (relax)
     C_FRONT = 4,  // sum of frontier vertices over rounds
     C_EDGES = 5,  // edges queued for relaxation (R)
-    C_IMPR = 6,   // improving relaxations (U)
+    C_IMPR = 6,   // improvement writes issued (U): candidate below the value read
     C_OVF = 7,    // queue overflow guard tripped (must stay 0)
     C_HEDGES = 8, // hidden edges queued
     C_DONE_E = 9, // workgroups finished (expand)

@@ -40,7 +40,7 @@ enum : int {
     B_DONE_E = 4,  // workgroups finished (expand)
     B_FRONT = 5,   // sum over rounds of activated (v,g)
     B_EDGES = 6,   // CSR edges streamed (per item, once for all lanes)
-    B_IMPR = 7,    // improving (query, edge) relaxations (U)
+    B_IMPR = 7,    // improvement writes issued (U): candidate below the value read
     B_OVF = 8,     // capacity guard tripped (must stay 0)
     B_LEDGES = 9,  // (query, edge) relaxation attempts
     B_ITEMS_TOT = 10,  // items processed
