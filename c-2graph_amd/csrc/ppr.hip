@@ -15,11 +15,12 @@
 // Single source, one round = two kernels:
 //   ppr_push_k   : items (u, e_begin, e_end, bits of r[u]) walked 64 edges at
 //                  a time per wave (shuffle owner search as in SSSP); one
-//                  hardware FP32 atomicAdd into m[v] per edge.
-//   ppr_expand_k : scans m[] (V floats); a touched vertex adds its message,
-//                  activates above threshold, and its out-row is cut into
-//                  items carrying the residual being pushed (wave scan + one
-//                  atomicAdd per wave).
+//                  hardware FP32 atomicAdd into m[v] per edge, and v marked in
+//                  a touched bitmap (instead of the reference's full-V scan).
+//   ppr_expand_k : walks the touched bitmap; a touched vertex adds its
+//                  message, activates above threshold, and its out-row is cut
+//                  into items carrying the residual being pushed (wave scan +
+//                  one atomicAdd per wave).
 // Pruned graphs (ppr/ variant): the first round's frontier is the source's
 // hidden and live neighbours with r = (1-alpha)*w (src/ppr/purn/Main.cu:78-101,
 // 193-198), activated unconditionally.
@@ -41,23 +42,31 @@ enum : int {
     P_OVF = 7, P_DONE_A = 8, P_N = 16
 };
 
-__global__ void ppr_init_k(float* __restrict__ p, float* __restrict__ r, float* __restrict__ m, int64_t n) {
+__global__ void ppr_init_k(float* __restrict__ p, float* __restrict__ r, float* __restrict__ m,
+                           uint32_t* __restrict__ tb, int64_t n) {
+    const int64_t nw = (n + 31) >> 5;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         p[i] = 0.0f;
         r[i] = 0.0f;
         m[i] = 0.0f;
+        if (i < nw) tb[i] = 0;
     }
 }
+
+__device__ __forceinline__ void touch(uint32_t* tb, int v) { atomicOr(&tb[v >> 5], 1u << (v & 31)); }
 
 // Seeds messages for the first round (activated unconditionally by the expand).
 __global__ void ppr_seed_k(const int32_t* __restrict__ row, const int2* __restrict__ adjf,
                            const int32_t* __restrict__ hrow, const int2* __restrict__ hadjf,
-                           float* __restrict__ p, float* __restrict__ m, int64_t* __restrict__ ctrl,
-                           int32_t src, float alpha, int pruned) {
+                           float* __restrict__ p, float* __restrict__ m, uint32_t* __restrict__ tb,
+                           int64_t* __restrict__ ctrl, int32_t src, float alpha, int pruned) {
     if (threadIdx.x == 0)
         for (int i = 0; i < P_N; ++i) ctrl[i] = 0;
     if (!pruned) {
-        if (threadIdx.x == 0) m[src] = 1.0f;
+        if (threadIdx.x == 0) {
+            m[src] = 1.0f;
+            touch(tb, src);
+        }
         return;
     }
     // Pruned graph: p[s] = alpha, and every hidden then live neighbour h gets
@@ -67,12 +76,15 @@ __global__ void ppr_seed_k(const int32_t* __restrict__ row, const int2* __restri
     for (int j = threadIdx.x; j < (h1 - h0) + (r1 - r0); j += blockDim.x) {
         const int2 a = j < h1 - h0 ? hadjf[h0 + j] : adjf[r0 + j - (h1 - h0)];
         atomicAdd(&m[a.x], (1.0f - alpha) * __int_as_float(a.y));
+        touch(tb, a.x);
     }
 }
 
+// Pushes the queued residuals along their edges: m[v] += (1-alpha) r_u f(u,v)
+// (FP32 atomics), and marks v in the touched bitmap for the expand.
 __global__ void __launch_bounds__(256)
-ppr_push_k(const int2* __restrict__ adjf, float* __restrict__ m, const int4* __restrict__ q,
-           int64_t* __restrict__ ctrl, float alpha) {
+ppr_push_k(const int2* __restrict__ adjf, float* __restrict__ m, uint32_t* __restrict__ tb,
+           const int4* __restrict__ q, int64_t* __restrict__ ctrl, float alpha) {
     const int64_t n = ctrl[P_CUR];
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -98,8 +110,14 @@ ppr_push_k(const int2* __restrict__ adjf, float* __restrict__ m, const int4* __r
             int2 a0 = make_int2(0, 0), a1 = make_int2(0, 0);
             if (k0 < tot) a0 = adjf[e0];
             if (k1 < tot) a1 = adjf[e1];
-            if (k0 < tot) unsafeAtomicAdd(&m[a0.x], m0 * __int_as_float(a0.y));
-            if (k1 < tot) unsafeAtomicAdd(&m[a1.x], m1 * __int_as_float(a1.y));
+            if (k0 < tot) {
+                unsafeAtomicAdd(&m[a0.x], m0 * __int_as_float(a0.y));
+                touch(tb, a0.x);
+            }
+            if (k1 < tot) {
+                unsafeAtomicAdd(&m[a1.x], m1 * __int_as_float(a1.y));
+                touch(tb, a1.x);
+            }
         }
     }
     __syncthreads();
@@ -116,47 +134,63 @@ ppr_push_k(const int2* __restrict__ adjf, float* __restrict__ m, const int4* __r
     }
 }
 
+// Threshold pass over the touched vertices only (the touched bitmap, one
+// 32-vertex word per lane): r[v] += m[v]; if r[v] > eps*outdeg(v) (or
+// `force`, the first round), p[v] += alpha*r[v], r[v] = 0 and v's items join
+// the next round carrying the residual to push.  Two passes per word: count
+// the items of the activated vertices, take the queue range with one atomic
+// per wave, then apply the updates and write the items.
 __global__ void __launch_bounds__(256)
 ppr_expand_k(const int32_t* __restrict__ row, float* __restrict__ p, float* __restrict__ r,
-             float* __restrict__ m, int4* __restrict__ q, int64_t* __restrict__ ctrl, int64_t V,
-             float alpha, float eps, int chunk, int64_t qcap, int force) {
+             float* __restrict__ m, uint32_t* __restrict__ tb, int4* __restrict__ q, int64_t* __restrict__ ctrl,
+             int64_t V, float alpha, float eps, int chunk, int64_t qcap, int force) {
     const int lane = lane_id();
-    const int64_t nn = (V + 63) & ~(int64_t)63;
+    const int64_t nw = (V + 31) >> 5;
+    const int64_t nwr = (nw + 63) & ~(int64_t)63;  // uniform trip count per wave
     int64_t front = 0, edges = 0;
-    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < nn; v += (int64_t)gridDim.x * blockDim.x) {
-        int cnt = 0, r0 = 0, r1 = 0;
-        float ru = 0.0f;
-        if (v < V) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nwr; i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t word = 0;
+        if (i < nw) {
+            word = tb[i];
+            if (word) tb[i] = 0;
+        }
+        int cnt = 0;
+        for (uint32_t b = word; b; b &= b - 1) {
+            const int64_t v = i * 32 + __builtin_ctz(b);
             const float mv = m[v];
-            if (mv != 0.0f) {
-                m[v] = 0.0f;
-                const float rv = r[v] + mv;
-                r0 = row[v];
-                r1 = row[v + 1];
-                if (force || rv > (float)(r1 - r0) * eps) {
-                    p[v] += alpha * rv;
-                    r[v] = 0.0f;
-                    ru = rv;
-                    cnt = (r1 - r0 + chunk - 1) / chunk;
-                    front += 1;
-                    edges += r1 - r0;
-                } else {
-                    r[v] = rv;
-                }
-            }
+            if (mv == 0.0f) continue;
+            const int d = row[v + 1] - row[v];
+            if (force || r[v] + mv > (float)d * eps) cnt += (d + chunk - 1) / chunk;
         }
         int tot;
         const int off = wave_excl_scan(cnt, &tot);
-        if (tot == 0) continue;
         int64_t base = 0;
-        if (lane == 0) base = (int64_t)atomicAdd((unsigned long long*)&ctrl[P_CUR], (unsigned long long)tot);
-        base = shfl64(base, 0);
-        if (base + tot > qcap) {
-            if (lane == 0) atomicOr((unsigned long long*)&ctrl[P_OVF], 1ull);
-            continue;
+        if (tot > 0) {
+            if (lane == 0) base = (int64_t)atomicAdd((unsigned long long*)&ctrl[P_CUR], (unsigned long long)tot);
+            base = shfl64(base, 0);
         }
+        const bool room = base + tot <= qcap;
+        if (!room && lane == 0) atomicOr((unsigned long long*)&ctrl[P_OVF], 1ull);
         int64_t pos = base + off;
-        for (int e = r0; e < r1 && cnt; e += chunk) q[pos++] = make_int4((int)v, e, min(e + chunk, r1), __float_as_int(ru));
+        for (uint32_t b = word; b; b &= b - 1) {
+            const int64_t v = i * 32 + __builtin_ctz(b);
+            const float mv = m[v];
+            if (mv == 0.0f) continue;
+            m[v] = 0.0f;
+            const float rv = r[v] + mv;
+            const int r0 = row[v], r1 = row[v + 1];
+            if (force || rv > (float)(r1 - r0) * eps) {
+                p[v] += alpha * rv;
+                r[v] = 0.0f;
+                front += 1;
+                edges += r1 - r0;
+                if (room)
+                    for (int e = r0; e < r1; e += chunk)
+                        q[pos++] = make_int4((int)v, e, min(e + chunk, r1), __float_as_int(rv));
+            } else {
+                r[v] = rv;
+            }
+        }
     }
     front = wave_sum64(front);
     edges = wave_sum64(edges);
@@ -370,6 +404,7 @@ pprb_emit_k(const int32_t* __restrict__ row, uint64_t* __restrict__ mask, uint32
 struct PprState {
     // single
     float *p = nullptr, *r = nullptr, *m = nullptr;
+    uint32_t* tb = nullptr;  // touched bitmap [V/32 + 1]
     int4* q = nullptr;
     int64_t qcap = 0;
     int64_t* ctrl = nullptr;
@@ -405,7 +440,7 @@ void ppr_release(Graph* g) {
     for (size_t i = 0; i < g_ps.size(); ++i)
         if (g_ps[i].first == g) {
             PprState* s = g_ps[i].second;
-            for (void* p : {(void*)s->p, (void*)s->r, (void*)s->m, (void*)s->q, (void*)s->ctrl, (void*)s->P,
+            for (void* p : {(void*)s->p, (void*)s->r, (void*)s->m, (void*)s->tb, (void*)s->q, (void*)s->ctrl, (void*)s->P,
                             (void*)s->R, (void*)s->M, (void*)s->RU, (void*)s->mask, (void*)s->tbits,
                             (void*)s->items, (void*)s->imask, (void*)s->srcs})
                 if (p) (void)hipFree(p);
@@ -437,6 +472,7 @@ static int ensure_ppr_single(Graph* g, PprState* s) {
         HIP_TRY(hipMalloc(&s->p, sizeof(float) * V));
         HIP_TRY(hipMalloc(&s->r, sizeof(float) * V));
         HIP_TRY(hipMalloc(&s->m, sizeof(float) * V));
+        HIP_TRY(hipMalloc(&s->tb, sizeof(uint32_t) * ((V + 31) / 32 + 1)));
         HIP_TRY(hipMalloc(&s->ctrl, sizeof(int64_t) * P_N));
         HIP_TRY(hipHostMalloc((void**)&s->h_ctrl, sizeof(int64_t) * P_N, hipHostMallocDefault));
     }
@@ -473,12 +509,13 @@ static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, f
     const int64_t V = g->V;
     const int grid = std::max(1, g->num_cus) * g->tune.grid_per_cu;
     const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (V + 255) / 256));
+    const int wgrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, ((V + 31) / 32 + 255) / 256));
     const double t0 = now_ms();
-    hipLaunchKernelGGL(ppr_init_k, dim3(egrid), dim3(256), 0, strm, s->p, s->r, s->m, V);
+    hipLaunchKernelGGL(ppr_init_k, dim3(egrid), dim3(256), 0, strm, s->p, s->r, s->m, s->tb, V);
     hipLaunchKernelGGL(ppr_seed_k, dim3(1), dim3(256), 0, strm, g->d_row, g->d_adjf,
                        pruned ? g->d_hrow : (const int32_t*)nullptr, pruned ? g->d_hadjf : (const int2*)nullptr,
-                       s->p, s->m, s->ctrl, src, alpha, pruned ? 1 : 0);
-    hipLaunchKernelGGL(ppr_expand_k, dim3(egrid), dim3(256), 0, strm, g->d_row, s->p, s->r, s->m, s->q,
+                       s->p, s->m, s->tb, s->ctrl, src, alpha, pruned ? 1 : 0);
+    hipLaunchKernelGGL(ppr_expand_k, dim3(wgrid), dim3(256), 0, strm, g->d_row, s->p, s->r, s->m, s->tb, s->q,
                        s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 1);
     HIP_TRY(hipGetLastError());
     int rounds = 0;
@@ -486,10 +523,10 @@ static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, f
     for (;;) {
         const int K = budget > 0 ? 1 : g->tune.iters_per_graph;
         for (int k = 0; k < K; ++k) {
-            hipLaunchKernelGGL(ppr_push_k, dim3(grid), dim3(256), 0, strm, g->d_adjf, s->m, s->q, s->ctrl,
-                               alpha);
-            hipLaunchKernelGGL(ppr_expand_k, dim3(egrid), dim3(256), 0, strm, g->d_row, s->p, s->r, s->m,
-                               s->q, s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 0);
+            hipLaunchKernelGGL(ppr_push_k, dim3(grid), dim3(256), 0, strm, g->d_adjf, s->m, s->tb, s->q,
+                               s->ctrl, alpha);
+            hipLaunchKernelGGL(ppr_expand_k, dim3(wgrid), dim3(256), 0, strm, g->d_row, s->p, s->r, s->m,
+                               s->tb, s->q, s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 0);
         }
         HIP_TRY(hipGetLastError());
         rounds += K;
