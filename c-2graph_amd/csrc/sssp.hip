@@ -64,6 +64,7 @@ enum : int {
 // target has out-edges (or hidden edges), i.e. when activating it can lead
 // anywhere.  Vertex ids are < 2^31, so the bit is otherwise unused.
 constexpr int kIdMask = 0x7fffffff;
+constexpr int kSteps = 4;  // 64-edge steps per lane whose loads are in flight together
 
 __global__ void sssp_init_k(int32_t* __restrict__ dist, uint32_t* __restrict__ bits,
                             uint32_t* __restrict__ hbits, uint8_t* __restrict__ hid, int64_t V) {
@@ -110,32 +111,6 @@ __global__ void sssp_seed_k(const int32_t* __restrict__ row, const int32_t* __re
     }
 }
 
-// Relaxation of one edge for the calling lane.
-template <bool PRUNED>
-__device__ __forceinline__ void relax_edge(const int2* __restrict__ adj, const int2* __restrict__ hadj,
-                                           int32_t* __restrict__ dist, uint32_t* __restrict__ bits,
-                                           uint32_t* __restrict__ hbits, uint8_t* __restrict__ hid,
-                                           int2 a, int d_u, int okind, int64_t& impr) {
-    const int v = a.x & kIdMask;
-    const int w = a.y;
-    if (d_u > INT32_MAX - w) return;
-    const int cand = d_u + w;
-    if (cand >= dist[v]) return;
-    // No wait for the old value: a lower dist[v] written since the read above
-    // comes from this round, whose writer marked the same activation bit.
-    atomicMin(&dist[v], cand);
-    ++impr;
-    if (a.x < 0) atomicOr(&bits[v >> 5], 1u << (v & 31));
-    if (PRUNED) {
-        if (okind) hid[v] = 1;
-        if (okind || hid[v]) atomicOr(&hbits[v >> 5], 1u << (v & 31));
-    }
-}
-
-// One round of relaxation.  A wave takes 64 items, scans their edge counts
-// and walks the concatenated edges 128 at a time (two 64-edge steps whose
-// loads are in flight together); improved targets are marked in the
-// activation bitmap with fire-and-forget atomicOr (no queue atomics here).
 // Items are int4(u, e_begin, e_end, kind | slot << 1): a single query is
 // slot 0; a slot pool runs several queries side by side, slot s owning
 // dist[s*V..], hid[s*V..] and the bitmap words [s*nwp..] of each map.
@@ -163,25 +138,48 @@ sssp_relax_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
         }
         int tot;
         const int pre = wave_excl_scan(cnt, &tot);
-        for (int s = 0; s < tot; s += 2 * kWave) {
-            const int k0 = s + lane, k1 = s + kWave + lane;
-            const int o0 = wave_owner(pre, k0), o1 = wave_owner(pre, k1);
-            const int e0 = __shfl(eb, o0, 64) + (k0 - __shfl(pre, o0, 64));
-            const int e1 = __shfl(eb, o1, 64) + (k1 - __shfl(pre, o1, 64));
-            const int d0 = __shfl(du, o0, 64), d1 = __shfl(du, o1, 64);
-            const int w0 = __shfl(kw, o0, 64), w1 = __shfl(kw, o1, 64);
-            const int kd0 = PRUNED ? (w0 & 1) : 0;
-            const int kd1 = PRUNED ? (w1 & 1) : 0;
-            int2 a0 = make_int2(0, 0), a1 = make_int2(0, 0);
-            if (k0 < tot) a0 = (PRUNED && kd0) ? hadj[e0] : adj[e0];
-            if (k1 < tot) a1 = (PRUNED && kd1) ? hadj[e1] : adj[e1];
-            const int64_t s0 = w0 >> 1, s1 = w1 >> 1;
-            if (k0 < tot)
-                relax_edge<PRUNED>(adj, hadj, dist + s0 * V, bits + s0 * nwp, PRUNED ? hbits + s0 * nwp : hbits,
-                                   PRUNED ? hid + s0 * V : hid, a0, d0, kd0, impr);
-            if (k1 < tot)
-                relax_edge<PRUNED>(adj, hadj, dist + s1 * V, bits + s1 * nwp, PRUNED ? hbits + s1 * nwp : hbits,
-                                   PRUNED ? hid + s1 * V : hid, a1, d1, kd1, impr);
+        // kSteps 64-edge steps per pass: every lane first loads its edges and
+        // the targets' current distances (all in flight together), then
+        // compares and issues the improving atomics.
+        for (int s = 0; s < tot; s += kSteps * kWave) {
+            int2 a[kSteps];
+            int du_j[kSteps], cur[kSteps], kd[kSteps];
+            int64_t sl[kSteps];
+#pragma unroll
+            for (int j = 0; j < kSteps; ++j) {
+                const int k = s + j * kWave + lane;
+                const int o = wave_owner(pre, k);
+                const int e = __shfl(eb, o, 64) + (k - __shfl(pre, o, 64));
+                const int w = __shfl(kw, o, 64);
+                du_j[j] = __shfl(du, o, 64);
+                kd[j] = PRUNED ? (w & 1) : 0;
+                sl[j] = w >> 1;
+                a[j] = make_int2(0, 0);
+                if (k < tot) a[j] = (PRUNED && kd[j]) ? hadj[e] : adj[e];
+            }
+#pragma unroll
+            for (int j = 0; j < kSteps; ++j) {
+                const bool ok = s + j * kWave + lane < tot && du_j[j] <= INT32_MAX - a[j].y;
+                cur[j] = ok ? dist[sl[j] * V + (a[j].x & kIdMask)] : INT32_MIN;
+            }
+#pragma unroll
+            for (int j = 0; j < kSteps; ++j) {
+                if (cur[j] == INT32_MIN) continue;
+                const int v = a[j].x & kIdMask;
+                const int cand = du_j[j] + a[j].y;
+                if (cand >= cur[j]) continue;
+                // No wait for the old value: a lower dist[v] written since the
+                // read above comes from this round, whose writer marked the same
+                // activation bit.
+                atomicMin(&dist[sl[j] * V + v], cand);
+                ++impr;
+                if (a[j].x < 0) atomicOr(&bits[sl[j] * nwp + (v >> 5)], 1u << (v & 31));
+                if (PRUNED) {
+                    uint8_t* h = hid + sl[j] * V;
+                    if (kd[j]) h[v] = 1;
+                    if (kd[j] || h[v]) atomicOr(&hbits[sl[j] * nwp + (v >> 5)], 1u << (v & 31));
+                }
+            }
         }
     }
     impr = wave_sum64(impr);

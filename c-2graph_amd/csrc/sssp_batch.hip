@@ -362,13 +362,22 @@ __global__ void batch_checksum_k(const int32_t* __restrict__ row, const int32_t*
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     int64_t s = 0, m = 0;
     int c = 0;
-    for (int64_t v = wave; v < V; v += nwaves) {
-        const int d = dist[v * 64 + lane];
-        if (d != INT32_MAX) {
-            s += d;
-            c += 1;
-            m += row[v + 1] - row[v];
+    // four rows per step, their loads in flight together
+    for (int64_t v0 = wave * 4; v0 < V; v0 += nwaves * 4) {
+        int d[4], deg[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int64_t v = v0 + k;
+            d[k] = v < V ? dist[v * 64 + lane] : INT32_MAX;
+            deg[k] = v < V ? row[v + 1] - row[v] : 0;
         }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (d[k] != INT32_MAX) {
+                s += d[k];
+                c += 1;
+                m += deg[k];
+            }
     }
     atomicAdd((unsigned long long*)&sums[lane], (unsigned long long)s);
     atomicAdd(&reached[lane], c);
@@ -599,7 +608,7 @@ static int batch_checksums(Graph* g, int n, int64_t* sums, int32_t* reached, int
     HIP_TRY(hipMemsetAsync(s->reach, 0, sizeof(int32_t) * 64 * groups, g->stream));
     HIP_TRY(hipMemsetAsync(s->medges, 0, sizeof(int64_t) * 64 * groups, g->stream));
     for (int gi = 0; gi < groups; ++gi)
-        hipLaunchKernelGGL(batch_checksum_k, dim3(std::max(1, g->num_cus) * 4), dim3(256), 0, g->stream,
+        hipLaunchKernelGGL(batch_checksum_k, dim3(std::max(1, g->num_cus) * 8), dim3(256), 0, g->stream,
                            g->d_row, w.dist + (int64_t)gi * V * 64, V, s->sums + 64 * gi,
                            s->reach + 64 * gi, s->medges + 64 * gi);
     HIP_TRY(hipGetLastError());
