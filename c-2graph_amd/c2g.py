@@ -374,6 +374,14 @@ class Graph:
         return p, r, st
 
     # ---- adsorption
+    def adsorption_push(self, src):
+        """The reference's single-source adsorption push (c2g_adsorption_push)."""
+        p = np.empty(self.V, np.float32)
+        r = np.empty(self.V, np.float32)
+        st = Stats()
+        check(lib().c2g_adsorption_push(self._h, int(src), _ptr(p, C.c_float), _ptr(r, C.c_float), C.byref(st)))
+        return p, r, st
+
     def adsorption(self, seed_v, seed_l, L=16, p_inj=0.25, p_cont=0.70, p_abnd=0.05, iters=20):
         seed_v = np.ascontiguousarray(seed_v, np.int32)
         seed_l = np.ascontiguousarray(seed_l, np.int32)

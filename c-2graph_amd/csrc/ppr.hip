@@ -59,12 +59,15 @@ __device__ __forceinline__ void touch(uint32_t* tb, int v) { atomicOr(&tb[v >> 5
 __global__ void ppr_seed_k(const int32_t* __restrict__ row, const int2* __restrict__ adjf,
                            const int32_t* __restrict__ hrow, const int2* __restrict__ hadjf,
                            float* __restrict__ p, float* __restrict__ m, uint32_t* __restrict__ tb,
-                           int64_t* __restrict__ ctrl, int32_t src, float alpha, int pruned) {
+                           int64_t* __restrict__ ctrl, int32_t src, float alpha, int pruned, float res0,
+                           float p_src) {
     if (threadIdx.x == 0)
         for (int i = 0; i < P_N; ++i) ctrl[i] = 0;
     if (!pruned) {
+        // residual res0 at the source; its first (forced) activation adds alpha*res0 to p_src
         if (threadIdx.x == 0) {
-            m[src] = 1.0f;
+            m[src] = res0;
+            p[src] = p_src;
             touch(tb, src);
         }
         return;
@@ -135,15 +138,15 @@ ppr_push_k(const int2* __restrict__ adjf, float* __restrict__ m, uint32_t* __res
 }
 
 // Threshold pass over the touched vertices only (the touched bitmap, one
-// 32-vertex word per lane): r[v] += m[v]; if r[v] > eps*outdeg(v) (or
-// `force`, the first round), p[v] += alpha*r[v], r[v] = 0 and v's items join
+// 32-vertex word per lane): r[v] += m[v]; if r[v] > eps*outdeg(v) (eps
+// alone when !degscale; always in the `force` first round), p[v] += alpha*r[v], r[v] = 0 and v's items join
 // the next round carrying the residual to push.  Two passes per word: count
 // the items of the activated vertices, take the queue range with one atomic
 // per wave, then apply the updates and write the items.
 __global__ void __launch_bounds__(256)
 ppr_expand_k(const int32_t* __restrict__ row, float* __restrict__ p, float* __restrict__ r,
              float* __restrict__ m, uint32_t* __restrict__ tb, int4* __restrict__ q, int64_t* __restrict__ ctrl,
-             int64_t V, float alpha, float eps, int chunk, int64_t qcap, int force) {
+             int64_t V, float alpha, float eps, int chunk, int64_t qcap, int force, int degscale) {
     const int lane = lane_id();
     const int64_t nw = (V + 31) >> 5;
     const int64_t nwr = (nw + 63) & ~(int64_t)63;  // uniform trip count per wave
@@ -160,7 +163,7 @@ ppr_expand_k(const int32_t* __restrict__ row, float* __restrict__ p, float* __re
             const float mv = m[v];
             if (mv == 0.0f) continue;
             const int d = row[v + 1] - row[v];
-            if (force || r[v] + mv > (float)d * eps) cnt += (d + chunk - 1) / chunk;
+            if (force || r[v] + mv > (degscale ? (float)d * eps : eps)) cnt += (d + chunk - 1) / chunk;
         }
         int tot;
         const int off = wave_excl_scan(cnt, &tot);
@@ -179,7 +182,7 @@ ppr_expand_k(const int32_t* __restrict__ row, float* __restrict__ p, float* __re
             m[v] = 0.0f;
             const float rv = r[v] + mv;
             const int r0 = row[v], r1 = row[v + 1];
-            if (force || rv > (float)(r1 - r0) * eps) {
+            if (force || rv > (degscale ? (float)(r1 - r0) * eps : eps)) {
                 p[v] += alpha * rv;
                 r[v] = 0.0f;
                 front += 1;
@@ -485,9 +488,19 @@ static int ensure_ppr_single(Graph* g, PprState* s) {
 }
 
 static const int kPprMaxRounds = 100000;
+
+// Seeding and threshold form of a single-source push: the workload's PPR is
+// {threshold eps*outdeg, r0 = 1, p[s] starts at 0}; the reference's adsorption
+// push is {constant threshold, r0 = 10, p[s] starts at (1-2*alpha)*r0}.
+struct PushSpec {
+    int degscale = 1;
+    float r0 = 1.0f;
+    float p_src = 0.0f;
+};
+
 static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, float eps, int32_t budget,
                          int32_t handoff_min, float* p_out, float* r_out, std::vector<int32_t>* frontier,
-                         std::vector<float>* fr_ru, c2g_stats* st);
+                         std::vector<float>* fr_ru, c2g_stats* st, const PushSpec& spec = PushSpec());
 
 int ppr_single(Graph* g, hipStream_t strm, int32_t src, float alpha, float eps, float* p_out, float* r_out,
                       c2g_stats* st) {
@@ -499,7 +512,7 @@ int ppr_single(Graph* g, hipStream_t strm, int32_t src, float alpha, float eps, 
 // restored to the residual it was about to push (p already credited).
 static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, float eps, int32_t budget,
                          int32_t handoff_min, float* p_out, float* r_out, std::vector<int32_t>* frontier,
-                         std::vector<float>* fr_ru, c2g_stats* st) {
+                         std::vector<float>* fr_ru, c2g_stats* st, const PushSpec& spec) {
     int rc = ppr_check(g, alpha, eps);
     if (rc) return rc;
     if (src < 0 || src >= g->V) return fail(C2G_EINVAL, "source %d out of range [0,%d)", src, g->V);
@@ -514,9 +527,9 @@ static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, f
     hipLaunchKernelGGL(ppr_init_k, dim3(egrid), dim3(256), 0, strm, s->p, s->r, s->m, s->tb, V);
     hipLaunchKernelGGL(ppr_seed_k, dim3(1), dim3(256), 0, strm, g->d_row, g->d_adjf,
                        pruned ? g->d_hrow : (const int32_t*)nullptr, pruned ? g->d_hadjf : (const int2*)nullptr,
-                       s->p, s->m, s->tb, s->ctrl, src, alpha, pruned ? 1 : 0);
+                       s->p, s->m, s->tb, s->ctrl, src, alpha, pruned ? 1 : 0, spec.r0, spec.p_src);
     hipLaunchKernelGGL(ppr_expand_k, dim3(wgrid), dim3(256), 0, strm, g->d_row, s->p, s->r, s->m, s->tb, s->q,
-                       s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 1);
+                       s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 1, spec.degscale);
     HIP_TRY(hipGetLastError());
     int rounds = 0;
     int64_t front_prev = 1;
@@ -526,7 +539,7 @@ static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, f
             hipLaunchKernelGGL(ppr_push_k, dim3(grid), dim3(256), 0, strm, g->d_adjf, s->m, s->tb, s->q,
                                s->ctrl, alpha);
             hipLaunchKernelGGL(ppr_expand_k, dim3(wgrid), dim3(256), 0, strm, g->d_row, s->p, s->r, s->m,
-                               s->tb, s->q, s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 0);
+                               s->tb, s->q, s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 0, spec.degscale);
         }
         HIP_TRY(hipGetLastError());
         rounds += K;
@@ -716,6 +729,24 @@ int c2g_ppr_migrate(c2g_graph* h, int32_t src, float alpha, float eps, int32_t b
         st->edges_relaxed += hs.edges_relaxed;
     }
     return C2G_OK;
+}
+
+int c2g_adsorption_push(c2g_graph* h, int32_t src, float* p_out, float* r_out, c2g_stats* st) {
+    if (!h || !p_out) return fail(C2G_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(h->g.mu);
+    Graph* g = &h->g;
+    if (st) memset(st, 0, sizeof *st);
+    if (ppr_pruned(g)) return fail(C2G_ENOTSUP, "the adsorption push runs on the full graph (origin variant)");
+    // src/adsorption/w-base/Main.cu:63-64 (alpha 0.2, rmax 1/V), :149-152
+    // (r[s] = 10, p[s] += 0.8 * r[s]), :248 (message (1-alpha) r w), :351-355
+    // (r += m; r > rmax activates with p += alpha r).
+    const float alpha = 0.2f;
+    PushSpec spec;
+    spec.degscale = 0;
+    spec.r0 = 10.0f;
+    spec.p_src = (1.0f - 2.0f * alpha) * spec.r0;  // the forced first activation adds alpha*r0
+    const float rmax = 1.0f / (float)std::max(g->V, 1);
+    return ppr_single_ex(g, g->stream, src, alpha, rmax, 0, 0, p_out, r_out, nullptr, nullptr, st, spec);
 }
 
 int c2g_ppr_batch(c2g_graph* h, const int32_t* srcs, int32_t n, float alpha, float eps, float* p_out,

@@ -287,6 +287,14 @@ int c2g_ppr_batch(c2g_graph* g, const int32_t* srcs, int32_t n, float alpha,
 /* ======================================================================= */
 /* Adsorption                                                                */
 /* ======================================================================= */
+/* The reference's own "adsorption" kernel, kept as a reference mode:
+ * calcuatePPR of src/adsorption/w-base/Main.cu:123-395, a single-source push
+ * on the full graph with alpha = 0.2, r[s] = 10, p[s] = 0.8 * r[s]
+ * (:63-64, :149-152), messages (1-alpha) r_u f(u,v) (:248) and a constant
+ * threshold rmax = 1/V (:351-355).  Rounds run to convergence (the reference
+ * caps them at 2000, :186).  p_out, r_out: float[V] (r_out nullable). */
+int c2g_adsorption_push(c2g_graph* g, int32_t src, float* p_out, float* r_out, c2g_stats* st);
+
 /* Multi-label adsorption (reference: adsorption/w-base/Main.cu:123-395 is a
  * single-source push; the multi-label recurrence here follows Baluja et al.
  * 2008 / Talukdar & Crammer 2009):

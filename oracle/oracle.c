@@ -28,6 +28,12 @@
  *                      certificate |p - pi|_1 <= sum(r) against orc_ppr_power.
  *   orc_adsorption     FP64 Jacobi of Y_{t+1} = p_inj*Y0 + p_cont*W*Y_t.  Pinned
  *                      by a dense numpy matrix-power evaluation on small graphs.
+ *   orc_adsorption_push  restatement of the reference's adsorption kernel
+ *                      (src/adsorption/w-base/Main.cu: alpha 0.2, r[s] = 10,
+ *                      p[s] = 0.8*r[s], constant threshold 1/V).  Pinned by the
+ *                      push invariant against orc_ppr_power: with
+ *                      p' = p - (0.8-alpha)*10*e_s, |10*pi_s - p'|_1 = sum(r)
+ *                      on graphs without dangling vertices, and r <= 1/V.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -189,10 +195,10 @@ int orc_ppr_power(int32_t V, const int32_t* vl, const int32_t* el, const float* 
  * r[v] > eps*outdeg(v) and a zero-initialised flag.  p, r updated in place
  * from the active list `frontier` (count entries); FP32 state like the
  * reference.  Returns the number of rounds. */
-int orc_forward_push(int32_t V, const int32_t* vl, const int32_t* el, const float* fl,
-                     float alpha, float eps, float* p, float* r,
-                     const int32_t* frontier, int32_t count, int32_t max_rounds,
-                     int64_t* push_out) {
+static int push_rounds(int32_t V, const int32_t* vl, const int32_t* el, const float* fl,
+                       float alpha, float eps, int degscale, float* p, float* r,
+                       const int32_t* frontier, int32_t count, int32_t max_rounds,
+                       int64_t* push_out) {
     if (count <= 0) return 0;
     float* msg = (float*)calloc((size_t)V, sizeof(float));
     uint8_t* flag = (uint8_t*)calloc((size_t)V, 1);
@@ -223,7 +229,7 @@ int orc_forward_push(int32_t V, const int32_t* vl, const int32_t* el, const floa
             r[v] += msg[v];
             msg[v] = 0.0f;
             flag[v] = 0;
-            if (r[v] > (float)(vl[v + 1] - vl[v]) * eps) {
+            if (r[v] > (degscale ? (float)(vl[v + 1] - vl[v]) * eps : eps)) {
                 p[v] += r[v] * alpha;
                 act[na++] = v;
             }
@@ -232,6 +238,26 @@ int orc_forward_push(int32_t V, const int32_t* vl, const int32_t* el, const floa
     free(msg); free(flag); free(act); free(tch);
     if (push_out) *push_out = pushes;
     return rounds;
+}
+
+int orc_forward_push(int32_t V, const int32_t* vl, const int32_t* el, const float* fl,
+                     float alpha, float eps, float* p, float* r,
+                     const int32_t* frontier, int32_t count, int32_t max_rounds,
+                     int64_t* push_out) {
+    return push_rounds(V, vl, el, fl, alpha, eps, 1, p, r, frontier, count, max_rounds, push_out);
+}
+
+/* The reference's adsorption kernel (src/adsorption/w-base/Main.cu): alpha
+ * 0.2 and rmax 1/V (:63-64), r[s] = 10 and p[s] += 0.8*r[s] (:149-152), then
+ * rounds of push (1-alpha) r_u f(u,v) (:208-320) and activation r > rmax with
+ * p += alpha*r (:351-355), the source active in the first round. */
+int orc_adsorption_push(int32_t V, const int32_t* vl, const int32_t* el, const float* fl,
+                        int32_t src, float* p, float* r, int32_t max_rounds, int64_t* push_out) {
+    const float alpha = 0.2f, rmax = 1.0f / (float)V;
+    for (int32_t v = 0; v < V; ++v) { p[v] = 0.0f; r[v] = 0.0f; }
+    r[src] = 10.0f;
+    p[src] += 0.8f * r[src];
+    return push_rounds(V, vl, el, fl, alpha, rmax, 0, p, r, &src, 1, max_rounds, push_out);
 }
 
 /* Seeds r[s]=1, p[s]=alpha (src/ppr/base/Main.cu:146-150) then pushes. */

@@ -38,6 +38,7 @@ def lib():
                                        C.c_int32, C.c_int32, i64]
         L.orc_ppr_push.argtypes = [C.c_int32, i32, i32, f32, C.c_int32, C.c_float, C.c_float, f32, f32,
                                    C.c_int32, i64]
+        L.orc_adsorption_push.argtypes = [C.c_int32, i32, i32, f32, C.c_int32, f32, f32, C.c_int32, i64]
         L.orc_adsorption.argtypes = [C.c_int32, i32, i32, f32, i32, i32, C.c_int32, C.c_int32, C.c_double,
                                      C.c_double, C.c_int32, f64]
         L.orc_adsorption.restype = None
@@ -86,6 +87,18 @@ def ppr_push(vlist, elist, flist, src, alpha=0.15, eps=1e-7, max_rounds=1 << 30)
     pushes = np.zeros(1, np.int64)
     rounds = lib().orc_ppr_push(V, _c(vlist, np.int32), _c(elist, np.int32), _c(flist, np.float32), int(src),
                                 float(alpha), float(eps), p, r, int(max_rounds), pushes)
+    return p, r, rounds, int(pushes[0])
+
+
+def adsorption_push(vlist, elist, flist, src, max_rounds=1 << 30):
+    """The reference's adsorption kernel (single-source push, alpha 0.2,
+    r[s] = 10, constant threshold 1/V): returns p, r, rounds, pushes."""
+    V = len(vlist) - 1
+    p = np.empty(V, np.float32)
+    r = np.empty(V, np.float32)
+    pushes = np.zeros(1, np.int64)
+    rounds = lib().orc_adsorption_push(V, _c(vlist, np.int32), _c(elist, np.int32), _c(flist, np.float32),
+                                       int(src), p, r, int(max_rounds), pushes)
     return p, r, rounds, int(pushes[0])
 
 

@@ -64,6 +64,21 @@ def random_graph(V, E, seed, wmax=10, zero_ok=False):
     return dict(V=V, E=len(v), vlist=vl, elist=v.astype(np.int32), wlist=w, flist=f)
 
 
+def no_dangling_graph(V, E, seed):
+    """Random digraph with the cycle i -> i+1 added, so no vertex is dangling;
+    row-normalised float weights."""
+    rng = np.random.default_rng(seed)
+    u = np.concatenate([np.arange(V), rng.integers(0, V, E)])
+    v = np.concatenate([(np.arange(V) + 1) % V, rng.integers(0, V, E)])
+    keep = u != v
+    key = np.unique(u[keep].astype(np.int64) * V + v[keep])
+    u, v = key // V, key % V
+    vl = np.concatenate([[0], np.cumsum(np.bincount(u, minlength=V))]).astype(np.int32)
+    f = rng.random(len(v)) + 1e-3
+    f = (f / np.bincount(u, weights=f, minlength=V)[u]).astype(np.float32)
+    return dict(V=V, E=len(v), vlist=vl, elist=v.astype(np.int32), flist=f)
+
+
 def check_sssp_certificate(vl, el, wl, src, d):
     """Exactness certificate for int SSSP with positive weights (Graph500-style):
     d[s]=0; d[v] <= d[u]+w on every edge out of a reached u; every reached v != s

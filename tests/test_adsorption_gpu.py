@@ -8,7 +8,7 @@ import pytest
 
 import c2g
 import oracle
-from conftest import random_graph, rmat_graph
+from conftest import no_dangling_graph, random_graph, rmat_graph
 
 pytestmark = pytest.mark.gpu
 
@@ -72,3 +72,34 @@ def test_multi_chunk_result_copy(gpu):
         Y, _ = g.adsorption(sv, sl, L=40, iters=4)
     assert 2 * (32 << 20) < Y.nbytes < 3 * (32 << 20)
     _check(Y, oracle.adsorption(G["vlist"], G["elist"], G["flist"], sv, sl, L=40, iters=4))
+
+
+def test_reference_push_mode(gpu):
+    # the reference's own adsorption kernel (src/adsorption/w-base/Main.cu):
+    # single-source push, alpha 0.2, r[s] = 10, p[s] = 0.8*10, threshold 1/V.
+    # Against the oracle's serial restatement (L1 <= 2e-4: FP32 atomics in any
+    # order, ten times the unit mass) and the push invariant
+    # |10 pi_s - (p - 6 e_s)|_1 = sum(r) with pi from the FP64 power iteration.
+    G = no_dangling_graph(3000, 30000, seed=3)
+    V = G["V"]
+    with c2g.Graph.from_csr(G["vlist"], G["elist"], flist=G["flist"], device=0) as g:
+        for s in (0, 5, V - 1):
+            p, r, st = g.adsorption_push(s)
+            po, ro, rounds, _ = oracle.adsorption_push(G["vlist"], G["elist"], G["flist"], s)
+            assert np.abs(p.astype(np.float64) - po).sum() <= 2e-4, s
+            assert r.max() <= 1.0 / V + 1e-7
+            pi = oracle.ppr_power(G["vlist"], G["elist"], G["flist"], s, alpha=0.2, tol=1e-15)
+            q = p.astype(np.float64)
+            q[s] -= 6.0
+            assert abs(np.abs(10.0 * pi - q).sum() - r.astype(np.float64).sum()) <= 1e-3, s
+            assert st.iterations > 1
+
+
+def test_reference_push_mode_refuses_pruned_graph(gpu):
+    G = random_graph(300, 900, seed=4)
+    P = c2g.prune_ppr_mem(G["vlist"], G["elist"], G["flist"])
+    with c2g.Graph.from_csr(P["vlist"], P["elist"], flist=P["wlist"], hvlist=P["hvlist"], helist=P["helist"],
+                            hwf=P["hwlist"], variant=c2g.VARIANT_PPR_PURN, device=0) as g:
+        with pytest.raises(c2g.C2GError) as e:
+            g.adsorption_push(0)
+        assert e.value.code == c2g.ENOTSUP

@@ -106,3 +106,11 @@ def test_ppr_tools(gpu, ds, tmp_path):
     run("ppr-purn-M", ds, 7, 2, "--reps", 1)
     p = run("adsorption", ds, "--labels", 4, "--iters", 3)
     assert "gpu runtime" in p.stdout
+    # the reference's adsorption main: dataset, source; first 11 p/r lines
+    txt = str(tmp_path / "push.txt")
+    p = run("adsorption-push", ds, 7, "--out", txt)
+    assert p.stdout.count("\tpagerank\t") == 11 and "gpu runtime:" in p.stdout
+    rows = np.loadtxt(txt)
+    po, ro, _, _ = oracle.adsorption_push(G["vlist"], G["elist"], G["flist"], 7)
+    assert np.array_equal(rows[:, 0], np.arange(G["V"]))
+    assert np.abs(rows[:, 1] - po).sum() <= 2e-3  # printed with 6 significant digits

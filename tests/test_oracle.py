@@ -9,7 +9,7 @@ import scipy.sparse.csgraph as cg
 import scipy.sparse.linalg as spla
 
 import oracle
-from conftest import INT_MAX, check_sssp_certificate, random_graph, rmat_graph
+from conftest import INT_MAX, check_sssp_certificate, no_dangling_graph, random_graph, rmat_graph
 
 
 def _scipy_dist(G, s):
@@ -153,3 +153,24 @@ def test_adsorption_vs_dense_matrix_power():
     # closed form of the limit: (I - 0.7 W)^-1 0.25 Y0 ; 20 steps are within 0.7^20
     Yinf = np.linalg.solve(np.eye(V) - 0.7 * W, 0.25 * Y0)
     assert np.abs(Y - Yinf).max() <= 0.7 ** 20 / 0.3 + 1e-12
+
+
+def test_adsorption_push_invariant():
+    # the reference's adsorption push (alpha 0.2, r[s] = 10, p[s] = 0.8*10,
+    # constant threshold 1/V): with p' = p - (0.8 - 0.2)*10 e_s the forward-push
+    # invariant gives |10 pi_s - p'|_1 = sum(r) on a graph without dangling
+    # vertices, pi from the FP64 power iteration
+    G = no_dangling_graph(400, 3000, seed=9)
+    assert np.all(np.diff(G["vlist"]) > 0)
+    for s in (0, 17, 399):
+        p, r, rounds, pushes = oracle.adsorption_push(G["vlist"], G["elist"], G["flist"], s)
+        pi = oracle.ppr_power(G["vlist"], G["elist"], G["flist"], s, alpha=0.2, tol=1e-15)
+        q = p.astype(np.float64)
+        q[s] -= (0.8 - 0.2) * 10.0
+        l1 = np.abs(10.0 * pi - q).sum()
+        assert abs(l1 - r.astype(np.float64).sum()) <= 1e-4 * 10, (s, l1, r.sum())
+        assert r.max() <= 1.0 / 400 + 1e-7
+        assert rounds > 1 and pushes > 0
+    # a plausible mistake fails: crediting the source alpha*r0 instead of 0.8*r0
+    q = p.astype(np.float64)
+    assert abs(np.abs(10.0 * pi - q).sum() - r.astype(np.float64).sum()) > 1.0
