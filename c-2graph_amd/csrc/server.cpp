@@ -26,6 +26,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <unordered_map>
 
 namespace c2g {
@@ -39,7 +40,7 @@ struct Request {
     bool done = false;
     double t_submit = 0, t_admit = 0, t_done = 0;
     int64_t bytes = 0;
-    std::vector<char> result;
+    std::unique_ptr<char[]> result;  // bytes long; from the server's buffer pool
 };
 
 }  // namespace c2g
@@ -51,6 +52,8 @@ struct c2g_server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<int64_t> q_sssp, q_aux, q_ads, q_host;
+    std::mutex pool_mu;  // taken after `mu` when both are held
+    std::vector<std::unique_ptr<char[]>> pool[3];  // free result buffers per query kind
     std::unordered_map<int64_t, c2g::Request> reqs;
     int64_t next_id = 0;
     int64_t inflight = 0;
@@ -64,6 +67,29 @@ struct c2g_server {
 };
 
 namespace c2g {
+
+// Result buffers are recycled per query kind: a fresh V-sized (or V*L-sized)
+// buffer would be zero-filled or page-faulted on every request.  Neither call
+// may hold `mu`'s critical section for the copy itself.
+constexpr size_t kPoolMax = 64;
+static void take_buffer(c2g_server* s, Request* r) {
+    std::unique_ptr<char[]> b;
+    {
+        std::lock_guard<std::mutex> lk(s->pool_mu);
+        auto& fl = s->pool[r->kind];
+        if (!fl.empty()) {
+            b = std::move(fl.back());
+            fl.pop_back();
+        }
+    }
+    if (!b) b.reset(new char[(size_t)r->bytes]);
+    r->result = std::move(b);
+}
+static void give_back(c2g_server* s, int kind, std::unique_ptr<char[]> b) {
+    if (!b) return;
+    std::lock_guard<std::mutex> lk(s->pool_mu);
+    if (s->pool[kind].size() < kPoolMax) s->pool[kind].push_back(std::move(b));
+}
 
 static int64_t result_bytes(const c2g_server* s, int kind) {
     const int64_t V = s->g->V;
@@ -157,30 +183,28 @@ static void sssp_worker(c2g_server* s) {
         if (lbusy && rc == C2G_OK) rc = bserve_step(g, s->s_sssp, &lwork);
         // Retire queries with no queued work (all of them after an error).
         std::vector<int> done_s, lanes;
+        std::vector<Request*> rs_s, rs_l;
         std::vector<int32_t*> outs;
         {
             std::lock_guard<std::mutex> lk(s->mu);
             for (int i = 0; i < nslots; ++i)
                 if (slot_req[i] >= 0 && (rc != C2G_OK || !((swork >> i) & 1ull))) {
-                    s->reqs[slot_req[i]].result.resize(sizeof(int32_t) * g->V);
                     done_s.push_back(i);
+                    rs_s.push_back(&s->reqs[slot_req[i]]);
                 }
             for (int l = 0; l < nlanes; ++l)
                 if (lane_req[l] >= 0 && (rc != C2G_OK || !((lwork >> l) & 1ull))) {
-                    Request& r = s->reqs[lane_req[l]];
-                    r.result.resize(sizeof(int32_t) * g->V);
                     lanes.push_back(l);
-                    outs.push_back((int32_t*)r.result.data());
+                    rs_l.push_back(&s->reqs[lane_req[l]]);
                 }
         }
-        for (int i : done_s) {
-            Request* r;
-            {
-                std::lock_guard<std::mutex> lk(s->mu);
-                r = &s->reqs[slot_req[i]];
-            }
-            if (rc == C2G_OK) rc = qserve_extract(g, s->s_sssp, i, (int32_t*)r->result.data());
+        for (Request* r : rs_s) take_buffer(s, r);
+        for (Request* r : rs_l) {
+            take_buffer(s, r);
+            outs.push_back((int32_t*)r->result.get());
         }
+        for (size_t k = 0; k < done_s.size(); ++k)
+            if (rc == C2G_OK) rc = qserve_extract(g, s->s_sssp, done_s[k], (int32_t*)rs_s[k]->result.get());
         if (!lanes.empty() && rc == C2G_OK) rc = bserve_extract(g, s->s_sssp, lanes.data(), (int)lanes.size(), outs.data());
         if (!done_s.empty() || !lanes.empty()) {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -235,7 +259,6 @@ static void aux_worker(c2g_server* s, bool ads) {
                 q.pop_front();
                 s->used += r.bytes;
                 r.t_admit = t_admit;
-                r.result.resize(r.bytes);
                 ids.push_back(r.id);
             }
         }
@@ -244,20 +267,21 @@ static void aux_worker(c2g_server* s, bool ads) {
             std::lock_guard<std::mutex> lk(s->mu);
             for (int64_t id : ids) rs.push_back(&s->reqs[id]);
         }
+        for (Request* r : rs) take_buffer(s, r);
         int rc;
         if (ads) {
             Request* r = rs[0];
             ads_seeds(g, r->src, s->o.ads_labels, sv, sl);
             rc = ads_run(g, strm, sv.data(), sl.data(), (int32_t)sv.size(), s->o.ads_labels, s->o.p_inj,
-                         s->o.p_cont, s->o.p_abnd, s->o.ads_iters, (float*)r->result.data(), nullptr);
+                         s->o.p_cont, s->o.p_abnd, s->o.ads_iters, (float*)r->result.get(), nullptr);
         } else if (rs.size() == 1) {
-            rc = ppr_single(g, strm, rs[0]->src, s->o.alpha, s->o.eps, (float*)rs[0]->result.data(), nullptr, nullptr);
+            rc = ppr_single(g, strm, rs[0]->src, s->o.alpha, s->o.eps, (float*)rs[0]->result.get(), nullptr, nullptr);
         } else {
             srcs.clear();
             outs.clear();
             for (Request* r : rs) {
                 srcs.push_back(r->src);
-                outs.push_back((float*)r->result.data());
+                outs.push_back((float*)r->result.get());
             }
             rc = ppr_batch_to(g, strm, srcs.data(), (int32_t)srcs.size(), s->o.alpha, s->o.eps, outs.data(),
                               nullptr, nullptr);
@@ -286,22 +310,22 @@ static void host_worker(c2g_server* s) {
             s->q_host.pop_front();
             r = &s->reqs[id];
             r->t_admit = now_ms() - s->t0;
-            r->result.resize(r->bytes);
         }
+        take_buffer(s, r);
         int rc = C2G_OK;
         if (r->kind == C2G_Q_SSSP) {
-            int32_t* d = (int32_t*)r->result.data();
+            int32_t* d = (int32_t*)r->result.get();
             for (int32_t v = 0; v < g->V; ++v) d[v] = INT32_MAX;
             d[r->src] = 0;
             std::vector<uint8_t> hid;
             if (!g->hvl.empty()) { hid.assign((size_t)g->V, 0); hid[r->src] = 1; }
             rc = host_bf(g, d, &r->src, 1, hid.empty() ? nullptr : hid.data(), nullptr);
         } else if (r->kind == C2G_Q_PPR) {
-            rc = c2g_host_ppr(s->h, r->src, s->o.alpha, s->o.eps, (float*)r->result.data(), nullptr, nullptr);
+            rc = c2g_host_ppr(s->h, r->src, s->o.alpha, s->o.eps, (float*)r->result.get(), nullptr, nullptr);
         } else {
             ads_seeds(g, r->src, s->o.ads_labels, sv, sl);
             rc = host_ads(g, sv.data(), sl.data(), (int32_t)sv.size(), s->o.ads_labels, s->o.p_inj, s->o.p_cont,
-                          s->o.ads_iters, (float*)r->result.data());
+                          s->o.ads_iters, (float*)r->result.get());
         }
         std::lock_guard<std::mutex> lk(s->mu);
         finish(s, *r, rc, C2G_LANE_HOST);
@@ -420,10 +444,14 @@ int c2g_server_poll(c2g_server* s, int64_t id, int32_t wait_ms, void* out, c2g_r
         info->t_admit_ms = r.t_admit;
         info->t_done_ms = r.t_done;
     }
-    if (out && r.status == C2G_OK) memcpy(out, r.result.data(), r.result.size());
-    const int st = r.status;
+    const int st = r.status, kind = r.kind;
+    const size_t n = (size_t)r.bytes;
+    std::unique_ptr<char[]> buf = std::move(r.result);
     s->reqs.erase(id);
     s->inflight--;
+    lk.unlock();  // the copy runs without blocking the lanes
+    if (out && st == C2G_OK && buf) memcpy(out, buf.get(), n);
+    give_back(s, kind, std::move(buf));
     if (st < 0) return fail(st, "request %lld failed", (long long)id);
     return 1;
 }
