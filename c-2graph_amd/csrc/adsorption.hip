@@ -155,6 +155,15 @@ ads_spmm4_k(const int2* __restrict__ adjf, const int4* __restrict__ items, int64
     }
 }
 
+// out[v][l] = Y[v][c0 + l] for l < L: one query's label block of a batched run.
+__global__ void ads_cols_k(const float* __restrict__ Y, int Lt, int c0, int L, float* __restrict__ out, int64_t V) {
+    const int64_t n = V * L;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = i / L;
+        out[i] = Y[v * Lt + c0 + (i - v * L)];
+    }
+}
+
 struct AdsState {
     int4* items = nullptr;
     int64_t nitems = 0;
@@ -251,17 +260,73 @@ static void launch_spmm4(Graph* g, hipStream_t strm, AdsState* s, const float* Y
                        (const float4*)Yt, (float4*)Yn, (const float4*)Yinj, L / 4, pc);
 }
 
-int ads_run(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t* sl, int32_t nseed, int32_t L, float p_inj,
-                   float p_cont, float p_abnd, int32_t iters, float* y_out, c2g_stats* st) {
+static int ads_core(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t* sl, int32_t nseed, int32_t L,
+                    float p_inj, float p_cont, int32_t iters, int* cur_out, c2g_stats* st);
+
+static int ads_check(Graph* g, int32_t L, float p_inj, float p_cont, float p_abnd, int32_t iters) {
     if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");
     if (!g->d_adjf) return fail(C2G_ENOTSUP, "graph has no float weights (flist) for adsorption");
     if (L < 1 || L > 64) return fail(C2G_EINVAL, "labels must be in [1,64]");
     if (iters < 0) return fail(C2G_EINVAL, "iters must be >= 0");
     if (p_inj < 0 || p_cont < 0 || p_abnd < 0 || std::fabs(p_inj + p_cont + p_abnd - 1.0f) > 1e-5f)
         return fail(C2G_EINVAL, "p_inj, p_cont, p_abnd must be >= 0 and sum to 1");
+    return C2G_OK;
+}
+
+// k queries of L labels each as one run over k*L labels (columns of the
+// recurrence never mix); query j's labels are columns [j*L, (j+1)*L) and its
+// result is copied out of that block into y_outs[j].
+int ads_run_batch(Graph* g, hipStream_t strm, int k, const std::vector<int32_t>* svs,
+                  const std::vector<int32_t>* sls, int32_t L, float p_inj, float p_cont, float p_abnd,
+                  int32_t iters, float* const* y_outs) {
+    if (k < 1 || k * L > 64) return fail(C2G_EINVAL, "batched adsorption needs 1 <= k and k*L <= 64");
+    int rc = ads_check(g, k * L, p_inj, p_cont, p_abnd, iters);
+    if (rc) return rc;
+    std::vector<int32_t> sv, sl;
+    for (int j = 0; j < k; ++j)
+        for (size_t i = 0; i < svs[j].size(); ++i) {
+            if (svs[j][i] < 0 || svs[j][i] >= g->V || sls[j][i] < 0 || sls[j][i] >= L)
+                return fail(C2G_EINVAL, "seed (%d,%d) out of range", svs[j][i], sls[j][i]);
+            sv.push_back(svs[j][i]);
+            sl.push_back(sls[j][i] + j * L);
+        }
+    int cur = 0;
+    if ((rc = ads_core(g, strm, sv.data(), sl.data(), (int32_t)sv.size(), k * L, p_inj, p_cont, iters, &cur, nullptr)))
+        return rc;
+    AdsState* s = astate(g);
+    const int64_t V = g->V;
+    const int grid = std::max(1, g->num_cus) * g->tune.grid_per_cu;
+    for (int j = 0; j < k; ++j) {
+        if (k == 1) {
+            if ((rc = s->out.copy_out(y_outs[0], s->Y[cur], sizeof(float) * V * L, strm))) return rc;
+            break;
+        }
+        // the injected image (Y[2]) is free once the iterations are done
+        hipLaunchKernelGGL(ads_cols_k, dim3(grid), dim3(256), 0, strm, s->Y[cur], k * L, j * L, L, s->Y[2], V);
+        HIP_TRY(hipGetLastError());
+        if ((rc = s->out.copy_out(y_outs[j], s->Y[2], sizeof(float) * V * L, strm))) return rc;
+    }
+    return C2G_OK;
+}
+
+int ads_run(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t* sl, int32_t nseed, int32_t L, float p_inj,
+                   float p_cont, float p_abnd, int32_t iters, float* y_out, c2g_stats* st) {
+    int rc = ads_check(g, L, p_inj, p_cont, p_abnd, iters);
+    if (rc) return rc;
     for (int32_t i = 0; i < nseed; ++i)
         if (sv[i] < 0 || sv[i] >= g->V || sl[i] < 0 || sl[i] >= L)
             return fail(C2G_EINVAL, "seed %d = (%d,%d) out of range", i, sv[i], sl[i]);
+    int cur = 0;
+    if ((rc = ads_core(g, strm, sv, sl, nseed, L, p_inj, p_cont, iters, &cur, st))) return rc;
+    AdsState* s = astate(g);
+    if (y_out && (rc = s->out.copy_out(y_out, s->Y[cur], sizeof(float) * (int64_t)g->V * L, strm))) return rc;
+    return C2G_OK;
+}
+
+// The recurrence on the device; the result stays in Y[*cur_out].
+static int ads_core(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t* sl, int32_t nseed, int32_t L,
+                    float p_inj, float p_cont, int32_t iters, int* cur_out, c2g_stats* st) {
+    if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");
     AdsState* s = astate(g);
     int rc = ads_prepare(g, s, L, nseed);
     if (rc) return rc;
@@ -312,7 +377,7 @@ int ads_run(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t* sl, in
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(strm));
     const double t1 = now_ms();
-    if (y_out && (rc = s->out.copy_out(y_out, s->Y[cur], sizeof(float) * n, strm))) return rc;
+    *cur_out = cur;
     if (st) {
         st->lane = C2G_LANE_GPU;
         st->iterations = iters;

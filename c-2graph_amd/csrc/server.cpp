@@ -13,8 +13,9 @@
 //     arrive while others run therefore overlap with them on the device.
 //   * PPR lane (stream B): the PPR requests queued when the lane frees up
 //     (up to 64) run as one batched push, lane = query.
-//   * adsorption lane (stream C): one query at a time.  Both run concurrently
-//     with the SSSP rounds on the device.
+//   * adsorption lane (stream C): the adsorption requests queued when the lane
+//     frees up run as one pass over their label sets side by side (up to 64
+//     labels).  Both run concurrently with the SSSP rounds on the device.
 //   * host lane (optional): the reference CPU worker's algorithms
 //     (Bellman-Ford / forward push / adsorption Jacobi) on host threads, used
 //     when the device is marked failed or a request cannot fit in HBM.
@@ -237,10 +238,14 @@ static void aux_worker(c2g_server* s, bool ads) {
     (void)hipSetDevice(g->device);
     std::deque<int64_t>& q = ads ? s->q_ads : s->q_aux;
     hipStream_t strm = ads ? s->s_ads : s->s_aux;
-    // PPR requests queued together run as one batched pass (lane = query);
-    // adsorption runs one query at a time.
-    const size_t cap = ads ? 1 : (size_t)std::min(64, std::max(1, s->o.max_batch));
-    std::vector<int32_t> sv, sl, srcs;
+    // Requests queued together run as one pass: PPR as one batched push (lane =
+    // query, up to 64), adsorption as one run over the queries' label sets side
+    // by side (up to 64 labels in all).
+    const size_t cap = ads ? (size_t)std::max(1, 64 / std::max(1, s->o.ads_labels))
+                           : (size_t)std::min(64, std::max(1, s->o.max_batch));
+    std::vector<std::vector<int32_t>> svs, sls;
+    std::vector<float*> youts;
+    std::vector<int32_t> srcs;
     std::vector<int64_t> ids;
     std::vector<float*> outs;
     while (true) {
@@ -270,10 +275,16 @@ static void aux_worker(c2g_server* s, bool ads) {
         for (Request* r : rs) take_buffer(s, r);
         int rc;
         if (ads) {
-            Request* r = rs[0];
-            ads_seeds(g, r->src, s->o.ads_labels, sv, sl);
-            rc = ads_run(g, strm, sv.data(), sl.data(), (int32_t)sv.size(), s->o.ads_labels, s->o.p_inj,
-                         s->o.p_cont, s->o.p_abnd, s->o.ads_iters, (float*)r->result.get(), nullptr);
+            const int k = (int)rs.size();
+            svs.resize(k);
+            sls.resize(k);
+            youts.clear();
+            for (int j = 0; j < k; ++j) {
+                ads_seeds(g, rs[j]->src, s->o.ads_labels, svs[j], sls[j]);
+                youts.push_back((float*)rs[j]->result.get());
+            }
+            rc = ads_run_batch(g, strm, k, svs.data(), sls.data(), s->o.ads_labels, s->o.p_inj, s->o.p_cont,
+                               s->o.p_abnd, s->o.ads_iters, youts.data());
         } else if (rs.size() == 1) {
             rc = ppr_single(g, strm, rs[0]->src, s->o.alpha, s->o.eps, (float*)rs[0]->result.get(), nullptr, nullptr);
         } else {

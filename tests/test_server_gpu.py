@@ -144,3 +144,21 @@ def test_pruned_graph_server(gpu):
                 for v, i in enumerate(ids):
                     info, d = s.poll(i)
                     assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], v)), (seed, v)
+
+
+def test_adsorption_burst_is_batched(gpu):
+    # a burst of adsorption requests runs as passes of up to 4 seed sets side by
+    # side (4 x 16 labels); each result equals the same seed set run alone
+    G = _G()
+    sets = list(range(9))
+    with _graph(G) as g:
+        with c2g.Server(g, ads_labels=16, ads_iters=20) as s:
+            ids = [s.submit(c2g.Q_ADS, k) for k in sets]
+            res = [s.poll(i) for i in ids]
+        with c2g.Server(g, ads_labels=16, ads_iters=20, device_failed=True, host_lane=True) as h:
+            alone = [h.poll(h.submit(c2g.Q_ADS, k))[1] for k in sets]
+    for (info, y), ya in zip(res, alone):
+        assert info["status"] == 0 and info["lane"] == c2g.LANE_GPU
+        assert np.abs(y - ya).max() <= 2e-6
+    admits = [i["t_admit_ms"] for i, _ in res]
+    assert len(set(admits)) < len(admits)  # a pass shares one admission time
