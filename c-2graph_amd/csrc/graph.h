@@ -43,13 +43,16 @@ struct Graph;
 // Per-query rows out of a lane-major result array [group][V][64] of 32-bit
 // words (query q = lane q%64 of group q/64): a device tile transpose of
 // kColStage columns at a time into [kColStage][V], one pinned transfer, and a
-// parallel host copy into outs[q] (null rows are skipped).  One instance per
-// lane or state (not shared between threads).
+// parallel host copy into outs[q] (null rows are skipped).  Two staging
+// buffers: the transpose and transfer of one chunk overlap the host copy of
+// the previous one.  One instance per lane or state (not shared between
+// threads).
 constexpr int kColStage = 8;
 struct ColStage {
-    uint32_t* dev = nullptr;   // [kColStage][V] device staging
-    uint32_t* host = nullptr;  // pinned mirror
-    int64_t cap = 0;           // words per staging buffer
+    uint32_t* dev[2] = {nullptr, nullptr};   // [kColStage][V] device staging, double-buffered
+    uint32_t* host[2] = {nullptr, nullptr};  // pinned mirrors
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int64_t cap = 0;                         // words per staging buffer
     int copy_out(Graph* g, hipStream_t strm, const uint32_t* A, int32_t n, void* const* outs);
     void release();
 };

@@ -232,37 +232,65 @@ int ColStage::copy_out(Graph* g, hipStream_t strm, const uint32_t* A, int32_t n,
     if (V == 0 || n <= 0) return C2G_OK;
     if (cap < kColStage * V) {
         release();
-        HIP_TRY(hipMalloc(&dev, sizeof(uint32_t) * kColStage * V));
-        HIP_TRY(hipHostMalloc((void**)&host, sizeof(uint32_t) * kColStage * V, hipHostMallocDefault));
+        for (int k = 0; k < 2; ++k) {
+            HIP_TRY(hipMalloc(&dev[k], sizeof(uint32_t) * kColStage * V));
+            HIP_TRY(hipHostMalloc((void**)&host[k], sizeof(uint32_t) * kColStage * V, hipHostMallocDefault));
+            HIP_TRY(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
+        }
         cap = kColStage * V;
     }
     const int grid = (int)std::min<int64_t>((V + 63) / 64, (int64_t)std::max(1, g->num_cus) * 16);
+    // chunks of up to kColStage queries that have a destination
+    std::vector<std::pair<int32_t, int>> chunks;
     for (int32_t q0 = 0; q0 < n; q0 += kColStage) {
-        const int gi = q0 / 64, l0 = q0 % 64;  // a chunk never spans two groups
         const int m = std::min<int32_t>(kColStage, n - q0);
         bool any = false;
         for (int k = 0; k < m; ++k) any |= outs[q0 + k] != nullptr;
-        if (!any) continue;
-        hipLaunchKernelGGL(extract_cols_k, dim3(grid), dim3(256), 0, strm, A + (int64_t)gi * V * 64, V, l0, m, dev);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(host, dev, sizeof(uint32_t) * V * m, hipMemcpyDeviceToHost, strm));
-        HIP_TRY(hipStreamSynchronize(strm));
+        if (any) chunks.emplace_back(q0, m);
+    }
+    auto host_copy = [&](size_t c) {
+        const int32_t q0 = chunks[c].first;
+        const int m = chunks[c].second;
+        const uint32_t* h = host[c & 1];
         parallel_for((int64_t)m * V, [&](int64_t lo, int64_t hi, int) {
             while (lo < hi) {
                 const int64_t k = lo / V, v0 = lo - k * V, len = std::min(hi - lo, V - v0);
-                if (outs[q0 + k]) memcpy((uint32_t*)outs[q0 + k] + v0, host + k * V + v0, sizeof(uint32_t) * len);
+                if (outs[q0 + k]) memcpy((uint32_t*)outs[q0 + k] + v0, h + k * V + v0, sizeof(uint32_t) * len);
                 lo += len;
             }
         }, 0, (int64_t)1 << 18);
+    };
+    for (size_t c = 0; c < chunks.size(); ++c) {
+        const int32_t q0 = chunks[c].first;
+        const int m = chunks[c].second;
+        const int gi = q0 / 64, l0 = q0 % 64;  // a chunk never spans two groups
+        // buffer c&1 was last used by chunk c-2, whose host copy has finished
+        hipLaunchKernelGGL(extract_cols_k, dim3(grid), dim3(256), 0, strm, A + (int64_t)gi * V * 64, V, l0, m,
+                           dev[c & 1]);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(host[c & 1], dev[c & 1], sizeof(uint32_t) * V * m, hipMemcpyDeviceToHost, strm));
+        HIP_TRY(hipEventRecord(ev[c & 1], strm));
+        if (c > 0) {
+            HIP_TRY(hipEventSynchronize(ev[(c - 1) & 1]));
+            host_copy(c - 1);
+        }
+    }
+    if (!chunks.empty()) {
+        HIP_TRY(hipEventSynchronize(ev[(chunks.size() - 1) & 1]));
+        host_copy(chunks.size() - 1);
     }
     return C2G_OK;
 }
 
 void ColStage::release() {
-    if (dev) (void)hipFree(dev);
-    if (host) (void)hipHostFree(host);
-    dev = nullptr;
-    host = nullptr;
+    for (int k = 0; k < 2; ++k) {
+        if (dev[k]) (void)hipFree(dev[k]);
+        if (host[k]) (void)hipHostFree(host[k]);
+        if (ev[k]) (void)hipEventDestroy(ev[k]);
+        dev[k] = nullptr;
+        host[k] = nullptr;
+        ev[k] = nullptr;
+    }
     cap = 0;
 }
 
