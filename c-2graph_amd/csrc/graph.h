@@ -60,14 +60,14 @@ struct ColStage {
 // Device workspace of single-query SSSP.
 struct SsspWs {
     int32_t* dist = nullptr;     // [V]
-    int32_t* stamp = nullptr;    // activation bitmaps: live [nw], hidden [nw]
+    uint32_t* bits = nullptr;    // activation bitmaps: live [nwp], hidden [nwp] (nwp padded to 64 words)
     int4* q[1] = {nullptr};      // frontier items (u, e_begin, e_end, kind)
     int64_t qcap = 0;
     int64_t* ctrl = nullptr;     // control block, see sssp.hip
     int64_t* h_ctrl = nullptr;   // pinned mirror
     uint8_t* hid = nullptr;      // [V] 1 = distance was set through a hidden edge
     hipGraphExec_t exec = nullptr;
-    int exec_iters = 0, exec_block = 0, exec_grid = 0, exec_chunk = 0;
+    int exec_iters = 0, exec_grid = 0, exec_chunk = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
     HostStage out;               // distance copies to the host
 };

@@ -172,7 +172,7 @@ void device_release(Graph* g) {
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     if (g->sssp.exec) (void)hipGraphExecDestroy(g->sssp.exec);
     if (g->batch.exec) (void)hipGraphExecDestroy(g->batch.exec);
-    dfree(g->sssp.dist); dfree(g->sssp.stamp); dfree(g->sssp.q[0]);
+    dfree(g->sssp.dist); dfree(g->sssp.bits); dfree(g->sssp.q[0]);
     dfree(g->sssp.ctrl); dfree(g->sssp.hid);
     if (g->sssp.h_ctrl) (void)hipHostFree(g->sssp.h_ctrl);
     g->sssp.out.release();

@@ -39,7 +39,7 @@ namespace c2g {
 
 enum : int {
     P_CUR = 0, P_ITER = 1, P_DONE = 2, P_DONE_E = 3, P_FRONT = 4, P_EDGES = 5, P_ITEMS = 6,
-    P_OVF = 7, P_DONE_A = 8, P_N = 16
+    P_OVF = 7, P_N = 16
 };
 
 __global__ void ppr_init_k(float* __restrict__ p, float* __restrict__ r, float* __restrict__ m,

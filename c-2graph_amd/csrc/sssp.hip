@@ -42,7 +42,6 @@ namespace c2g {
 
 enum : int {
     C_CUR = 0,    // items in the current round's queue
-    C_SPARE = 1,
     C_ITER = 2,   // rounds with work
     C_DONE = 3,   // workgroups finished (relax)
     C_FRONT = 4,  // sum of frontier vertices over rounds
@@ -362,7 +361,7 @@ int ensure_sssp_ws(Graph* g) {
     if (!w.dist) {
         HIP_TRY(hipMalloc(&w.dist, sizeof(int32_t) * std::max<int64_t>(V, 1)));
         // live and hidden activation bitmaps
-        HIP_TRY(hipMalloc(&w.stamp, sizeof(uint32_t) * nwp * 2));
+        HIP_TRY(hipMalloc(&w.bits, sizeof(uint32_t) * nwp * 2));
         HIP_TRY(hipMalloc(&w.ctrl, sizeof(int64_t) * C_N));
         HIP_TRY(hipEventCreate(&w.ev[0]));
         HIP_TRY(hipEventCreate(&w.ev[1]));
@@ -392,7 +391,7 @@ static int launch_rounds(Graph* g, int k, double* relax_ms = nullptr) {
     const bool pruned = is_pruned(g);
     const int grid = relax_grid(g);
     const int64_t nwp = bitmap_words(g->V);
-    uint32_t* bits = (uint32_t*)w.stamp;
+    uint32_t* bits = w.bits;
     uint32_t* hbits = bits + nwp;
     const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (nwp + 255) / 256));
     for (int r = 0; r < k; ++r) {
@@ -476,7 +475,7 @@ int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* 
     SsspWs& w = g->sssp;
     const bool pruned = is_pruned(g);
     const int64_t nwp = bitmap_words(g->V);
-    uint32_t* bits = (uint32_t*)w.stamp;
+    uint32_t* bits = w.bits;
     const double t0 = now_ms();
     const int tb = 256, gridv = (int)std::min<int64_t>((g->V + tb - 1) / tb, 65535);
     hipLaunchKernelGGL(sssp_init_k, dim3(std::max(gridv, 1)), dim3(tb), 0, g->stream, w.dist, bits,

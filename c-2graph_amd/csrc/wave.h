@@ -41,12 +41,6 @@ __device__ __forceinline__ int64_t shfl64(int64_t x, int src) {
     return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-// Number of set bits of `mask` below this lane (ballot prefix).
-__device__ __forceinline__ int lanes_below(uint64_t mask) {
-    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                          __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
 // Largest j in [0,64) with pre_j <= k, where pre is this wave's non-decreasing
 // exclusive prefix held one value per lane (six shuffles).
 __device__ __forceinline__ int wave_owner(int pre, int k) {
