@@ -110,9 +110,11 @@ typedef struct {
 /* ======================================================================= */
 /* Message of the last failing call on this thread ("" if none). */
 const char* c2g_last_error(void);
-/* Library version string. */
+/* Library version string (no reference counterpart). */
 const char* c2g_version(void);
-/* 1 if a HIP device with a gfx950 kernel image is usable, else 0. */
+/* 1 if a HIP device with a gfx950 kernel image is usable, else 0 (the
+ * reference's mains call cudaGetDeviceCount/cudaSetDevice and exit on failure,
+ * src/sssp/base/Main.cu:26-30). */
 int c2g_device_available(void);
 
 /* ======================================================================= */
@@ -122,9 +124,11 @@ int c2g_device_available(void);
  * filesize/4.  Returns count >= 0, or a negative error.  If buf is NULL only
  * the count is returned. */
 int64_t c2g_read_i32(const char* path, int32_t* buf, int64_t cap);
+/* read_binary2vector<float> (core/io.hpp:25-40), for flist / hwlist(f32). */
 int64_t c2g_read_f32(const char* path, float* buf, int64_t cap);
 /* Headerless little-endian writers (convert2binary.cpp:118-134). */
 int c2g_write_i32(const char* path, const int32_t* buf, int64_t n);
+/* Float form of the writer (convert2binary.cpp:130-134 writes flist). */
 int c2g_write_f32(const char* path, const float* buf, int64_t n);
 
 /* Seeded R-MAT generator (replaces convert2binary's text ingestion for the
@@ -142,7 +146,8 @@ int c2g_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
              int32_t** vlist, int32_t** elist, int32_t** wlist, float** flist,
              int64_t* out_E);
 
-/* Writes <ds>/origin/{vlist,elist,wlist[,flist]}.bin (creates directories). */
+/* Writes <ds>/origin/{vlist,elist,wlist[,flist]}.bin (creates directories),
+ * the files convert2binary writes (utils/convert2binary.cpp:118-134). */
 int c2g_write_dataset(const char* ds, int32_t V, const int32_t* vlist,
                       const int32_t* elist, const int32_t* wlist, const float* flist);
 
@@ -165,8 +170,10 @@ int c2g_prune_sssp(const char* ds, int64_t* stats_out);
 /* purn-ppr <ds> (src/purn-ppr.cpp:63-344): reads origin/{vlist,elist,flist},
  * writes ppr/{vlist,elist,wlist(f32),hvlist,helist,hwlist(f32)}.        */
 int c2g_prune_ppr(const char* ds, int64_t* stats_out);
-/* In-memory forms used by the library and tests: inputs are an origin CSR;
- * outputs are malloc'ed by the library and released with c2g_free.      */
+/* In-memory forms of the same two passes (src/purn-sssp.cpp:109-349,
+ * src/purn-ppr.cpp:113-335) used by the library and tests: inputs are an
+ * origin CSR; outputs are malloc'ed by the library and released with
+ * c2g_free.                                                              */
 int c2g_prune_sssp_mem(int32_t V, const int32_t* vlist, const int32_t* elist,
                        const int32_t* wlist, int32_t** out_vlist,
                        int32_t** out_elist, int32_t** out_wlist,
@@ -178,6 +185,7 @@ int c2g_prune_ppr_mem(int32_t V, const int32_t* vlist, const int32_t* elist,
                       int32_t** out_elist, float** out_wlist,
                       int32_t** out_hvlist, int32_t** out_helist,
                       float** out_hwlist);
+/* Releases a buffer the library malloc'ed (no reference counterpart). */
 void c2g_free(void* p);
 
 /* ======================================================================= */
@@ -190,21 +198,30 @@ void c2g_free(void* p);
  * compacted sssp/hvlist is rejected with C2G_EFORMAT) and uploads the graph
  * to HBM once. */
 int c2g_open(const char* ds, int32_t variant, const c2g_opts* opts, c2g_graph** out);
-/* Same from caller arrays (copied).  wlist/flist/hidden arrays are nullable;
- * for the PPR_PURN variant pass flist (live edge floats) and hwf (hidden
- * floats).  */
+/* Same from caller arrays (copied), i.e. DeviceMemory + CudaMemcpyGraph
+ * without the file loader (src/sssp/base/DeviceMemory.cuh:47-65).
+ * wlist/flist/hidden arrays are nullable; for the PPR_PURN variant pass
+ * flist (live edge floats) and hwf (hidden floats).  */
 int c2g_graph_from_csr(int32_t V, const int32_t* vlist, const int32_t* elist,
                        const int32_t* wlist, const float* flist,
                        const int32_t* hvlist, const int32_t* helist,
                        const int32_t* hwlist, const float* hwf,
                        int32_t variant, const c2g_opts* opts, c2g_graph** out);
+/* Frees the handle and its device memory (~DeviceMemory,
+ * src/sssp/base/DeviceMemory.cuh:41-44, 103-122). */
 int c2g_close(c2g_graph* g);
+/* vert_num / edge_num of the loaded Graph (src/sssp/base/Graph.h:26-27, 33-34);
+ * nullable outputs. */
 int c2g_graph_info(const c2g_graph* g, int32_t* V, int64_t* E, int64_t* hidden_E,
                    int32_t* variant);
-/* Host copies of the loaded arrays (nullable outputs; sized V+1 / E). */
+/* Host copies of the loaded arrays, Graph's csr_v / csr_e / csr_w
+ * (src/sssp/base/Graph.h:33-35); nullable outputs, sized V+1 / E. */
 int c2g_graph_host_csr(const c2g_graph* g, const int32_t** vlist,
                        const int32_t** elist, const int32_t** wlist,
                        const float** flist);
+/* Launch shape and scheduling tunables, which the reference compiles in
+ * (THREADS_PER_BLOCK 512, MAX_BLOCKS_NUM 20: core/Util.cuh:31-35).  Setting
+ * them drops the captured launch sequences that hold the old values. */
 int c2g_set_tune(c2g_graph* g, const c2g_tune* t);
 int c2g_get_tune(const c2g_graph* g, c2g_tune* t);
 
@@ -218,8 +235,10 @@ int c2g_get_tune(const c2g_graph* g, c2g_tune* t);
  * whose distance was lowered through a hidden edge, is relaxed, so the
  * distances equal those of the full graph.  dist_out: int32[V]. */
 int c2g_sssp(c2g_graph* g, int32_t src, int32_t* dist_out, c2g_stats* st);
-/* Batched multi-source SSSP: n sources in one captured launch sequence with
- * per-query frontiers kept as a 64-bit lane mask per vertex.  dist_out is
+/* Batched multi-source SSSP (no reference counterpart: the reference runs
+ * one query per launch, src/sssp/base/Main.cu:59-76): n sources in one
+ * captured launch sequence with per-query frontiers kept as a 64-bit lane
+ * mask per vertex.  dist_out is
  * n*V int32 (query-major, row q = query q).  Results are bit-identical to
  * n calls of c2g_sssp.  st (nullable) receives whole-batch totals. */
 int c2g_sssp_batch(c2g_graph* g, const int32_t* srcs, int32_t n,
@@ -254,13 +273,16 @@ int c2g_sssp_migrate(c2g_graph* g, int32_t src, int32_t budget, int32_t* dist_ou
  * On SSSP_PURN graphs hidden lists are relaxed by the same rule as c2g_sssp. */
 int c2g_host_bellman_ford(const c2g_graph* g, int32_t* dist, const int32_t* frontier,
                           int32_t count, const uint8_t* via_hidden, c2g_stats* st);
-/* Full single-source query on the host lane (seeding + Bellman-Ford). */
+/* Full single-source query on the host lane: the reference CPU worker's
+ * entry (SSSP_CPU, src/sssp/purn-perk/SSSP.h:31-37) from {s:0},{s}. */
 int c2g_host_sssp(const c2g_graph* g, int32_t src, int32_t* dist_out, c2g_stats* st);
 /* Forward push from (p, r, frontier) (src/ppr/w-base-perk/PPR.cpp:178-236)
  * with threshold r[v] > eps*outdeg(v); flag array zero-initialised. */
 int c2g_host_forward_push(const c2g_graph* g, float alpha, float eps, float* p,
                           float* r, const int32_t* frontier, int32_t count,
                           c2g_stats* st);
+/* Full single-source push on the host lane (ppr_CPU entry,
+ * src/ppr/w-base-perk/PPR.h:38-46) from r[s]=1, p[s]=alpha. */
 int c2g_host_ppr(const c2g_graph* g, int32_t src, float alpha, float eps,
                  float* p_out, float* r_out, c2g_stats* st);
 
@@ -280,7 +302,9 @@ int c2g_ppr(c2g_graph* g, int32_t src, float alpha, float eps, float* p_out,
  * the handed-over (p, r, frontier).  handoff_min = 10000 in the reference. */
 int c2g_ppr_migrate(c2g_graph* g, int32_t src, float alpha, float eps, int32_t budget,
                     int32_t handoff_min, float* p_out, float* r_out, c2g_stats* st);
-/* n sources in one launch sequence; p_out/r_out are n*V (query-major). */
+/* n sources in one launch sequence, lane = query (no reference counterpart:
+ * one calcuatePPR launch per query, src/ppr/base/Main.cu:79-83); p_out/r_out
+ * are n*V (query-major). */
 int c2g_ppr_batch(c2g_graph* g, const int32_t* srcs, int32_t n, float alpha,
                   float eps, float* p_out, float* r_out, c2g_stats* st);
 
@@ -310,9 +334,12 @@ int c2g_adsorption(c2g_graph* g, const int32_t* seed_v, const int32_t* seed_labe
 /* ======================================================================= */
 /* Autotuner                                                                 */
 /* ======================================================================= */
-/* Times the SSSP kernels over a sweep of block shapes, chunk sizes and
- * iterations-per-graph on the given sample sources and stores the fastest
- * setting in the handle (returned in best_out, nullable). */
+/* Times the SSSP kernels over a sweep of block shapes, chunk sizes (the
+ * degree at which a row is split across waves) and iterations-per-graph on
+ * the given sample sources and stores the fastest setting in the handle
+ * (returned in best_out, nullable).  Replaces the reference's compile-time
+ * launch constants (core/Util.cuh:31-35) and its fixed block/warp/thread
+ * degree tiers (src/sssp/base/Main.cu:197-293). */
 int c2g_autotune(c2g_graph* g, const int32_t* srcs, int32_t n, int32_t batched,
                  c2g_tune* best_out);
 
@@ -349,15 +376,24 @@ typedef struct {
     double  t_submit_ms, t_admit_ms, t_done_ms;  /* server clock */
 } c2g_result_info;
 
+/* Starts the lanes over an open graph.  The server replaces the reference's
+ * CPU/GPU collaboration loop (src/sssp/purn-perk/Main.cu:87-246: two GPU
+ * streams and one CPU worker thread serving one repeated query) with
+ * continuously batched lanes per query kind plus the host-core lane. */
 int c2g_server_start(c2g_graph* g, const c2g_server_opts* o, c2g_server** out);
 /* Submit one query; returns its id (>= 0) or a negative error (C2G_EBUSY when
- * admission is refused).  For C2G_Q_ADS, src is the seed-set index. */
+ * admission is refused).  For C2G_Q_ADS, src is the seed-set index.  The
+ * reference hands work to its CPU worker through a pinned ring buffer
+ * (initQueue / outQueue, src/sssp/purn-perk/SSSP.h:13-37); here requests
+ * enter per-kind queues admitted by free HBM. */
 int64_t c2g_server_submit(c2g_server* s, int32_t kind, int32_t src);
 /* Poll a finished query: 1 = done (info filled, result copied into out when
  * non-null: int32[V] for SSSP, float[V] for PPR, float[V*L] for adsorption),
  * 0 = not done, <0 error.  wait_ms < 0 blocks. */
 int c2g_server_poll(c2g_server* s, int64_t id, int32_t wait_ms, void* out,
                     c2g_result_info* info);
+/* Stops the lanes, cancels outstanding requests and frees the server (the
+ * reference joins its CPU worker thread, src/sssp/purn-perk/Main.cu:255). */
 int c2g_server_stop(c2g_server* s);
 
 #ifdef __cplusplus
