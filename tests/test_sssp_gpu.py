@@ -146,3 +146,22 @@ def test_multi_chunk_distance_copy(gpu):
         d, st = g.sssp(0)
     assert 2 * (32 << 20) < d.nbytes < 3 * (32 << 20)
     assert np.array_equal(d, want.astype(np.int32))
+
+
+def test_distances_near_int32_max(gpu):
+    # chain 0 -> 1 -> 2 -> 3 with weights 2^30 - 1, 2^30 - 1, 5 and a shortcut
+    # 0 -> 3 of weight 2^31 - 10: every sum stays below INT32_MAX except
+    # 1 -> 2 -> 3, which must not wrap (a wrapped sum would be negative and win)
+    vl = np.array([0, 2, 3, 4, 4], np.int32)
+    el = np.array([1, 3, 2, 3], np.int32)
+    big = 2**30 - 1
+    wl = np.array([big, 2**31 - 10, big, 5], np.int32)
+    ref = oracle.dijkstra(vl, el, wl, 0)
+    assert ref.tolist() == [0, big, 2 * big, 2**31 - 10]  # 2*big + 5 exceeds INT32_MAX
+    with c2g.Graph.from_csr(vl, el, wl, device=0) as g:
+        d, _ = g.sssp(0)
+        db, _ = g.sssp_batch(np.array([0, 1, 2], np.int32))
+    assert np.array_equal(d, ref)
+    assert np.array_equal(db[0], ref)
+    assert db[1].tolist() == [INT_MAX, 0, big, big + 5]
+    assert db[2].tolist() == [INT_MAX, INT_MAX, 0, 5]
