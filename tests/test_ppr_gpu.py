@@ -127,3 +127,18 @@ def test_pruned_graph_seeding(gpu):
                                               np.zeros(1, np.int64))
             p, r, _ = g.ppr(s, alpha=ALPHA, eps=1e-7)
             assert np.abs(p - p0).sum() <= 2e-5, s
+
+
+def test_pruned_graph_batched_equals_single(gpu):
+    # batched seeding on the ppr/ variant (H(s) U N'(s) with r = (1-alpha) w)
+    # must give the single-source results for every lane
+    G = random_graph(300, 900, seed=7)
+    P = c2g.prune_ppr_mem(G["vlist"], G["elist"], G["flist"])
+    assert len(P["helist"]) > 0
+    srcs = np.arange(0, 300, 7, dtype=np.int32)
+    with c2g.Graph.from_csr(P["vlist"], P["elist"], flist=P["wlist"], hvlist=P["hvlist"], helist=P["helist"],
+                            hwf=P["hwlist"], variant=c2g.VARIANT_PPR_PURN, device=0) as g:
+        Pb, Rb, _ = g.ppr_batch(srcs, alpha=ALPHA, eps=1e-7)
+        for q, s in enumerate(srcs):
+            p, r, _ = g.ppr(int(s), alpha=ALPHA, eps=1e-7)
+            assert np.abs(Pb[q].astype(np.float64) - p).sum() <= 2e-5, s
