@@ -129,3 +129,45 @@ def test_prune_rejects_duplicate_targets():
     with pytest.raises(c2g.C2GError) as e:
         c2g.prune_sssp_mem(vl, np.array([1, 1], np.int32), np.array([3, 4], np.int32))
     assert e.value.code == c2g.EFORMAT
+
+
+def _stanford():
+    F = np.load(os.path.join(os.path.dirname(__file__), "golden", "stanford_layout.npz"))
+    return {k: np.concatenate([[0], np.cumsum(F[k])]).astype(np.int32) for k in F.files}
+
+
+def test_reference_dataset_fixture():
+    # dataset/stanford as shipped by the reference (vlist/hvlist only): V, E,
+    # the kept-edge ratios of its own purn-sssp / purn-ppr run, and the two
+    # hidden-list layouts (sssp/: one entry per processed vertex in prune
+    # order, src/purn-sssp.cpp:283-284; ppr/: V+1, src/purn-ppr.cpp:275-283)
+    S = _stanford()
+    V = len(S["origin_vlist"]) - 1
+    E = int(S["origin_vlist"][-1])
+    assert (V, E) == (685231, 7600595)
+    assert len(S["sssp_vlist"]) == V + 1 and len(S["ppr_vlist"]) == V + 1
+    assert round(S["sssp_vlist"][-1] / E, 3) == 0.691 and round(S["ppr_vlist"][-1] / E, 3) == 0.721
+    assert len(S["ppr_hvlist"]) == V + 1
+    assert len(S["sssp_hvlist"]) == 595077 < V + 1  # 1 + processed vertices
+
+
+def test_loader_on_reference_dataset_layouts(tmp_path):
+    # the reference's compacted sssp/hvlist is refused with C2G_EFORMAT (it
+    # cannot be indexed by vertex); its per-vertex ppr/hvlist loads.  Edge
+    # arrays are not shipped, so they are stood in by arrays of the right length.
+    S = _stanford()
+    V = len(S["origin_vlist"]) - 1
+    ds = str(tmp_path / "stanford")
+    for sub, wdt in (("sssp", np.int32), ("ppr", np.float32)):
+        os.makedirs(os.path.join(ds, sub))
+        vl, hv = S[sub + "_vlist"], S[sub + "_hvlist"]
+        e, he = int(vl[-1]), int(hv[-1])
+        files = {"vlist": vl, "elist": np.zeros(e, np.int32), "wlist": np.ones(e, wdt), "hvlist": hv,
+                 "helist": np.zeros(he, np.int32), "hwlist": np.ones(he, wdt)}
+        for name, a in files.items():
+            a.tofile(os.path.join(ds, sub, name + ".bin"))
+    with pytest.raises(c2g.C2GError) as e:
+        c2g.Graph.open(ds, variant=c2g.VARIANT_SSSP_PURN, no_device=True)
+    assert e.value.code == c2g.EFORMAT and "compacted" in c2g.lib().c2g_last_error().decode()
+    with c2g.Graph.open(ds, variant=c2g.VARIANT_PPR_PURN, no_device=True) as g:
+        assert g.V == V
