@@ -204,6 +204,47 @@ def aux_phase(args):
     return out
 
 
+def ppr_purn_study(args):
+    """Approximation of the pruned PPR graph (purn-ppr's additive 0.8*w
+    compensation, src/purn-ppr.cpp:243-259, with the ppr-purn seeding
+    H(s) U N'(s)): PPR on the pruned graph against PPR on the full graph, both
+    on the GPU (alpha 0.15, eps 1e-7*outdeg), next to the kept-edge ratio."""
+    import c2g
+
+    G = c2g.rmat(args.study_scale, 16, 0.57, 0.19, 0.19, seed=5, wmax=255, want_f=True)
+    t = time.time()
+    P = c2g.prune_ppr_mem(G["vlist"], G["elist"], G["flist"])
+    t_prune = time.time() - t
+    deg = np.diff(G["vlist"])
+    rng = np.random.default_rng(5)
+    srcs = rng.choice(np.flatnonzero(deg > 0), 32, replace=False)
+    gf = c2g.Graph.from_csr(G["vlist"], G["elist"], flist=G["flist"], device=0)
+    gp = c2g.Graph.from_csr(P["vlist"], P["elist"], flist=P["wlist"], hvlist=P["hvlist"], helist=P["helist"],
+                            hwf=P["hwlist"], variant=c2g.VARIANT_PPR_PURN, device=0)
+    gf.ppr(int(srcs[0]))
+    gp.ppr(int(srcs[0]))
+    l1, mx, tf, tp = [], [], 0.0, 0.0
+    hidden_src = 0
+    for s_ in srcs:
+        pf, _, sf = gf.ppr(int(s_))
+        pp, _, sp = gp.ppr(int(s_))
+        tf += sf.ms
+        tp += sp.ms
+        d = np.abs(pp.astype(np.float64) - pf)
+        l1.append(float(d.sum()))
+        mx.append(float(d.max()))
+        hidden_src += int(P["hvlist"][s_ + 1] > P["hvlist"][s_])
+    gf.close()
+    gp.close()
+    return {"workload": f"R-MAT scale {args.study_scale} seed 5 directed (V={G['V']}, E={G['E']}), 32 sources, "
+                        "alpha 0.15, eps 1e-7*outdeg",
+            "kept_edge_ratio": round(len(P["elist"]) / G["E"], 4), "hidden_edges": int(len(P["helist"])),
+            "prune_s": round(t_prune, 1), "sources_with_hidden_list": hidden_src,
+            "l1_pruned_vs_full": {"mean": float(np.mean(l1)), "max": float(np.max(l1))},
+            "maxabs_pruned_vs_full": float(np.max(mx)),
+            "ms_per_query": {"full": round(tf / len(srcs), 3), "pruned": round(tp / len(srcs), 3)}}
+
+
 def serve_phase(args):
     """Served mixed stream (60% SSSP, 30% PPR, 10% adsorption) with Poisson arrivals."""
     import c2g
@@ -294,6 +335,9 @@ def main():
     ap.add_argument("--serve-scale", type=int, default=23)
     ap.add_argument("--serve-slots", type=int, default=0, help="SSSP query-major slots (0 = library default)")
     ap.add_argument("--aux", action="store_true", help="also time PPR (scale 22) and adsorption (scale 23)")
+    ap.add_argument("--ppr-purn-study", action="store_true",
+                    help="only the pruned-PPR approximation study (PPR on the purn-ppr graph vs the full graph)")
+    ap.add_argument("--study-scale", type=int, default=20)
     ap.add_argument("--pcie", action="store_true",
                     help="also time one batched step with every distance vector copied to host memory")
     ap.add_argument("--aux-only", action="store_true", help="only the PPR and adsorption timings")
@@ -321,6 +365,9 @@ def main():
 
     if args.serve_only:
         print(json.dumps({"serving": serve_phase(args)}), flush=True)
+        return
+    if args.ppr_purn_study:
+        print(json.dumps({"ppr_purn_study": ppr_purn_study(args)}), flush=True)
         return
     if args.aux_only:
         args.skip_cpu = True  # kernels only (the profiled runs)
