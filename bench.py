@@ -245,6 +245,41 @@ def ppr_purn_study(args):
             "ms_per_query": {"full": round(tf / len(srcs), 3), "pruned": round(tp / len(srcs), 3)}}
 
 
+def c2_phase(args):
+    """Pruned-graph SSSP: R-MAT scale 22 seed 2, weights U[1,255], preprocessed
+    by purn-sssp (c2g_prune_sssp); 32 sources drawn uniformly from vertices
+    with out-degree >= 1 in the full graph, one query per launch sequence on
+    the pruned graph and on the full graph.  Distances must agree exactly
+    (checksum and reached count of every query)."""
+    import c2g
+
+    G = c2g.rmat(22, 16, 0.57, 0.19, 0.19, seed=2, wmax=255)
+    t = time.time()
+    P = c2g.prune_sssp_mem(G["vlist"], G["elist"], G["wlist"])
+    t_prune = time.time() - t
+    deg = np.diff(G["vlist"])
+    rng = np.random.default_rng(2)
+    srcs = rng.choice(np.flatnonzero(deg > 0), 32, replace=False).astype(np.int32)
+    gf = c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], device=0)
+    gp = c2g.Graph.from_csr(P["vlist"], P["elist"], P["wlist"], hvlist=P["hvlist"], helist=P["helist"],
+                            hwlist=P["hwlist"], variant=c2g.VARIANT_SSSP_PURN, device=0)
+    out = {"workload": f"R-MAT scale 22 seed 2 (V={G['V']}, E={G['E']}), w U[1,255], purn-sssp, 32 sources",
+           "kept_edge_ratio": round(len(P["elist"]) / G["E"], 4), "hidden_edges": int(len(P["helist"])),
+           "prune_s": round(t_prune, 1)}
+    res = {}
+    for name, g in (("full", gf), ("pruned", gp)):
+        g.sssp_each_resident(srcs[:2])
+        t = time.perf_counter()
+        sums, reach, st = g.sssp_each_resident(srcs)
+        res[name] = (sums, reach)
+        out[f"ms_per_query_{name}"] = round((time.perf_counter() - t) / len(srcs) * 1e3, 3)
+    out["distances_equal"] = bool(np.array_equal(res["full"][0], res["pruned"][0])
+                                  and np.array_equal(res["full"][1], res["pruned"][1]))
+    gf.close()
+    gp.close()
+    return out
+
+
 def serve_phase(args):
     """Served mixed stream (60% SSSP, 30% PPR, 10% adsorption) with Poisson arrivals."""
     import c2g
@@ -338,6 +373,7 @@ def main():
     ap.add_argument("--ppr-purn-study", action="store_true",
                     help="only the pruned-PPR approximation study (PPR on the purn-ppr graph vs the full graph)")
     ap.add_argument("--study-scale", type=int, default=20)
+    ap.add_argument("--c2", action="store_true", help="only the pruned-graph SSSP workload (scale 22 seed 2)")
     ap.add_argument("--pcie", action="store_true",
                     help="also time one batched step with every distance vector copied to host memory")
     ap.add_argument("--aux-only", action="store_true", help="only the PPR and adsorption timings")
@@ -365,6 +401,9 @@ def main():
 
     if args.serve_only:
         print(json.dumps({"serving": serve_phase(args)}), flush=True)
+        return
+    if args.c2:
+        print(json.dumps({"sssp_purn": c2_phase(args)}), flush=True)
         return
     if args.ppr_purn_study:
         print(json.dumps({"ppr_purn_study": ppr_purn_study(args)}), flush=True)
