@@ -142,3 +142,26 @@ def test_pruned_graph_batched_equals_single(gpu):
         for q, s in enumerate(srcs):
             p, r, _ = g.ppr(int(s), alpha=ALPHA, eps=1e-7)
             assert np.abs(Pb[q].astype(np.float64) - p).sum() <= 2e-5, s
+
+
+@pytest.mark.slow
+def test_c3_full_scale_against_power_iteration(gpu):
+    # the PPR workload at full size: symmetrised R-MAT scale 22, seed 3,
+    # alpha 0.15, eps 1e-7*outdeg, one source, against the FP64 power
+    # iteration.  The certificate |p - pi|_1 = sum(r) holds for the push
+    # whatever eps is; L1 and max-abs are printed for the report.
+    G = rmat_graph(22, 3, wmax=0, symmetric=True, want_f=True)
+    deg = np.diff(G["vlist"])
+    # symmetrised: a vertex without out-edges has no in-edges either, so no
+    # reachable vertex is dangling and no mass is dropped
+    assert np.all(np.bincount(G["elist"], minlength=G["V"])[deg == 0] == 0)
+    rng = np.random.default_rng(3)
+    s = int(rng.choice(np.flatnonzero(deg > 0)))
+    with _g(G) as g:
+        p, r, st = g.ppr(s, alpha=ALPHA, eps=1e-7)
+    pi = oracle.ppr_power(G["vlist"], G["elist"], G["flist"], s, alpha=ALPHA, tol=1e-13)
+    d = np.abs(p.astype(np.float64) - pi)
+    l1, mx, sr = d.sum(), d.max(), r.astype(np.float64).sum()
+    print(f"\nC3 scale 22 source {s}: L1 {l1:.6g} max-abs {mx:.6g} sum(r) {sr:.6g} rounds {st.iterations}")
+    assert abs(l1 - sr) <= 1e-5
+    assert np.all(r <= deg * 1e-7 + 1e-12)
