@@ -169,24 +169,41 @@ int orc_bellman_ford(int32_t V, const int32_t* vl, const int32_t* el, const int3
  * Iterates until the L1 change is below tol or max_iter.  Returns iterations. */
 int orc_ppr_power(int32_t V, const int32_t* vl, const int32_t* el, const float* fl,
                   int32_t src, double alpha, double tol, int32_t max_iter, double* x) {
+    /* pull form y[v] = alpha*[v==s] + (1-alpha) * sum_{u->v} f(u,v) x[u]
+     * over the in-edges (built once), so threads split the rows v */
+    const int64_t E = vl[V];
+    int64_t* tv = (int64_t*)calloc((size_t)V + 1, sizeof(int64_t));
+    int32_t* tu = (int32_t*)malloc(sizeof(int32_t) * (size_t)(E > 0 ? E : 1));
+    double* tf = (double*)malloc(sizeof(double) * (size_t)(E > 0 ? E : 1));
     double* y = (double*)malloc(sizeof(double) * (size_t)V);
+    for (int64_t e = 0; e < E; ++e) tv[el[e] + 1]++;
+    for (int32_t v = 0; v < V; ++v) tv[v + 1] += tv[v];
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)(V > 0 ? V : 1));
+    for (int32_t v = 0; v < V; ++v) fill[v] = tv[v];
+    for (int32_t u = 0; u < V; ++u)
+        for (int32_t e = vl[u]; e < vl[u + 1]; ++e) {
+            const int64_t k = fill[el[e]]++;
+            tu[k] = u;
+            tf[k] = (double)fl[e];
+        }
+    free(fill);
     for (int32_t v = 0; v < V; ++v) x[v] = 0.0;
     x[src] = alpha;  /* first iterate from x=0 */
     int it = 1;
     for (; it < max_iter; ++it) {
-        for (int32_t v = 0; v < V; ++v) y[v] = 0.0;
-        y[src] = alpha;
-        for (int32_t u = 0; u < V; ++u) {
-            double xu = x[u];
-            if (xu == 0.0) continue;
-            double m = (1.0 - alpha) * xu;
-            for (int32_t e = vl[u]; e < vl[u + 1]; ++e) y[el[e]] += m * (double)fl[e];
-        }
         double d = 0;
-        for (int32_t v = 0; v < V; ++v) { d += fabs(y[v] - x[v]); x[v] = y[v]; }
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : d)
+        for (int32_t v = 0; v < V; ++v) {
+            double acc = 0.0;
+            for (int64_t k = tv[v]; k < tv[v + 1]; ++k) acc += tf[k] * x[tu[k]];
+            const double yv = (v == src ? alpha : 0.0) + (1.0 - alpha) * acc;
+            d += fabs(yv - x[v]);
+            y[v] = yv;
+        }
+        memcpy(x, y, sizeof(double) * (size_t)V);
         if (d < tol) { ++it; break; }
     }
-    free(y);
+    free(y); free(tv); free(tu); free(tf);
     return it;
 }
 
@@ -283,6 +300,8 @@ void orc_adsorption(int32_t V, const int32_t* vl, const int32_t* el, const float
     for (int32_t i = 0; i < nseed; ++i) Y0[(size_t)seed_v[i] * L + seed_l[i]] = 1.0;
     memcpy(Y, Y0, sizeof(double) * n);
     for (int32_t t = 0; t < iters; ++t) {
+        /* rows are independent: threads split them, each row summed in edge order */
+#pragma omp parallel for schedule(dynamic, 4096)
         for (int32_t v = 0; v < V; ++v) {
             double* z = Z + (size_t)v * L;
             for (int32_t l = 0; l < L; ++l) z[l] = 0.0;

@@ -103,3 +103,18 @@ def test_reference_push_mode_refuses_pruned_graph(gpu):
         with pytest.raises(c2g.C2GError) as e:
             g.adsorption_push(0)
         assert e.value.code == c2g.ENOTSUP
+
+
+@pytest.mark.slow
+def test_c5_full_scale_against_fp64(gpu):
+    # the adsorption workload at full size: R-MAT scale 23, seed 5, 16 labels
+    # on 1% of the vertices, 0.25/0.70/0.05, 20 iterations, against the FP64
+    # Jacobi of the same recurrence; errors are printed for the report
+    G = rmat_graph(23, 5, wmax=255, want_f=True)
+    sv, sl = _seeds(G["V"], 16, 0.01, 5)
+    with c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], G["flist"], device=0) as g:
+        Y, _ = g.adsorption(sv, sl, L=16, p_inj=0.25, p_cont=0.70, p_abnd=0.05, iters=20)
+    Yo = oracle.adsorption(G["vlist"], G["elist"], G["flist"], sv, sl, L=16, p_inj=0.25, p_cont=0.70, iters=20)
+    d = np.abs(Y.astype(np.float64) - Yo)
+    print(f"\nC5 scale 23: max-abs {d.max():.3g} mean-abs {d.mean():.3g} L1 per label {d.sum() / 16:.3g}")
+    _check(Y, Yo)
