@@ -20,12 +20,15 @@
 
 namespace c2g {
 
+// Failing HIP calls return C2G_ENOMEM for an allocation that does not fit
+// (device or pinned host memory) and C2G_EDEVICE otherwise.
 #define HIP_TRY(expr)                                                                  \
     do {                                                                               \
         hipError_t _e = (expr);                                                        \
         if (_e != hipSuccess)                                                          \
-            return ::c2g::fail(C2G_EDEVICE, "%s failed: %s (%s:%d)", #expr,            \
-                               hipGetErrorString(_e), __FILE__, __LINE__);             \
+            return ::c2g::fail(_e == hipErrorOutOfMemory ? C2G_ENOMEM : C2G_EDEVICE,  \
+                               "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                               __FILE__, __LINE__);                                    \
     } while (0)
 
 // Device-to-host copies into pageable memory through two pinned chunks: the
