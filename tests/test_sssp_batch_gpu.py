@@ -117,3 +117,28 @@ def test_retune_between_runs_takes_effect(gpu):
         d, st = g.sssp(int(srcs[0]))
     assert s1.iterations > s0.iterations  # narrow buckets need more rounds
     assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], int(srcs[0])))
+
+
+def test_batch_duplicate_edges(gpu):
+    # repeated (u, v) edges with different weights (graphs given as CSR are
+    # not deduplicated): several lanes and steps update the same distance
+    rng = np.random.default_rng(3)
+    V, E = 400, 6000
+    u = rng.integers(0, V, E)
+    v = rng.integers(0, V, E)
+    keep = u != v
+    u, v = np.concatenate([u[keep], u[keep]]), np.concatenate([v[keep], v[keep]])
+    w = rng.integers(1, 50, len(u))
+    order = np.lexsort((v, u))
+    u, v, w = u[order], v[order], w[order]
+    vl = np.zeros(V + 1, np.int64)
+    np.add.at(vl, u + 1, 1)
+    G = dict(V=V, E=len(v), vlist=np.cumsum(vl).astype(np.int32), elist=v.astype(np.int32),
+             wlist=w.astype(np.int32))
+    srcs = np.arange(0, V, 5).astype(np.int32)
+    with _graph(G) as g:
+        d, _ = g.sssp_batch(srcs)
+        ds, _ = g.sssp(int(srcs[0]))
+    assert np.array_equal(ds, d[0])
+    for q, s in enumerate(srcs):
+        assert np.array_equal(d[q], oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], int(s))), q
