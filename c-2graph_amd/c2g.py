@@ -33,10 +33,11 @@ class Stats(C.Structure):
     _fields_ = [("lane", C.c_int32), ("iterations", C.c_int32), ("host_iterations", C.c_int32),
                 ("reached", C.c_int32), ("frontier_total", C.c_int64), ("edges_relaxed", C.c_int64),
                 ("improvements", C.c_int64), ("reached_edges", C.c_int64), ("csr_edges", C.c_int64),
-                ("items", C.c_int64), ("ms", C.c_double), ("kernel_ms", C.c_double)]
+                ("items", C.c_int64), ("ms", C.c_double), ("kernel_ms", C.c_double),
+                ("pull_rounds", C.c_int32), ("pad", C.c_int32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
 class Opts(C.Structure):
@@ -48,7 +49,7 @@ class Tune(C.Structure):
     _fields_ = [("block", C.c_int32), ("grid_per_cu", C.c_int32), ("chunk", C.c_int32),
                 ("batch_chunk", C.c_int32), ("iters_per_graph", C.c_int32), ("use_graph", C.c_int32),
                 ("delta", C.c_int32), ("profile", C.c_int32), ("grid_per_cu_single", C.c_int32),
-                ("reserved", C.c_int32 * 7)]
+                ("ppr_pull", C.c_int32), ("reserved", C.c_int32 * 6)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -356,6 +357,15 @@ class Graph:
                             _ptr(r, C.c_float), C.byref(st)))
         return p, r, st
 
+    def ppr_migrate(self, src, budget, handoff_min=0, alpha=0.15, eps=1e-7):
+        """GPU rounds up to `budget`, then the host-core push finishes (c2g_ppr_migrate)."""
+        p = np.empty(self.V, np.float32)
+        r = np.empty(self.V, np.float32)
+        st = Stats()
+        check(lib().c2g_ppr_migrate(self._h, int(src), C.c_float(alpha), C.c_float(eps), int(budget),
+                                    int(handoff_min), _ptr(p, C.c_float), _ptr(r, C.c_float), C.byref(st)))
+        return p, r, st
+
     def ppr_batch(self, srcs, alpha=0.15, eps=1e-7):
         srcs = np.ascontiguousarray(srcs, np.int32)
         p = np.empty((len(srcs), self.V), np.float32)
@@ -396,6 +406,13 @@ class Graph:
         srcs = np.ascontiguousarray(srcs, np.int32)
         t = Tune()
         check(lib().c2g_autotune(self._h, _ptr(srcs, C.c_int32), len(srcs), int(batched), C.byref(t)))
+        return t
+
+    def autotune_ppr(self, srcs, alpha=0.15, eps=1e-7):
+        srcs = np.ascontiguousarray(srcs, np.int32)
+        t = Tune()
+        check(lib().c2g_autotune_ppr(self._h, _ptr(srcs, C.c_int32), len(srcs), C.c_float(alpha), C.c_float(eps),
+                                     C.byref(t)))
         return t
 
 

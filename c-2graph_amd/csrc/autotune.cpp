@@ -1,5 +1,6 @@
 // Autotuner: picks launch shape, work-item chunk size and rounds per hipGraph
-// replay for the SSSP kernels by timing them on sample sources.
+// replay for the SSSP kernels, and the push/pull switch point of single-source
+// PPR, by timing them on sample sources.
 //
 // The reference compiles in one shape (20 workgroups x 512 threads,
 // core/Util.cuh:31-35).  Here the candidates are timed on the caller's sample
@@ -41,7 +42,54 @@ double measure(Graph* g, const int32_t* srcs, int32_t n, bool batched, int* rc) 
     return best;
 }
 
+double time_ppr(Graph* g, const int32_t* srcs, int32_t n, float alpha, float eps, int* rc) {
+    double best = 1e30;
+    for (int r = 0; r < 2 && *rc == C2G_OK; ++r) {
+        const double t0 = now_ms();
+        for (int32_t i = 0; i < n && *rc == C2G_OK; ++i)
+            *rc = ppr_single(g, g->stream, srcs[i], alpha, eps, nullptr, nullptr, nullptr);
+        best = std::min(best, now_ms() - t0);
+    }
+    return best;
+}
+
 }  // namespace
+
+// PPR push/pull switch point: the share of all out-edges (in thousandths)
+// above which a round's frontier is propagated by a pull sweep over in-edges
+// instead of scattered pushes, timed on the caller's sources, alpha and eps.
+// Both forms compute the same round, so the choice changes FP32 summation
+// order only.
+extern "C" int c2g_autotune_ppr(c2g_graph* h, const int32_t* srcs, int32_t n, float alpha, float eps,
+                                c2g_tune* best_out) {
+    if (!h || !srcs || n < 1) return fail(C2G_EINVAL, "c2g_autotune_ppr: need at least one sample source");
+    std::lock_guard<std::mutex> lk(h->g.mu);
+    Graph* g = &h->g;
+    if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");
+    for (int32_t i = 0; i < n; ++i)
+        if (srcs[i] < 0 || srcs[i] >= g->V) return fail(C2G_EINVAL, "source %d out of range", srcs[i]);
+    const int32_t m = std::min<int32_t>(n, 16);
+    c2g_tune best = g->tune;
+    best.profile = 0;
+    set_tune_locked(g, best);
+    int rc = C2G_OK;
+    (void)time_ppr(g, srcs, 1, alpha, eps, &rc);  // warm: allocations, in-edge rows
+    double tbest = time_ppr(g, srcs, m, alpha, eps, &rc);
+    for (int v : {0, 25, 50, 100, 200, 400, 800}) {
+        if (rc) break;
+        c2g_tune t = best;
+        t.ppr_pull = v;
+        if (!memcmp(&t, &best, sizeof t)) continue;
+        set_tune_locked(g, t);
+        const double tt = time_ppr(g, srcs, m, alpha, eps, &rc);
+        if (rc) break;
+        if (tt < tbest * 0.98) { tbest = tt; best = t; }
+    }
+    set_tune_locked(g, best);
+    if (rc) return rc;
+    if (best_out) *best_out = best;
+    return C2G_OK;
+}
 
 extern "C" int c2g_autotune(c2g_graph* h, const int32_t* srcs, int32_t n, int32_t batched, c2g_tune* best_out) {
     if (!h || !srcs || n < 1) return fail(C2G_EINVAL, "c2g_autotune: need at least one sample source");

@@ -17,6 +17,13 @@
 //                  a time per wave (shuffle owner search as in SSSP); one
 //                  hardware FP32 atomicAdd into m[v] per edge, and v marked in
 //                  a touched bitmap (instead of the reference's full-V scan).
+//                  On dense rounds (the previous expand activated more than
+//                  the tunable share of all out-edges) the same kernel runs
+//                  the pull form instead: a coalesced sweep over the in-edge
+//                  rows, m[v] = (1-alpha) * sum over activated in-neighbours u
+//                  of r_u * f(u,v), one segmented wave sum and one add per
+//                  row segment.  Activated u are recognised by a round stamp
+//                  next to the residual they push.
 //   ppr_expand_k : walks the touched bitmap; a touched vertex adds its
 //                  message, activates above threshold, and its out-row is cut
 //                  into items carrying the residual being pushed (wave scan +
@@ -39,8 +46,15 @@ namespace c2g {
 
 enum : int {
     P_CUR = 0, P_ITER = 1, P_DONE = 2, P_DONE_E = 3, P_FRONT = 4, P_EDGES = 5, P_ITEMS = 6,
-    P_OVF = 7, P_N = 16
+    P_OVF = 7,
+    P_MODE = 8,    // single source: 1 = the next round runs as a pull sweep
+    P_REDGES = 9,  // single source: out-edges of the vertices the running expand activates
+    P_PULLS = 10,  // single source: rounds run as pull sweeps
+    P_N = 16
 };
+
+constexpr int kPullChunk = 64;  // in-edges per pull item
+constexpr int kPullWin = 4;     // 64-edge windows of a pull step
 
 __global__ void ppr_init_k(float* __restrict__ p, float* __restrict__ r, float* __restrict__ m,
                            uint32_t* __restrict__ tb, int64_t n) {
@@ -87,12 +101,76 @@ __global__ void ppr_seed_k(const int32_t* __restrict__ row, const int2* __restri
 // (FP32 atomics), and marks v in the touched bitmap for the expand.
 __global__ void __launch_bounds__(256)
 ppr_push_k(const int2* __restrict__ adjf, float* __restrict__ m, uint32_t* __restrict__ tb,
-           const int4* __restrict__ q, int64_t* __restrict__ ctrl, float alpha) {
+           const int4* __restrict__ q, int64_t* __restrict__ ctrl, float alpha,
+           const int2* __restrict__ tadjf, const int4* __restrict__ titems, int64_t nt,
+           const int2* __restrict__ ru, const uint32_t* __restrict__ stamp) {
     const int64_t n = ctrl[P_CUR];
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     const int lane = lane_id();
-    for (int64_t base = wave * kWave; base < n; base += nwaves * kWave) {
+    const bool pull = ctrl[P_MODE] != 0;
+    if (pull) {
+        // Pull sweep over the in-edge items (v, e_begin, e_end): the edges of
+        // 64 items are walked 64 at a time; lanes of one item form a segment
+        // whose sum goes to m[v] with one atomic (an item can continue in the
+        // next window, so its parts are added).
+        const int S = (int)*stamp;
+        for (int64_t base = wave * kWave; base < nt; base += nwaves * kWave) {
+            const int64_t i = base + lane;
+            int eb = 0, cnt = 0, v = 0;
+            if (i < nt) {
+                const int4 it = titems[i];
+                v = it.x;
+                eb = it.y;
+                cnt = it.z - it.y;
+            }
+            int tot;
+            const int pre = wave_excl_scan(cnt, &tot);
+            // four 64-edge windows per step: their edge loads, then their
+            // residual gathers, are in flight together
+            for (int s0 = 0; s0 < tot; s0 += kPullWin * kWave) {
+                int o[kPullWin], vo[kPullWin];
+                int2 a[kPullWin];
+                float val[kPullWin];
+#pragma unroll
+                for (int w = 0; w < kPullWin; ++w) {
+                    const int k = s0 + w * kWave + lane;
+                    o[w] = wave_owner(pre, k);
+                    const int e = __shfl(eb, o[w], 64) + (k - __shfl(pre, o[w], 64));
+                    vo[w] = __shfl(v, o[w], 64);
+                    a[w] = make_int2(0, 0);
+                    if (k < tot) a[w] = tadjf[e];
+                }
+#pragma unroll
+                for (int w = 0; w < kPullWin; ++w) {
+                    const int k = s0 + w * kWave + lane;
+                    val[w] = 0.0f;
+                    if (k < tot) {
+                        const int2 x = ru[a[w].x];
+                        if (x.x == S) val[w] = __int_as_float(x.y) * __int_as_float(a[w].y);
+                    }
+                }
+#pragma unroll
+                for (int w = 0; w < kPullWin; ++w) {
+                    const int k = s0 + w * kWave + lane;
+                    if (s0 + w * kWave >= tot) break;  // uniform: no edges left in this window
+#pragma unroll
+                    for (int d = 1; d < kWave; d <<= 1) {
+                        const float t = __shfl_up(val[w], d, 64);
+                        const int to = __shfl_up(o[w], d, 64);
+                        if (lane >= d && to == o[w]) val[w] += t;
+                    }
+                    const int on = __shfl_down(o[w], 1, 64);
+                    const bool last = k < tot && (lane == kWave - 1 || on != o[w] || k + 1 == tot);
+                    if (last && val[w] != 0.0f) {
+                        unsafeAtomicAdd(&m[vo[w]], (1.0f - alpha) * val[w]);
+                        touch(tb, vo[w]);
+                    }
+                }
+            }
+        }
+    }
+    for (int64_t base = wave * kWave; !pull && base < n; base += nwaves * kWave) {
         const int64_t i = base + lane;
         int eb = 0, cnt = 0;
         float msg = 0.0f;
@@ -130,6 +208,7 @@ ppr_push_k(const int2* __restrict__ adjf, float* __restrict__ m, uint32_t* __res
         if (prev == (unsigned long long)gridDim.x - 1) {
             __threadfence();
             if (n > 0) ctrl[P_ITER] += 1;
+            if (n > 0 && pull) ctrl[P_PULLS] += 1;
             ctrl[P_ITEMS] += n;
             ctrl[P_CUR] = 0;
             ctrl[P_DONE] = 0;
@@ -142,12 +221,17 @@ ppr_push_k(const int2* __restrict__ adjf, float* __restrict__ m, uint32_t* __res
 // alone when !degscale; always in the `force` first round), p[v] += alpha*r[v], r[v] = 0 and v's items join
 // the next round carrying the residual to push.  Two passes per word: count
 // the items of the activated vertices, take the queue range with one atomic
-// per wave, then apply the updates and write the items.
+// per wave, then apply the updates and write the items.  With ru set, each
+// activated v also gets ru[v] = (stamp+1, residual) for a pull round, and the
+// last workgroup picks the next round's form: pull when the activated
+// vertices hold more than pull_edges out-edges.
 __global__ void __launch_bounds__(256)
 ppr_expand_k(const int32_t* __restrict__ row, float* __restrict__ p, float* __restrict__ r,
              float* __restrict__ m, uint32_t* __restrict__ tb, int4* __restrict__ q, int64_t* __restrict__ ctrl,
-             int64_t V, float alpha, float eps, int chunk, int64_t qcap, int force, int degscale) {
+             int64_t V, float alpha, float eps, int chunk, int64_t qcap, int force, int degscale,
+             int2* __restrict__ ru, uint32_t* __restrict__ stamp, int64_t pull_edges) {
     const int lane = lane_id();
+    const int S1 = ru ? (int)*stamp + 1 : 0;
     const int64_t nw = (V + 31) >> 5;
     const int64_t nwr = (nw + 63) & ~(int64_t)63;  // uniform trip count per wave
     int64_t front = 0, edges = 0;
@@ -185,6 +269,7 @@ ppr_expand_k(const int32_t* __restrict__ row, float* __restrict__ p, float* __re
             if (force || rv > (degscale ? (float)(r1 - r0) * eps : eps)) {
                 p[v] += alpha * rv;
                 r[v] = 0.0f;
+                if (ru) ru[v] = make_int2(S1, __float_as_int(rv));
                 front += 1;
                 edges += r1 - r0;
                 if (room)
@@ -200,6 +285,7 @@ ppr_expand_k(const int32_t* __restrict__ row, float* __restrict__ p, float* __re
     if (lane == 0 && front) {
         atomicAdd((unsigned long long*)&ctrl[P_FRONT], (unsigned long long)front);
         atomicAdd((unsigned long long*)&ctrl[P_EDGES], (unsigned long long)edges);
+        atomicAdd((unsigned long long*)&ctrl[P_REDGES], (unsigned long long)edges);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -210,6 +296,10 @@ ppr_expand_k(const int32_t* __restrict__ row, float* __restrict__ p, float* __re
             ctrl[P_DONE_E] = 0;
             int64_t c = (int64_t)atomicAdd((unsigned long long*)&ctrl[P_CUR], 0ull);
             if (c > qcap) { ctrl[P_OVF] = 1; ctrl[P_CUR] = 0; }
+            const int64_t re = (int64_t)atomicAdd((unsigned long long*)&ctrl[P_REDGES], 0ull);
+            ctrl[P_MODE] = (ru && pull_edges > 0 && re > pull_edges) ? 1 : 0;
+            ctrl[P_REDGES] = 0;
+            if (ru) *stamp = (uint32_t)S1;
         }
     }
 }
@@ -424,6 +514,14 @@ struct PprState {
     int32_t* srcs = nullptr;
     ColStage cols;             // per-query rows out of P and R
     HostStage out;             // single-query p / r copies to the host
+    // single-source pull rounds: in-edge rows (u, bits of f(u,v)) cut into
+    // items of <= kPullChunk edges, and the stamped residuals
+    int2* tadjf = nullptr;
+    int4* titems = nullptr;
+    int64_t nt = 0;
+    int2* ru = nullptr;        // [V] (stamp of the activating expand, bits of the pushed residual)
+    uint32_t* stamp = nullptr; // stamp of the last expand (device)
+    int64_t stamp_est = 0;     // upper bound of *stamp kept on the host
 };
 
 static std::mutex g_ps_mu;
@@ -445,7 +543,8 @@ void ppr_release(Graph* g) {
             PprState* s = g_ps[i].second;
             for (void* p : {(void*)s->p, (void*)s->r, (void*)s->m, (void*)s->tb, (void*)s->q, (void*)s->ctrl, (void*)s->P,
                             (void*)s->R, (void*)s->M, (void*)s->RU, (void*)s->mask, (void*)s->tbits,
-                            (void*)s->items, (void*)s->imask, (void*)s->srcs})
+                            (void*)s->items, (void*)s->imask, (void*)s->srcs, (void*)s->tadjf,
+                            (void*)s->titems, (void*)s->ru, (void*)s->stamp})
                 if (p) (void)hipFree(p);
             if (s->h_ctrl) (void)hipHostFree(s->h_ctrl);
             s->cols.release();
@@ -487,6 +586,67 @@ static int ensure_ppr_single(Graph* g, PprState* s) {
     return C2G_OK;
 }
 
+// In-edge rows of the PPR graph for pull rounds, built once per handle on
+// the host: thread t owns a range of target vertices and scans every row in
+// source order, so each in-row lists its sources in increasing order and the
+// layout does not depend on the thread count.
+static int ensure_pull(Graph* g, PprState* s) {
+    const int64_t V = g->V, E = g->E;
+    if (!s->ru) {
+        HIP_TRY(hipMalloc(&s->ru, sizeof(int2) * (size_t)std::max<int64_t>(V, 1)));
+        HIP_TRY(hipMemset(s->ru, 0, sizeof(int2) * (size_t)std::max<int64_t>(V, 1)));
+        HIP_TRY(hipMalloc(&s->stamp, sizeof(uint32_t)));
+        HIP_TRY(hipMemset(s->stamp, 0, sizeof(uint32_t)));
+        s->stamp_est = 0;
+    }
+    if (s->stamp_est > ((int64_t)1 << 30)) {  // restart the stamps long before they wrap
+        HIP_TRY(hipMemset(s->ru, 0, sizeof(int2) * (size_t)std::max<int64_t>(V, 1)));
+        HIP_TRY(hipMemset(s->stamp, 0, sizeof(uint32_t)));
+        s->stamp_est = 0;
+    }
+    if (s->tadjf) return C2G_OK;
+    const int32_t* vl = g->vl.data();
+    const int32_t* el = g->el.data();
+    const float* fl = g->fl.data();
+    std::vector<int64_t> trow((size_t)V + 1, 0);
+    const int T = std::max(1, std::min(host_threads(), 16));
+    parallel_for(T, [&](int64_t t0, int64_t t1, int) {
+        for (int64_t t = t0; t < t1; ++t) {
+            const int64_t lo = V * t / T, hi = V * (t + 1) / T;
+            for (int64_t e = 0; e < E; ++e)
+                if (el[e] >= lo && el[e] < hi) trow[(size_t)el[e] + 1] += 1;
+        }
+    }, T, 1);
+    for (int64_t v = 0; v < V; ++v) trow[v + 1] += trow[v];
+    std::vector<int2> tadj((size_t)std::max<int64_t>(E, 1));
+    parallel_for(T, [&](int64_t t0, int64_t t1, int) {
+        for (int64_t t = t0; t < t1; ++t) {
+            const int64_t lo = V * t / T, hi = V * (t + 1) / T;
+            std::vector<int64_t> pos(trow.begin() + lo, trow.begin() + hi);
+            for (int64_t u = 0; u < V; ++u)
+                for (int64_t e = vl[u]; e < vl[u + 1]; ++e) {
+                    const int32_t v = el[e];
+                    if (v < lo || v >= hi) continue;
+                    int32_t fb;
+                    memcpy(&fb, &fl[e], sizeof fb);
+                    tadj[(size_t)pos[v - lo]++] = make_int2((int32_t)u, fb);
+                }
+        }
+    }, T, 1);
+    std::vector<int4> items;
+    items.reserve((size_t)V + (size_t)(E / kPullChunk) + 1);
+    for (int64_t v = 0; v < V; ++v)
+        for (int64_t e = trow[v]; e < trow[v + 1]; e += kPullChunk)
+            items.push_back(make_int4((int32_t)v, (int32_t)e, (int32_t)std::min<int64_t>(e + kPullChunk, trow[v + 1]), 0));
+    HIP_TRY(hipMalloc(&s->tadjf, sizeof(int2) * tadj.size()));
+    HIP_TRY(hipMemcpy(s->tadjf, tadj.data(), sizeof(int2) * tadj.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&s->titems, sizeof(int4) * std::max<size_t>(items.size(), 1)));
+    if (!items.empty())
+        HIP_TRY(hipMemcpy(s->titems, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
+    s->nt = (int64_t)items.size();
+    return C2G_OK;
+}
+
 static const int kPprMaxRounds = 100000;
 
 // Seeding and threshold form of a single-source push: the workload's PPR is
@@ -523,26 +683,35 @@ static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, f
     const int grid = std::max(1, g->num_cus) * g->tune.grid_per_cu;
     const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (V + 255) / 256));
     const int wgrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, ((V + 31) / 32 + 255) / 256));
+    // Pull rounds: a round whose frontier holds more than ppr_pull/1000 of E
+    // out-edges gathers over in-edges instead of scattering.
+    const bool use_pull = g->tune.ppr_pull > 0 && g->E > 0;
+    if (use_pull && (rc = ensure_pull(g, s))) return rc;
+    int2* ru = use_pull ? s->ru : nullptr;
+    const int64_t pull_edges = use_pull ? std::max<int64_t>(1, g->E * g->tune.ppr_pull / 1000) : 0;
     const double t0 = now_ms();
     hipLaunchKernelGGL(ppr_init_k, dim3(egrid), dim3(256), 0, strm, s->p, s->r, s->m, s->tb, V);
     hipLaunchKernelGGL(ppr_seed_k, dim3(1), dim3(256), 0, strm, g->d_row, g->d_adjf,
                        pruned ? g->d_hrow : (const int32_t*)nullptr, pruned ? g->d_hadjf : (const int2*)nullptr,
                        s->p, s->m, s->tb, s->ctrl, src, alpha, pruned ? 1 : 0, spec.r0, spec.p_src);
     hipLaunchKernelGGL(ppr_expand_k, dim3(wgrid), dim3(256), 0, strm, g->d_row, s->p, s->r, s->m, s->tb, s->q,
-                       s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 1, spec.degscale);
+                       s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 1, spec.degscale, ru, s->stamp, pull_edges);
     HIP_TRY(hipGetLastError());
+    if (use_pull) s->stamp_est += 1;
     int rounds = 0;
     int64_t front_prev = 1;
     for (;;) {
         const int K = budget > 0 ? 1 : g->tune.iters_per_graph;
         for (int k = 0; k < K; ++k) {
             hipLaunchKernelGGL(ppr_push_k, dim3(grid), dim3(256), 0, strm, g->d_adjf, s->m, s->tb, s->q,
-                               s->ctrl, alpha);
+                               s->ctrl, alpha, s->tadjf, s->titems, use_pull ? s->nt : 0, s->ru, s->stamp);
             hipLaunchKernelGGL(ppr_expand_k, dim3(wgrid), dim3(256), 0, strm, g->d_row, s->p, s->r, s->m,
-                               s->tb, s->q, s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 0, spec.degscale);
+                               s->tb, s->q, s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 0, spec.degscale,
+                               ru, s->stamp, pull_edges);
         }
         HIP_TRY(hipGetLastError());
         rounds += K;
+        if (use_pull) s->stamp_est += K;
         HIP_TRY(hipMemcpyAsync(s->h_ctrl, s->ctrl, sizeof(int64_t) * P_N, hipMemcpyDeviceToHost, strm));
         HIP_TRY(hipStreamSynchronize(strm));
         if (s->h_ctrl[P_OVF]) return fail(C2G_EDEVICE, "PPR frontier queue overflow (internal)");
@@ -580,6 +749,7 @@ static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, f
         st->frontier_total = s->h_ctrl[P_FRONT];
         st->edges_relaxed = s->h_ctrl[P_EDGES];
         st->items = s->h_ctrl[P_ITEMS];
+        st->pull_rounds = (int32_t)s->h_ctrl[P_PULLS];
         st->ms = t1 - t0;
     }
     return C2G_OK;

@@ -87,6 +87,8 @@ typedef struct {
     int64_t items;           /* frontier work items processed                     */
     double  ms;              /* wall time of the call's device work               */
     double  kernel_ms;       /* profile runs: summed duration of the relax kernel */
+    int32_t pull_rounds;     /* single-source PPR: rounds run as pull sweeps      */
+    int32_t pad;
 } c2g_stats;
 
 /* Tunables picked by the autotuner (c2g_autotune) or set by hand. */
@@ -102,7 +104,11 @@ typedef struct {
                                kernel; the summed time lands in c2g_stats.kernel_ms */
     int32_t grid_per_cu_single; /* workgroups per CU for the single-source SSSP
                                kernels (0 = grid_per_cu)                      */
-    int32_t reserved[7];
+    int32_t ppr_pull;       /* single-source PPR push/pull switch: a round whose
+                               frontier holds more than ppr_pull/1000 of all
+                               out-edges runs as a pull sweep over in-edges
+                               (0 = push only; 0..1000)                       */
+    int32_t reserved[6];
 } c2g_tune;
 
 /* ======================================================================= */
@@ -342,6 +348,14 @@ int c2g_adsorption(c2g_graph* g, const int32_t* seed_v, const int32_t* seed_labe
  * degree tiers (src/sssp/base/Main.cu:197-293). */
 int c2g_autotune(c2g_graph* g, const int32_t* srcs, int32_t n, int32_t batched,
                  c2g_tune* best_out);
+
+/* Picks c2g_tune.ppr_pull, the push/pull switch point of single-source PPR,
+ * by timing c2g_ppr on the sample sources with the caller's alpha and eps,
+ * and stores it in the handle (best_out nullable).  The reference only
+ * pushes (calcuatePPR<float>, src/ppr/base/Main.cu:208-320); the pull form
+ * computes the same round. */
+int c2g_autotune_ppr(c2g_graph* g, const int32_t* srcs, int32_t n, float alpha, float eps,
+                     c2g_tune* best_out);
 
 /* Diagnostic (no reference counterpart): a plain device copy of `bytes`
  * bytes repeated `reps` times, used to calibrate HBM byte counters; returns

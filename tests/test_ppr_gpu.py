@@ -165,3 +165,99 @@ def test_c3_full_scale_against_power_iteration(gpu):
     print(f"\nC3 scale 22 source {s}: L1 {l1:.6g} max-abs {mx:.6g} sum(r) {sr:.6g} rounds {st.iterations}")
     assert abs(l1 - sr) <= 1e-5
     assert np.all(r <= deg * 1e-7 + 1e-12)
+
+
+def _l1(a, b):
+    return float(np.abs(a.astype(np.float64) - b.astype(np.float64)).sum())
+
+
+@pytest.mark.parametrize("pull", [1, 50, 1000])
+def test_pull_rounds_match_push(gpu, pull):
+    # rounds whose frontier holds more than pull/1000 of the out-edges gather
+    # over in-edges; the result is the push's up to FP32 summation order
+    G = rmat_graph(16, 3, wmax=0, symmetric=True, want_f=True)
+    deg = np.diff(G["vlist"])
+    rng = np.random.default_rng(5)
+    srcs = [int(np.argmax(deg))] + [int(x) for x in rng.choice(np.flatnonzero(deg > 0), 2)]
+    with _g(G) as g:
+        for s in srcs:
+            g.set_tune(ppr_pull=0)
+            p0, r0, st0 = g.ppr(s, alpha=ALPHA, eps=1e-9)
+            g.set_tune(ppr_pull=pull)
+            p1, r1, st1 = g.ppr(s, alpha=ALPHA, eps=1e-9)
+            assert st0.pull_rounds == 0
+            assert (st1.pull_rounds > 0) == (pull < 1000), (s, st1.pull_rounds)
+            assert _l1(p1, p0) <= 2e-5 and _l1(r1, r0) <= 2e-5, s
+            pi = oracle.ppr_power(G["vlist"], G["elist"], G["flist"], s, alpha=ALPHA, tol=1e-15)
+            assert abs(_l1(p1, pi) - r1.astype(np.float64).sum()) <= 1e-5
+            assert np.all(r1 <= 1e-9 * deg * (1 + 1e-5) + 1e-30)
+
+
+def test_pull_rounds_directed_and_small(gpu):
+    # dangling vertices, vertices without in-edges (no pull items), a graph
+    # whose V is not a multiple of 32, and the accuracy bound at small eps
+    G = rmat_graph(14, 5, wmax=0, want_f=True)
+    s = int(np.argmax(np.diff(G["vlist"])))
+    pi = oracle.ppr_power(G["vlist"], G["elist"], G["flist"], s, alpha=ALPHA, tol=1e-15)
+    with _g(G) as g:
+        g.set_tune(ppr_pull=1)
+        p, r, st = g.ppr(s, alpha=ALPHA, eps=1e-11)
+    assert st.pull_rounds > 0
+    assert _l1(p, pi) <= 1e-5
+    H = random_graph(301, 2500, seed=6)
+    with _g(H) as g:
+        for s in (0, 150, 300):
+            p0, r0, _ = g.ppr(s, alpha=ALPHA, eps=1e-9)
+            g.set_tune(ppr_pull=1)
+            p1, r1, st = g.ppr(s, alpha=ALPHA, eps=1e-9)
+            g.set_tune(ppr_pull=0)
+            assert st.pull_rounds > 0 and _l1(p1, p0) <= 2e-5 and _l1(r1, r0) <= 2e-5, s
+
+
+def test_pull_rounds_pruned_graph_and_reference_modes(gpu):
+    # pruned ppr/ variant (one-hop seeding, then the live edges), the
+    # reference's adsorption push, and the budgeted GPU -> host hand-over
+    G = random_graph(300, 900, seed=4)
+    P = c2g.prune_ppr_mem(G["vlist"], G["elist"], G["flist"])
+    with c2g.Graph.from_csr(P["vlist"], P["elist"], flist=P["wlist"], hvlist=P["hvlist"], helist=P["helist"],
+                            hwf=P["hwlist"], variant=c2g.VARIANT_PPR_PURN, device=0) as g:
+        for s in range(0, 300, 37):
+            p0, r0, _ = g.ppr(s, alpha=ALPHA, eps=1e-9)
+            g.set_tune(ppr_pull=1)
+            p1, r1, _ = g.ppr(s, alpha=ALPHA, eps=1e-9)
+            g.set_tune(ppr_pull=0)
+            assert _l1(p1, p0) <= 2e-5 and _l1(r1, r0) <= 2e-5, s
+    H = rmat_graph(14, 3, wmax=0, symmetric=True, want_f=True)
+    s = int(np.argmax(np.diff(H["vlist"])))
+    with _g(H) as g:
+        a0, b0, _ = g.adsorption_push(s)
+        m0, n0, sm0 = g.ppr_migrate(s, 3, 1 << 30, alpha=ALPHA, eps=1e-9)
+        g.set_tune(ppr_pull=1)
+        a1, b1, sa = g.adsorption_push(s)
+        m1, n1, sm1 = g.ppr_migrate(s, 3, 1 << 30, alpha=ALPHA, eps=1e-9)
+    assert sa.pull_rounds > 0
+    # the adsorption push seeds residual 10: ten times the unit-mass bound 2e-5
+    assert _l1(a1, a0) <= 2e-4 and _l1(b1, b0) <= 2e-4
+    assert _l1(m1, m0) <= 2e-5 and _l1(n1, n0) <= 2e-5
+
+
+@pytest.mark.slow
+def test_c3_full_scale_l1_within_1e5(gpu):
+    # the C3 graph and source again, at eps = 1e-13*outdeg: the push's own
+    # truncation sum(r) falls below 1e-5, and the FP32 result meets L1 <= 1e-5
+    # and max-abs <= 1e-6 against the FP64 power iteration.  Most rounds are
+    # dense and run as pull sweeps at the default switch point.
+    G = rmat_graph(22, 3, wmax=0, symmetric=True, want_f=True)
+    deg = np.diff(G["vlist"])
+    rng = np.random.default_rng(3)
+    s = int(rng.choice(np.flatnonzero(deg > 0)))
+    with _g(G) as g:
+        p, r, st = g.ppr(s, alpha=ALPHA, eps=1e-13)
+    pi = oracle.ppr_power(G["vlist"], G["elist"], G["flist"], s, alpha=ALPHA, tol=1e-13)
+    d = np.abs(p.astype(np.float64) - pi)
+    l1, mx, sr = d.sum(), d.max(), r.astype(np.float64).sum()
+    print(f"\nC3 scale 22 source {s} eps 1e-13: L1 {l1:.6g} max-abs {mx:.6g} sum(r) {sr:.6g} "
+          f"rounds {st.iterations} (pull {st.pull_rounds}) {st.ms:.1f} ms")
+    assert st.pull_rounds > 0
+    assert l1 <= 1e-5 and mx <= 1e-6
+    assert abs(l1 - sr) <= 1e-6
