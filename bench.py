@@ -204,6 +204,30 @@ def aux_phase(args):
     return out
 
 
+def ppr_accuracy_phase(args):
+    """Single-source PPR on the C3 graph (R-MAT scale 22 symmetrised, seed 3,
+    alpha 0.15) at eps = 1e-13*outdeg, where the push's own truncation sum(r)
+    falls below the 1e-5 L1 target: push-only rounds against the default
+    push/pull switch.  The L1 to the FP64 power iteration is checked in
+    tests/test_ppr_gpu.py::test_c3_full_scale_l1_within_1e5."""
+    import c2g
+
+    G3 = c2g.rmat(22, 16, 0.57, 0.19, 0.19, seed=3, wmax=0, symmetric=True, want_f=True)
+    deg = np.diff(G3["vlist"])
+    s = int(np.random.default_rng(3).choice(np.flatnonzero(deg > 0)))
+    out = {"graph": "R-MAT scale 22 symmetrised seed 3", "source": s, "alpha": 0.15, "eps": "1e-13*outdeg"}
+    with c2g.Graph.from_csr(G3["vlist"], G3["elist"], flist=G3["flist"], device=0) as g:
+        default = g.tune().ppr_pull
+        g.ppr(s)  # warm: allocations and the in-edge rows
+        for name, pull in (("push_only", 0), ("push_pull", default)):
+            g.set_tune(ppr_pull=pull)
+            _, r, st = g.ppr(s, eps=1e-13)
+            out[name] = {"ms": round(st.ms, 2), "rounds": st.iterations, "pull_rounds": st.pull_rounds,
+                         "ppr_pull": pull, "sum_r": float(r.astype(np.float64).sum())}
+            log(f"ppr accuracy {name}: {out[name]}")
+    return out
+
+
 def ppr_purn_study(args):
     """Approximation of the pruned PPR graph (purn-ppr's additive 0.8*w
     compensation, src/purn-ppr.cpp:243-259, with the ppr-purn seeding
@@ -373,6 +397,8 @@ def main():
     ap.add_argument("--ppr-purn-study", action="store_true",
                     help="only the pruned-PPR approximation study (PPR on the purn-ppr graph vs the full graph)")
     ap.add_argument("--study-scale", type=int, default=20)
+    ap.add_argument("--ppr-accuracy", action="store_true",
+                    help="only single-source PPR on the C3 graph at eps 1e-13*outdeg, push-only vs push/pull")
     ap.add_argument("--c2", action="store_true", help="only the pruned-graph SSSP workload (scale 22 seed 2)")
     ap.add_argument("--pcie", action="store_true",
                     help="also time one batched step with every distance vector copied to host memory")
@@ -407,6 +433,9 @@ def main():
         return
     if args.ppr_purn_study:
         print(json.dumps({"ppr_purn_study": ppr_purn_study(args)}), flush=True)
+        return
+    if args.ppr_accuracy:
+        print(json.dumps({"ppr_accuracy": ppr_accuracy_phase(args)}), flush=True)
         return
     if args.aux_only:
         args.skip_cpu = True  # kernels only (the profiled runs)
