@@ -325,7 +325,7 @@ def serve_phase(args):
     lat = []
     lanes = {}
     with c2g.Server(g, max_batch=64, ads_labels=16, ads_iters=20, alpha=0.15, eps=1e-7,
-                    sssp_slots=args.serve_slots) as s:
+                    sssp_slots=args.serve_slots, ads_grid_per_cu=args.serve_ads_grid) as s:
         # warm the lanes (allocations, first kernel loads)
         for k in (c2g.Q_SSSP, c2g.Q_PPR, c2g.Q_ADS):
             s.poll(s.submit(k, int(pool[0]) if k != c2g.Q_ADS else 0), want_result=False)
@@ -397,6 +397,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=15.0)
     ap.add_argument("--serve-scale", type=int, default=23)
     ap.add_argument("--serve-slots", type=int, default=0, help="SSSP query-major slots (0 = library default)")
+    ap.add_argument("--serve-ads-grid", type=int, default=0,
+                    help="workgroups per CU of the adsorption lane (0 = library default)")
     ap.add_argument("--aux", action="store_true", help="also time PPR (scale 22) and adsorption (scale 23)")
     ap.add_argument("--ppr-purn-study", action="store_true",
                     help="only the pruned-PPR approximation study (PPR on the purn-ppr graph vs the full graph)")

@@ -60,7 +60,8 @@ class ServerOpts(C.Structure):
                 ("fake_free_hbm", C.c_int64), ("host_lane", C.c_int32), ("device_failed", C.c_int32),
                 ("alpha", C.c_float), ("eps", C.c_float), ("ads_labels", C.c_int32),
                 ("ads_iters", C.c_int32), ("p_inj", C.c_float), ("p_cont", C.c_float),
-                ("p_abnd", C.c_float), ("sssp_slots", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("p_abnd", C.c_float), ("sssp_slots", C.c_int32), ("ads_grid_per_cu", C.c_int32),
+                ("reserved", C.c_int32 * 6)]
 
 
 class ResultInfo(C.Structure):
@@ -421,7 +422,7 @@ class Server:
 
     def __init__(self, graph, max_batch=64, max_inflight=0, hbm_reserve=0, fake_free_hbm=0, host_lane=False,
                  device_failed=False, alpha=0.15, eps=1e-7, ads_labels=16, ads_iters=20, p_inj=0.25, p_cont=0.70,
-                 p_abnd=0.05, sssp_slots=0):
+                 p_abnd=0.05, sssp_slots=0, ads_grid_per_cu=0):
         self.graph = graph
         self.L = ads_labels
         o = ServerOpts()
@@ -429,6 +430,7 @@ class Server:
         o.host_lane, o.device_failed, o.alpha, o.eps = int(host_lane), int(device_failed), alpha, eps
         o.ads_labels, o.ads_iters, o.p_inj, o.p_cont, o.p_abnd = ads_labels, ads_iters, p_inj, p_cont, p_abnd
         o.sssp_slots = sssp_slots
+        o.ads_grid_per_cu = ads_grid_per_cu
         self._s = C.c_void_p()
         check(lib().c2g_server_start(graph._h, C.byref(o), C.byref(self._s)))
 

@@ -244,24 +244,31 @@ static int ads_prepare(Graph* g, AdsState* s, int L, int nseed) {
     return C2G_OK;
 }
 
+// Workgroups per CU of the adsorption kernels: gpc > 0 overrides the graph's
+// grid_per_cu (the server runs this lane below full occupancy so that the
+// latency-bound SSSP and PPR lanes keep room on every CU).
+static int ads_grid(const Graph* g, int gpc) {
+    return std::max(1, g->num_cus) * (gpc > 0 ? gpc : g->tune.grid_per_cu);
+}
+
 template <int G>
 static void launch_spmm(Graph* g, hipStream_t strm, AdsState* s, const float* Yt, float* Yn, const float* Yinj,
-                        int L, float pc) {
-    const int grid = std::max(1, g->num_cus) * g->tune.grid_per_cu;
+                        int L, float pc, int gpc) {
+    const int grid = ads_grid(g, gpc);
     hipLaunchKernelGGL(ads_spmm_k<G>, dim3(grid), dim3(256), 0, strm, g->d_adjf, s->items, s->nitems, Yt, Yn,
                        Yinj, L, pc);
 }
 
 template <int G>
 static void launch_spmm4(Graph* g, hipStream_t strm, AdsState* s, const float* Yt, float* Yn, const float* Yinj,
-                         int L, float pc) {
-    const int grid = std::max(1, g->num_cus) * g->tune.grid_per_cu;
+                         int L, float pc, int gpc) {
+    const int grid = ads_grid(g, gpc);
     hipLaunchKernelGGL(ads_spmm4_k<G>, dim3(grid), dim3(256), 0, strm, g->d_adjf, s->items, s->nitems,
                        (const float4*)Yt, (float4*)Yn, (const float4*)Yinj, L / 4, pc);
 }
 
 static int ads_core(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t* sl, int32_t nseed, int32_t L,
-                    float p_inj, float p_cont, int32_t iters, int* cur_out, c2g_stats* st);
+                    float p_inj, float p_cont, int32_t iters, int* cur_out, c2g_stats* st, int gpc);
 
 static int ads_check(Graph* g, int32_t L, float p_inj, float p_cont, float p_abnd, int32_t iters) {
     if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");
@@ -278,7 +285,7 @@ static int ads_check(Graph* g, int32_t L, float p_inj, float p_cont, float p_abn
 // result is copied out of that block into y_outs[j].
 int ads_run_batch(Graph* g, hipStream_t strm, int k, const std::vector<int32_t>* svs,
                   const std::vector<int32_t>* sls, int32_t L, float p_inj, float p_cont, float p_abnd,
-                  int32_t iters, float* const* y_outs) {
+                  int32_t iters, float* const* y_outs, int gpc) {
     if (k < 1 || k * L > 64) return fail(C2G_EINVAL, "batched adsorption needs 1 <= k and k*L <= 64");
     int rc = ads_check(g, k * L, p_inj, p_cont, p_abnd, iters);
     if (rc) return rc;
@@ -291,11 +298,12 @@ int ads_run_batch(Graph* g, hipStream_t strm, int k, const std::vector<int32_t>*
             sl.push_back(sls[j][i] + j * L);
         }
     int cur = 0;
-    if ((rc = ads_core(g, strm, sv.data(), sl.data(), (int32_t)sv.size(), k * L, p_inj, p_cont, iters, &cur, nullptr)))
+    if ((rc = ads_core(g, strm, sv.data(), sl.data(), (int32_t)sv.size(), k * L, p_inj, p_cont, iters, &cur, nullptr,
+                       gpc)))
         return rc;
     AdsState* s = astate(g);
     const int64_t V = g->V;
-    const int grid = std::max(1, g->num_cus) * g->tune.grid_per_cu;
+    const int grid = ads_grid(g, gpc);
     for (int j = 0; j < k; ++j) {
         if (k == 1) {
             if ((rc = s->out.copy_out(y_outs[0], s->Y[cur], sizeof(float) * V * L, strm))) return rc;
@@ -317,7 +325,7 @@ int ads_run(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t* sl, in
         if (sv[i] < 0 || sv[i] >= g->V || sl[i] < 0 || sl[i] >= L)
             return fail(C2G_EINVAL, "seed %d = (%d,%d) out of range", i, sv[i], sl[i]);
     int cur = 0;
-    if ((rc = ads_core(g, strm, sv, sl, nseed, L, p_inj, p_cont, iters, &cur, st))) return rc;
+    if ((rc = ads_core(g, strm, sv, sl, nseed, L, p_inj, p_cont, iters, &cur, st, 0))) return rc;
     AdsState* s = astate(g);
     if (y_out && (rc = s->out.copy_out(y_out, s->Y[cur], sizeof(float) * (int64_t)g->V * L, strm))) return rc;
     return C2G_OK;
@@ -325,13 +333,13 @@ int ads_run(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t* sl, in
 
 // The recurrence on the device; the result stays in Y[*cur_out].
 static int ads_core(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t* sl, int32_t nseed, int32_t L,
-                    float p_inj, float p_cont, int32_t iters, int* cur_out, c2g_stats* st) {
+                    float p_inj, float p_cont, int32_t iters, int* cur_out, c2g_stats* st, int gpc) {
     if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");
     AdsState* s = astate(g);
     int rc = ads_prepare(g, s, L, nseed);
     if (rc) return rc;
     const int64_t n = (int64_t)g->V * L;
-    const int grid = std::max(1, g->num_cus) * g->tune.grid_per_cu;
+    const int grid = ads_grid(g, gpc);
     const double t0 = now_ms();
     float* Yinj = s->Y[2];
     // Yinj = p_inj * Y0 ; Y_0 = Y0
@@ -364,14 +372,14 @@ static int ads_core(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t
             hipLaunchKernelGGL(ads_split_init_k, dim3((int)std::min<int64_t>(grid, (s->nsplit * L + 255) / 256)),
                                dim3(256), 0, strm, Yn, Yinj, s->splitv, s->nsplit, L);
         if (L % 4 == 0) {  // 16-byte rows per lane
-            if (L <= 4) launch_spmm4<1>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
-            else if (L <= 8) launch_spmm4<2>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
-            else if (L <= 16) launch_spmm4<4>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
-            else if (L <= 32) launch_spmm4<8>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
-            else launch_spmm4<16>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
-        } else if (L <= 16) launch_spmm<16>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
-        else if (L <= 32) launch_spmm<32>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
-        else launch_spmm<64>(g, strm, s, Yt, Yn, Yinj, L, p_cont);
+            if (L <= 4) launch_spmm4<1>(g, strm, s, Yt, Yn, Yinj, L, p_cont, gpc);
+            else if (L <= 8) launch_spmm4<2>(g, strm, s, Yt, Yn, Yinj, L, p_cont, gpc);
+            else if (L <= 16) launch_spmm4<4>(g, strm, s, Yt, Yn, Yinj, L, p_cont, gpc);
+            else if (L <= 32) launch_spmm4<8>(g, strm, s, Yt, Yn, Yinj, L, p_cont, gpc);
+            else launch_spmm4<16>(g, strm, s, Yt, Yn, Yinj, L, p_cont, gpc);
+        } else if (L <= 16) launch_spmm<16>(g, strm, s, Yt, Yn, Yinj, L, p_cont, gpc);
+        else if (L <= 32) launch_spmm<32>(g, strm, s, Yt, Yn, Yinj, L, p_cont, gpc);
+        else launch_spmm<64>(g, strm, s, Yt, Yn, Yinj, L, p_cont, gpc);
         cur ^= 1;
     }
     HIP_TRY(hipGetLastError());

@@ -143,7 +143,7 @@ int ads_run(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t* sl, in
             float p_inj, float p_cont, float p_abnd, int32_t iters, float* y_out, c2g_stats* st);
 int ads_run_batch(Graph* g, hipStream_t strm, int k, const std::vector<int32_t>* svs,
                   const std::vector<int32_t>* sls, int32_t L, float p_inj, float p_cont, float p_abnd,
-                  int32_t iters, float* const* y_outs);
+                  int32_t iters, float* const* y_outs, int grid_per_cu = 0);
 int host_ads(const Graph* g, const int32_t* sv, const int32_t* sl, int32_t nseed, int32_t L, float p_inj,
              float p_cont, int32_t iters, float* y_out);
 

@@ -284,7 +284,7 @@ static void aux_worker(c2g_server* s, bool ads) {
                 youts.push_back((float*)rs[j]->result.get());
             }
             rc = ads_run_batch(g, strm, k, svs.data(), sls.data(), s->o.ads_labels, s->o.p_inj, s->o.p_cont,
-                               s->o.p_abnd, s->o.ads_iters, youts.data());
+                               s->o.p_abnd, s->o.ads_iters, youts.data(), s->o.ads_grid_per_cu);
         } else if (rs.size() == 1) {
             rc = ppr_single(g, strm, rs[0]->src, s->o.alpha, s->o.eps, (float*)rs[0]->result.get(), nullptr, nullptr);
         } else {
@@ -364,6 +364,10 @@ int c2g_server_start(c2g_graph* h, const c2g_server_opts* o, c2g_server** out) {
     if (s->o.ads_labels <= 0) s->o.ads_labels = 16;
     if (s->o.ads_iters <= 0) s->o.ads_iters = 20;
     if (s->o.p_inj <= 0 && s->o.p_cont <= 0) { s->o.p_inj = 0.25f; s->o.p_cont = 0.70f; s->o.p_abnd = 0.05f; }
+    if (s->o.ads_grid_per_cu < 0 || s->o.ads_grid_per_cu > 64) {
+        delete s;
+        return fail(C2G_EINVAL, "server: ads_grid_per_cu must be in [0,64]");
+    }
     const bool device_ok = g->on_device && !s->o.device_failed;
     if (!device_ok && !s->o.host_lane) {
         delete s;
@@ -376,8 +380,14 @@ int c2g_server_start(c2g_graph* h, const c2g_server_opts* o, c2g_server** out) {
         if (s->o.fake_free_hbm > 0) fr = (size_t)s->o.fake_free_hbm;
         else if (hipMemGetInfo(&fr, &tot) != hipSuccess) { delete s; return fail(C2G_EDEVICE, "hipMemGetInfo failed"); }
         s->budget = (int64_t)fr - s->o.hbm_reserve;
-        if (hipStreamCreateWithFlags(&s->s_sssp, hipStreamNonBlocking) != hipSuccess ||
-            hipStreamCreateWithFlags(&s->s_aux, hipStreamNonBlocking) != hipSuccess ||
+        // The SSSP lane runs on the graph's own stream (idle while the server
+        // owns the handle).  The HIP runtime maps a process's streams onto a
+        // few hardware queues (4 by default, the null stream included); a
+        // lane whose stream shares a queue with another lane waits behind
+        // that lane's kernels in order.  With no extra SSSP stream, the
+        // PPR and adsorption lanes each keep a queue of their own.
+        s->s_sssp = g->stream;
+        if (hipStreamCreateWithFlags(&s->s_aux, hipStreamNonBlocking) != hipSuccess ||
             hipStreamCreateWithFlags(&s->s_ads, hipStreamNonBlocking) != hipSuccess) {
             delete s;
             return fail(C2G_EDEVICE, "stream creation failed");
@@ -479,7 +489,7 @@ int c2g_server_stop(c2g_server* s) {
     if (s->t_aux.joinable()) s->t_aux.join();
     if (s->t_ads.joinable()) s->t_ads.join();
     for (auto& t : s->t_host) t.join();
-    if (s->s_sssp) { (void)hipStreamSynchronize(s->s_sssp); (void)hipStreamDestroy(s->s_sssp); }
+    if (s->s_sssp) (void)hipStreamSynchronize(s->s_sssp);  // the graph's stream: not ours to destroy
     if (s->s_aux) { (void)hipStreamSynchronize(s->s_aux); (void)hipStreamDestroy(s->s_aux); }
     if (s->s_ads) { (void)hipStreamSynchronize(s->s_ads); (void)hipStreamDestroy(s->s_ads); }
     delete s;

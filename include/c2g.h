@@ -378,7 +378,11 @@ typedef struct {
     int32_t sssp_slots;       /* SSSP queries run side by side in query-major
                                  slots before the 64-lane group takes the
                                  overflow: 0 = 16, <0 = lane group only, <=64 */
-    int32_t reserved[7];
+    int32_t ads_grid_per_cu;  /* workgroups per CU of the adsorption lane's
+                                 kernels, below full occupancy so the SSSP
+                                 and PPR lanes share every CU: 0 = the
+                                 graph's grid_per_cu, else 1..64             */
+    int32_t reserved[6];
 } c2g_server_opts;
 
 typedef struct {
