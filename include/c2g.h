@@ -45,7 +45,7 @@ extern "C" {
 #define C2G_EBUSY    (-7)  /* admission refused (server)                    */
 #define C2G_ETIMEOUT (-8)  /* lane timeout / cancellation                   */
 
-/* ---- dataset variants (directory layout, SURVEY of reference) ---------- */
+/* ---- dataset variants (the reference's directory layout) ----------------- */
 /* <ds>/origin/{vlist,elist,wlist[,flist]}.bin  utils/convert2binary.cpp:122-125 */
 #define C2G_VARIANT_ORIGIN    0
 /* <ds>/sssp/{vlist,elist,wlist,hvlist,helist,hwlist}.bin  src/purn-sssp.cpp:72-87,342-344 */
