@@ -156,11 +156,15 @@ ads_spmm4_k(const int2* __restrict__ adjf, const int4* __restrict__ items, int64
 }
 
 // out[v][l] = Y[v][c0 + l] for l < L: one query's label block of a batched run.
-__global__ void ads_cols_k(const float* __restrict__ Y, int Lt, int c0, int L, float* __restrict__ out, int64_t V) {
+// With inv set the run used relabelled vertices: row v of the result is row
+// inv[v] of Y.
+__global__ void ads_cols_k(const float* __restrict__ Y, int Lt, int c0, int L, float* __restrict__ out, int64_t V,
+                           const int32_t* __restrict__ inv) {
     const int64_t n = V * L;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t v = i / L;
-        out[i] = Y[v * Lt + c0 + (i - v * L)];
+        const int64_t src = inv ? (int64_t)inv[v] : v;
+        out[i] = Y[src * Lt + c0 + (i - v * L)];
     }
 }
 
@@ -174,6 +178,13 @@ struct AdsState {
     int32_t* splitv = nullptr;  // vertices whose row is split across items
     int64_t nsplit = 0;
     HostStage out;              // result copies to the host
+    // Relabelled copy of the weighted CSR: vertices renumbered by descending
+    // in-degree, so the rows gathered most often share cache lines.
+    int relabeled = -1;         // layout the items were built for (-1: none yet)
+    int2* radjf = nullptr;      // [E] (new target id, bits of f) in new row order
+    int32_t* inv = nullptr;     // [V] old id -> new id (device)
+    std::vector<int32_t> h_inv; // the same on the host (seed mapping)
+    std::vector<int32_t> h_row; // [V+1] row offsets of the relabelled CSR
 };
 
 static std::mutex g_as_mu;
@@ -197,6 +208,8 @@ void ads_release(Graph* g) {
             for (auto* y : s->Y) if (y) (void)hipFree(y);
             if (s->seeds) (void)hipFree(s->seeds);
             if (s->splitv) (void)hipFree(s->splitv);
+            if (s->radjf) (void)hipFree(s->radjf);
+            if (s->inv) (void)hipFree(s->inv);
             s->out.release();
             delete s;
             g_as.erase(g_as.begin() + i);
@@ -204,14 +217,57 @@ void ads_release(Graph* g) {
         }
 }
 
+// Vertices renumbered by descending in-degree (ties by id): new vertex n is
+// old vertex perm[n]; its row is the old row in the old edge order with
+// targets renamed, so every row sums the same terms in the same order.
+static int ads_build_relabel(Graph* g, AdsState* s) {
+    const int64_t V = g->V, E = g->E;
+    std::vector<int64_t> indeg((size_t)V, 0);
+    for (int64_t e = 0; e < E; ++e) indeg[(size_t)g->el[e]] += 1;
+    std::vector<int32_t> perm((size_t)V);
+    for (int64_t v = 0; v < V; ++v) perm[v] = (int32_t)v;
+    std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) { return indeg[a] > indeg[b]; });
+    s->h_inv.assign((size_t)V, 0);
+    for (int64_t n = 0; n < V; ++n) s->h_inv[(size_t)perm[n]] = (int32_t)n;
+    s->h_row.assign((size_t)V + 1, 0);
+    for (int64_t n = 0; n < V; ++n) s->h_row[n + 1] = s->h_row[n] + (g->vl[perm[n] + 1] - g->vl[perm[n]]);
+    std::vector<int2> radj((size_t)std::max<int64_t>(E, 1));
+    parallel_for(V, [&](int64_t lo, int64_t hi, int) {
+        for (int64_t n = lo; n < hi; ++n) {
+            const int32_t v = perm[n];
+            int64_t o = s->h_row[n];
+            for (int32_t e = g->vl[v]; e < g->vl[v + 1]; ++e, ++o) {
+                int32_t fb;
+                memcpy(&fb, &g->fl[e], sizeof fb);
+                radj[(size_t)o] = make_int2(s->h_inv[(size_t)g->el[e]], fb);
+            }
+        }
+    });
+    HIP_TRY(hipMalloc(&s->radjf, sizeof(int2) * radj.size()));
+    HIP_TRY(hipMemcpy(s->radjf, radj.data(), sizeof(int2) * radj.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&s->inv, sizeof(int32_t) * (size_t)std::max<int64_t>(V, 1)));
+    if (V) HIP_TRY(hipMemcpy(s->inv, s->h_inv.data(), sizeof(int32_t) * V, hipMemcpyHostToDevice));
+    return C2G_OK;
+}
+
 static int ads_prepare(Graph* g, AdsState* s, int L, int nseed) {
     HIP_TRY(hipSetDevice(g->device));
-    if (!s->items) {
+    const int want = g->tune.ads_relabel ? 1 : 0;
+    if (want && !s->radjf) {
+        int rc = ads_build_relabel(g, s);
+        if (rc) return rc;
+    }
+    if (s->relabeled != want) {
+        if (s->items) (void)hipFree(s->items);
+        if (s->splitv) (void)hipFree(s->splitv);
+        s->items = nullptr;
+        s->splitv = nullptr;
+        const int32_t* row = want ? s->h_row.data() : g->vl.data();
         std::vector<int4> it;
         std::vector<int32_t> split;
         it.reserve((size_t)g->V + g->E / kAdsChunk + 1);
         for (int32_t v = 0; v < g->V; ++v) {
-            const int r0 = g->vl[v], r1 = g->vl[v + 1];
+            const int r0 = row[v], r1 = row[v + 1];
             if (r1 - r0 <= kAdsChunk) {
                 it.push_back(make_int4(v, r0, r1, 0));  // whole row (also deg 0: writes p_inj*Y0)
             } else {
@@ -229,6 +285,7 @@ static int ads_prepare(Graph* g, AdsState* s, int L, int nseed) {
         HIP_TRY(hipMalloc(&s->splitv, sizeof(int32_t) * std::max<int64_t>(s->nsplit, 1)));
         if (s->nsplit)
             HIP_TRY(hipMemcpy(s->splitv, split.data(), sizeof(int32_t) * s->nsplit, hipMemcpyHostToDevice));
+        s->relabeled = want;
     }
     const int64_t need = (int64_t)std::max(g->V, 1) * L;
     if (s->cap < need) {
@@ -255,7 +312,8 @@ template <int G>
 static void launch_spmm(Graph* g, hipStream_t strm, AdsState* s, const float* Yt, float* Yn, const float* Yinj,
                         int L, float pc, int gpc) {
     const int grid = ads_grid(g, gpc);
-    hipLaunchKernelGGL(ads_spmm_k<G>, dim3(grid), dim3(256), 0, strm, g->d_adjf, s->items, s->nitems, Yt, Yn,
+    hipLaunchKernelGGL(ads_spmm_k<G>, dim3(grid), dim3(256), 0, strm, s->relabeled == 1 ? s->radjf : g->d_adjf,
+                       s->items, s->nitems, Yt, Yn,
                        Yinj, L, pc);
 }
 
@@ -263,7 +321,8 @@ template <int G>
 static void launch_spmm4(Graph* g, hipStream_t strm, AdsState* s, const float* Yt, float* Yn, const float* Yinj,
                          int L, float pc, int gpc) {
     const int grid = ads_grid(g, gpc);
-    hipLaunchKernelGGL(ads_spmm4_k<G>, dim3(grid), dim3(256), 0, strm, g->d_adjf, s->items, s->nitems,
+    hipLaunchKernelGGL(ads_spmm4_k<G>, dim3(grid), dim3(256), 0, strm, s->relabeled == 1 ? s->radjf : g->d_adjf,
+                       s->items, s->nitems,
                        (const float4*)Yt, (float4*)Yn, (const float4*)Yinj, L / 4, pc);
 }
 
@@ -304,13 +363,14 @@ int ads_run_batch(Graph* g, hipStream_t strm, int k, const std::vector<int32_t>*
     AdsState* s = astate(g);
     const int64_t V = g->V;
     const int grid = ads_grid(g, gpc);
+    const int32_t* inv = s->relabeled == 1 ? s->inv : nullptr;
     for (int j = 0; j < k; ++j) {
-        if (k == 1) {
+        if (k == 1 && !inv) {
             if ((rc = s->out.copy_out(y_outs[0], s->Y[cur], sizeof(float) * V * L, strm))) return rc;
             break;
         }
         // the injected image (Y[2]) is free once the iterations are done
-        hipLaunchKernelGGL(ads_cols_k, dim3(grid), dim3(256), 0, strm, s->Y[cur], k * L, j * L, L, s->Y[2], V);
+        hipLaunchKernelGGL(ads_cols_k, dim3(grid), dim3(256), 0, strm, s->Y[cur], k * L, j * L, L, s->Y[2], V, inv);
         HIP_TRY(hipGetLastError());
         if ((rc = s->out.copy_out(y_outs[j], s->Y[2], sizeof(float) * V * L, strm))) return rc;
     }
@@ -327,7 +387,15 @@ int ads_run(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t* sl, in
     int cur = 0;
     if ((rc = ads_core(g, strm, sv, sl, nseed, L, p_inj, p_cont, iters, &cur, st, 0))) return rc;
     AdsState* s = astate(g);
-    if (y_out && (rc = s->out.copy_out(y_out, s->Y[cur], sizeof(float) * (int64_t)g->V * L, strm))) return rc;
+    if (!y_out) return C2G_OK;
+    const float* Y = s->Y[cur];
+    if (s->relabeled == 1) {  // rows back to the caller's vertex ids, via the free injected image
+        hipLaunchKernelGGL(ads_cols_k, dim3(ads_grid(g, 0)), dim3(256), 0, strm, s->Y[cur], L, 0, L, s->Y[2],
+                           (int64_t)g->V, s->inv);
+        HIP_TRY(hipGetLastError());
+        Y = s->Y[2];
+    }
+    if ((rc = s->out.copy_out(y_out, Y, sizeof(float) * (int64_t)g->V * L, strm))) return rc;
     return C2G_OK;
 }
 
@@ -349,7 +417,7 @@ static int ads_core(Graph* g, hipStream_t strm, const int32_t* sv, const int32_t
         // duplicate (v,l) seeds count once: Y0 is an indicator
         std::vector<std::pair<int32_t, int32_t>> u;
         u.reserve(nseed);
-        for (int32_t i = 0; i < nseed; ++i) u.emplace_back(sv[i], sl[i]);
+        for (int32_t i = 0; i < nseed; ++i) u.emplace_back(s->relabeled == 1 ? s->h_inv[sv[i]] : sv[i], sl[i]);
         std::sort(u.begin(), u.end());
         u.erase(std::unique(u.begin(), u.end()), u.end());
         std::vector<int32_t> hv(u.size()), hl(u.size());

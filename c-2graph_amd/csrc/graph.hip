@@ -58,6 +58,7 @@ void default_tune(Graph* g) {
     t.grid_per_cu = 8;
     t.grid_per_cu_single = 4;
     t.ppr_pull = 200;
+    t.ads_relabel = 1;
     t.chunk = 32;
     t.batch_chunk = 64;
     t.iters_per_graph = 8;
@@ -456,6 +457,7 @@ int c2g_set_tune(c2g_graph* h, const c2g_tune* t) {
     if (t->grid_per_cu_single < 0 || t->grid_per_cu_single > 64)
         return fail(C2G_EINVAL, "grid_per_cu_single in [0,64]");
     if (t->ppr_pull < 0 || t->ppr_pull > 1000) return fail(C2G_EINVAL, "ppr_pull in [0,1000]");
+    if (t->ads_relabel != 0 && t->ads_relabel != 1) return fail(C2G_EINVAL, "ads_relabel must be 0 or 1");
     if (t->chunk < 1 || t->chunk > 4096) return fail(C2G_EINVAL, "chunk in [1,4096]");
     if (t->batch_chunk < 1 || t->batch_chunk > 4096) return fail(C2G_EINVAL, "batch_chunk in [1,4096]");
     if (t->iters_per_graph < 2 || t->iters_per_graph > 256 || (t->iters_per_graph & 1))

@@ -108,7 +108,11 @@ typedef struct {
                                frontier holds more than ppr_pull/1000 of all
                                out-edges runs as a pull sweep over in-edges
                                (0 = push only; 0..1000)                       */
-    int32_t reserved[6];
+    int32_t ads_relabel;    /* adsorption on a copy of the weighted CSR with
+                               vertices renumbered by descending in-degree
+                               (rows gathered most often share cache lines);
+                               results come back in the caller's ids: 0 / 1 */
+    int32_t reserved[5];
 } c2g_tune;
 
 /* ======================================================================= */

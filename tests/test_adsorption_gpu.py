@@ -118,3 +118,33 @@ def test_c5_full_scale_against_fp64(gpu):
     d = np.abs(Y.astype(np.float64) - Yo)
     print(f"\nC5 scale 23: max-abs {d.max():.3g} mean-abs {d.mean():.3g} L1 per label {d.sum() / 16:.3g}")
     _check(Y, Yo)
+
+
+@pytest.mark.parametrize("L", [16, 5, 40])
+def test_relabelled_layout(gpu, L):
+    # the run on the copy of the CSR renumbered by in-degree returns rows in the
+    # caller's ids: against the oracle, and against the identity layout (rows
+    # of at most 256 edges sum the same terms in the same order; longer rows
+    # are split and added atomically)
+    G = rmat_graph(12, 7, wmax=0, symmetric=True, want_f=True)
+    sv, sl = _seeds(G["V"], L, 0.05, 3 + L)
+    sv[0] = int(np.argmax(np.diff(G["vlist"])))  # a seed on a split hub row
+    with c2g.Graph.from_csr(G["vlist"], G["elist"], flist=G["flist"], device=0) as g:
+        Y0, _ = g.adsorption(sv, sl, L=L, iters=9)
+        g.set_tune(ads_relabel=1)
+        Y1, _ = g.adsorption(sv, sl, L=L, iters=9)
+        Y2, _ = g.adsorption(sv, sl, L=L, iters=9)  # relabelled copy reused
+        g.set_tune(ads_relabel=0)
+        Y3, _ = g.adsorption(sv, sl, L=L, iters=9)  # and back
+    _check(Y1, oracle.adsorption(G["vlist"], G["elist"], G["flist"], sv, sl, L=L, iters=9))
+    assert np.abs(Y1 - Y0).max() <= 1e-6
+    assert np.abs(Y2 - Y1).max() <= 1e-6
+    assert np.abs(Y3 - Y0).max() <= 1e-6
+    # no row above 256 edges: no atomics, so both layouts give the same bits
+    H = random_graph(500, 4000, seed=L)
+    hv, hl = _seeds(500, L, 0.05, L)
+    with c2g.Graph.from_csr(H["vlist"], H["elist"], flist=H["flist"], device=0) as g:
+        Z0, _ = g.adsorption(hv, hl, L=L, iters=9)
+        g.set_tune(ads_relabel=1)
+        Z1, _ = g.adsorption(hv, hl, L=L, iters=9)
+    assert np.array_equal(Z0, Z1)

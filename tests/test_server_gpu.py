@@ -146,12 +146,15 @@ def test_pruned_graph_server(gpu):
                     assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], v)), (seed, v)
 
 
-def test_adsorption_burst_is_batched(gpu):
+@pytest.mark.parametrize("relabel", [0, 1])
+def test_adsorption_burst_is_batched(gpu, relabel):
     # a burst of adsorption requests runs as passes of up to 4 seed sets side by
-    # side (4 x 16 labels); each result equals the same seed set run alone
+    # side (4 x 16 labels); each result equals the same seed set run alone, also
+    # when the passes run on the in-degree-relabelled copy of the graph
     G = _G()
     sets = list(range(9))
     with _graph(G) as g:
+        g.set_tune(ads_relabel=relabel)
         with c2g.Server(g, ads_labels=16, ads_iters=20) as s:
             ids = [s.submit(c2g.Q_ADS, k) for k in sets]
             res = [s.poll(i) for i in ids]
