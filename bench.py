@@ -221,7 +221,7 @@ def ppr_accuracy_phase(args):
     out = {"graph": "R-MAT scale 22 symmetrised seed 3", "source": s, "alpha": 0.15, "eps": "1e-13*outdeg"}
     with c2g.Graph.from_csr(G3["vlist"], G3["elist"], flist=G3["flist"], device=0) as g:
         default = g.tune().ppr_pull
-        g.ppr(s)  # warm: allocations and the in-edge rows
+        g.ppr(s, eps=1e-13)  # warm: allocations, and the in-edge rows (built once a round is dense)
         for name, pull in (("push_only", 0), ("push_pull", default)):
             g.set_tune(ppr_pull=pull)
             _, r, st = g.ppr(s, eps=1e-13)

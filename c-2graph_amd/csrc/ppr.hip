@@ -50,6 +50,7 @@ enum : int {
     P_MODE = 8,    // single source: 1 = the next round runs as a pull sweep
     P_REDGES = 9,  // single source: out-edges of the vertices the running expand activates
     P_PULLS = 10,  // single source: rounds run as pull sweeps
+    P_WANT = 11,   // single source: some round qualified for a pull sweep
     P_N = 16
 };
 
@@ -224,12 +225,14 @@ ppr_push_k(const int2* __restrict__ adjf, float* __restrict__ m, uint32_t* __res
 // per wave, then apply the updates and write the items.  With ru set, each
 // activated v also gets ru[v] = (stamp+1, residual) for a pull round, and the
 // last workgroup picks the next round's form: pull when the activated
-// vertices hold more than pull_edges out-edges.
+// vertices hold more than pull_edges out-edges and the in-edge rows exist
+// (can_pull); a qualifying round without them sets P_WANT, and the host
+// builds them at its next check.
 __global__ void __launch_bounds__(256)
 ppr_expand_k(const int32_t* __restrict__ row, float* __restrict__ p, float* __restrict__ r,
              float* __restrict__ m, uint32_t* __restrict__ tb, int4* __restrict__ q, int64_t* __restrict__ ctrl,
              int64_t V, float alpha, float eps, int chunk, int64_t qcap, int force, int degscale,
-             int2* __restrict__ ru, uint32_t* __restrict__ stamp, int64_t pull_edges) {
+             int2* __restrict__ ru, uint32_t* __restrict__ stamp, int64_t pull_edges, int can_pull) {
     const int lane = lane_id();
     const int S1 = ru ? (int)*stamp + 1 : 0;
     const int64_t nw = (V + 31) >> 5;
@@ -297,7 +300,9 @@ ppr_expand_k(const int32_t* __restrict__ row, float* __restrict__ p, float* __re
             int64_t c = (int64_t)atomicAdd((unsigned long long*)&ctrl[P_CUR], 0ull);
             if (c > qcap) { ctrl[P_OVF] = 1; ctrl[P_CUR] = 0; }
             const int64_t re = (int64_t)atomicAdd((unsigned long long*)&ctrl[P_REDGES], 0ull);
-            ctrl[P_MODE] = (ru && pull_edges > 0 && re > pull_edges) ? 1 : 0;
+            const bool want = ru && pull_edges > 0 && re > pull_edges;
+            ctrl[P_MODE] = (want && can_pull) ? 1 : 0;
+            if (want) ctrl[P_WANT] = 1;
             ctrl[P_REDGES] = 0;
             if (ru) *stamp = (uint32_t)S1;
         }
@@ -586,12 +591,10 @@ static int ensure_ppr_single(Graph* g, PprState* s) {
     return C2G_OK;
 }
 
-// In-edge rows of the PPR graph for pull rounds, built once per handle on
-// the host: thread t owns a range of target vertices and scans every row in
-// source order, so each in-row lists its sources in increasing order and the
-// layout does not depend on the thread count.
+// Stamped residuals for pull rounds (cheap: 8 bytes per vertex); the stamps
+// persist across queries and restart long before the 32-bit counter wraps.
 static int ensure_pull(Graph* g, PprState* s) {
-    const int64_t V = g->V, E = g->E;
+    const int64_t V = g->V;
     if (!s->ru) {
         HIP_TRY(hipMalloc(&s->ru, sizeof(int2) * (size_t)std::max<int64_t>(V, 1)));
         HIP_TRY(hipMemset(s->ru, 0, sizeof(int2) * (size_t)std::max<int64_t>(V, 1)));
@@ -604,6 +607,15 @@ static int ensure_pull(Graph* g, PprState* s) {
         HIP_TRY(hipMemset(s->stamp, 0, sizeof(uint32_t)));
         s->stamp_est = 0;
     }
+    return C2G_OK;
+}
+
+// In-edge rows of the PPR graph for pull rounds, built once per handle on
+// the host: thread t owns a range of target vertices and scans every row in
+// source order, so each in-row lists its sources in increasing order and the
+// layout does not depend on the thread count.
+static int ensure_transpose(Graph* g, PprState* s) {
+    const int64_t V = g->V, E = g->E;
     if (s->tadjf) return C2G_OK;
     const int32_t* vl = g->vl.data();
     const int32_t* el = g->el.data();
@@ -687,6 +699,7 @@ static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, f
     // out-edges gathers over in-edges instead of scattering.
     const bool use_pull = g->tune.ppr_pull > 0 && g->E > 0;
     if (use_pull && (rc = ensure_pull(g, s))) return rc;
+    int can_pull = use_pull && s->tadjf ? 1 : 0;  // in-edge rows are built once a round qualifies
     int2* ru = use_pull ? s->ru : nullptr;
     const int64_t pull_edges = use_pull ? std::max<int64_t>(1, g->E * g->tune.ppr_pull / 1000) : 0;
     const double t0 = now_ms();
@@ -695,7 +708,8 @@ static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, f
                        pruned ? g->d_hrow : (const int32_t*)nullptr, pruned ? g->d_hadjf : (const int2*)nullptr,
                        s->p, s->m, s->tb, s->ctrl, src, alpha, pruned ? 1 : 0, spec.r0, spec.p_src);
     hipLaunchKernelGGL(ppr_expand_k, dim3(wgrid), dim3(256), 0, strm, g->d_row, s->p, s->r, s->m, s->tb, s->q,
-                       s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 1, spec.degscale, ru, s->stamp, pull_edges);
+                       s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 1, spec.degscale, ru, s->stamp, pull_edges,
+                       can_pull);
     HIP_TRY(hipGetLastError());
     if (use_pull) s->stamp_est += 1;
     int rounds = 0;
@@ -704,10 +718,10 @@ static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, f
         const int K = budget > 0 ? 1 : g->tune.iters_per_graph;
         for (int k = 0; k < K; ++k) {
             hipLaunchKernelGGL(ppr_push_k, dim3(grid), dim3(256), 0, strm, g->d_adjf, s->m, s->tb, s->q,
-                               s->ctrl, alpha, s->tadjf, s->titems, use_pull ? s->nt : 0, s->ru, s->stamp);
+                               s->ctrl, alpha, s->tadjf, s->titems, can_pull ? s->nt : 0, s->ru, s->stamp);
             hipLaunchKernelGGL(ppr_expand_k, dim3(wgrid), dim3(256), 0, strm, g->d_row, s->p, s->r, s->m,
                                s->tb, s->q, s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 0, spec.degscale,
-                               ru, s->stamp, pull_edges);
+                               ru, s->stamp, pull_edges, can_pull);
         }
         HIP_TRY(hipGetLastError());
         rounds += K;
@@ -716,6 +730,12 @@ static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, f
         HIP_TRY(hipStreamSynchronize(strm));
         if (s->h_ctrl[P_OVF]) return fail(C2G_EDEVICE, "PPR frontier queue overflow (internal)");
         if (s->h_ctrl[P_CUR] == 0) break;
+        if (use_pull && !can_pull && s->h_ctrl[P_WANT]) {
+            // a round was dense enough to pull: build the in-edge rows now;
+            // the next expands may then switch (their stamps are in place)
+            if ((rc = ensure_transpose(g, s))) return rc;
+            can_pull = 1;
+        }
         if (budget > 0) {
             const int64_t fsize = s->h_ctrl[P_FRONT] - front_prev;  // vertices activated this round
             front_prev = s->h_ctrl[P_FRONT];

@@ -180,6 +180,10 @@ def test_pull_rounds_match_push(gpu, pull):
     rng = np.random.default_rng(5)
     srcs = [int(np.argmax(deg))] + [int(x) for x in rng.choice(np.flatnonzero(deg > 0), 2)]
     with _g(G) as g:
+        # the in-edge rows are built at the first host check after a round
+        # qualified for pull, so the first query may run all its rounds as pushes
+        g.set_tune(ppr_pull=pull)
+        g.ppr(srcs[0], alpha=ALPHA, eps=1e-9)
         for s in srcs:
             g.set_tune(ppr_pull=0)
             p0, r0, st0 = g.ppr(s, alpha=ALPHA, eps=1e-9)
@@ -201,11 +205,15 @@ def test_pull_rounds_directed_and_small(gpu):
     pi = oracle.ppr_power(G["vlist"], G["elist"], G["flist"], s, alpha=ALPHA, tol=1e-15)
     with _g(G) as g:
         g.set_tune(ppr_pull=1)
+        g.ppr(s, alpha=ALPHA, eps=1e-11)  # builds the in-edge rows
         p, r, st = g.ppr(s, alpha=ALPHA, eps=1e-11)
     assert st.pull_rounds > 0
     assert _l1(p, pi) <= 1e-5
     H = random_graph(301, 2500, seed=6)
     with _g(H) as g:
+        g.set_tune(ppr_pull=1)
+        g.ppr(0, alpha=ALPHA, eps=1e-9)  # builds the in-edge rows
+        g.set_tune(ppr_pull=0)
         for s in (0, 150, 300):
             p0, r0, _ = g.ppr(s, alpha=ALPHA, eps=1e-9)
             g.set_tune(ppr_pull=1)
@@ -221,6 +229,9 @@ def test_pull_rounds_pruned_graph_and_reference_modes(gpu):
     P = c2g.prune_ppr_mem(G["vlist"], G["elist"], G["flist"])
     with c2g.Graph.from_csr(P["vlist"], P["elist"], flist=P["wlist"], hvlist=P["hvlist"], helist=P["helist"],
                             hwf=P["hwlist"], variant=c2g.VARIANT_PPR_PURN, device=0) as g:
+        g.set_tune(ppr_pull=1)
+        g.ppr(0, alpha=ALPHA, eps=1e-9)  # builds the in-edge rows
+        g.set_tune(ppr_pull=0)
         for s in range(0, 300, 37):
             p0, r0, _ = g.ppr(s, alpha=ALPHA, eps=1e-9)
             g.set_tune(ppr_pull=1)
@@ -233,6 +244,7 @@ def test_pull_rounds_pruned_graph_and_reference_modes(gpu):
         a0, b0, _ = g.adsorption_push(s)
         m0, n0, sm0 = g.ppr_migrate(s, 3, 1 << 30, alpha=ALPHA, eps=1e-9)
         g.set_tune(ppr_pull=1)
+        g.adsorption_push(s)  # builds the in-edge rows
         a1, b1, sa = g.adsorption_push(s)
         m1, n1, sm1 = g.ppr_migrate(s, 3, 1 << 30, alpha=ALPHA, eps=1e-9)
     assert sa.pull_rounds > 0
