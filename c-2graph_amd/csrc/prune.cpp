@@ -77,14 +77,24 @@ struct Pruner {
             for (size_t k = 1; k < graph[u].size(); ++k)
                 if (graph[u][k].first == graph[u][k - 1].first)
                     return fail(C2G_EFORMAT, "vertex %d lists target %d twice", u, graph[u][k].first);
-        // In-neighbour lists in increasing source order (distinct targets per source).
-        std::vector<int32_t> cnt((size_t)V, 0);
-        for (int32_t u = 0; u < V; ++u)
-            for (const auto& p : graph[u]) cnt[p.first]++;
-        for (int32_t v = 0; v < V; ++v) graphT[v].reserve(cnt[v]);
-        for (int32_t u = 0; u < V; ++u)
-            for (const auto& p : graph[u]) graphT[p.first].emplace_back(u, 1);
-        for (int32_t v = 0; v < V; ++v) indeg[v] = (int32_t)graphT[v].size();
+        // In-neighbour lists in increasing source order (distinct targets per
+        // source): each thread owns a range of targets and scans the rows in
+        // source order, so the lists do not depend on the thread count.
+        const int T = std::max(1, std::min(host_threads(), 16));
+        parallel_for(T, [&](int64_t t0, int64_t t1, int) {
+            for (int64_t t = t0; t < t1; ++t) {
+                const int32_t lo = (int32_t)((int64_t)V * t / T), hi = (int32_t)((int64_t)V * (t + 1) / T);
+                std::vector<int32_t> cnt((size_t)(hi - lo), 0);
+                for (int32_t u = 0; u < V; ++u)
+                    for (const auto& p : graph[u])
+                        if (p.first >= lo && p.first < hi) cnt[p.first - lo]++;
+                for (int32_t v = lo; v < hi; ++v) graphT[v].reserve(cnt[v - lo]);
+                for (int32_t u = 0; u < V; ++u)
+                    for (const auto& p : graph[u])
+                        if (p.first >= lo && p.first < hi) graphT[p.first].emplace_back(u, 1);
+                for (int32_t v = lo; v < hi; ++v) indeg[v] = (int32_t)graphT[v].size();
+            }
+        }, T, 1);
         return C2G_OK;
     }
 
@@ -189,12 +199,14 @@ struct Pruner {
         wl.resize(vl[V]);
         hel.resize(hvl[V]);
         hwl.resize(hvl[V]);
-        for (int32_t v = 0; v < V; ++v) {
-            int32_t o = vl[v];
-            for (const auto& p : graph[v]) { el[o] = p.first; wl[o++] = p.second; }
-            o = hvl[v];
-            for (const auto& p : hidden[v]) { hel[o] = p.first; hwl[o++] = p.second; }
-        }
+        parallel_for(V, [&](int64_t lo, int64_t hi, int) {
+            for (int64_t v = lo; v < hi; ++v) {
+                int32_t o = vl[v];
+                for (const auto& p : graph[v]) { el[o] = p.first; wl[o++] = p.second; }
+                o = hvl[v];
+                for (const auto& p : hidden[v]) { hel[o] = p.first; hwl[o++] = p.second; }
+            }
+        });
     }
 };
 
