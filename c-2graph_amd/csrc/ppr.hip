@@ -353,7 +353,8 @@ pprb_push_k(const int2* __restrict__ adjf, float* __restrict__ M, const float* _
             int64_t* __restrict__ ctrl, int64_t V, int64_t nw, float alpha) {
     const int64_t n = ctrl[P_CUR];
     const int lane = lane_id();
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    // uniform wave index: item headers through scalar loads
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     for (int64_t i = wave; i < n; i += nwaves) {
         const int4 it = items[i];
