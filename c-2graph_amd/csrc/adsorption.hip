@@ -59,7 +59,7 @@ ads_spmm_k(const int2* __restrict__ adjf, const int4* __restrict__ items, int64_
     constexpr int PER = 64 / G;
     const int lane = lane_id();
     const int sub = lane / G, l = lane % G;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t wave = wave_index();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     for (int64_t base = wave * PER; base < nitems; base += nwaves * PER) {
         const int64_t i = base + sub;
@@ -103,7 +103,7 @@ ads_spmm4_k(const int2* __restrict__ adjf, const int4* __restrict__ items, int64
     const int lane = lane_id();
     const int sub = lane / G, l = lane % G;
     const bool on = l < L4;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t wave = wave_index();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     for (int64_t base = wave * PER; base < nitems; base += nwaves * PER) {
         const int64_t i = base + sub;

@@ -106,7 +106,7 @@ ppr_push_k(const int2* __restrict__ adjf, float* __restrict__ m, uint32_t* __res
            const int2* __restrict__ tadjf, const int4* __restrict__ titems, int64_t nt,
            const int2* __restrict__ ru, const uint32_t* __restrict__ stamp) {
     const int64_t n = ctrl[P_CUR];
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t wave = wave_index();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     const int lane = lane_id();
     const bool pull = ctrl[P_MODE] != 0;
@@ -353,8 +353,7 @@ pprb_push_k(const int2* __restrict__ adjf, float* __restrict__ M, const float* _
             int64_t* __restrict__ ctrl, int64_t V, int64_t nw, float alpha) {
     const int64_t n = ctrl[P_CUR];
     const int lane = lane_id();
-    // uniform wave index: item headers through scalar loads
-    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t wave = wave_index();  // item headers through scalar loads
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     for (int64_t i = wave; i < n; i += nwaves) {
         const int4 it = items[i];
@@ -396,7 +395,7 @@ pprb_activate_k(const int32_t* __restrict__ row, float* __restrict__ P, float* _
                 const uint32_t* __restrict__ tbits, int64_t V, int64_t nw, int groups, float alpha, float eps,
                 int force) {
     const int lane = lane_id();
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t wave = wave_index();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     const int64_t words = nw * groups;
     for (int64_t wi = wave; wi < words; wi += nwaves) {

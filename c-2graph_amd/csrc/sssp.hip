@@ -121,7 +121,7 @@ sssp_relax_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
              const int4* __restrict__ q, int64_t* __restrict__ ctrl, int64_t V, int64_t nwp) {
     const int64_t n = ctrl[C_CUR];
     const int lane = lane_id();
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t wave = wave_index();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     int64_t impr = 0;
 

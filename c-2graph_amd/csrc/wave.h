@@ -11,6 +11,13 @@ constexpr int kWave = 64;
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63u); }
 
+// Global index of this wave, visibly uniform to the compiler (readfirstlane),
+// so loads indexed by it alone (item headers, bitmap words, row offsets)
+// become scalar loads into SGPRs.  Assumes blockDim.x is a multiple of 64.
+__device__ __forceinline__ int64_t wave_index() {
+    return (int64_t)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
+
 // Exclusive prefix sum over the 64 lanes of a wave; *total gets the wave sum.
 // Every lane of the wave must call it (uniform control flow).
 __device__ __forceinline__ int wave_excl_scan(int x, int* total) {
