@@ -304,6 +304,35 @@ def c2_phase(args):
                                   and np.array_equal(res["full"][1], res["pruned"][1]))
     gf.close()
     gp.close()
+    # CPU baseline of the preprocessing: the reference's own purn-sssp
+    # (src/purn-sssp.cpp, compiled from source into oracle/_ref), timed on the
+    # same recipe at scale 20 (about a minute; scale 22 would take several),
+    # beside c2g_prune_sssp, and its pruned CSR compared byte for byte.
+    ref = os.path.join(ROOT, "oracle", "_ref", "purn-sssp")
+    if os.path.exists(ref):
+        import shutil
+        import subprocess
+        import tempfile
+
+        H = c2g.rmat(20, 16, 0.57, 0.19, 0.19, seed=2, wmax=255)
+        ds = tempfile.mkdtemp()
+        try:
+            c2g.write_dataset(ds, H["vlist"], H["elist"], H["wlist"])
+            log("prune baseline: reference purn-sssp at scale 20")
+            t = time.time()
+            subprocess.run([ref, ds], check=True, capture_output=True, timeout=900)
+            t_ref = time.time() - t
+            t = time.time()
+            Q = c2g.prune_sssp_mem(H["vlist"], H["elist"], H["wlist"])
+            t_ours = time.time() - t
+            same = all(Q[f].tobytes() == np.fromfile(f"{ds}/sssp/{f}.bin", np.int32).tobytes()
+                       for f in ("vlist", "elist", "wlist"))
+        finally:
+            shutil.rmtree(ds, ignore_errors=True)
+        out["prune_cpu_baseline"] = {
+            "kind": "reference", "cores": 1, "sample": "R-MAT scale 20 seed 2, w U[1,255]",
+            "reference_s": round(t_ref, 2), "c2g_prune_sssp_s": round(t_ours, 2), "pruned_csr_identical": same}
+        log(f"prune baseline: {out['prune_cpu_baseline']}")
     return out
 
 
@@ -578,6 +607,7 @@ def main():
                    "tune": tune_used,
                    "autotune_s": tune_s},
         "gteps": round(gteps, 3),
+        "query_edge_relaxations_per_s": round(st.edges_relaxed * args.steps / total, 1) if times else 0.0,
     }
     out.update(extra)
     print(json.dumps(out), flush=True)
