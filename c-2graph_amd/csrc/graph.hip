@@ -56,7 +56,7 @@ void default_tune(Graph* g) {
     memset(&t, 0, sizeof t);
     t.block = 256;
     t.grid_per_cu = 8;
-    t.grid_per_cu_single = 4;
+    t.grid_per_cu_single = 2;
     t.ppr_pull = 200;
     t.ads_relabel = 1;
     t.chunk = 32;
