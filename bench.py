@@ -417,6 +417,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--skip-profile", action="store_true", help="skip the profiled roofline pass")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default tunables")
+    ap.add_argument("--batch-sweep", action="store_true",
+                    help="only batched SSSP at 64..1024 sources (same recipe), best of two runs each")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--single-profile", action="store_true", help="profile two single-source queries")
     ap.add_argument("--skip-serve", action="store_true", help="skip the served mixed stream")
@@ -496,6 +498,23 @@ def main():
         log(f"autotuned in {tune_s}s: {g.tune().as_dict()}")
     tune_used = {k: v for k, v in g.tune().as_dict().items() if k != "profile"}
 
+    if args.batch_sweep:
+        # throughput against batch size, same source recipe and graph
+        res = []
+        for n in (64, 128, 256, 512, 1024):
+            sn = pick_sources(G["vlist"], n, seed=args.seed)
+            g.sssp_batch_resident(sn)
+            ts = []
+            for _ in range(2):
+                t0 = time.perf_counter()
+                _, _, stn = g.sssp_batch_resident(sn)
+                ts.append(time.perf_counter() - t0)
+            best = min(ts)
+            res.append({"queries": n, "ms": round(best * 1e3, 1), "queries_per_s": round(n / best, 1),
+                        "rounds": stn.iterations, "csr_edges": stn.csr_edges})
+            log(f"batch sweep: {res[-1]}")
+        print(json.dumps({"batch_sweep": res}), flush=True)
+        return
     if args.profile_only:
         args.warmup = args.steps = 0
         args.skip_each = args.skip_cpu = args.skip_serve = True
