@@ -276,8 +276,9 @@ def c2_phase(args):
     """Pruned-graph SSSP: R-MAT scale 22 seed 2, weights U[1,255], preprocessed
     by purn-sssp (c2g_prune_sssp); 32 sources drawn uniformly from vertices
     with out-degree >= 1 in the full graph, one query per launch sequence on
-    the pruned graph and on the full graph.  Distances must agree exactly
-    (checksum and reached count of every query)."""
+    the pruned graph and on the full graph, then 256 sources of the headline
+    recipe batched on both.  Distances must agree exactly (checksum and
+    reached count of every query)."""
     import c2g
 
     G = c2g.rmat(22, 16, 0.57, 0.19, 0.19, seed=2, wmax=255)
@@ -302,6 +303,19 @@ def c2_phase(args):
         out[f"ms_per_query_{name}"] = round((time.perf_counter() - t) / len(srcs) * 1e3, 3)
     out["distances_equal"] = bool(np.array_equal(res["full"][0], res["pruned"][0])
                                   and np.array_equal(res["full"][1], res["pruned"][1]))
+    # the same graphs batched: 256 sources of the headline recipe in one call
+    bs = pick_sources(G["vlist"], 256, seed=2)
+    resb = {}
+    for name, g in (("full", gf), ("pruned", gp)):
+        g.sssp_batch_resident(bs)
+        t = time.perf_counter()
+        sums, reach, st = g.sssp_batch_resident(bs)
+        dt = time.perf_counter() - t
+        resb[name] = (sums, reach)
+        out[f"batched_256_ms_{name}"] = round(dt * 1e3, 1)
+        out[f"batched_256_queries_per_s_{name}"] = round(256 / dt, 1)
+    out["batched_distances_equal"] = bool(np.array_equal(resb["full"][0], resb["pruned"][0])
+                                          and np.array_equal(resb["full"][1], resb["pruned"][1]))
     gf.close()
     gp.close()
     # CPU baseline of the preprocessing: the reference's own purn-sssp
