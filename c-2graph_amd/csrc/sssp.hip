@@ -219,9 +219,17 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
     const int32_t T = (int32_t)ctrl[C_THR];
     int64_t front = 0, edges = 0, hedges = 0, near_n = 0;
     int32_t fmin = INT32_MAX;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nwr; i += stride) {
-        const int slot = (int)(i / nwp);
-        const int64_t wi = i - (int64_t)slot * nwp;
+    // The loop's trip count is the same for every thread of a workgroup, so
+    // its barriers are reached by all of them.
+    __shared__ int64_t s_tot[16];
+    __shared__ int64_t s_base;
+    const int wid = threadIdx.x >> 6;
+    const int nwv = blockDim.x >> 6;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < nwr; i0 += stride) {
+        const int64_t i = i0 + threadIdx.x;
+        const bool inr = i < nwr;
+        const int slot = inr ? (int)(i / nwp) : 0;
+        const int64_t wi = inr ? i - (int64_t)slot * nwp : nw;
         uint32_t wl = 0, wh = 0;
         if (wi < nw) {
             wl = bits[i];
@@ -261,13 +269,20 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
         near_n += __popc(nl | nh);
         int tot;
         const int off = wave_excl_scan(cnt, &tot);
-        if (tot == 0) continue;
-        int64_t base = 0;
-        if (lane == 0) {
-            base = (int64_t)atomicAdd((unsigned long long*)&ctrl[C_CUR], (unsigned long long)tot);
-            atomicOr((unsigned long long*)&ctrl[C_SLOTS], 1ull << slot);
+        // one queue reservation per workgroup: wave totals meet in LDS
+        if (lane == 0) s_tot[wid] = tot;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int64_t sum = 0;
+            for (int w = 0; w < nwv; ++w) sum += s_tot[w];
+            s_base = sum > 0 ? (int64_t)atomicAdd((unsigned long long*)&ctrl[C_CUR], (unsigned long long)sum) : 0;
         }
-        base = shfl64(base, 0);
+        __syncthreads();
+        int64_t base = s_base;
+        for (int w = 0; w < wid; ++w) base += s_tot[w];
+        __syncthreads();  // s_tot and s_base are rewritten in the next iteration
+        if (tot == 0) continue;
+        if (lane == 0) atomicOr((unsigned long long*)&ctrl[C_SLOTS], 1ull << slot);
         if (base + tot > qcap) {
             if (lane == 0) atomicOr((unsigned long long*)&ctrl[C_OVF], 1ull);
             continue;
