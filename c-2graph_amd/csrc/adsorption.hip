@@ -243,10 +243,20 @@ static int ads_build_relabel(Graph* g, AdsState* s) {
             }
         }
     });
-    HIP_TRY(hipMalloc(&s->radjf, sizeof(int2) * radj.size()));
-    HIP_TRY(hipMemcpy(s->radjf, radj.data(), sizeof(int2) * radj.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMalloc(&s->inv, sizeof(int32_t) * (size_t)std::max<int64_t>(V, 1)));
-    if (V) HIP_TRY(hipMemcpy(s->inv, s->h_inv.data(), sizeof(int32_t) * V, hipMemcpyHostToDevice));
+    // both arrays complete before s->radjf marks them ready
+    int2* ra = nullptr;
+    int32_t* iv = nullptr;
+    hipError_t e = hipMalloc(&ra, sizeof(int2) * radj.size());
+    if (e == hipSuccess) e = hipMemcpy(ra, radj.data(), sizeof(int2) * radj.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&iv, sizeof(int32_t) * (size_t)std::max<int64_t>(V, 1));
+    if (e == hipSuccess && V) e = hipMemcpy(iv, s->h_inv.data(), sizeof(int32_t) * V, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (ra) (void)hipFree(ra);
+        if (iv) (void)hipFree(iv);
+        return hip_fail(e, "adsorption relabelled CSR");
+    }
+    s->inv = iv;
+    s->radjf = ra;
     return C2G_OK;
 }
 
@@ -262,6 +272,7 @@ static int ads_prepare(Graph* g, AdsState* s, int L, int nseed) {
         if (s->splitv) (void)hipFree(s->splitv);
         s->items = nullptr;
         s->splitv = nullptr;
+        s->relabeled = -1;  // no items until the new ones are in place
         const int32_t* row = want ? s->h_row.data() : g->vl.data();
         std::vector<int4> it;
         std::vector<int32_t> split;
@@ -290,11 +301,14 @@ static int ads_prepare(Graph* g, AdsState* s, int L, int nseed) {
     const int64_t need = (int64_t)std::max(g->V, 1) * L;
     if (s->cap < need) {
         for (auto*& y : s->Y) { if (y) (void)hipFree(y); y = nullptr; }
+        s->cap = 0;  // empty until all three are in place
         for (auto*& y : s->Y) HIP_TRY(hipMalloc(&y, sizeof(float) * need));
         s->cap = need;
     }
     if (s->seedcap < 2 * (int64_t)nseed) {
         if (s->seeds) (void)hipFree(s->seeds);
+        s->seeds = nullptr;
+        s->seedcap = 0;
         HIP_TRY(hipMalloc(&s->seeds, sizeof(int32_t) * 2 * std::max(nseed, 1)));
         s->seedcap = 2 * (int64_t)std::max(nseed, 1);
     }

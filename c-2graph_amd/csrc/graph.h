@@ -20,15 +20,26 @@
 
 namespace c2g {
 
+// A failed HIP call `what` as a C2G error (ENOMEM for memory, else EDEVICE),
+// with the runtime's last-error state consumed like HIP_TRY does.
+inline int hip_fail(hipError_t e, const char* what) {
+    (void)hipGetLastError();
+    return ::c2g::fail(e == hipErrorOutOfMemory ? C2G_ENOMEM : C2G_EDEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
 // Failing HIP calls return C2G_ENOMEM for an allocation that does not fit
-// (device or pinned host memory) and C2G_EDEVICE otherwise.
+// (device or pinned host memory) and C2G_EDEVICE otherwise.  The runtime's
+// last-error state is consumed on the way out, so the launch checks of a
+// later call (hipGetLastError) do not report this failure again.
 #define HIP_TRY(expr)                                                                  \
     do {                                                                               \
         hipError_t _e = (expr);                                                        \
-        if (_e != hipSuccess)                                                          \
+        if (_e != hipSuccess) {                                                        \
+            (void)hipGetLastError();                                                   \
             return ::c2g::fail(_e == hipErrorOutOfMemory ? C2G_ENOMEM : C2G_EDEVICE,  \
                                "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
                                __FILE__, __LINE__);                                    \
+        }                                                                              \
     } while (0)
 
 // Device-to-host copies into pageable memory through two pinned chunks: the

@@ -299,11 +299,9 @@ void ColStage::release() {
 
 int HostStage::copy_out(void* dst, const void* src, size_t bytes, hipStream_t strm) {
     if (bytes == 0) return C2G_OK;
-    if (!buf[0]) {
-        for (int k = 0; k < 2; ++k) {
-            HIP_TRY(hipHostMalloc((void**)&buf[k], kStageChunk, hipHostMallocDefault));
-            HIP_TRY(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
-        }
+    for (int k = 0; k < 2; ++k) {  // each piece on its own: a failed attempt is completed next time
+        if (!buf[k]) HIP_TRY(hipHostMalloc((void**)&buf[k], kStageChunk, hipHostMallocDefault));
+        if (!ev[k]) HIP_TRY(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
     }
     const size_t nchunks = (bytes + kStageChunk - 1) / kStageChunk;
     auto host_copy = [&](size_t c) {

@@ -575,16 +575,17 @@ static int ensure_ppr_single(Graph* g, PprState* s) {
     const int64_t V = std::max<int64_t>(g->V, 1);
     const int chunk = std::max(1, g->tune.chunk);
     const int64_t need = V + g->E / chunk + 64;
-    if (!s->p) {
-        HIP_TRY(hipMalloc(&s->p, sizeof(float) * V));
-        HIP_TRY(hipMalloc(&s->r, sizeof(float) * V));
-        HIP_TRY(hipMalloc(&s->m, sizeof(float) * V));
-        HIP_TRY(hipMalloc(&s->tb, sizeof(uint32_t) * ((V + 31) / 32 + 1)));
-        HIP_TRY(hipMalloc(&s->ctrl, sizeof(int64_t) * P_N));
-        HIP_TRY(hipHostMalloc((void**)&s->h_ctrl, sizeof(int64_t) * P_N, hipHostMallocDefault));
-    }
+    // each piece on its own, so a failed attempt is completed by the next call
+    if (!s->p) HIP_TRY(hipMalloc(&s->p, sizeof(float) * V));
+    if (!s->r) HIP_TRY(hipMalloc(&s->r, sizeof(float) * V));
+    if (!s->m) HIP_TRY(hipMalloc(&s->m, sizeof(float) * V));
+    if (!s->tb) HIP_TRY(hipMalloc(&s->tb, sizeof(uint32_t) * ((V + 31) / 32 + 1)));
+    if (!s->ctrl) HIP_TRY(hipMalloc(&s->ctrl, sizeof(int64_t) * P_N));
+    if (!s->h_ctrl) HIP_TRY(hipHostMalloc((void**)&s->h_ctrl, sizeof(int64_t) * P_N, hipHostMallocDefault));
     if (s->qcap < need) {
         if (s->q) (void)hipFree(s->q);
+        s->q = nullptr;
+        s->qcap = 0;
         HIP_TRY(hipMalloc(&s->q, sizeof(int4) * need));
         s->qcap = need;
     }
@@ -595,10 +596,10 @@ static int ensure_ppr_single(Graph* g, PprState* s) {
 // persist across queries and restart long before the 32-bit counter wraps.
 static int ensure_pull(Graph* g, PprState* s) {
     const int64_t V = g->V;
-    if (!s->ru) {
-        HIP_TRY(hipMalloc(&s->ru, sizeof(int2) * (size_t)std::max<int64_t>(V, 1)));
+    if (!s->ru || !s->stamp) {
+        if (!s->ru) HIP_TRY(hipMalloc(&s->ru, sizeof(int2) * (size_t)std::max<int64_t>(V, 1)));
+        if (!s->stamp) HIP_TRY(hipMalloc(&s->stamp, sizeof(uint32_t)));
         HIP_TRY(hipMemset(s->ru, 0, sizeof(int2) * (size_t)std::max<int64_t>(V, 1)));
-        HIP_TRY(hipMalloc(&s->stamp, sizeof(uint32_t)));
         HIP_TRY(hipMemset(s->stamp, 0, sizeof(uint32_t)));
         s->stamp_est = 0;
     }
@@ -650,12 +651,22 @@ static int ensure_transpose(Graph* g, PprState* s) {
     for (int64_t v = 0; v < V; ++v)
         for (int64_t e = trow[v]; e < trow[v + 1]; e += kPullChunk)
             items.push_back(make_int4((int32_t)v, (int32_t)e, (int32_t)std::min<int64_t>(e + kPullChunk, trow[v + 1]), 0));
-    HIP_TRY(hipMalloc(&s->tadjf, sizeof(int2) * tadj.size()));
-    HIP_TRY(hipMemcpy(s->tadjf, tadj.data(), sizeof(int2) * tadj.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMalloc(&s->titems, sizeof(int4) * std::max<size_t>(items.size(), 1)));
-    if (!items.empty())
-        HIP_TRY(hipMemcpy(s->titems, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
+    // both arrays complete before s->tadjf marks them ready
+    int2* ta = nullptr;
+    int4* ti = nullptr;
+    hipError_t e = hipMalloc(&ta, sizeof(int2) * tadj.size());
+    if (e == hipSuccess) e = hipMemcpy(ta, tadj.data(), sizeof(int2) * tadj.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&ti, sizeof(int4) * std::max<size_t>(items.size(), 1));
+    if (e == hipSuccess && !items.empty())
+        e = hipMemcpy(ti, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (ta) (void)hipFree(ta);
+        if (ti) (void)hipFree(ti);
+        return hip_fail(e, "PPR in-edge rows");
+    }
+    s->titems = ti;
     s->nt = (int64_t)items.size();
+    s->tadjf = ta;
     return C2G_OK;
 }
 
@@ -780,15 +791,15 @@ static int ensure_ppr_batch(Graph* g, PprState* s, int groups) {
     const int64_t V = std::max<int64_t>(g->V, 1);
     const int chunk = std::max(1, g->tune.batch_chunk);
     const int64_t needi = groups * (V + g->E / chunk) + 64;
-    if (!s->ctrl) {
-        HIP_TRY(hipMalloc(&s->ctrl, sizeof(int64_t) * P_N));
-        HIP_TRY(hipHostMalloc((void**)&s->h_ctrl, sizeof(int64_t) * P_N, hipHostMallocDefault));
-    }
+    if (!s->ctrl) HIP_TRY(hipMalloc(&s->ctrl, sizeof(int64_t) * P_N));
+    if (!s->h_ctrl) HIP_TRY(hipHostMalloc((void**)&s->h_ctrl, sizeof(int64_t) * P_N, hipHostMallocDefault));
     if (groups > s->groups || s->itemcap < needi) {
         const int G = std::max(groups, s->groups);
         for (void** p : {(void**)&s->P, (void**)&s->R, (void**)&s->M, (void**)&s->RU, (void**)&s->mask,
                          (void**)&s->tbits, (void**)&s->items, (void**)&s->imask, (void**)&s->srcs})
             if (*p) { (void)hipFree(*p); *p = nullptr; }
+        s->groups = 0;  // empty until every piece is in place: a failure is retried, never used
+        s->itemcap = 0;
         const size_t n = (size_t)G * V * 64;
         HIP_TRY(hipMalloc(&s->P, sizeof(float) * n));
         HIP_TRY(hipMalloc(&s->R, sizeof(float) * n));

@@ -373,19 +373,20 @@ int ensure_sssp_ws(Graph* g) {
     const int64_t need = 2 * V + (g->E + g->HE) / chunk + 64;
     const int64_t nwp = bitmap_words(V);
     HIP_TRY(hipSetDevice(g->device));
-    if (!w.dist) {
-        HIP_TRY(hipMalloc(&w.dist, sizeof(int32_t) * std::max<int64_t>(V, 1)));
-        // live and hidden activation bitmaps
-        HIP_TRY(hipMalloc(&w.bits, sizeof(uint32_t) * nwp * 2));
-        HIP_TRY(hipMalloc(&w.ctrl, sizeof(int64_t) * C_N));
-        HIP_TRY(hipEventCreate(&w.ev[0]));
-        HIP_TRY(hipEventCreate(&w.ev[1]));
-        HIP_TRY(hipHostMalloc((void**)&w.h_ctrl, sizeof(int64_t) * C_N, hipHostMallocDefault));
-        if (g->d_hrow) HIP_TRY(hipMalloc(&w.hid, std::max<int64_t>(V, 1)));
-    }
+    // Each piece is checked on its own: after a failed allocation the next
+    // call allocates what is missing instead of using it.
+    if (!w.dist) HIP_TRY(hipMalloc(&w.dist, sizeof(int32_t) * std::max<int64_t>(V, 1)));
+    // live and hidden activation bitmaps
+    if (!w.bits) HIP_TRY(hipMalloc(&w.bits, sizeof(uint32_t) * nwp * 2));
+    if (!w.ctrl) HIP_TRY(hipMalloc(&w.ctrl, sizeof(int64_t) * C_N));
+    if (!w.ev[0]) HIP_TRY(hipEventCreate(&w.ev[0]));
+    if (!w.ev[1]) HIP_TRY(hipEventCreate(&w.ev[1]));
+    if (!w.h_ctrl) HIP_TRY(hipHostMalloc((void**)&w.h_ctrl, sizeof(int64_t) * C_N, hipHostMallocDefault));
+    if (g->d_hrow && !w.hid) HIP_TRY(hipMalloc(&w.hid, std::max<int64_t>(V, 1)));
     if (w.qcap < need) {
         if (w.exec) { (void)hipGraphExecDestroy(w.exec); w.exec = nullptr; }
         if (w.q[0]) { (void)hipFree(w.q[0]); w.q[0] = nullptr; }
+        w.qcap = 0;
         HIP_TRY(hipMalloc(&w.q[0], sizeof(int4) * need));
         w.qcap = need;
     }
@@ -449,11 +450,17 @@ static int ensure_exec(Graph* g) {
     HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
     int rc = launch_rounds(g, K);
     hipError_t e = hipStreamEndCapture(g->stream, &graph);
-    if (rc) return rc;
-    if (e != hipSuccess) return fail(C2G_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    if (rc) {
+        if (e == hipSuccess) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
     e = hipGraphInstantiate(&w.exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) return fail(C2G_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    if (e != hipSuccess) {
+        w.exec = nullptr;
+        return hip_fail(e, "hipGraphInstantiate");
+    }
     w.exec_iters = K;
     w.exec_grid = relax_grid(g);
     w.exec_chunk = g->tune.chunk;
@@ -648,6 +655,11 @@ int qserve_init(Graph* g, hipStream_t strm, int slots) {
     SlotPool* s = spool(g);
     if (s->slots) return s->slots == slots ? C2G_OK : fail(C2G_EINVAL, "slot pool already sized %d", s->slots);
     HIP_TRY(hipSetDevice(g->device));
+    // pieces left by an earlier attempt that failed part way
+    for (void* p : {(void*)s->dist, (void*)s->bits, (void*)s->hid, (void*)s->q, (void*)s->ctrl})
+        if (p) (void)hipFree(p);
+    if (s->h_ctrl) (void)hipHostFree(s->h_ctrl);
+    s->dist = nullptr; s->bits = nullptr; s->hid = nullptr; s->q = nullptr; s->ctrl = nullptr; s->h_ctrl = nullptr;
     const int64_t V = std::max<int64_t>(g->V, 1);
     const int chunk = std::max(1, g->tune.chunk);
     s->nwp = bitmap_words(V);

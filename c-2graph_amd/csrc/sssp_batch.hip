@@ -439,13 +439,14 @@ static int ensure_batch(Graph* g, int groups) {
     HIP_TRY(hipSetDevice(g->device));
     const int64_t V = g->V;
     const int chunk = std::max(1, g->tune.batch_chunk);
-    if (!s->ev[0]) {
-        HIP_TRY(hipEventCreate(&s->ev[0]));
-        HIP_TRY(hipEventCreate(&s->ev[1]));
+    // each piece on its own, so a failed attempt is completed by the next call
+    if (!s->ev[0]) HIP_TRY(hipEventCreate(&s->ev[0]));
+    if (!s->ev[1]) HIP_TRY(hipEventCreate(&s->ev[1]));
+    if (!w.ctrl) {
         HIP_TRY(hipMalloc(&w.ctrl, sizeof(int64_t) * B_N));
         HIP_TRY(hipMemset(w.ctrl, 0, sizeof(int64_t) * B_N));
-        HIP_TRY(hipHostMalloc((void**)&w.h_ctrl, sizeof(int64_t) * B_N, hipHostMallocDefault));
     }
+    if (!w.h_ctrl) HIP_TRY(hipHostMalloc((void**)&w.h_ctrl, sizeof(int64_t) * B_N, hipHostMallocDefault));
     // Each (v,g) is activated at most once per round: at most
     // ceil(deg/chunk) + ceil(hdeg/chunk) <= 2 + (deg + hdeg)/chunk items.
     const int64_t needi = groups * (2 * V + (g->E + g->HE) / chunk) + 64;
@@ -457,6 +458,12 @@ static int ensure_batch(Graph* g, int groups) {
         (void)hipFree(s->sums); (void)hipFree(s->reach); (void)hipFree(s->medges);
         (void)hipFree(s->thr); (void)hipFree(s->fmin); (void)hipFree(s->lor);
         w.dist = nullptr; w.mask[0] = nullptr; w.hmask = nullptr;
+        s->abits = nullptr; s->items = nullptr; s->imask = nullptr; s->srcs = nullptr;
+        s->sums = nullptr; s->reach = nullptr; s->medges = nullptr;
+        s->thr = nullptr; s->fmin = nullptr; s->lor = nullptr;
+        // until every piece is in place the workspace counts as empty, so a
+        // failed allocation is retried instead of used
+        s->groups = 0; s->itemcap = 0; w.groups = 0;
         s->nw = (((V + 31) / 32 + 1) + 63) & ~(int64_t)63;  // whole waves per group
         HIP_TRY(hipMalloc(&s->thr, sizeof(int32_t) * 64 * G));
         HIP_TRY(hipMalloc(&s->fmin, sizeof(int32_t) * 64 * G));
@@ -539,11 +546,17 @@ static int ensure_exec_b(Graph* g, int groups) {
     int rc = C2G_OK;
     for (int r = 0; r < K && rc == C2G_OK; ++r) rc = launch_round_b(g, groups, nullptr);
     hipError_t e = hipStreamEndCapture(g->stream, &graph);
-    if (rc) return rc;
-    if (e != hipSuccess) return fail(C2G_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    if (rc) {
+        if (e == hipSuccess) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
     e = hipGraphInstantiate(&w.exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) return fail(C2G_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    if (e != hipSuccess) {
+        w.exec = nullptr;
+        return hip_fail(e, "hipGraphInstantiate");
+    }
     w.exec_iters = K;
     w.exec_grid = relax_grid_b(g);
     w.exec_chunk = g->tune.batch_chunk;
@@ -747,12 +760,11 @@ int bserve_init(Graph* g, hipStream_t strm) {
     int rc = ensure_batch(g, 1);
     if (rc) return rc;
     ServeState* ss = sstate(g);
-    if (!ss->out) {
-        HIP_TRY(hipMalloc(&ss->out, sizeof(int32_t) * (size_t)kServeStage * std::max(g->V, 1)));
+    if (!ss->out) HIP_TRY(hipMalloc(&ss->out, sizeof(int32_t) * (size_t)kServeStage * std::max(g->V, 1)));
+    if (!ss->h_out)
         HIP_TRY(hipHostMalloc((void**)&ss->h_out, sizeof(int32_t) * (size_t)kServeStage * std::max(g->V, 1),
                               hipHostMallocDefault));
-        HIP_TRY(hipMalloc(&ss->slot, 64));
-    }
+    if (!ss->slot) HIP_TRY(hipMalloc(&ss->slot, 64));
     BatchWs& w = g->batch;
     BatchState* s = bstate(g);
     const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;

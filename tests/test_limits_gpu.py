@@ -65,3 +65,33 @@ def test_largest_batches(gpu):
         with pytest.raises(c2g.C2GError) as e:
             g.sssp_batch_resident(srcs[:16385])
         assert e.value.code == c2g.EINVAL
+
+
+def test_allocation_failure_leaves_handle_usable(gpu):
+    # a batch whose lane rows cannot fit in HBM (1,024 groups x 4 M vertices x
+    # 256 bytes, about 1 TB) fails with C2G_ENOMEM; afterwards the same handle
+    # runs smaller batches and the other paths correctly: no half-built
+    # workspace of the failed call is used and no stale runtime error remains.
+    # (The batched PPR's host outputs for that many sources would not fit in
+    # host memory, so its own allocation path is not driven to failure here.)
+    G = random_graph(1 << 22, 1 << 23, seed=11, wmax=50)
+    big = np.arange(65536, dtype=np.int32)
+    srcs = np.array([0, 1, 2, 3, 12345, 4194303], np.int32)
+    with c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], G["flist"], device=0) as g:
+        d0, _ = g.sssp_batch(srcs[:2])  # a workspace exists before the failure
+        with pytest.raises(c2g.C2GError) as e:
+            g.sssp_batch_resident(big)
+        assert e.value.code == c2g.ENOMEM, e.value
+        d, _ = g.sssp_batch(srcs)
+        assert np.array_equal(d[:2], d0)
+        for q in (0, 5):
+            assert np.array_equal(d[q], oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], int(srcs[q]))), q
+        with pytest.raises(c2g.C2GError) as e:
+            g.sssp_batch_resident(big)
+        assert e.value.code == c2g.ENOMEM, e.value
+        dq, _ = g.sssp(int(srcs[5]))
+        assert np.array_equal(dq, d[5])
+        P, R, _ = g.ppr_batch(srcs)
+        for q in (0, 5):
+            p, r, _ = g.ppr(int(srcs[q]))
+            assert np.abs(P[q].astype(np.float64) - p).sum() <= 2e-5, q
