@@ -50,6 +50,7 @@ enum : int {
 };
 
 constexpr int kJ = 8;  // edges whose row loads are in flight together
+static_assert(kJ - 1 <= kEdgePad, "relax_b_k reads up to kJ-1 edges past a chunk");
 constexpr int kIdMaskB = 0x7fffffff;  // bit 31 of an edge target = target has edges
 
 __global__ void batch_fill_k(int32_t* __restrict__ dist, int64_t n_dist, uint64_t* __restrict__ mask,
@@ -86,18 +87,82 @@ __global__ void batch_seed_k(const int32_t* __restrict__ srcs, int n, const int3
     atomicOr(&abits[(int64_t)g * nw + (s >> 5)], 1u << (s & 31));
 }
 
+typedef __attribute__((address_space(1))) int32_t gint32;
+typedef __attribute__((address_space(1))) char gchar;
+
+// `p` as an SGPR pair in the global address space: a value the compiler can
+// no longer fold into per-lane address arithmetic, so `p` plus a 32-bit lane
+// offset is one load with a scalar base (no 64-bit vector add per access).
+__device__ __forceinline__ gchar* sgpr_ptr(int32_t* p) {
+    const uint64_t a = (uint64_t)p;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32));
+    return (gchar*)(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ gint32* lane_slot(gchar* row, uint32_t loff) { return (gint32*)(row + loff); }
+
+// Relaxes edges j0 .. j0+kJ-1 of a chunk for the lanes in exec.  Lane j of
+// `my` holds edge j (readlane ignores exec; every lane of `my` holds a value);
+// everything about an edge (target, weight, row address, which lanes
+// improved) is wave-uniform and lives in SGPRs, so per lane there is one add,
+// one compare, the row load and the atomic.  Distances are compared unsigned:
+// a candidate of INT32_MAX or more (unreached source, or past the end of the
+// chunk) never beats a stored distance, which is at most INT32_MAX.  TAIL:
+// some of the kJ edges may lie at or past `ne`; they read the source's own row
+// and relax nothing.
+template <bool TAIL>
+__device__ __forceinline__ void relax_step_b(const int2 my, int j0, int ne, int u, uint32_t du,
+                                             int32_t* __restrict__ dG, uint32_t loff,
+                                             uint64_t* __restrict__ bits) {
+    gchar* rj[kJ];
+    uint32_t cj[kJ], oj[kJ];
+#pragma unroll
+    for (int jj = 0; jj < kJ; ++jj) {
+        int v = __builtin_amdgcn_readlane(my.x, j0 + jj) & kIdMaskB;
+        uint32_t w = (uint32_t)__builtin_amdgcn_readlane(my.y, j0 + jj);
+        if (TAIL) {
+            const bool valid = j0 + jj < ne;
+            v = valid ? v : u;
+            w = valid ? w : 0x7fffffffu;
+        }
+        rj[jj] = sgpr_ptr(dG + (int64_t)v * 64);
+        cj[jj] = du + w;
+        oj[jj] = (uint32_t)*lane_slot(rj[jj], loff);
+    }
+    // Fire-and-forget atomicMin: a lane whose candidate beats the value it
+    // read marks (v, lane) active.  If another relaxation lowered
+    // dist[v][lane] below the candidate since the read, it was made in this
+    // round and marked the same bit, so waiting for the old value would change
+    // nothing but the latency.
+#pragma unroll
+    for (int jj = 0; jj < kJ; ++jj) {
+        // lanes (in exec) whose candidate is below the stored value: the
+        // compare's own mask, so the ballot costs no extra instruction
+        const uint64_t b = __builtin_amdgcn_uicmp(cj[jj], oj[jj], 36 /* unsigned < */);
+        if (cj[jj] < oj[jj])
+            (void)__hip_atomic_fetch_min(lane_slot(rj[jj], loff), (int)cj[jj], __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+        bits[j0 + jj] = b;
+    }
+}
+
 template <bool PRUNED>
 __global__ void __launch_bounds__(256)
 relax_b_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
           int32_t* __restrict__ dist, uint64_t* __restrict__ mask, uint64_t* __restrict__ hmask,
           const int4* __restrict__ items, const uint64_t* __restrict__ imask,
           uint32_t* __restrict__ abits, int64_t* __restrict__ ctrl, int64_t V, int64_t nw) {
+    // improved-lane mask of each edge of the current 64-edge chunk, per wave
+    __shared__ uint64_t s_bits[4][64];
     const int64_t n = ctrl[B_ITEMS];
     const int lane = lane_id();
-    // uniform wave index: each item header comes through scalar loads into
-    // SGPRs (8 fewer VGPRs, and 5 % faster rounds)
+    // uniform wave index: item headers and edge entries come through scalar
+    // loads, and each target row's base address is an SGPR pair
     const int64_t wave = wave_index();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    uint64_t* __restrict__ bits = s_bits[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
+    const uint32_t loff = (uint32_t)lane * 4u;  // this lane's byte offset in a 256-byte row
     int64_t impr = 0;
 
     for (int64_t i = wave; i < n; i += nwaves) {
@@ -108,50 +173,32 @@ relax_b_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
         const bool hid_item = PRUNED && (it.w >> 16);
         int32_t* __restrict__ dG = dist + (int64_t)g * V * 64;
         const bool act = (m >> lane) & 1ull;
-        const int du = act ? dG[(int64_t)u * 64 + lane] : INT32_MAX;
         const int2* __restrict__ A = hid_item ? hadj : adj;
         for (int e0 = it.y; e0 < it.z; e0 += 64) {
             const int ne = min(64, it.z - e0);
             int2 my = make_int2(0, 0);
             if (lane < ne) my = A[e0 + lane];
-            uint64_t mybits = 0;
-            for (int j0 = 0; j0 < ne; j0 += kJ) {
-                int vj[kJ], cj[kJ], oj[kJ];
-#pragma unroll
-                for (int jj = 0; jj < kJ; ++jj) {
-                    const bool valid = j0 + jj < ne;
-                    const int j = valid ? j0 + jj : ne - 1;
-                    vj[jj] = __shfl(my.x, j, 64) & kIdMaskB;
-                    const int w = __shfl(my.y, j, 64);
-                    cj[jj] = (valid && act && du <= INT32_MAX - w) ? du + w : INT32_MAX;
-                    oj[jj] = INT32_MAX;
-                    if (cj[jj] != INT32_MAX) oj[jj] = dG[(int64_t)vj[jj] * 64 + lane];
-                }
-                // Fire-and-forget atomicMin: a lane whose candidate beats the
-                // value it read marks (v, lane) active.  If another relaxation
-                // lowered dist[v][lane] below the candidate since the read, it
-                // was made in this round and marked the same bit, so waiting
-                // for the old value would change nothing but the latency.
-#pragma unroll
-                for (int jj = 0; jj < kJ; ++jj)
-                    if (cj[jj] < oj[jj]) atomicMin(&dG[(int64_t)vj[jj] * 64 + lane], cj[jj]);
-#pragma unroll
-                for (int jj = 0; jj < kJ; ++jj) {
-                    const bool imp = cj[jj] < oj[jj];
-                    const uint64_t b = __ballot(imp);
-                    impr += imp;
-                    if (lane == j0 + jj) mybits = b;
-                }
+            // The item's pending lanes relax the chunk's edges kJ at a time.
+            if (act) {
+                const uint32_t du = (uint32_t)dG[(int64_t)u * 64 + lane];
+                wait_vmem();  // `my` and du, once per chunk rather than at every loop head
+                int j0 = 0;
+                for (; j0 + kJ <= ne; j0 += kJ) relax_step_b<false>(my, j0, ne, u, du, dG, loff, bits);
+                if (j0 < ne) relax_step_b<true>(my, j0, ne, u, du, dG, loff, bits);
             }
-            // Lane l now holds the improved-lane mask of edge l: record the
-            // lanes at v and mark (v,g) active (both fire-and-forget).
-            if (mybits) {
-                const int v = my.x & kIdMaskB;
-                if (PRUNED && hid_item)
-                    atomicOr((unsigned long long*)&hmask[(int64_t)g * V + v], (unsigned long long)mybits);
-                if (my.x < 0) {
-                    atomicOr((unsigned long long*)&mask[(int64_t)g * V + v], (unsigned long long)mybits);
-                    atomicOr(&abits[(int64_t)g * nw + (v >> 5)], 1u << (v & 31));
+            // Lane l applies edge l's improved lanes: record them at v and
+            // mark (v,g) active (both fire-and-forget).
+            if (lane < ne) {
+                const uint64_t mybits = m ? bits[lane] : 0;
+                impr += __popcll(mybits);
+                if (mybits) {
+                    const int v = my.x & kIdMaskB;
+                    if (PRUNED && hid_item)
+                        atomicOr((unsigned long long*)&hmask[(int64_t)g * V + v], (unsigned long long)mybits);
+                    if (my.x < 0) {
+                        atomicOr((unsigned long long*)&mask[(int64_t)g * V + v], (unsigned long long)mybits);
+                        atomicOr(&abits[(int64_t)g * nw + (v >> 5)], 1u << (v & 31));
+                    }
                 }
             }
         }

@@ -32,6 +32,14 @@ __device__ __forceinline__ int wave_excl_scan(int x, int* total) {
     return v - x;
 }
 
+// s_waitcnt vmcnt(0): waits for every vector-memory operation this wave has
+// in flight.  On gfx950 stores and atomics without return count in vmcnt too,
+// and with both kinds pending the compiler can only wait for zero.  A loop
+// whose first use of a loaded register sits at its head therefore waits for
+// the previous trip's atomics on every trip; waiting once before the loop
+// clears that register and leaves the loop head without a wait.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 __device__ __forceinline__ int64_t wave_sum64(int64_t x) {
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) {
