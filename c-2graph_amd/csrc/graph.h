@@ -20,9 +20,6 @@
 
 namespace c2g {
 
-// Zeroed int2 entries allocated past the end of every device edge array.
-constexpr int64_t kEdgePad = 16;
-
 // A failed HIP call `what` as a C2G error (ENOMEM for memory, else EDEVICE),
 // with the runtime's last-error state consumed like HIP_TRY does.
 inline int hip_fail(hipError_t e, const char* what) {

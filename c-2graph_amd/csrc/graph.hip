@@ -127,11 +127,7 @@ int device_upload(Graph* g, int device) {
     }
     auto pack = [&](const std::vector<int32_t>& e, const void* w, int64_t n, int2** dst,
                     bool flag) -> int {
-        // kEdgePad zeroed entries past the end: the batched relax kernel
-        // reads edges in groups of 8 through scalar loads and may read past
-        // the last chunk of the array (it ignores what it reads there)
-        HIP_TRY(hipMalloc(dst, sizeof(int2) * (size_t)(std::max<int64_t>(n, 1) + kEdgePad)));
-        HIP_TRY(hipMemset(*dst + std::max<int64_t>(n, 1), 0, sizeof(int2) * kEdgePad));
+        HIP_TRY(hipMalloc(dst, sizeof(int2) * (size_t)std::max<int64_t>(n, 1)));
         const int64_t step = 1 << 24;  // pack and copy in 128 MiB slices
         tmp.resize((size_t)std::min<int64_t>(std::max<int64_t>(n, 1), step));
         for (int64_t o = 0; o < n; o += step) {

@@ -50,7 +50,6 @@ enum : int {
 };
 
 constexpr int kJ = 8;  // edges whose row loads are in flight together
-static_assert(kJ - 1 <= kEdgePad, "relax_b_k reads up to kJ-1 edges past a chunk");
 constexpr int kIdMaskB = 0x7fffffff;  // bit 31 of an edge target = target has edges
 
 __global__ void batch_fill_k(int32_t* __restrict__ dist, int64_t n_dist, uint64_t* __restrict__ mask,
@@ -119,8 +118,9 @@ __device__ __forceinline__ void relax_step_b(const int2 my, int j0, int ne, int 
     uint32_t cj[kJ], oj[kJ];
 #pragma unroll
     for (int jj = 0; jj < kJ; ++jj) {
-        int v = __builtin_amdgcn_readlane(my.x, j0 + jj) & kIdMaskB;
-        uint32_t w = (uint32_t)__builtin_amdgcn_readlane(my.y, j0 + jj);
+        const int j = TAIL ? min(j0 + jj, 63) : j0 + jj;  // a lane index
+        int v = __builtin_amdgcn_readlane(my.x, j) & kIdMaskB;
+        uint32_t w = (uint32_t)__builtin_amdgcn_readlane(my.y, j);
         if (TAIL) {
             const bool valid = j0 + jj < ne;
             v = valid ? v : u;
@@ -154,7 +154,8 @@ relax_b_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
           const int4* __restrict__ items, const uint64_t* __restrict__ imask,
           uint32_t* __restrict__ abits, int64_t* __restrict__ ctrl, int64_t V, int64_t nw) {
     // improved-lane mask of each edge of the current 64-edge chunk, per wave
-    __shared__ uint64_t s_bits[4][64];
+    // (a last step of kJ edges may run up to kJ-1 entries past the chunk)
+    __shared__ uint64_t s_bits[4][64 + kJ];
     const int64_t n = ctrl[B_ITEMS];
     const int lane = lane_id();
     // uniform wave index: item headers and edge entries come through scalar
