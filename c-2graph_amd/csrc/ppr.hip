@@ -353,6 +353,7 @@ pprb_push_k(const int2* __restrict__ adjf, float* __restrict__ M, const float* _
             int64_t* __restrict__ ctrl, int64_t V, int64_t nw, float alpha) {
     const int64_t n = ctrl[P_CUR];
     const int lane = lane_id();
+    const uint32_t loff = (uint32_t)lane * 4u;  // this lane's byte offset in a 256-byte row
     const int64_t wave = wave_index();  // item headers through scalar loads
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     for (int64_t i = wave; i < n; i += nwaves) {
@@ -367,10 +368,22 @@ pprb_push_k(const int2* __restrict__ adjf, float* __restrict__ M, const float* _
             int2 my = make_int2(0, 0);
             if (lane < ne) my = adjf[e0 + lane];
             if (lane < ne) atomicOr(&tbits[(int64_t)g * nw + (my.x >> 5)], 1u << (my.x & 31));
-            for (int j = 0; j < ne; ++j) {
-                const int v = __shfl(my.x, j, 64);
-                const float f = __int_as_float(__shfl(my.y, j, 64));
-                if (act) unsafeAtomicAdd(&Mg[(int64_t)v * 64 + lane], msg * f);
+            // The item's pending lanes push along the chunk's edges.  An
+            // edge's target, weight and row address are wave-uniform (SGPRs,
+            // the edge read from the lane that loaded it: readlane ignores
+            // exec), so per lane there is one multiply and the atomic.
+            if (act) {
+                gchar* MgS = sgpr_ptr(Mg);  // the group's rows, base in SGPRs
+#pragma unroll 8
+                for (int j = 0; j < ne; ++j) {
+                    const int v = __builtin_amdgcn_readlane(my.x, j);
+                    const float f = __int_as_float(__builtin_amdgcn_readlane(my.y, j));
+                    gchar* row = MgS + (int64_t)v * 256;
+                    [[clang::atomic(ignore_denormal_mode)]] {
+                        (void)__hip_atomic_fetch_add(lane_slot<float>(row, loff), msg * f, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                }
             }
         }
     }

@@ -86,21 +86,6 @@ __global__ void batch_seed_k(const int32_t* __restrict__ srcs, int n, const int3
     atomicOr(&abits[(int64_t)g * nw + (s >> 5)], 1u << (s & 31));
 }
 
-typedef __attribute__((address_space(1))) int32_t gint32;
-typedef __attribute__((address_space(1))) char gchar;
-
-// `p` as an SGPR pair in the global address space: a value the compiler can
-// no longer fold into per-lane address arithmetic, so `p` plus a 32-bit lane
-// offset is one load with a scalar base (no 64-bit vector add per access).
-__device__ __forceinline__ gchar* sgpr_ptr(int32_t* p) {
-    const uint64_t a = (uint64_t)p;
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32));
-    return (gchar*)(((uint64_t)hi << 32) | lo);
-}
-
-__device__ __forceinline__ gint32* lane_slot(gchar* row, uint32_t loff) { return (gint32*)(row + loff); }
-
 // Relaxes edges j0 .. j0+kJ-1 of a chunk for the lanes in exec.  Lane j of
 // `my` holds edge j (readlane ignores exec; every lane of `my` holds a value);
 // everything about an edge (target, weight, row address, which lanes
@@ -128,7 +113,7 @@ __device__ __forceinline__ void relax_step_b(const int2 my, int j0, int ne, int 
         }
         rj[jj] = sgpr_ptr(dG + (int64_t)v * 64);
         cj[jj] = du + w;
-        oj[jj] = (uint32_t)*lane_slot(rj[jj], loff);
+        oj[jj] = (uint32_t)*lane_slot<int32_t>(rj[jj], loff);
     }
     // Fire-and-forget atomicMin: a lane whose candidate beats the value it
     // read marks (v, lane) active.  If another relaxation lowered
@@ -141,7 +126,7 @@ __device__ __forceinline__ void relax_step_b(const int2 my, int j0, int ne, int 
         // compare's own mask, so the ballot costs no extra instruction
         const uint64_t b = __builtin_amdgcn_uicmp(cj[jj], oj[jj], 36 /* unsigned < */);
         if (cj[jj] < oj[jj])
-            (void)__hip_atomic_fetch_min(lane_slot(rj[jj], loff), (int)cj[jj], __ATOMIC_RELAXED,
+            (void)__hip_atomic_fetch_min(lane_slot<int32_t>(rj[jj], loff), (int)cj[jj], __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT);
         bits[j0 + jj] = b;
     }

@@ -40,6 +40,25 @@ __device__ __forceinline__ int wave_excl_scan(int x, int* total) {
 // clears that register and leaves the loop head without a wait.
 __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+typedef __attribute__((address_space(1))) char gchar;
+
+// `p` as an SGPR pair in the global address space: a value the compiler can
+// no longer fold into per-lane address arithmetic, so a row at `p` plus a
+// lane's 32-bit offset costs no 64-bit vector add per row.  `p` must be
+// wave-uniform.
+__device__ __forceinline__ gchar* sgpr_ptr(const void* p) {
+    const uint64_t a = (uint64_t)p;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32));
+    return (gchar*)(((uint64_t)hi << 32) | lo);
+}
+
+// This lane's element of the row at `row` (`loff` = its byte offset).
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* lane_slot(gchar* row, uint32_t loff) {
+    return (__attribute__((address_space(1))) T*)(row + loff);
+}
+
 __device__ __forceinline__ int64_t wave_sum64(int64_t x) {
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) {
