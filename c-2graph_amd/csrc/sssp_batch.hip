@@ -143,8 +143,8 @@ relax_b_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
     __shared__ uint64_t s_bits[4][64 + kJ];
     const int64_t n = ctrl[B_ITEMS];
     const int lane = lane_id();
-    // uniform wave index: item headers and edge entries come through scalar
-    // loads, and each target row's base address is an SGPR pair
+    // uniform wave index: item headers come through scalar loads, and each
+    // target row's base address is an SGPR pair
     const int64_t wave = wave_index();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     uint64_t* __restrict__ bits = s_bits[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
