@@ -227,17 +227,26 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
         __syncthreads();
     }
     int64_t front = 0, edges = 0, ledges = 0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t n = nw * groups;  // nw is a multiple of 64: a wave never straddles groups
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint32_t word = abits[i];
-        const int g = (int)(i / nw);
-        const int64_t vb = (i - (int64_t)g * nw) * 32;
-        const uint64_t* mg = mask + (int64_t)g * V;
+    // Work unit: a block of 64 bitmap words (2048 vertices of one group; nw
+    // is a multiple of 64).  The wave loads the block's words in one step and
+    // walks its nonzero words two at a time, lanes 0-31 on one word and 32-63
+    // on the other, so each active vertex has a lane of its own and its mask
+    // and row offsets arrive in coalesced loads.  Pass 1 counts each lane's
+    // items, one atomic reserves the block's item range, pass 2 writes the
+    // items and takes the expanded lanes out of the mask.
+    const int64_t nblk = (nw * groups) >> 6;
+    const int64_t nwav = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int half = lane >> 5, bit = lane & 31;
+    for (int64_t blk = wave_index(); blk < nblk; blk += nwav) {
+        const int64_t w0 = blk << 6;
+        const uint32_t myword = abits[w0 + lane];
+        const uint64_t nzw = __ballot(myword != 0);
+        if (!nzw) continue;
+        const int g = (int)(w0 / nw);
+        const int64_t vb0 = (w0 - (int64_t)g * nw) * 32;  // first vertex of the block
+        uint64_t* __restrict__ mg = mask + (int64_t)g * V;
         const int32_t* dG = dist + (int64_t)g * V * 64;
         const int32_t* tg = thr + g * 64;
-        uint64_t nor = 0, forr = 0;
-        uint32_t keep = 0;
         // Near lanes of v (and far lanes, recorded once in pass 1).
         auto split = [&](int v, uint64_t m, bool record) -> uint64_t {
             if (delta <= 0) return m;
@@ -257,21 +266,27 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
             }
             return near;
         };
+        uint64_t nor = 0, forr = 0;
         int cnt = 0;
-        for (uint32_t b = word; b; b &= b - 1) {
-            const int v = (int)(vb + __builtin_ctz(b));
+        for (uint64_t rest = nzw; rest;) {
+            const int k0 = __builtin_ctzll(rest);
+            rest &= rest - 1;
+            const int k1 = rest ? __builtin_ctzll(rest) : k0;
+            const uint32_t wb = rest ? (uint32_t)__builtin_amdgcn_readlane((int)myword, k1) : 0u;
+            if (rest) rest &= rest - 1;
+            const uint32_t wa = (uint32_t)__builtin_amdgcn_readlane((int)myword, k0);
+            const uint32_t word = half ? wb : wa;
+            if (!((word >> bit) & 1u)) continue;
+            const int v = (int)(vb0 + (int64_t)(half ? k1 : k0) * 32 + bit);
             const uint64_t m = mg[v];
             const uint64_t near = split(v, m, true);
-            const uint64_t far = m & ~near;
-            forr |= far;
-            if (far) keep |= b & (0u - b);
+            forr |= m & ~near;
             if (!near) continue;
             nor |= near;
             const int r0 = row[v], r1 = row[v + 1];
             cnt += (r1 - r0 + chunk - 1) / chunk;
-            uint64_t hm = 0;
             if (hrow) {
-                hm = near & hmask[(int64_t)g * V + v];
+                const uint64_t hm = near & hmask[(int64_t)g * V + v];
                 if (hm) {
                     const int h0 = hrow[v], h1 = hrow[v + 1];
                     cnt += (h1 - h0 + chunk - 1) / chunk;
@@ -293,31 +308,47 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
         const bool room = base + tot <= itemcap;
         if (!room && lane == 0) atomicOr((unsigned long long*)&ctrl[B_OVF], 1ull);
         int64_t p = base + off;
-        uint64_t* mw = mask + (int64_t)g * V;
-        for (uint32_t b = word; b; b &= b - 1) {
-            const int v = (int)(vb + __builtin_ctz(b));
-            const uint64_t m = mw[v];
-            const uint64_t near = split(v, m, false);
-            mw[v] = m & ~near;
-            if (!near || !room) continue;
-            const int r0 = row[v], r1 = row[v + 1];
-            for (int e = r0; e < r1; e += chunk, ++p) {
-                items[p] = make_int4(v, e, min(e + chunk, r1), g);
-                imask[p] = near;
-            }
-            if (hrow) {
-                const uint64_t hm = near & hmask[(int64_t)g * V + v];
-                if (hm) {
-                    const int h0 = hrow[v], h1 = hrow[v + 1];
-                    for (int e = h0; e < h1; e += chunk, ++p) {
-                        items[p] = make_int4(v, e, min(e + chunk, h1), g | (1 << 16));
-                        imask[p] = hm;
+        for (uint64_t rest = nzw; rest;) {
+            const int k0 = __builtin_ctzll(rest);
+            rest &= rest - 1;
+            const bool two = rest != 0;
+            const int k1 = two ? __builtin_ctzll(rest) : k0;
+            const uint32_t wb = two ? (uint32_t)__builtin_amdgcn_readlane((int)myword, k1) : 0u;
+            if (two) rest &= rest - 1;
+            const uint32_t wa = (uint32_t)__builtin_amdgcn_readlane((int)myword, k0);
+            const uint32_t word = half ? wb : wa;
+            const bool on = (word >> bit) & 1u;
+            bool keep = false;  // v stays active: lanes left for a later bucket
+            if (on) {
+                const int v = (int)(vb0 + (int64_t)(half ? k1 : k0) * 32 + bit);
+                const uint64_t m = mg[v];
+                const uint64_t near = split(v, m, false);
+                mg[v] = m & ~near;
+                keep = (m & ~near) != 0;
+                if (near && room) {
+                    const int r0 = row[v], r1 = row[v + 1];
+                    for (int e = r0; e < r1; e += chunk, ++p) {
+                        items[p] = make_int4(v, e, min(e + chunk, r1), g);
+                        imask[p] = near;
+                    }
+                    if (hrow) {
+                        const uint64_t hm = near & hmask[(int64_t)g * V + v];
+                        if (hm) {
+                            const int h0 = hrow[v], h1 = hrow[v + 1];
+                            for (int e = h0; e < h1; e += chunk, ++p) {
+                                items[p] = make_int4(v, e, min(e + chunk, h1), g | (1 << 16));
+                                imask[p] = hm;
+                            }
+                        }
                     }
                 }
             }
+            // the two words keep only the bits of vertices with far lanes
+            const uint64_t kb = __ballot(keep);
+            if (lane == 0 && (uint32_t)kb != wa) abits[w0 + k0] = (uint32_t)kb;
+            if (two && lane == 32 && (uint32_t)(kb >> 32) != wb) abits[w0 + k1] = (uint32_t)(kb >> 32);
         }
-        if (word != keep) abits[i] = keep;
-        // Lane bookkeeping of this group (the wave's 64 words share g).
+        // Lane bookkeeping of this group.
 #pragma unroll
         for (int d = 32; d > 0; d >>= 1) {
             nor |= (uint64_t)shfl64((int64_t)nor, lane ^ d);
