@@ -28,6 +28,17 @@
  *                      certificate |p - pi|_1 <= sum(r) against orc_ppr_power.
  *   orc_adsorption     FP64 Jacobi of Y_{t+1} = p_inj*Y0 + p_cont*W*Y_t.  Pinned
  *                      by a dense numpy matrix-power evaluation on small graphs.
+ *   orc_sssp_certificate  exactness check of an int32 distance vector for
+ *                      weights >= 1 (Graph500-style): d[s] = 0; d[v] <= d[u]+w
+ *                      on every edge out of a reached u; every reached v != s
+ *                      has a tight in-edge from a reached u; unreached vertices
+ *                      hold INT32_MAX.  Tight in-edges strictly decrease d, so
+ *                      they lead back to s and every finite d[v] is a real path
+ *                      length; feasibility makes it no longer than the shortest
+ *                      one.  Pinned by: Dijkstra outputs pass, and each single
+ *                      perturbation (+1, -1, an unreached vertex marked reached,
+ *                      a reached one marked unreached, d[s] != 0) fails with
+ *                      its own code (test_oracle.py).
  *   orc_adsorption_push  restatement of the reference's adsorption kernel
  *                      (src/adsorption/w-base/Main.cu: alpha 0.2, r[s] = 10,
  *                      p[s] = 0.8*r[s], constant threshold 1/V).  Pinned by the
@@ -121,6 +132,43 @@ int orc_dijkstra(int32_t V, const int32_t* vl, const int32_t* el, const int32_t*
     }
     free(h.heap); free(h.pos); free(h.key);
     return overflow ? -1 : 0;
+}
+
+/* Returns 0 when d is the exact distance vector from src (weights >= 1), else
+ * -1 (d[src] != 0 or src out of range), -2 (an edge out of a reached vertex
+ * violates d[v] <= d[u] + w), -3 (a reached v != src without a tight in-edge),
+ * -4 (a weight below 1), -5 (no memory).  Vertices are split over OpenMP
+ * threads; `tight` marks are idempotent byte stores. */
+int orc_sssp_certificate(int32_t V, const int32_t* vl, const int32_t* el, const int32_t* wl,
+                         int32_t src, const int32_t* d) {
+    if (src < 0 || src >= V || d[src] != 0) return -1;
+    uint8_t* tight = (uint8_t*)calloc((size_t)V, 1);
+    if (!tight) return -5;
+    int bad_w = 0, bad_f = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(|: bad_w, bad_f)
+    for (int32_t u = 0; u < V; ++u) {
+        if (d[u] == INT32_MAX) continue;
+        const int64_t du = d[u];
+        for (int32_t e = vl[u]; e < vl[u + 1]; ++e) {
+            const int32_t v = el[e];
+            if (wl[e] < 1) { bad_w = 1; continue; }
+            const int64_t c = du + (int64_t)wl[e];
+            const int64_t dv = d[v];  /* INT32_MAX for an unreached v */
+            if (dv > c) bad_f = 1;
+            else if (dv == c) {
+#pragma omp atomic write
+                tight[v] = 1;
+            }
+        }
+    }
+    int rc = 0;
+    if (bad_w) rc = -4;
+    else if (bad_f) rc = -2;
+    else
+        for (int32_t v = 0; v < V; ++v)
+            if (v != src && d[v] != INT32_MAX && !tight[v]) { rc = -3; break; }
+    free(tight);
+    return rc;
 }
 
 /* ---------------------------------------------------------- Bellman-Ford lane

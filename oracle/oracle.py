@@ -32,6 +32,7 @@ def lib():
         i32, f32, f64, i64 = (P(np.int32, flags="C"), P(np.float32, flags="C"), P(np.float64, flags="C"),
                               P(np.int64, flags="C"))
         L.orc_dijkstra.argtypes = [C.c_int32, i32, i32, i32, C.c_int32, i32]
+        L.orc_sssp_certificate.argtypes = [C.c_int32, i32, i32, i32, C.c_int32, i32]
         L.orc_bellman_ford.argtypes = [C.c_int32, i32, i32, i32, i32, i32, C.c_int32, i64]
         L.orc_ppr_power.argtypes = [C.c_int32, i32, i32, f32, C.c_int32, C.c_double, C.c_double, C.c_int32, f64]
         L.orc_forward_push.argtypes = [C.c_int32, i32, i32, f32, C.c_float, C.c_float, f32, f32, i32,
@@ -59,6 +60,25 @@ def dijkstra(vlist, elist, wlist, src):
     if rc != 0:
         raise ValueError(f"orc_dijkstra rc={rc}")
     return d
+
+
+def sssp_certificate(vlist, elist, wlist, src, dist):
+    """0 if dist is the exact int32 distance vector from src (weights >= 1),
+    else the negative code of the first check that fails (see oracle.c)."""
+    V = len(vlist) - 1
+    return lib().orc_sssp_certificate(V, _c(vlist, np.int32), _c(elist, np.int32), _c(wlist, np.int32), int(src),
+                                      _c(dist, np.int32))
+
+
+def dijkstra_many(vlist, elist, wlist, srcs, threads=None):
+    """orc_dijkstra for several sources on host threads (ctypes releases the
+    GIL during each call); returns a list of distance vectors."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    vl, el, wl = _c(vlist, np.int32), _c(elist, np.int32), _c(wlist, np.int32)
+    threads = threads or min(16, os.cpu_count() or 4, max(1, len(srcs)))
+    with ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(lambda s: dijkstra(vl, el, wl, int(s)), srcs))
 
 
 def bellman_ford(vlist, elist, wlist, src):

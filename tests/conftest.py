@@ -97,3 +97,21 @@ def check_sssp_certificate(vl, el, wl, src, d):
     reached = d64 != INT_MAX
     reached[src] = False
     assert np.all(tight[reached]), "a reached vertex has no tight in-edge"
+
+
+def headline_sources(vlist, n, seed):
+    """The batched workload's source recipe: n/2 drawn from the top-1%
+    out-degree vertices, n/2 from out-degree-1 vertices (seeded)."""
+    deg = np.diff(vlist)
+    rng = np.random.default_rng(seed)
+    order = np.argsort(-deg, kind="stable")
+    top = order[: max(1, len(deg) // 100)]
+    ones = np.flatnonzero(deg == 1)
+    a = rng.choice(top, n // 2, replace=False)
+    b = rng.choice(ones, n - n // 2, replace=len(ones) < n - n // 2)
+    return np.concatenate([a, b]).astype(np.int32)
+
+
+def checksum(d):
+    fin = d != INT_MAX
+    return int(d[fin].astype(np.int64).sum()), int(fin.sum())

@@ -174,3 +174,48 @@ def test_adsorption_push_invariant():
     # a plausible mistake fails: crediting the source alpha*r0 instead of 0.8*r0
     q = p.astype(np.float64)
     assert abs(np.abs(10.0 * pi - q).sum() - r.astype(np.float64).sum()) > 1.0
+
+
+def test_sssp_certificate_accepts_exact_and_rejects_each_perturbation():
+    # the C certificate used on full-size GPU results: Dijkstra output passes
+    # (scipy agrees with it above); every single slip fails with its own code
+    G = rmat_graph(14, 1, wmax=64)
+    vl, el, wl = G["vlist"], G["elist"], G["wlist"]
+    s = int(np.argmax(np.diff(vl)))
+    d = oracle.dijkstra(vl, el, wl, s)
+    assert oracle.sssp_certificate(vl, el, wl, s, d) == 0
+    fin = np.flatnonzero((d != INT_MAX) & (np.arange(G["V"]) != s))
+    unr = np.flatnonzero(d == INT_MAX)
+    rng = np.random.default_rng(0)
+    v = int(rng.choice(fin))
+    for delta, code in ((+1, None), (-1, None)):
+        e = d.copy()
+        e[v] += delta
+        # +1: the tight in-edge of v now over-shoots (feasibility); -1: no tight in-edge
+        assert oracle.sssp_certificate(vl, el, wl, s, e) in ((-2,) if delta > 0 else (-3, -2))
+    e = d.copy()
+    e[v] = INT_MAX  # a reachable vertex reported unreached
+    assert oracle.sssp_certificate(vl, el, wl, s, e) == -2
+    if len(unr):
+        e = d.copy()
+        e[int(unr[0])] = 5  # an unreachable vertex reported reached: no tight in-edge
+        assert oracle.sssp_certificate(vl, el, wl, s, e) == -3
+    e = d.copy()
+    e[s] = 1
+    assert oracle.sssp_certificate(vl, el, wl, s, e) == -1
+    # every distance off by the same +k except the source: no tight edge out of s
+    e = np.where(d == INT_MAX, d, d + 3).astype(np.int32)
+    e[s] = 0
+    assert oracle.sssp_certificate(vl, el, wl, s, e) != 0
+    # a zero weight on an edge out of a reached vertex voids the proof: refused
+    w0 = wl.copy()
+    w0[vl[s]] = 0
+    assert oracle.sssp_certificate(vl, el, w0, s, d) == -4
+
+
+def test_dijkstra_many_equals_single():
+    G = rmat_graph(14, 1, wmax=64)
+    srcs = [0, 5, int(np.argmax(np.diff(G["vlist"])))]
+    many = oracle.dijkstra_many(G["vlist"], G["elist"], G["wlist"], srcs, threads=3)
+    for s, d in zip(srcs, many):
+        assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], s))
