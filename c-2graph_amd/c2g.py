@@ -217,6 +217,60 @@ def prune_ppr_mem(vlist, elist, flist):
                 hvlist=hv, helist=_take(ohe, int(hv[V]), np.int32), hwlist=_take(ohw, int(hv[V]), np.float32))
 
 
+_hip = None
+
+
+def _hip_rt():
+    """The HIP runtime libc2g.so links (for caller-owned device buffers)."""
+    global _hip
+    if _hip is None:
+        lib()
+        try:
+            H = C.CDLL("libamdhip64.so.7")
+        except OSError:
+            H = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+        H.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        H.hipFree.argtypes = [C.c_void_p]
+        H.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        H.hipSetDevice.argtypes = [C.c_int]
+        _hip = H
+    return _hip
+
+
+class DeviceBuffer:
+    """A caller-owned device allocation (hipMalloc through the HIP runtime that
+    libc2g.so uses), for entry points that write into device memory."""
+
+    def __init__(self, nbytes, device=0):
+        H = _hip_rt()
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        if H.hipSetDevice(int(device)) != 0 or H.hipMalloc(C.byref(self.ptr), max(self.nbytes, 1)) != 0:
+            raise C2GError(ENOMEM, f"hipMalloc of {nbytes} bytes failed")
+
+    def to_numpy(self, dtype, n):
+        a = np.empty(int(n), dtype)
+        if _hip_rt().hipMemcpy(a.ctypes.data_as(C.c_void_p), self.ptr, a.nbytes, 2) != 0:  # device to host
+            raise C2GError(EDEVICE, "hipMemcpy device to host failed")
+        return a
+
+    def fill(self, arr):
+        arr = np.ascontiguousarray(arr)
+        if _hip_rt().hipMemcpy(self.ptr, arr.ctypes.data_as(C.c_void_p), arr.nbytes, 1) != 0:  # host to device
+            raise C2GError(EDEVICE, "hipMemcpy host to device failed")
+
+    def free(self):
+        if self.ptr:
+            _hip_rt().hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.free()
+
+
 class Graph:
     """Owning wrapper of a c2g_graph handle."""
 
@@ -293,6 +347,14 @@ class Graph:
         st = Stats()
         check(lib().c2g_sssp(self._h, int(src), _ptr(d, C.c_int32), C.byref(st)))
         return d, st
+
+    def sssp_noalloc(self, src, dbuf):
+        """c2g_sssp_noalloc: distances into the DeviceBuffer dbuf (int32[V])."""
+        if dbuf.nbytes < 4 * self.V:
+            raise C2GError(EINVAL, "device buffer smaller than V int32")
+        st = Stats()
+        check(lib().c2g_sssp_noalloc(self._h, int(src), C.cast(dbuf.ptr, C.POINTER(C.c_int32)), C.byref(st)))
+        return st
 
     def sssp_batch(self, srcs):
         srcs = np.ascontiguousarray(srcs, np.int32)

@@ -139,6 +139,10 @@ int ensure_sssp_ws(Graph* g);
 int host_bf(const Graph* g, int32_t* dist, const int32_t* frontier, int32_t count,
             const uint8_t* via_hidden, c2g_stats* st);
 
+// Looks up `device`'s copy of the static scratch of c2g_sssp_noalloc (the
+// device must be current); once per device, at upload.
+int sssp_scratch_init(int device);
+
 // Used by the server and batched callers.
 int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* frontier_out,
              int32_t* count_out, uint8_t* hid_out, c2g_stats* st, bool copy_out);

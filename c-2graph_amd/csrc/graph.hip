@@ -113,6 +113,8 @@ int device_upload(Graph* g, int device) {
     g->num_cus = prop.multiProcessorCount;
     g->on_device = false;
     HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    int rc0 = sssp_scratch_init(device);  // this device's copy of the no-alloc scratch
+    if (rc0) return rc0;
     const int64_t V = g->V, E = g->E;
     HIP_TRY(hipMalloc(&g->d_row, sizeof(int32_t) * (V + 1)));
     HIP_TRY(hipMemcpy(g->d_row, g->vl.data(), sizeof(int32_t) * (V + 1), hipMemcpyHostToDevice));

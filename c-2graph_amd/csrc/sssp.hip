@@ -400,33 +400,50 @@ static int relax_grid(const Graph* g) {
 
 static bool is_pruned(const Graph* g) { return g->d_hrow != nullptr && g->d_hadj != nullptr; }
 
-// Launches k rounds (relax + expand).  With relax_ms non-null (profiling),
-// HIP events bracket each relax kernel and the time is accumulated.
-static int launch_rounds(Graph* g, int k, double* relax_ms = nullptr) {
+// Device buffers of one single-query traversal: the handle's workspace, or
+// the caller's distance array plus the static scratch (c2g_sssp_noalloc).
+struct SsspBufs {
+    int32_t* dist;     // [V]
+    uint32_t* bits;    // live [nwp], then hidden [nwp]
+    uint8_t* hid;      // [V] (pruned graphs)
+    int4* q;           // frontier items
+    int64_t qcap;
+    int64_t* ctrl;     // [C_N]
+};
+
+static SsspBufs ws_bufs(Graph* g) {
+    SsspWs& w = g->sssp;
+    return SsspBufs{w.dist, w.bits, w.hid, w.q[0], w.qcap, w.ctrl};
+}
+
+// Launches k rounds (relax + expand) on `b`.  With relax_ms non-null
+// (profiling), the workspace's HIP events bracket each relax kernel and the
+// time is accumulated.
+static int launch_rounds_on(Graph* g, const SsspBufs& b, int k, double* relax_ms) {
     SsspWs& w = g->sssp;
     const bool pruned = is_pruned(g);
     const int grid = relax_grid(g);
     const int64_t nwp = bitmap_words(g->V);
-    uint32_t* bits = w.bits;
+    uint32_t* bits = b.bits;
     uint32_t* hbits = bits + nwp;
     const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (nwp + 255) / 256));
     for (int r = 0; r < k; ++r) {
         if (relax_ms) HIP_TRY(hipEventRecord(w.ev[0], g->stream));
         if (pruned) {
             hipLaunchKernelGGL(sssp_relax_k<true>, dim3(grid), dim3(256), 0, g->stream, g->d_adj,
-                               g->d_hadj, w.dist, bits, hbits, w.hid, w.q[0], w.ctrl, (int64_t)g->V, nwp);
+                               g->d_hadj, b.dist, bits, hbits, b.hid, b.q, b.ctrl, (int64_t)g->V, nwp);
             if (relax_ms) HIP_TRY(hipEventRecord(w.ev[1], g->stream));
             hipLaunchKernelGGL(sssp_expand_k<true>, dim3(egrid), dim3(256), 0, g->stream, g->d_row,
-                               g->d_hrow, bits, hbits, w.q[0], w.ctrl, (int64_t)g->V, g->tune.chunk, w.qcap,
-                               (const int32_t*)w.dist, g->tune.delta, 1, nwp);
+                               g->d_hrow, bits, hbits, b.q, b.ctrl, (int64_t)g->V, g->tune.chunk, b.qcap,
+                               (const int32_t*)b.dist, g->tune.delta, 1, nwp);
         } else {
             hipLaunchKernelGGL(sssp_relax_k<false>, dim3(grid), dim3(256), 0, g->stream, g->d_adj,
-                               (const int2*)nullptr, w.dist, bits, (uint32_t*)nullptr, (uint8_t*)nullptr,
-                               w.q[0], w.ctrl, (int64_t)g->V, nwp);
+                               (const int2*)nullptr, b.dist, bits, (uint32_t*)nullptr, (uint8_t*)nullptr,
+                               b.q, b.ctrl, (int64_t)g->V, nwp);
             if (relax_ms) HIP_TRY(hipEventRecord(w.ev[1], g->stream));
             hipLaunchKernelGGL(sssp_expand_k<false>, dim3(egrid), dim3(256), 0, g->stream, g->d_row,
-                               (const int32_t*)nullptr, bits, (uint32_t*)nullptr, w.q[0], w.ctrl,
-                               (int64_t)g->V, g->tune.chunk, w.qcap, (const int32_t*)w.dist, g->tune.delta, 1, nwp);
+                               (const int32_t*)nullptr, bits, (uint32_t*)nullptr, b.q, b.ctrl,
+                               (int64_t)g->V, g->tune.chunk, b.qcap, (const int32_t*)b.dist, g->tune.delta, 1, nwp);
         }
         if (relax_ms) {
             HIP_TRY(hipEventSynchronize(w.ev[1]));
@@ -437,6 +454,23 @@ static int launch_rounds(Graph* g, int k, double* relax_ms = nullptr) {
     }
     HIP_TRY(hipGetLastError());
     return C2G_OK;
+}
+
+static int launch_rounds(Graph* g, int k, double* relax_ms = nullptr) {
+    return launch_rounds_on(g, ws_bufs(g), k, relax_ms);
+}
+
+// Clears `b` for a query from src and queues the source's items.
+static void launch_seed(Graph* g, const SsspBufs& b, int32_t src) {
+    const bool pruned = is_pruned(g);
+    const int64_t nwp = bitmap_words(g->V);
+    const int tb = 256, gridv = (int)std::min<int64_t>((g->V + tb - 1) / tb, 65535);
+    hipLaunchKernelGGL(sssp_init_k, dim3(std::max(gridv, 1)), dim3(tb), 0, g->stream, b.dist, b.bits,
+                       pruned ? b.bits + nwp : (uint32_t*)nullptr, pruned ? b.hid : (uint8_t*)nullptr,
+                       (int64_t)g->V);
+    hipLaunchKernelGGL(sssp_seed_k, dim3(1), dim3(256), 0, g->stream, g->d_row,
+                       pruned ? g->d_hrow : (const int32_t*)nullptr, b.dist,
+                       pruned ? b.hid : (uint8_t*)nullptr, b.q, b.ctrl, src, g->tune.chunk, g->tune.delta);
 }
 
 // A hipGraph holding `iters_per_graph` rounds.
@@ -499,13 +533,7 @@ int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* 
     const int64_t nwp = bitmap_words(g->V);
     uint32_t* bits = w.bits;
     const double t0 = now_ms();
-    const int tb = 256, gridv = (int)std::min<int64_t>((g->V + tb - 1) / tb, 65535);
-    hipLaunchKernelGGL(sssp_init_k, dim3(std::max(gridv, 1)), dim3(tb), 0, g->stream, w.dist, bits,
-                       pruned ? bits + nwp : (uint32_t*)nullptr, pruned ? w.hid : (uint8_t*)nullptr,
-                       (int64_t)g->V);
-    hipLaunchKernelGGL(sssp_seed_k, dim3(1), dim3(256), 0, g->stream, g->d_row,
-                       pruned ? g->d_hrow : (const int32_t*)nullptr, w.dist,
-                       pruned ? w.hid : (uint8_t*)nullptr, w.q[0], w.ctrl, src, g->tune.chunk, g->tune.delta);
+    launch_seed(g, ws_bufs(g), src);
     HIP_TRY(hipGetLastError());
     const bool prof = g->tune.profile != 0;
     const bool use_graph = g->tune.use_graph && budget <= 0 && !prof;
@@ -730,6 +758,113 @@ int qserve_extract(Graph* g, hipStream_t strm, int slot, int32_t* out) {
     return s->out.copy_out(out, s->dist + (int64_t)slot * g->V, sizeof(int32_t) * g->V, strm);
 }
 
+// ------------------------------------------------------- no-alloc variant
+// Scratch of c2g_sssp_noalloc: control block, checksum words, activation
+// bitmaps, hidden flags and the frontier queue of one traversal, in one
+// file-scope device array.  Each device holds its own copy for the life of
+// the process; host code reaches it only through the address that
+// hipGetSymbolAddress returned on that device (kept in g_scr), never through
+// the array's name.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scr_mu;
+static void* g_scr[kMaxDevices];
+
+// Looks up this device's copy of the scratch array; `device` must be current.
+// Called once per device when a handle is uploaded (c2g_open and
+// c2g_graph_from_csr), before any no-alloc query can run there.
+int sssp_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(C2G_EINVAL, "device %d beyond the scratch table", device);
+    std::lock_guard<std::mutex> lk(g_scr_mu);
+    if (g_scr[device]) return C2G_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scr[device] = p;
+    return C2G_OK;
+}
+
+static void* scratch_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scr_mu);
+    return g_scr[device];
+}
+
+// Byte layout of the scratch for one graph; returns the total.
+struct ScratchLayout {
+    size_t ctrl, sums, bits, hid, q, total;
+    int64_t qcap;
+};
+static ScratchLayout scratch_layout(const Graph* g) {
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    ScratchLayout L{};
+    const int64_t V = std::max<int64_t>(g->V, 1);
+    const int chunk = std::max(1, g->tune.chunk);
+    L.qcap = 2 * V + (g->E + g->HE) / chunk + 64;  // the workspace's own bound (ensure_sssp_ws)
+    L.ctrl = 0;
+    L.sums = up(sizeof(int64_t) * C_N);
+    L.bits = L.sums + up(sizeof(int64_t) * 4);
+    L.hid = L.bits + up(sizeof(uint32_t) * 2 * bitmap_words(V));
+    L.q = L.hid + (is_pruned(g) ? up((size_t)V) : 0);
+    L.total = L.q + sizeof(int4) * (size_t)L.qcap;
+    return L;
+}
+
+__global__ void zero_i64_k(int64_t* __restrict__ p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+// One query into the caller's device array d_dist[V], all other state in the
+// scratch.  Plain launches (no captured graph), counters read back into host
+// stack memory.
+static int sssp_noalloc_run(Graph* g, int32_t src, int32_t* d_dist, c2g_stats* st) {
+    if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");
+    if (!g->d_adj) return fail(C2G_ENOTSUP, "graph has no int weights (wlist) for SSSP");
+    if (src < 0 || src >= g->V) return fail(C2G_EINVAL, "source %d out of range [0,%d)", src, g->V);
+    const ScratchLayout L = scratch_layout(g);
+    if (L.total > kScratchBytes)
+        return fail(C2G_EINVAL,
+                    "graph too large for c2g_sssp_noalloc: needs %zu bytes of scratch (V=%d, E=%lld, "
+                    "hidden E=%lld, chunk %d), the static scratch holds %zu",
+                    L.total, g->V, (long long)g->E, (long long)g->HE, std::max(1, g->tune.chunk), kScratchBytes);
+    unsigned char* base = (unsigned char*)scratch_ptr(g->device);
+    if (!base) return fail(C2G_EDEVICE, "no-alloc scratch was not set up on device %d", g->device);
+    HIP_TRY(hipSetDevice(g->device));
+    const SsspBufs b{d_dist, (uint32_t*)(base + L.bits), is_pruned(g) ? (uint8_t*)(base + L.hid) : nullptr,
+                     (int4*)(base + L.q), L.qcap, (int64_t*)(base + L.ctrl)};
+    int64_t* sums = (int64_t*)(base + L.sums);
+    const double t0 = now_ms();
+    launch_seed(g, b, src);  // clears the bitmaps, hidden flags and control block
+    HIP_TRY(hipGetLastError());
+    int64_t c[C_N];
+    for (;;) {
+        int rc = launch_rounds_on(g, b, std::max(1, g->tune.iters_per_graph), nullptr);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(c, b.ctrl, sizeof c, hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        if (c[C_OVF]) return fail(C2G_EDEVICE, "frontier queue overflow (internal)");
+        if (c[C_CUR] == 0 && c[C_FAR] == 0) break;
+    }
+    const double t1 = now_ms();
+    if (st) {
+        // sums[0] = sum of finite distances, sums[1] = reached count (int32
+        // in its low word), sums[2] = out-degree sum of reached vertices
+        hipLaunchKernelGGL(zero_i64_k, dim3(1), dim3(64), 0, g->stream, sums, 4);
+        hipLaunchKernelGGL(dist_checksum_k, dim3(std::max(1, g->num_cus) * 4), dim3(256), 0, g->stream, g->d_row,
+                           (const int32_t*)d_dist, (int64_t)g->V, (int64_t)1, sums, (int32_t*)(sums + 1), sums + 2);
+        HIP_TRY(hipGetLastError());
+        int64_t h[4];
+        HIP_TRY(hipMemcpyAsync(h, sums, sizeof h, hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        fill_stats(st, c, t1 - t0, C2G_LANE_GPU);
+        st->reached = (int32_t)h[1];
+        st->reached_edges = h[2];
+    }
+    return C2G_OK;
+}
+
 }  // namespace c2g
 
 using namespace c2g;
@@ -741,6 +876,13 @@ int c2g_sssp(c2g_graph* h, int32_t src, int32_t* dist_out, c2g_stats* st) {
     std::lock_guard<std::mutex> lk(h->g.mu);
     if (st) memset(st, 0, sizeof *st);
     return sssp_run(&h->g, src, 0, dist_out, nullptr, nullptr, nullptr, st, true);
+}
+
+int c2g_sssp_noalloc(c2g_graph* h, int32_t src, int32_t* d_dist, c2g_stats* st) {
+    if (!h || !d_dist) return fail(C2G_EINVAL, "c2g_sssp_noalloc: null argument");
+    std::lock_guard<std::mutex> lk(h->g.mu);
+    if (st) memset(st, 0, sizeof *st);
+    return sssp_noalloc_run(&h->g, src, d_dist, st);
 }
 
 int c2g_sssp_budget(c2g_graph* h, int32_t src, int32_t budget, int32_t* dist_out,

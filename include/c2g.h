@@ -269,6 +269,19 @@ int c2g_sssp_each_resident(c2g_graph* g, const int32_t* srcs, int32_t n,
  * frontier_out has room for V entries. */
 int c2g_sssp_budget(c2g_graph* g, int32_t src, int32_t budget, int32_t* dist_out,
                     int32_t* frontier_out, int32_t* count_out, c2g_stats* st);
+/* c2g_sssp without any allocation (no reference counterpart: it is for callers
+ * that cannot allocate, such as tight serving loops or code that must not
+ * call hipMalloc).  The distances land in the caller's DEVICE array
+ * d_dist (int32[V], on the handle's device); the traversal's other state (bitmaps,
+ * hidden flags, frontier queue, control block) lives in a static 64 MiB
+ * device array that each device holds for the life of the process.  Graphs
+ * whose state does not fit (about 2V + (E + hidden E)/chunk frontier items
+ * of 16 bytes, e.g. up to R-MAT scale 20 at edge factor 16) are refused with
+ * C2G_EINVAL.  Rounds run as plain launches on the handle's stream; the call
+ * returns when the query has converged.  Every call on one device shares that
+ * scratch: calls that overlap in time on one device (from several threads or
+ * handles) race and give wrong results.  Distances equal c2g_sssp's. */
+int c2g_sssp_noalloc(c2g_graph* g, int32_t src, int32_t* d_dist, c2g_stats* st);
 /* GPU head for `budget` iterations, then the host-core lane finishes the
  * frontier (the C^2graph-PM handoff, src/sssp/purn-perk/Main.cu:111-182 with
  * Bellman_ford of src/sssp/base-perk/SSSP.cpp:88-119). */
