@@ -61,7 +61,24 @@ class ServerOpts(C.Structure):
                 ("alpha", C.c_float), ("eps", C.c_float), ("ads_labels", C.c_int32),
                 ("ads_iters", C.c_int32), ("p_inj", C.c_float), ("p_cont", C.c_float),
                 ("p_abnd", C.c_float), ("sssp_slots", C.c_int32), ("ads_grid_per_cu", C.c_int32),
-                ("reserved", C.c_int32 * 6)]
+                ("deadline_ms", C.c_int32), ("fail_inject", C.c_int32), ("fail_after", C.c_int32),
+                ("dry_device", C.c_int32), ("dry_step_ms", C.c_int32), ("reserved", C.c_int32 * 1)]
+
+
+SLANE_SLOTS, SLANE_GROUP, SLANE_PPR, SLANE_ADS = 1, 2, 4, 8
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("budget", C.c_int64), ("charged", C.c_int64), ("lane_bytes", C.c_int64 * 4),
+                ("sssp_slots", C.c_int32), ("lanes_enabled", C.c_int32), ("submitted", C.c_int64),
+                ("completed", C.c_int64), ("failed", C.c_int64), ("cancelled", C.c_int64),
+                ("to_host", C.c_int64), ("lane_failures", C.c_int64), ("queued", C.c_int32),
+                ("running", C.c_int32)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["lane_bytes"] = list(self.lane_bytes)
+        return d
 
 
 class ResultInfo(C.Structure):
@@ -99,6 +116,8 @@ def lib():
         L.c2g_server_stop.argtypes = [C.c_void_p]
         L.c2g_server_start.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]
         L.c2g_server_poll.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.POINTER(ResultInfo)]
+        L.c2g_server_cancel.argtypes = [C.c_void_p, C.c_int64]
+        L.c2g_server_get_stats.argtypes = [C.c_void_p, C.POINTER(ServerStats)]
         _lib = L
     return _lib
 
@@ -484,7 +503,8 @@ class Server:
 
     def __init__(self, graph, max_batch=64, max_inflight=0, hbm_reserve=0, fake_free_hbm=0, host_lane=False,
                  device_failed=False, alpha=0.15, eps=1e-7, ads_labels=16, ads_iters=20, p_inj=0.25, p_cont=0.70,
-                 p_abnd=0.05, sssp_slots=0, ads_grid_per_cu=0):
+                 p_abnd=0.05, sssp_slots=0, ads_grid_per_cu=0, deadline_ms=0, fail_inject=0, fail_after=0,
+                 dry_device=False, dry_step_ms=0):
         self.graph = graph
         self.L = ads_labels
         o = ServerOpts()
@@ -493,6 +513,8 @@ class Server:
         o.ads_labels, o.ads_iters, o.p_inj, o.p_cont, o.p_abnd = ads_labels, ads_iters, p_inj, p_cont, p_abnd
         o.sssp_slots = sssp_slots
         o.ads_grid_per_cu = ads_grid_per_cu
+        o.deadline_ms, o.fail_inject, o.fail_after = deadline_ms, fail_inject, fail_after
+        o.dry_device, o.dry_step_ms = int(dry_device), dry_step_ms
         self._s = C.c_void_p()
         check(lib().c2g_server_start(graph._h, C.byref(o), C.byref(self._s)))
 
@@ -521,6 +543,15 @@ class Server:
             else:
                 res = out[: V * self.L].reshape(V, self.L).copy()
         return d, res
+
+    def cancel(self, rid):
+        """1 if the request was cancelled, 0 if it had already finished."""
+        return check(lib().c2g_server_cancel(self._s, rid))
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().c2g_server_get_stats(self._s, C.byref(st)))
+        return st.as_dict()
 
     def stop(self):
         if self._s:

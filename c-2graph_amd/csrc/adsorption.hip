@@ -315,6 +315,20 @@ static int ads_prepare(Graph* g, AdsState* s, int L, int nseed) {
     return C2G_OK;
 }
 
+// Device bytes the server's adsorption lane (three V x `labels` iterates, the
+// work items and, with ads_relabel, the relabelled CSR copy) would add to
+// what the handle holds.
+int64_t ads_lane_ws_bytes(Graph* g, int labels) {
+    AdsState* s = astate(g);
+    const int64_t V = std::max<int64_t>(g->V, 1);
+    int64_t b = 0;
+    if (s->cap < V * labels) b += 3 * 4 * V * labels;
+    if (!s->items) b += 16 * (V + g->E / kAdsChunk + 1) + 4 * V;
+    if (g->tune.ads_relabel && !s->radjf) b += 8 * g->E + 4 * V;
+    b += 8 * (V / 100 + 1) * (64 / std::max(1, labels));  // seed lists of one batched pass
+    return b;
+}
+
 // Workgroups per CU of the adsorption kernels: gpc > 0 overrides the graph's
 // grid_per_cu (the server runs this lane below full occupancy so that the
 // latency-bound SSSP and PPR lanes keep room on every CU).
@@ -486,7 +500,7 @@ extern "C" int c2g_adsorption(c2g_graph* h, const int32_t* seed_v, const int32_t
                               int32_t L, float p_inj, float p_cont, float p_abnd, int32_t iters, float* y_out,
                               c2g_stats* st) {
     if (!h || (nseed > 0 && (!seed_v || !seed_label)) || nseed < 0) return fail(C2G_EINVAL, "bad arguments");
-    std::lock_guard<std::mutex> lk(h->g.mu);
+    C2G_HANDLE_LOCK(h);
     if (st) memset(st, 0, sizeof *st);
     return ads_run(&h->g, h->g.stream, seed_v, seed_label, nseed, L, p_inj, p_cont, p_abnd, iters, y_out, st);
 }

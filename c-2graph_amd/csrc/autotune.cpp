@@ -63,7 +63,7 @@ double time_ppr(Graph* g, const int32_t* srcs, int32_t n, float alpha, float eps
 extern "C" int c2g_autotune_ppr(c2g_graph* h, const int32_t* srcs, int32_t n, float alpha, float eps,
                                 c2g_tune* best_out) {
     if (!h || !srcs || n < 1) return fail(C2G_EINVAL, "c2g_autotune_ppr: need at least one sample source");
-    std::lock_guard<std::mutex> lk(h->g.mu);
+    C2G_HANDLE_LOCK(h);
     Graph* g = &h->g;
     if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");
     for (int32_t i = 0; i < n; ++i)
@@ -93,7 +93,7 @@ extern "C" int c2g_autotune_ppr(c2g_graph* h, const int32_t* srcs, int32_t n, fl
 
 extern "C" int c2g_autotune(c2g_graph* h, const int32_t* srcs, int32_t n, int32_t batched, c2g_tune* best_out) {
     if (!h || !srcs || n < 1) return fail(C2G_EINVAL, "c2g_autotune: need at least one sample source");
-    std::lock_guard<std::mutex> lk(h->g.mu);
+    C2G_HANDLE_LOCK(h);
     Graph* g = &h->g;
     if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");
     for (int32_t i = 0; i < n; ++i)

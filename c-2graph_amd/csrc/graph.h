@@ -16,6 +16,7 @@
 #include <string>
 #include <memory>
 #include <mutex>
+#include <atomic>
 #include "common.h"
 
 namespace c2g {
@@ -120,7 +121,11 @@ struct Graph {
     SsspWs sssp;
     BatchWs batch;
     c2g_tune tune{};
-    std::mutex mu;  // serialises device use of this handle (server vs caller)
+    std::mutex mu;  // serialises the direct entry points on this handle
+    // 1 while a query server owns the handle: its lanes use the handle's
+    // stream and workspaces without `mu`, so every direct entry point that
+    // would touch the device refuses with C2G_EBUSY until c2g_server_stop.
+    std::atomic<int> serving{0};
 
     bool has_w() const { return !wl.empty() || E == 0; }
     bool has_f() const { return !fl.empty() || E == 0; }
@@ -173,6 +178,13 @@ int bserve_extract(Graph* g, hipStream_t strm, const int* lanes, int n, int32_t*
 // (at most 64), query-major state, items tagged with the slot.  Used while
 // few queries are in flight, when the 64-lane rows above would be mostly idle.
 int qserve_init(Graph* g, hipStream_t strm, int slots);
+// Device bytes each server lane's workspace would add to what the handle
+// holds now (0 when it is already allocated at that size); the server
+// charges them against its HBM budget before a lane takes work.
+int64_t qserve_ws_bytes(Graph* g, int slots);
+int64_t bserve_ws_bytes(Graph* g);
+int64_t ppr_lane_ws_bytes(Graph* g);
+int64_t ads_lane_ws_bytes(Graph* g, int labels);
 int qserve_admit(Graph* g, hipStream_t strm, int slot, int32_t src);
 int qserve_step(Graph* g, hipStream_t strm, uint64_t* slots_with_work);
 int qserve_extract(Graph* g, hipStream_t strm, int slot, int32_t* out);
@@ -183,3 +195,11 @@ void qserve_release(Graph* g);
 struct c2g_graph {
     c2g::Graph g;
 };
+
+// Opening of every direct entry point that uses a handle's device state:
+// serialises callers and refuses while a query server owns the handle.
+#define C2G_HANDLE_LOCK(h)                                                               \
+    std::lock_guard<std::mutex> lk((h)->g.mu);                                           \
+    if ((h)->g.serving.load())                                                           \
+        return ::c2g::fail(C2G_EBUSY, "the handle is owned by a running query server "   \
+                                      "(call c2g_server_stop first)")

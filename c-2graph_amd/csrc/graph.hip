@@ -166,7 +166,16 @@ static void dfree(T*& p) {
 }
 
 void device_release(Graph* g) {
-    if (!g->stream && !g->d_row) return;
+    if (!g->stream && !g->d_row) {
+        // host-only handle: per-graph state records a server's workspace
+        // estimates may have created (they hold no device memory)
+        batch_release(g);
+        ppr_release(g);
+        ads_release(g);
+        serve_release(g);
+        qserve_release(g);
+        return;
+    }
     (void)hipSetDevice(g->device);
     batch_release(g);
     ppr_release(g);
@@ -425,6 +434,7 @@ int c2g_graph_from_csr(int32_t V, const int32_t* vlist, const int32_t* elist, co
 
 int c2g_close(c2g_graph* h) {
     if (!h) return C2G_OK;
+    if (h->g.serving.load()) return fail(C2G_EBUSY, "the handle is owned by a running query server");
     device_release(&h->g);
     delete h;
     return C2G_OK;
@@ -463,7 +473,7 @@ int c2g_set_tune(c2g_graph* h, const c2g_tune* t) {
     if (t->iters_per_graph < 2 || t->iters_per_graph > 256 || (t->iters_per_graph & 1))
         return fail(C2G_EINVAL, "iters_per_graph must be even, in [2,256]");
     if (t->delta < 0) return fail(C2G_EINVAL, "delta >= 0");
-    std::lock_guard<std::mutex> lk(h->g.mu);
+    C2G_HANDLE_LOCK(h);
     if (memcmp(&h->g.tune, t, sizeof *t) != 0) {
         // Captured launch sequences hold the old values as kernel arguments.
         if (h->g.sssp.exec) { (void)hipGraphExecDestroy(h->g.sssp.exec); h->g.sssp.exec = nullptr; }

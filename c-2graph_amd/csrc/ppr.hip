@@ -573,6 +573,24 @@ void ppr_release(Graph* g) {
         }
 }
 
+// Device bytes the server's PPR lane (single-source state, one 64-lane group
+// of the batched state, its column staging and, when pull rounds are on, the
+// in-edge rows) would add to what the handle holds.
+int64_t ppr_lane_ws_bytes(Graph* g) {
+    PprState* s = pstate(g);
+    const int64_t V = std::max<int64_t>(g->V, 1);
+    const int64_t nw = (V + 31) / 32 + 1;
+    int64_t b = 0;
+    if (!s->p) b += 12 * V + 4 * nw + 8 * P_N;
+    const int64_t needq = V + g->E / std::max(1, g->tune.chunk) + 64;
+    if (s->qcap < needq) b += 16 * needq;
+    const int64_t needi = (V + g->E / std::max(1, g->tune.batch_chunk)) + 64;
+    if (s->groups < 1 || s->itemcap < needi) b += 4 * 4 * 64 * V + 8 * V + 4 * nw + 24 * needi + 4 * 64;
+    if (s->cols.cap < kColStage * V) b += 2 * 4 * (int64_t)kColStage * V;
+    if (g->tune.ppr_pull > 0 && !s->tadjf) b += 8 * g->E + 16 * (V + g->E / 64 + 1) + 8 * V;
+    return b;
+}
+
 static bool ppr_pruned(const Graph* g) { return g->d_hrow != nullptr && g->d_hadjf != nullptr; }
 
 static int ppr_check(Graph* g, float alpha, float eps) {
@@ -910,7 +928,7 @@ extern "C" {
 
 int c2g_ppr(c2g_graph* h, int32_t src, float alpha, float eps, float* p_out, float* r_out, c2g_stats* st) {
     if (!h || !p_out) return fail(C2G_EINVAL, "null argument");
-    std::lock_guard<std::mutex> lk(h->g.mu);
+    C2G_HANDLE_LOCK(h);
     if (st) memset(st, 0, sizeof *st);
     return ppr_single(&h->g, h->g.stream, src, alpha, eps, p_out, r_out, st);
 }
@@ -919,7 +937,7 @@ int c2g_ppr_migrate(c2g_graph* h, int32_t src, float alpha, float eps, int32_t b
                     float* p_out, float* r_out, c2g_stats* st) {
     if (!h || !p_out) return fail(C2G_EINVAL, "null argument");
     if (budget < 1) return fail(C2G_EINVAL, "budget must be >= 1");
-    std::lock_guard<std::mutex> lk(h->g.mu);
+    C2G_HANDLE_LOCK(h);
     Graph* g = &h->g;
     if (st) memset(st, 0, sizeof *st);
     std::vector<float> r((size_t)g->V);
@@ -947,7 +965,7 @@ int c2g_ppr_migrate(c2g_graph* h, int32_t src, float alpha, float eps, int32_t b
 
 int c2g_adsorption_push(c2g_graph* h, int32_t src, float* p_out, float* r_out, c2g_stats* st) {
     if (!h || !p_out) return fail(C2G_EINVAL, "null argument");
-    std::lock_guard<std::mutex> lk(h->g.mu);
+    C2G_HANDLE_LOCK(h);
     Graph* g = &h->g;
     if (st) memset(st, 0, sizeof *st);
     if (ppr_pruned(g)) return fail(C2G_ENOTSUP, "the adsorption push runs on the full graph (origin variant)");
@@ -966,7 +984,7 @@ int c2g_adsorption_push(c2g_graph* h, int32_t src, float* p_out, float* r_out, c
 int c2g_ppr_batch(c2g_graph* h, const int32_t* srcs, int32_t n, float alpha, float eps, float* p_out,
                   float* r_out, c2g_stats* st) {
     if (!h || !srcs || !p_out) return fail(C2G_EINVAL, "null argument");
-    std::lock_guard<std::mutex> lk(h->g.mu);
+    C2G_HANDLE_LOCK(h);
     if (st) memset(st, 0, sizeof *st);
     if (n < 1 || n > 64 * 1024) return fail(C2G_EINVAL, "need 1..65536 sources");
     const int64_t V = h->g.V;

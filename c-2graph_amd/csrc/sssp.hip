@@ -676,13 +676,41 @@ void qserve_release(Graph* g) {
         }
 }
 
+// Device bytes of a slot pool of `slots` queries on this graph.
+static int64_t qserve_bytes(const Graph* g, int slots) {
+    const int64_t V = std::max<int64_t>(g->V, 1);
+    const int chunk = std::max(1, g->tune.chunk);
+    const int64_t qcap = (int64_t)slots * (2 * V + (g->E + g->HE) / chunk) + 64;
+    return 4 * slots * V + 8 * slots * bitmap_words(V) + (g->hvl.empty() ? 0 : slots * V) + 16 * qcap +
+           8 * C_N;
+}
+
+// Device bytes a slot pool of `slots` would add to what the handle holds.
+int64_t qserve_ws_bytes(Graph* g, int slots) {
+    if (slots <= 0) return 0;
+    std::lock_guard<std::mutex> lk(g_sp_mu);
+    for (auto& p : g_sp)
+        if (p.first == g && p.second->slots == slots) return 0;
+    return qserve_bytes(g, slots);
+}
+
+// Sizes the pool for `slots` queries (a pool of another size is freed
+// first) and clears its control block and bitmaps, so a pool left by an
+// earlier server, stopped with queries in flight, starts empty.
 int qserve_init(Graph* g, hipStream_t strm, int slots) {
     if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");
     if (!g->d_adj) return fail(C2G_ENOTSUP, "graph has no int weights (wlist) for SSSP");
     if (slots < 1 || slots > 64) return fail(C2G_EINVAL, "slot pool size must be in [1,64]");
     SlotPool* s = spool(g);
-    if (s->slots) return s->slots == slots ? C2G_OK : fail(C2G_EINVAL, "slot pool already sized %d", s->slots);
     HIP_TRY(hipSetDevice(g->device));
+    const int64_t needq = (int64_t)slots * (2 * std::max<int64_t>(g->V, 1) + (g->E + g->HE) / std::max(1, g->tune.chunk)) + 64;
+    if (s->slots == slots && s->qcap >= needq) {
+        HIP_TRY(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * 2 * slots * s->nwp, strm));
+        HIP_TRY(hipMemsetAsync(s->ctrl, 0, sizeof(int64_t) * C_N, strm));
+        HIP_TRY(hipStreamSynchronize(strm));
+        return C2G_OK;
+    }
+    s->slots = 0;  // unusable until every piece below is in place
     // pieces left by an earlier attempt that failed part way
     for (void* p : {(void*)s->dist, (void*)s->bits, (void*)s->hid, (void*)s->q, (void*)s->ctrl})
         if (p) (void)hipFree(p);
@@ -873,14 +901,14 @@ extern "C" {
 
 int c2g_sssp(c2g_graph* h, int32_t src, int32_t* dist_out, c2g_stats* st) {
     if (!h || !dist_out) return fail(C2G_EINVAL, "c2g_sssp: null argument");
-    std::lock_guard<std::mutex> lk(h->g.mu);
+    C2G_HANDLE_LOCK(h);
     if (st) memset(st, 0, sizeof *st);
     return sssp_run(&h->g, src, 0, dist_out, nullptr, nullptr, nullptr, st, true);
 }
 
 int c2g_sssp_noalloc(c2g_graph* h, int32_t src, int32_t* d_dist, c2g_stats* st) {
     if (!h || !d_dist) return fail(C2G_EINVAL, "c2g_sssp_noalloc: null argument");
-    std::lock_guard<std::mutex> lk(h->g.mu);
+    C2G_HANDLE_LOCK(h);
     if (st) memset(st, 0, sizeof *st);
     return sssp_noalloc_run(&h->g, src, d_dist, st);
 }
@@ -889,7 +917,7 @@ int c2g_sssp_budget(c2g_graph* h, int32_t src, int32_t budget, int32_t* dist_out
                     int32_t* frontier_out, int32_t* count_out, c2g_stats* st) {
     if (!h || !dist_out || !frontier_out || !count_out) return fail(C2G_EINVAL, "null argument");
     if (budget < 1) return fail(C2G_EINVAL, "budget must be >= 1");
-    std::lock_guard<std::mutex> lk(h->g.mu);
+    C2G_HANDLE_LOCK(h);
     if (st) memset(st, 0, sizeof *st);
     return sssp_run(&h->g, src, budget, dist_out, frontier_out, count_out, nullptr, st, true);
 }
@@ -897,7 +925,7 @@ int c2g_sssp_budget(c2g_graph* h, int32_t src, int32_t budget, int32_t* dist_out
 int c2g_sssp_migrate(c2g_graph* h, int32_t src, int32_t budget, int32_t* dist_out, c2g_stats* st) {
     if (!h || !dist_out) return fail(C2G_EINVAL, "null argument");
     if (budget < 1) return fail(C2G_EINVAL, "budget must be >= 1");
-    std::lock_guard<std::mutex> lk(h->g.mu);
+    C2G_HANDLE_LOCK(h);
     Graph* g = &h->g;
     std::vector<int32_t> fr((size_t)g->V);
     std::vector<uint8_t> hid((size_t)g->V, 0);
@@ -925,7 +953,7 @@ int c2g_sssp_migrate(c2g_graph* h, int32_t src, int32_t budget, int32_t* dist_ou
 int c2g_sssp_each_resident(c2g_graph* h, const int32_t* srcs, int32_t n, int64_t* sums,
                            int32_t* reached, c2g_stats* st) {
     if (!h || !srcs || n < 0) return fail(C2G_EINVAL, "null argument");
-    std::lock_guard<std::mutex> lk(h->g.mu);
+    C2G_HANDLE_LOCK(h);
     Graph* g = &h->g;
     if (st) memset(st, 0, sizeof *st);
     if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");

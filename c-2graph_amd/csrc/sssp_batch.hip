@@ -818,6 +818,33 @@ void serve_release(Graph* g) {
 }
 constexpr int kServeStage = 8;  // lanes copied out per transfer
 
+// Device bytes the server's 64-lane group (one group of the batched
+// workspace plus the extraction staging) would add to what the handle holds.
+int64_t bserve_ws_bytes(Graph* g) {
+    const int64_t V = std::max<int64_t>(g->V, 1);
+    const int chunk = std::max(1, g->tune.batch_chunk);
+    const int64_t nw = (((V + 31) / 32 + 1) + 63) & ~(int64_t)63;
+    const int64_t needi = (2 * V + (g->E + g->HE) / chunk) + 64;
+    int64_t b = 0;
+    {
+        std::lock_guard<std::mutex> lk(g_bs_mu);
+        bool have = false;
+        for (auto& p : g_bs)
+            if (p.first == g && p.second->groups >= 1 && p.second->itemcap >= needi) have = true;
+        if (!have)
+            b += 256 * V + 8 * V * (g->hvl.empty() ? 1 : 2) + 4 * nw + 24 * needi + 64 * (4 + 4 + 8 + 8 + 4 + 8) +
+                 8 * B_N;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_ss_mu);
+        bool have = false;
+        for (auto& p : g_ss)
+            if (p.first == g && p.second->out) have = true;
+        if (!have) b += 4 * (int64_t)kServeStage * V + 64;
+    }
+    return b;
+}
+
 int bserve_init(Graph* g, hipStream_t strm) {
     if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");
     if (!g->d_adj) return fail(C2G_ENOTSUP, "graph has no int weights (wlist) for SSSP");
@@ -899,7 +926,7 @@ extern "C" {
 
 int c2g_sssp_batch(c2g_graph* h, const int32_t* srcs, int32_t n, int32_t* dist_out, c2g_stats* st) {
     if (!h || !srcs || !dist_out) return fail(C2G_EINVAL, "null argument");
-    std::lock_guard<std::mutex> lk(h->g.mu);
+    C2G_HANDLE_LOCK(h);
     if (st) memset(st, 0, sizeof *st);
     return sssp_batch_group(&h->g, srcs, n, dist_out, nullptr, nullptr, st);
 }
@@ -907,7 +934,7 @@ int c2g_sssp_batch(c2g_graph* h, const int32_t* srcs, int32_t n, int32_t* dist_o
 int c2g_sssp_batch_resident(c2g_graph* h, const int32_t* srcs, int32_t n, int64_t* sums,
                             int32_t* reached, c2g_stats* st) {
     if (!h || !srcs) return fail(C2G_EINVAL, "null argument");
-    std::lock_guard<std::mutex> lk(h->g.mu);
+    C2G_HANDLE_LOCK(h);
     if (st) memset(st, 0, sizeof *st);
     return sssp_batch_group(&h->g, srcs, n, nullptr, sums, reached, st);
 }

@@ -399,8 +399,41 @@ typedef struct {
                                  kernels, below full occupancy so the SSSP
                                  and PPR lanes share every CU: 0 = the
                                  graph's grid_per_cu, else 1..64             */
-    int32_t reserved[6];
+    int32_t deadline_ms;      /* >0: a request not done this many ms after its
+                                 submission completes with C2G_ETIMEOUT     */
+    int32_t fail_inject;      /* tests: C2G_SLANE_* bits of device lanes whose
+                                 step reports a device error, once each      */
+    int32_t fail_after;       /* tests: successful steps of an injected lane
+                                 before its failure                          */
+    int32_t dry_device;       /* tests: the device lanes (admission, budget,
+                                 slots, failover, cancellation) run with host
+                                 computations in place of kernels, so they work
+                                 without a GPU; fake_free_hbm is the budget  */
+    int32_t dry_step_ms;      /* tests: duration of one dry lane step         */
+    int32_t reserved[1];
 } c2g_server_opts;
+
+/* Device lanes of the server, as bits (c2g_server_opts.fail_inject,
+ * c2g_server_stats.lanes_enabled). */
+#define C2G_SLANE_SLOTS 1   /* SSSP query-major slot pool       */
+#define C2G_SLANE_GROUP 2   /* SSSP 64-lane group (overflow)    */
+#define C2G_SLANE_PPR   4   /* batched PPR lane                 */
+#define C2G_SLANE_ADS   8   /* adsorption lane                  */
+
+typedef struct {
+    int64_t budget;          /* HBM budget: free HBM at start (or fake_free_hbm)
+                                minus hbm_reserve                            */
+    int64_t charged;         /* workspace bytes the enabled lanes charged      */
+    int64_t lane_bytes[4];   /* charged per lane (slots, group, PPR, ads)      */
+    int32_t sssp_slots;      /* slots of the SSSP pool after budget sizing     */
+    int32_t lanes_enabled;   /* C2G_SLANE_* bits of device lanes in service    */
+    int64_t submitted, completed, failed, cancelled;  /* cancelled includes
+                                                         deadline expiries */
+    int64_t to_host;         /* requests handed to the host lane              */
+    int64_t lane_failures;   /* device-lane errors the server recovered from  */
+    int32_t queued;          /* waiting for a lane                            */
+    int32_t running;         /* admitted to a lane, not finished              */
+} c2g_server_stats;
 
 typedef struct {
     int64_t id;
@@ -414,7 +447,21 @@ typedef struct {
 /* Starts the lanes over an open graph.  The server replaces the reference's
  * CPU/GPU collaboration loop (src/sssp/purn-perk/Main.cu:87-246: two GPU
  * streams and one CPU worker thread serving one repeated query) with
- * continuously batched lanes per query kind plus the host-core lane. */
+ * continuously batched lanes per query kind plus the host-core lane.
+ *
+ * Admission by device memory: at start every device lane's workspace is
+ * charged against the HBM budget in the order SSSP slot pool (its slot count
+ * halved until it fits), SSSP lane group, PPR lane, adsorption lane; a lane
+ * that does not fit is not started.  Requests of a kind with no device lane
+ * go to the host lane when it is enabled and are refused with C2G_EBUSY
+ * otherwise; the others wait for a free slot, lane or pass.
+ *
+ * Lane failure: when a device lane's step fails, its running requests move
+ * to the host lane (or finish with the error when the host lane is off), the
+ * lane is reset and keeps serving; a lane whose reset fails is retired.
+ *
+ * While the server runs it owns the handle: every other entry point on it
+ * returns C2G_EBUSY until c2g_server_stop. */
 int c2g_server_start(c2g_graph* g, const c2g_server_opts* o, c2g_server** out);
 /* Submit one query; returns its id (>= 0) or a negative error (C2G_EBUSY when
  * admission is refused).  For C2G_Q_ADS, src is the seed-set index.  The
@@ -427,6 +474,14 @@ int64_t c2g_server_submit(c2g_server* s, int32_t kind, int32_t src);
  * 0 = not done, <0 error.  wait_ms < 0 blocks. */
 int c2g_server_poll(c2g_server* s, int64_t id, int32_t wait_ms, void* out,
                     c2g_result_info* info);
+/* Cancels request `id` (no reference counterpart: the reference's CPU worker
+ * has no cancellation, src/sssp/purn-perk/SSSP.cpp:140-159).  A queued
+ * request finishes at once, a running one at its lane's next step boundary;
+ * either way its poll reports C2G_ETIMEOUT.  Returns 1 if the request was
+ * cancelled, 0 if it had already finished. */
+int c2g_server_cancel(c2g_server* s, int64_t id);
+/* Counters and admission state of a running server. */
+int c2g_server_get_stats(c2g_server* s, c2g_server_stats* out);
 /* Stops the lanes, cancels outstanding requests and frees the server (the
  * reference joins its CPU worker thread, src/sssp/purn-perk/Main.cu:255). */
 int c2g_server_stop(c2g_server* s);

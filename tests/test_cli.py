@@ -114,3 +114,69 @@ def test_ppr_tools(gpu, ds, tmp_path):
     po, ro, _, _ = oracle.adsorption_push(G["vlist"], G["elist"], G["flist"], 7)
     assert np.array_equal(rows[:, 0], np.arange(G["V"]))
     assert np.abs(rows[:, 1] - po).sum() <= 2e-3  # printed with 6 significant digits
+
+
+def _adj_texts(tmp_path):
+    """Adjacency texts in the reference's input grammar ("src d1:d2:..."),
+    with duplicates, self-loops, lines that do not parse, out-of-order rows,
+    a vertex that appears only as a target, and one large random graph."""
+    rng = np.random.default_rng(7)
+    texts = {"tiny": "0 1:2:2:0\n2 0\nbad\n5 3:1\n7\n3 3:3:4\n"}
+    lines = []
+    for u in rng.permutation(300):
+        k = int(rng.integers(1, 12))
+        d = rng.integers(0, 320, k)  # ids up to 319: some vertices only appear as targets
+        if rng.random() < 0.2:
+            d = np.append(d, u)  # self-loop
+        if rng.random() < 0.3:
+            d = np.append(d, d[:2])  # duplicates
+        lines.append(f"{u} " + ":".join(map(str, d)))
+        if rng.random() < 0.05:
+            lines.append("# comment line")
+    texts["mixed"] = "\n".join(lines) + "\n"
+    lines = [f"{u} " + ":".join(map(str, rng.integers(0, 5000, int(rng.integers(1, 40)))))
+             for u in range(5000) if rng.random() < 0.9]
+    texts["large"] = "\n".join(lines) + "\n"
+    texts["dup_src"] = "4 1:2\n4 3:1\n0 4\n"  # a source on two lines: both lines' targets kept
+    out = {}
+    for name, t in texts.items():
+        p = tmp_path / f"{name}.txt"
+        p.write_text(t)
+        out[name] = str(p)
+    return out
+
+
+def test_convert2binary_byte_identical_to_reference(tmp_path):
+    # utils/convert2binary.cpp:10-135, compiled from the reference's source
+    # into oracle/_ref: all four origin files must match byte for byte
+    ref = os.path.join(ROOT, "oracle", "_ref", "convert2binary")
+    if not os.path.exists(ref):
+        pytest.skip("oracle/_ref/convert2binary not built (reference tree absent)")
+    for name, txt in _adj_texts(tmp_path).items():
+        a, b = str(tmp_path / f"{name}_ref"), str(tmp_path / f"{name}_ours")
+        os.makedirs(a + "/origin")  # the reference writes into an existing origin/
+        subprocess.run([ref, txt, a], check=True, capture_output=True, timeout=120)
+        run("convert2binary", txt, b)
+        for f in ("vlist", "elist", "wlist", "flist"):
+            ra = open(f"{a}/origin/{f}.bin", "rb").read()
+            rb = open(f"{b}/origin/{f}.bin", "rb").read()
+            assert ra == rb, (name, f, len(ra), len(rb))
+
+
+def test_convert2binary_weights_follow_libc_rand(tmp_path):
+    # the weights are (rand() % 100) + 1 in CSR order from the C library's
+    # unseeded generator (convert2binary.cpp:108): compare with libc itself
+    import ctypes
+    libc = ctypes.CDLL("libc.so.6")
+    txt = _adj_texts(tmp_path)["large"]
+    d = str(tmp_path / "w")
+    run("convert2binary", txt, d)
+    w = c2g.read_i32(d + "/origin/wlist.bin")
+    libc.srand(1)  # rand() before any srand() behaves as srand(1)
+    expect = np.array([libc.rand() % 100 + 1 for _ in range(len(w))], np.int32)
+    assert len(w) > 50000 and np.array_equal(w, expect)
+    vl = c2g.read_i32(d + "/origin/vlist.bin")
+    f = c2g.read_f32(d + "/origin/flist.bin")
+    row = np.repeat(np.arange(len(vl) - 1), np.diff(vl))
+    sums = np.bincount(row, weights=w, minlength=len(vl) - 1)
+    assert np.array_equal(f, (w / sums[row]).astype(np.float32))  # 1.0*w/sum in double, stored as float

@@ -56,16 +56,23 @@ def test_mixed_stream_matches_alone(gpu):
 
 
 def test_memory_admission_and_fallback(gpu):
+    # lanes are admitted by their device workspace: a budget of one slot's
+    # bytes starts the SSSP slot pool with one slot and no other lane
     G = _G()
-    V = G["V"]
     with _graph(G) as g:
-        # room for two SSSP results at a time: everything still completes (queued)
-        with c2g.Server(g, fake_free_hbm=2 * 4 * V + 16) as s:
-            ids = [s.submit(c2g.Q_SSSP, v) for v in range(0, 400, 40)]
+        with c2g.Server(g, sssp_slots=1) as s:
+            one_slot = s.stats()["lane_bytes"][0]
+        with c2g.Server(g, fake_free_hbm=one_slot) as s:
+            st = s.stats()
+            assert st["lanes_enabled"] == c2g.SLANE_SLOTS and st["sssp_slots"] == 1
+            ids = [s.submit(c2g.Q_SSSP, v) for v in range(0, 400, 40)]  # queued behind the one slot
             for i, v in zip(ids, range(0, 400, 40)):
                 info, d = s.poll(i)
                 assert info["lane"] == c2g.LANE_GPU
                 assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], v))
+            with pytest.raises(c2g.C2GError) as e:  # no PPR lane fits, no host lane
+                s.submit(c2g.Q_PPR, 0)
+            assert e.value.code == c2g.EBUSY
         # no room at all: refused without the host lane, host lane with it
         with c2g.Server(g, fake_free_hbm=1024) as s:
             with pytest.raises(c2g.C2GError) as e:
@@ -79,6 +86,63 @@ def test_memory_admission_and_fallback(gpu):
         with c2g.Server(g, device_failed=True, host_lane=True) as s:
             info, p = s.poll(s.submit(c2g.Q_PPR, 3))
             assert info["lane"] == c2g.LANE_HOST
+
+
+def test_two_servers_on_one_handle(gpu):
+    # the slot pool is sized per server (16 slots, then 4) and starts empty
+    G = _G()
+    srcs = [int(v) for v in np.flatnonzero(np.diff(G["vlist"]) > 0)[::53][:40]]
+    with _graph(G) as g:
+        for slots in (16, 4):
+            with c2g.Server(g, sssp_slots=slots) as s:
+                assert s.stats()["sssp_slots"] == slots
+                ids = [s.submit(c2g.Q_SSSP, v) for v in srcs]
+                for i, v in zip(ids, srcs):
+                    info, d = s.poll(i)
+                    assert info["status"] == 0 and info["lane"] == c2g.LANE_GPU
+                    assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], v)), (slots, v)
+        d, _ = g.sssp(srcs[0])  # the handle is the caller's again
+        assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], srcs[0]))
+
+
+def test_handle_is_owned_while_serving(gpu):
+    G = _G()
+    with _graph(G) as g:
+        with c2g.Server(g) as s:
+            for call in (lambda: g.sssp(0), lambda: g.sssp_batch([0, 1]), lambda: g.ppr(0),
+                         lambda: g.adsorption([0], [0], L=1)):
+                with pytest.raises(c2g.C2GError) as e:
+                    call()
+                assert e.value.code == c2g.EBUSY
+            info, d = s.poll(s.submit(c2g.Q_SSSP, 5))
+            assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], 5))
+
+
+@pytest.mark.parametrize("lane,slots,kind", [(c2g.SLANE_SLOTS, 0, c2g.Q_SSSP), (c2g.SLANE_GROUP, -1, c2g.Q_SSSP),
+                                             (c2g.SLANE_PPR, 0, c2g.Q_PPR)])
+def test_injected_lane_failure_recovers(gpu, lane, slots, kind):
+    # a device lane's step reports an error (forced from the host; every kernel
+    # still runs normally): its running requests finish on the host lane, the
+    # lane is reset and serves the next requests on the GPU
+    G = _G()
+    deg = np.diff(G["vlist"])
+    srcs = [int(v) for v in np.flatnonzero(deg > 0)[::101][:12]]
+    with _graph(G) as g:
+        with c2g.Server(g, host_lane=True, fail_inject=lane, fail_after=2, sssp_slots=slots) as s:
+            res = []
+            for v in srcs:  # one at a time: a known request runs into the failing step
+                res.append((v,) + s.poll(s.submit(kind, v)))
+            st = s.stats()
+        lanes = [info["lane"] for _, info, _ in res]
+        assert st["lane_failures"] == 1 and st["failed"] == 0 and st["to_host"] == 1
+        assert lanes.count(c2g.LANE_HOST) == 1 and lanes[-1] == c2g.LANE_GPU
+        for v, info, r in res:
+            assert info["status"] == 0
+            if kind == c2g.Q_SSSP:
+                assert np.array_equal(r, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], v)), v
+            else:
+                p, _, _ = g.ppr(v, alpha=0.15, eps=1e-7)
+                assert np.abs(r.astype(np.float64) - p).sum() <= 2e-5, v
 
 
 @pytest.mark.parametrize("delta", [0, 100])
