@@ -5,9 +5,10 @@ Workload (the metric's configuration): directed R-MAT scale 24, edge factor 16,
 int32 weights U[1,1023]; 256 sources = 128 drawn from the top-1% out-degree
 vertices + 128 from out-degree-1 vertices (seeded).  A step is one batched
 traversal of all 256 sources to convergence with the graph resident in HBM.
-Before timing, c2g_autotune picks the launch shapes, bin thresholds (work-item
-chunk sizes) and rounds per hipGraph replay on these sources (single-source
-settings on 16 of them, then the batched ones on all 256).
+Before timing, c2g_autotune picks the launch shapes (workgroups per CU and
+threads per workgroup), bin thresholds (work-item chunk sizes) and rounds per
+hipGraph replay on 256 other sources of the same recipe (seed + 1):
+single-source settings on 16 of them, then the batched ones on all 256.
 
 Reported beside it:
   * per-query: the same 256 sources run one query per launch sequence with the
@@ -131,16 +132,88 @@ def cpu_baseline_ppr(G, srcs, budget_s, threads, alpha, eps):
                       "src/ppr/w-base-perk/PPR.cpp:178-236 (threshold eps*outdeg), one query per thread"}
 
 
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "relax_b_c4_scale24_traffic.json")
+KERNEL_SRC = os.path.join(ROOT, "c-2graph_amd", "csrc", "sssp_batch.hip")
+
+
+def _sha256(path):
+    import hashlib
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
 def measured_traffic(kernel, args):
-    """HBM bytes per launch of `kernel` on this workload from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (scaled by the copy-kernel
-    calibration), or None when no summary for this kernel and workload exists."""
-    path = os.path.join(ROOT, "profiles", "relax_b_c4_scale24_traffic.json")
-    if (args.scale, args.seed, args.queries, args.wmax) != (24, 4, 256, 1023) or not os.path.exists(path):
-        return None
-    with open(path) as f:
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 counter summary, with its provenance.  The summary records the
+    sha256 of the kernel's source file at the counter run; when the current
+    file differs, the figure no longer describes this kernel and traffic is
+    None (the note says why)."""
+    if (args.scale, args.seed, args.queries, args.wmax) != (24, 4, 256, 1023) or not os.path.exists(TRAFFIC_JSON):
+        return None, {"note": "no counter summary for this workload"}
+    with open(TRAFFIC_JSON) as f:
         t = json.load(f)
-    return t["traffic_bytes_per_launch"] if t.get("kernel") == kernel else None
+    if t.get("kernel") != kernel:
+        return None, {"note": f"counter summary is for {t.get('kernel')}"}
+    prov = {"profile": t.get("source"), "kernel_source_sha256": t.get("kernel_source_sha256"),
+            "counters": t.get("counters")}
+    if t.get("kernel_source_sha256") != _sha256(KERNEL_SRC):
+        prov["note"] = "stale: the kernel source changed since the counter run"
+        return None, prov
+    return t["traffic_bytes_per_launch"], prov
+
+
+def ingest_counters(dirs, out_path):
+    """Sums rocprofv3 --pmc CSV outputs (one directory per counter pass) per
+    dispatch of relax_b_k<false> and writes the traffic summary: read bytes
+    = 32*RDREQ_32B + 64*RDREQ_64B + 128*RDREQ_128B, write bytes = 32*(WRREQ -
+    WRREQ_64B) + 64*WRREQ_64B, per launch averaged over the dispatches."""
+    import csv
+    import glob
+    # per[kind][counter] = list of per-dispatch sums (a counter may come in
+    # several rows per dispatch, one per hardware instance)
+    per = {"relax": {}, "copy": {}}
+    for d in dirs:
+        for fn in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            acc = {}
+            with open(fn) as f:
+                for r in csv.DictReader(f):
+                    k = r["Kernel_Name"]
+                    kind = "relax" if "relax_b_k<false>" in k else "copy" if "diag_copy_k" in k else None
+                    if kind:
+                        key = (kind, r["Dispatch_Id"], r["Counter_Name"])
+                        acc[key] = acc.get(key, 0.0) + float(r["Counter_Value"])
+            for (kind, _, cname), v in acc.items():
+                per[kind].setdefault(cname, []).append(v)
+    copy = per["copy"]
+    per = per["relax"]
+
+    def avg(dct, name):
+        v = dct.get(name) or dct.get(name + "_sum")
+        return sum(v) / len(v) if v else None
+
+    def traffic(dct):
+        r32, r64, r128 = (avg(dct, f"TCC_EA0_RDREQ_{b}B") for b in (32, 64, 128))
+        wr, w64 = avg(dct, "TCC_EA0_WRREQ"), avg(dct, "TCC_EA0_WRREQ_64B")
+        rd = 32 * r32 + 64 * r64 + 128 * r128
+        wb = 32 * (wr - w64) + 64 * w64
+        return rd, wb, {"rdreq": avg(dct, "TCC_EA0_RDREQ"), "rdreq_32b": r32, "rdreq_64b": r64, "rdreq_128b": r128,
+                        "wrreq": wr, "wrreq_64b": w64, "bubble": avg(dct, "TCC_BUBBLE"),
+                        "rdreq_dram": avg(dct, "TCC_EA0_RDREQ_DRAM")}
+
+    rd, wb, cnt = traffic(per)
+    out = {"kernel": "relax_b_k<false>", "workload": "R-MAT scale 24 seed 4, 256 sources",
+           "dispatches": len(per.get("TCC_EA0_RDREQ_128B", per.get("TCC_EA0_RDREQ_128B_sum", []))),
+           "read_bytes_per_launch": int(rd), "write_bytes_per_launch": int(wb),
+           "traffic_bytes_per_launch": int(rd + wb), "counters": {k: v for k, v in cnt.items()},
+           "kernel_source_sha256": _sha256(KERNEL_SRC),
+           "source": "profiles/relax_b_c4_scale24_hbm_traffic.md"}
+    if copy:
+        crd, cwb, ccnt = traffic(copy)
+        out["copy_check"] = {"read_bytes_per_dispatch": int(crd), "write_bytes_per_dispatch": int(cwb),
+                             "counters": ccnt}
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    return out
 
 
 def aux_phase(args):
@@ -231,14 +304,57 @@ def ppr_accuracy_phase(args):
     return out
 
 
+def site_graph(scale, nav=8, rnd=1.0, seed=5, mean_site=32, wmax=255):
+    """Web-like directed graph of sites (the shape purn-sssp / purn-ppr were
+    built for; R-MAT keeps 99.8 % of its edges under them).  V = 2^scale pages
+    split into sites of mean size `mean_site`.  Each site's home page links to
+    every page of the site and to `nav` navigation targets (other homes, or
+    pages of its own site); every page links to the same navigation targets and
+    to Poisson(`rnd`) random pages.  A page reached only from its home has
+    in-degree 1 and its navigation links duplicate the home's: pruning bypasses
+    it without adding edges.  Self-loops and duplicates are removed, rows
+    sorted; int weights U[1,wmax] and row-normalised float weights U(0,1]."""
+    V = 1 << scale
+    rng = np.random.default_rng(seed)
+    nsite = max(1, V // mean_site)
+    cuts = np.sort(rng.choice(np.arange(1, V), nsite - 1, replace=False))
+    starts = np.concatenate([[0], cuts])
+    ends = np.concatenate([cuts, [V]])
+    site = np.repeat(np.arange(nsite), ends - starts)
+    pages = np.arange(V)
+    nh = pages[pages != starts[site]]
+    navt = np.where(rng.random((nsite, nav)) < 0.5, starts[rng.integers(0, nsite, (nsite, nav))],
+                    starts[:, None] + (rng.random((nsite, nav)) * (ends - starts)[:, None]).astype(np.int64))
+    k = rng.poisson(rnd, V)
+    src = np.repeat(pages, k)
+    u = np.concatenate([starts[site[nh]], np.repeat(starts, nav), np.repeat(nh, nav), src])
+    v = np.concatenate([nh, navt.ravel(), navt[site[nh]].ravel(), rng.integers(0, V, len(src))])
+    keep = u != v
+    key = np.unique(u[keep].astype(np.int64) * V + v[keep])
+    u, v = key // V, key % V
+    vl = np.concatenate([[0], np.cumsum(np.bincount(u, minlength=V))]).astype(np.int32)
+    w = rng.integers(1, wmax + 1, len(v)).astype(np.int32)
+    f = rng.random(len(v)) + 1e-3
+    f = (f / np.bincount(u, weights=f, minlength=V)[u]).astype(np.float32)
+    return {"V": V, "E": len(v), "vlist": vl, "elist": v.astype(np.int32), "wlist": w, "flist": f}
+
+
 def ppr_purn_study(args):
     """Approximation of the pruned PPR graph (purn-ppr's additive 0.8*w
     compensation, src/purn-ppr.cpp:243-259, with the ppr-purn seeding
     H(s) U N'(s)): PPR on the pruned graph against PPR on the full graph, both
-    on the GPU (alpha 0.15, eps 1e-7*outdeg), next to the kept-edge ratio."""
+    on the GPU (alpha 0.15, eps 1e-7*outdeg), next to the kept-edge ratio.  On
+    the site graph (default) the pruning keeps about 0.74 of the edges, near
+    the reference's 0.721 on web-BerkStan; the same graph's purn-sssp output is
+    also run (pruned SSSP is exact: checksums must equal the full graph's)."""
     import c2g
 
-    G = c2g.rmat(args.study_scale, 16, 0.57, 0.19, 0.19, seed=5, wmax=255, want_f=True)
+    if args.study_graph == "rmat":
+        G = c2g.rmat(args.study_scale, 16, 0.57, 0.19, 0.19, seed=5, wmax=255, want_f=True)
+        gname = f"R-MAT scale {args.study_scale} seed 5 directed"
+    else:
+        G = site_graph(args.study_scale, seed=5)
+        gname = f"site graph scale {args.study_scale} seed 5 (nav 8, random 1.0, mean site 32)"
     t = time.time()
     P = c2g.prune_ppr_mem(G["vlist"], G["elist"], G["flist"])
     t_prune = time.time() - t
@@ -261,15 +377,40 @@ def ppr_purn_study(args):
         l1.append(float(d.sum()))
         mx.append(float(d.max()))
         hidden_src += int(P["hvlist"][s_ + 1] > P["hvlist"][s_])
+    sr = [float(gf.ppr(int(s_))[1].astype(np.float64).sum()) for s_ in srcs[:8]]  # the push's own truncation
     gf.close()
     gp.close()
-    return {"workload": f"R-MAT scale {args.study_scale} seed 5 directed (V={G['V']}, E={G['E']}), 32 sources, "
-                        "alpha 0.15, eps 1e-7*outdeg",
-            "kept_edge_ratio": round(len(P["elist"]) / G["E"], 4), "hidden_edges": int(len(P["helist"])),
-            "prune_s": round(t_prune, 1), "sources_with_hidden_list": hidden_src,
-            "l1_pruned_vs_full": {"mean": float(np.mean(l1)), "max": float(np.max(l1))},
-            "maxabs_pruned_vs_full": float(np.max(mx)),
-            "ms_per_query": {"full": round(tf / len(srcs), 3), "pruned": round(tp / len(srcs), 3)}}
+    out = {"workload": f"{gname} (V={G['V']}, E={G['E']}), 32 sources, alpha 0.15, eps 1e-7*outdeg",
+           "kept_edge_ratio": round(len(P["elist"]) / G["E"], 4), "hidden_edges": int(len(P["helist"])),
+           "prune_s": round(t_prune, 1), "sources_with_hidden_list": hidden_src,
+           "l1_pruned_vs_full": {"mean": float(np.mean(l1)), "max": float(np.max(l1))},
+           "maxabs_pruned_vs_full": float(np.max(mx)), "push_sum_r_full_mean": float(np.mean(sr)),
+           "ms_per_query": {"full": round(tf / len(srcs), 3), "pruned": round(tp / len(srcs), 3)}}
+    # purn-sssp on the same graph: pruned SSSP must equal the full graph's
+    Q = c2g.prune_sssp_mem(G["vlist"], G["elist"], G["wlist"])
+    gf = c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], device=0)
+    gq = c2g.Graph.from_csr(Q["vlist"], Q["elist"], Q["wlist"], hvlist=Q["hvlist"], helist=Q["helist"],
+                            hwlist=Q["hwlist"], variant=c2g.VARIANT_SSSP_PURN, device=0)
+    bs = pick_sources(G["vlist"], 256, seed=5)
+    res = {}
+    for name, g in (("full", gf), ("pruned", gq)):
+        g.sssp_each_resident(srcs[:2])
+        t = time.perf_counter()
+        se, re_, _ = g.sssp_each_resident(srcs)
+        te = time.perf_counter() - t
+        g.sssp_batch_resident(bs)
+        t = time.perf_counter()
+        sb, rb, _ = g.sssp_batch_resident(bs)
+        tb = time.perf_counter() - t
+        res[name] = (se, re_, sb, rb, te, tb)
+    gf.close()
+    gq.close()
+    out["sssp"] = {
+        "kept_edge_ratio": round(len(Q["elist"]) / G["E"], 4), "hidden_edges": int(len(Q["helist"])),
+        "distances_equal": bool(all(np.array_equal(a, b) for a, b in zip(res["full"][:4], res["pruned"][:4]))),
+        "ms_per_query": {k: round(v[4] / len(srcs) * 1e3, 3) for k, v in res.items()},
+        "batched_256_ms": {k: round(v[5] * 1e3, 1) for k, v in res.items()}}
+    return out
 
 
 def c2_phase(args):
@@ -448,6 +589,9 @@ def main():
     ap.add_argument("--ppr-purn-study", action="store_true",
                     help="only the pruned-PPR approximation study (PPR on the purn-ppr graph vs the full graph)")
     ap.add_argument("--study-scale", type=int, default=20)
+    ap.add_argument("--study-graph", choices=["sites", "rmat"], default="sites",
+                    help="graph of the ppr-purn study: the web-like site graph (pruning keeps ~0.74 of the "
+                         "edges) or directed R-MAT (keeps 0.998)")
     ap.add_argument("--ppr-accuracy", action="store_true",
                     help="only single-source PPR on the C3 graph at eps 1e-13*outdeg, push-only vs push/pull")
     ap.add_argument("--c2", action="store_true", help="only the pruned-graph SSSP workload (scale 22 seed 2)")
@@ -456,7 +600,17 @@ def main():
     ap.add_argument("--aux-only", action="store_true", help="only the PPR and adsorption timings")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the profiled pass (for rocprofv3: it then sees exactly those launches)")
+    ap.add_argument("--set-tune", default="", metavar="K=V,...",
+                    help="pin tunables (e.g. grid_per_cu=16,block=256) instead of running the autotuner, so a "
+                         "profiled run repeats the timed run's launch shapes")
+    ap.add_argument("--ingest-counters", nargs="+", metavar="DIR",
+                    help="summarise rocprofv3 --pmc CSV directories of --profile-only (and --copy-calib) "
+                         "runs into profiles/relax_b_c4_scale24_traffic.json, then exit")
     args = ap.parse_args()
+
+    if args.ingest_counters:
+        print(json.dumps(ingest_counters(args.ingest_counters, TRAFFIC_JSON)), flush=True)
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -501,13 +655,18 @@ def main():
     g = c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], device=0)
     log(f"uploaded in {time.time() - t:.1f}s; tune={g.tune().as_dict()}")
     tune_s = None
-    if not args.no_autotune and not args.profile_only:
+    if args.set_tune:
+        g.set_tune(**{k: int(v) for k, v in (kv.split("=") for kv in args.set_tune.split(","))})
+        log(f"tunables pinned: {g.tune().as_dict()}")
+    elif not args.no_autotune and not args.profile_only:
         # the workload's launch shapes and bin thresholds come from the
-        # autotuner: single-source settings on 16 of the sources, then the
-        # batched ones on all of them (which also settle rounds per replay)
+        # autotuner, on a sample of the same recipe drawn with another seed
+        # (not the timed sources): single-source settings on 16 sources, then
+        # the batched ones on all 256 (which also settle rounds per replay)
         t = time.time()
-        g.autotune(srcs[:16], batched=False)
-        g.autotune(srcs, batched=True)
+        tune_srcs = pick_sources(G["vlist"], args.queries, seed=args.seed + 1)
+        g.autotune(tune_srcs[:16], batched=False)
+        g.autotune(tune_srcs, batched=True)
         tune_s = round(time.time() - t, 1)
         log(f"autotuned in {tune_s}s: {g.tune().as_dict()}")
     tune_used = {k: v for k, v in g.tune().as_dict().items() if k != "profile"}
@@ -585,13 +744,16 @@ def main():
         g.set_tune(profile=0)
         algo_bytes = 8 * sp.csr_edges + 8 * sp.frontier_total + 4 * sp.edges_relaxed + 4 * sp.improvements
         achieved = algo_bytes / (sp.kernel_ms * 1e-3) / 1e9
-        traffic = measured_traffic("relax_b_k<false>", args)
+        traffic, prov = measured_traffic("relax_b_k<false>", args)
+        algo_per_launch = algo_bytes / max(1, sp.iterations)
         extra["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-            "frac": round(achieved / 8000.0, 4), "traffic": traffic,
+            "frac": round(achieved / 8000.0, 4), "traffic": traffic, "traffic_source": prov,
+            "traffic_over_algorithmic": round(traffic / algo_per_launch, 3) if traffic else None,
             "kernel": "relax_b_k<false>", "launches": sp.iterations,
             "avg_launch_us": round(sp.kernel_ms * 1e3 / max(1, sp.iterations), 2),
-            "algo_bytes_per_launch": int(algo_bytes / max(1, sp.iterations)),
+            "algo_bytes_per_launch": int(algo_per_launch),
+            "lane_relaxations_per_reached_edge": round(sp.edges_relaxed / max(1, sp.reached_edges), 3),
             "counters": {"csr_edges": sp.csr_edges, "vertex_group_activations": sp.frontier_total,
                          "query_edge_relaxations": sp.edges_relaxed, "improvements": sp.improvements,
                          "items": sp.items, "rounds": sp.iterations},
@@ -610,10 +772,11 @@ def main():
         g.set_tune(profile=0)
 
     if not args.skip_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = len(os.sched_getaffinity(0))  # every host core this process may use
         cb = cpu_baseline(G, srcs, args.cpu_seconds, threads)
         extra["cpu_baseline"] = {
             "value": round(cb["n"] / cb["wall"], 4), "unit": "queries/s", "cores": threads, "kind": "port",
+            "host_cores": os.cpu_count(),
             "gteps": round(cb["gteps"], 4), "single_thread_s_per_query": round(cb["single_s"], 2),
             "sample": f"{cb['n']} of the 256 sources (hub/out-degree-1 interleaved), frontier Bellman-Ford "
                       f"restated from src/sssp/base-perk/SSSP.cpp:88-119, one query per thread, "

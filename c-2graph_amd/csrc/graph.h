@@ -82,7 +82,7 @@ struct SsspWs {
     int64_t* h_ctrl = nullptr;   // pinned mirror
     uint8_t* hid = nullptr;      // [V] 1 = distance was set through a hidden edge
     hipGraphExec_t exec = nullptr;
-    int exec_iters = 0, exec_grid = 0, exec_chunk = 0;
+    int exec_iters = 0, exec_grid = 0, exec_chunk = 0, exec_tb = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
     HostStage out;               // distance copies to the host
 };
@@ -96,7 +96,7 @@ struct BatchWs {
     int64_t* ctrl = nullptr;
     int64_t* h_ctrl = nullptr;
     hipGraphExec_t exec = nullptr;
-    int exec_iters = 0, exec_block = 0, exec_grid = 0, exec_chunk = 0;
+    int exec_iters = 0, exec_groups = 0, exec_grid = 0, exec_chunk = 0, exec_tb = 0;
     int groups = 0;              // number of groups allocated
 };
 

@@ -461,8 +461,10 @@ int c2g_graph_host_csr(const c2g_graph* h, const int32_t** vl, const int32_t** e
 
 int c2g_set_tune(c2g_graph* h, const c2g_tune* t) {
     if (!h || !t) return fail(C2G_EINVAL, "null argument");
-    if (t->block != 64 && t->block != 128 && t->block != 256 && t->block != 512 && t->block != 1024)
-        return fail(C2G_EINVAL, "block must be one of 64..1024 (power of two)");
+    if (t->block != 128 && t->block != 256 && t->block != 512)
+        return fail(C2G_EINVAL, "block (threads per workgroup of the relax kernels) must be 128, 256 or 512");
+    if (t->block_single != 0 && t->block_single != 128 && t->block_single != 256 && t->block_single != 512)
+        return fail(C2G_EINVAL, "block_single must be 0, 128, 256 or 512");
     if (t->grid_per_cu < 1 || t->grid_per_cu > 64) return fail(C2G_EINVAL, "grid_per_cu in [1,64]");
     if (t->grid_per_cu_single < 0 || t->grid_per_cu_single > 64)
         return fail(C2G_EINVAL, "grid_per_cu_single in [0,64]");

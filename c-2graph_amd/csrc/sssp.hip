@@ -37,6 +37,7 @@
 #include "graph.h"
 #include "wave.h"
 #include <cstring>
+#include <type_traits>
 
 namespace c2g {
 
@@ -113,8 +114,9 @@ __global__ void sssp_seed_k(const int32_t* __restrict__ row, const int32_t* __re
 // Items are int4(u, e_begin, e_end, kind | slot << 1): a single query is
 // slot 0; a slot pool runs several queries side by side, slot s owning
 // dist[s*V..], hid[s*V..] and the bitmap words [s*nwp..] of each map.
-template <bool PRUNED>
-__global__ void __launch_bounds__(256)
+// BLOCK: threads per workgroup (128, 256 or 512; c2g_tune.block).
+template <bool PRUNED, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
 sssp_relax_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
              int32_t* __restrict__ dist, uint32_t* __restrict__ bits,
              uint32_t* __restrict__ hbits, uint8_t* __restrict__ hid,
@@ -416,6 +418,27 @@ static SsspBufs ws_bufs(Graph* g) {
     return SsspBufs{w.dist, w.bits, w.hid, w.q[0], w.qcap, w.ctrl};
 }
 
+// One relax launch with the tuned workgroup size; hbits/hid null on full
+// graphs.  Items may belong to several slots (slot pools).
+static void launch_relax(Graph* g, hipStream_t strm, int grid, int32_t* dist, uint32_t* bits, uint32_t* hbits,
+                         uint8_t* hid, const int4* q, int64_t* ctrl, int64_t nwp) {
+    auto go = [&](auto pruned, auto block) {
+        constexpr bool P = decltype(pruned)::value;
+        constexpr int B = decltype(block)::value;
+        hipLaunchKernelGGL((sssp_relax_k<P, B>), dim3(grid), dim3(B), 0, strm, g->d_adj,
+                           P ? g->d_hadj : (const int2*)nullptr, dist, bits, hbits, hid, q, ctrl, (int64_t)g->V, nwp);
+    };
+    auto tb = [&](auto pruned) {
+        switch (g->tune.block_single > 0 ? g->tune.block_single : g->tune.block) {
+            case 128: go(pruned, std::integral_constant<int, 128>()); break;
+            case 512: go(pruned, std::integral_constant<int, 512>()); break;
+            default: go(pruned, std::integral_constant<int, 256>()); break;
+        }
+    };
+    if (hbits) tb(std::true_type());
+    else tb(std::false_type());
+}
+
 // Launches k rounds (relax + expand) on `b`.  With relax_ms non-null
 // (profiling), the workspace's HIP events bracket each relax kernel and the
 // time is accumulated.
@@ -429,18 +452,14 @@ static int launch_rounds_on(Graph* g, const SsspBufs& b, int k, double* relax_ms
     const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (nwp + 255) / 256));
     for (int r = 0; r < k; ++r) {
         if (relax_ms) HIP_TRY(hipEventRecord(w.ev[0], g->stream));
+        launch_relax(g, g->stream, grid, b.dist, bits, pruned ? hbits : nullptr, pruned ? b.hid : nullptr, b.q, b.ctrl,
+                     nwp);
+        if (relax_ms) HIP_TRY(hipEventRecord(w.ev[1], g->stream));
         if (pruned) {
-            hipLaunchKernelGGL(sssp_relax_k<true>, dim3(grid), dim3(256), 0, g->stream, g->d_adj,
-                               g->d_hadj, b.dist, bits, hbits, b.hid, b.q, b.ctrl, (int64_t)g->V, nwp);
-            if (relax_ms) HIP_TRY(hipEventRecord(w.ev[1], g->stream));
             hipLaunchKernelGGL(sssp_expand_k<true>, dim3(egrid), dim3(256), 0, g->stream, g->d_row,
                                g->d_hrow, bits, hbits, b.q, b.ctrl, (int64_t)g->V, g->tune.chunk, b.qcap,
                                (const int32_t*)b.dist, g->tune.delta, 1, nwp);
         } else {
-            hipLaunchKernelGGL(sssp_relax_k<false>, dim3(grid), dim3(256), 0, g->stream, g->d_adj,
-                               (const int2*)nullptr, b.dist, bits, (uint32_t*)nullptr, (uint8_t*)nullptr,
-                               b.q, b.ctrl, (int64_t)g->V, nwp);
-            if (relax_ms) HIP_TRY(hipEventRecord(w.ev[1], g->stream));
             hipLaunchKernelGGL(sssp_expand_k<false>, dim3(egrid), dim3(256), 0, g->stream, g->d_row,
                                (const int32_t*)nullptr, bits, (uint32_t*)nullptr, b.q, b.ctrl,
                                (int64_t)g->V, g->tune.chunk, b.qcap, (const int32_t*)b.dist, g->tune.delta, 1, nwp);
@@ -477,7 +496,8 @@ static void launch_seed(Graph* g, const SsspBufs& b, int32_t src) {
 static int ensure_exec(Graph* g) {
     SsspWs& w = g->sssp;
     const int K = g->tune.iters_per_graph;
-    if (w.exec && w.exec_iters == K && w.exec_grid == relax_grid(g) && w.exec_chunk == g->tune.chunk)
+    if (w.exec && w.exec_iters == K && w.exec_grid == relax_grid(g) && w.exec_chunk == g->tune.chunk &&
+        w.exec_tb == (g->tune.block_single > 0 ? g->tune.block_single : g->tune.block))
         return C2G_OK;
     if (w.exec) { (void)hipGraphExecDestroy(w.exec); w.exec = nullptr; }
     hipGraph_t graph;
@@ -498,6 +518,7 @@ static int ensure_exec(Graph* g) {
     w.exec_iters = K;
     w.exec_grid = relax_grid(g);
     w.exec_chunk = g->tune.chunk;
+    w.exec_tb = g->tune.block_single > 0 ? g->tune.block_single : g->tune.block;
     return C2G_OK;
 }
 
@@ -759,15 +780,13 @@ int qserve_step(Graph* g, hipStream_t strm, uint64_t* slots_with_work) {
     const int64_t words = (int64_t)s->slots * s->nwp;
     const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (words + 255) / 256));
     uint32_t* hb = s->bits + (int64_t)s->slots * s->nwp;
+    launch_relax(g, strm, grid, s->dist, s->bits, pruned ? hb : nullptr, pruned ? s->hid : nullptr, s->q, s->ctrl,
+                 s->nwp);
     if (pruned) {
-        hipLaunchKernelGGL(sssp_relax_k<true>, dim3(grid), dim3(256), 0, strm, g->d_adj, g->d_hadj, s->dist,
-                           s->bits, hb, s->hid, s->q, s->ctrl, V, s->nwp);
         hipLaunchKernelGGL(sssp_expand_k<true>, dim3(egrid), dim3(256), 0, strm, g->d_row, g->d_hrow, s->bits, hb,
                            s->q, s->ctrl, V, std::max(1, g->tune.chunk), s->qcap, (const int32_t*)s->dist, 0,
                            s->slots, s->nwp);
     } else {
-        hipLaunchKernelGGL(sssp_relax_k<false>, dim3(grid), dim3(256), 0, strm, g->d_adj, (const int2*)nullptr,
-                           s->dist, s->bits, (uint32_t*)nullptr, (uint8_t*)nullptr, s->q, s->ctrl, V, s->nwp);
         hipLaunchKernelGGL(sssp_expand_k<false>, dim3(egrid), dim3(256), 0, strm, g->d_row, (const int32_t*)nullptr,
                            s->bits, (uint32_t*)nullptr, s->q, s->ctrl, V, std::max(1, g->tune.chunk), s->qcap,
                            (const int32_t*)s->dist, 0, s->slots, s->nwp);

@@ -13,15 +13,19 @@
 //                            queries is one coalesced 256-byte row access
 //   mask  [g][V]     u64     lanes of v improved this round, not yet expanded
 //   hmask [g][V]     u64     lanes whose value at v came through a hidden edge
-// Per round (two kernels, captured in a hipGraph):
-//   relax_b  : each wave takes one item (u, e_begin, e_end, g|kind) with its
-//              lane mask; edges are loaded 64 at a time lane-parallel, then
-//              walked in groups of 8 (8 independent row loads in flight);
-//              improved lanes are OR-ed into mask[g][v] lane-parallel and
-//              (v,g) is marked in the group's activation bitmap.
+// Per round (kernels captured in a hipGraph):
+//   relax_b  : each wave takes one item (u, e_begin, e_end, g) with its lane
+//              mask; edges are loaded 64 at a time lane-parallel, then walked
+//              in groups of 8 (8 independent row loads in flight); improved
+//              lanes are OR-ed into mask[g][v] lane-parallel and (v,g) is
+//              marked in the group's activation bitmap.  Pruned graphs run the
+//              same kernel twice per round: over the hidden-list items first
+//              (edges from hadj, improved lanes also recorded in hmask), then
+//              over the live items with exactly the full graph's code.
 //   expand_b : scans the bitmaps; for each active (v,g) takes mask[g][v] (clearing it),
-//              splits v's row into items of <= batch_chunk edges, and adds a
-//              hidden-list item for lanes in hmask (pruned graphs).
+//              splits v's row into items of <= batch_chunk edges, and writes a
+//              hidden-list item for the lanes in hmask to the hidden queue
+//              (pruned graphs).
 // Vertices with neither out-edges nor hidden edges are never activated (flag
 // bit 31 of the edge target), which skips about half of all activations on
 // R-MAT graphs.  Activation is recorded in a per-group bitmap with
@@ -29,6 +33,7 @@
 #include "graph.h"
 #include "wave.h"
 #include <cstring>
+#include <type_traits>
 
 namespace c2g {
 
@@ -46,6 +51,8 @@ enum : int {
     B_ITEMS_TOT = 10,  // items processed
     B_LANEOR = 11,     // server lane: lanes of group 0 with pending work after expand
     B_FAR = 12,        // 1 while some lane has deferred (far) work
+    B_HITEMS = 13,     // hidden-list items for the next relax round (pruned graphs)
+    B_DONE_H = 14,     // workgroups finished (hidden relax)
     B_N = 16
 };
 
@@ -132,16 +139,19 @@ __device__ __forceinline__ void relax_step_b(const int2 my, int j0, int ne, int 
     }
 }
 
-template <bool PRUNED>
-__global__ void __launch_bounds__(256)
-relax_b_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
-          int32_t* __restrict__ dist, uint64_t* __restrict__ mask, uint64_t* __restrict__ hmask,
-          const int4* __restrict__ items, const uint64_t* __restrict__ imask,
+// HIDDEN: the items are hidden-list items (pruned graphs): edges come from
+// `adj` = hadj and every improved lane is also recorded in hmask.  The live
+// items of any graph run with HIDDEN = false, which never touches hmask.
+// BLOCK: threads per workgroup (128, 256 or 512; c2g_tune.block).
+template <bool HIDDEN, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+relax_b_k(const int2* __restrict__ adj, int32_t* __restrict__ dist, uint64_t* __restrict__ mask,
+          uint64_t* __restrict__ hmask, const int4* __restrict__ items, const uint64_t* __restrict__ imask,
           uint32_t* __restrict__ abits, int64_t* __restrict__ ctrl, int64_t V, int64_t nw) {
     // improved-lane mask of each edge of the current 64-edge chunk, per wave
     // (a last step of kJ edges may run up to kJ-1 entries past the chunk)
-    __shared__ uint64_t s_bits[4][64 + kJ];
-    const int64_t n = ctrl[B_ITEMS];
+    __shared__ uint64_t s_bits[BLOCK / 64][64 + kJ];
+    const int64_t n = ctrl[HIDDEN ? B_HITEMS : B_ITEMS];
     const int lane = lane_id();
     // uniform wave index: item headers come through scalar loads, and each
     // target row's base address is an SGPR pair
@@ -155,15 +165,13 @@ relax_b_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
         const int4 it = items[i];
         const uint64_t m = imask[i];
         const int u = it.x;
-        const int g = it.w & 0xffff;
-        const bool hid_item = PRUNED && (it.w >> 16);
+        const int g = it.w;
         int32_t* __restrict__ dG = dist + (int64_t)g * V * 64;
         const bool act = (m >> lane) & 1ull;
-        const int2* __restrict__ A = hid_item ? hadj : adj;
         for (int e0 = it.y; e0 < it.z; e0 += 64) {
             const int ne = min(64, it.z - e0);
             int2 my = make_int2(0, 0);
-            if (lane < ne) my = A[e0 + lane];
+            if (lane < ne) my = adj[e0 + lane];
             // The item's pending lanes relax the chunk's edges kJ at a time.
             if (act) {
                 const uint32_t du = (uint32_t)dG[(int64_t)u * 64 + lane];
@@ -179,7 +187,7 @@ relax_b_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
                 impr += __popcll(mybits);
                 if (mybits) {
                     const int v = my.x & kIdMaskB;
-                    if (PRUNED && hid_item)
+                    if (HIDDEN)
                         atomicOr((unsigned long long*)&hmask[(int64_t)g * V + v], (unsigned long long)mybits);
                     if (my.x < 0) {
                         atomicOr((unsigned long long*)&mask[(int64_t)g * V + v], (unsigned long long)mybits);
@@ -194,13 +202,15 @@ relax_b_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
-        unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[B_DONE_R], 1ull);
+        unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[HIDDEN ? B_DONE_H : B_DONE_R], 1ull);
         if (prev == (unsigned long long)gridDim.x - 1) {
             __threadfence();
-            if (n > 0) ctrl[B_ITER] += 1;
+            // a round counts once: the hidden pass runs first and counts only
+            // rounds whose live queue is empty
+            if (HIDDEN ? (n > 0 && ctrl[B_ITEMS] == 0) : n > 0) ctrl[B_ITER] += 1;
             ctrl[B_ITEMS_TOT] += n;
-            ctrl[B_ITEMS] = 0;
-            ctrl[B_DONE_R] = 0;
+            ctrl[HIDDEN ? B_HITEMS : B_ITEMS] = 0;
+            ctrl[HIDDEN ? B_DONE_H : B_DONE_R] = 0;
         }
     }
 }
@@ -217,8 +227,9 @@ relax_b_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
 __global__ void __launch_bounds__(256)
 expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, uint64_t* __restrict__ mask,
            const uint64_t* __restrict__ hmask, uint32_t* __restrict__ abits, int4* __restrict__ items,
-           uint64_t* __restrict__ imask, int64_t* __restrict__ ctrl, int64_t V, int64_t nw, int groups,
-           int chunk, int64_t itemcap, const int32_t* __restrict__ dist, int32_t* __restrict__ thr,
+           uint64_t* __restrict__ imask, int4* __restrict__ hitems, uint64_t* __restrict__ himask,
+           int64_t* __restrict__ ctrl, int64_t V, int64_t nw, int groups,
+           int chunk, int64_t itemcap, int64_t hitemcap, const int32_t* __restrict__ dist, int32_t* __restrict__ thr,
            int32_t* __restrict__ fmin, uint64_t* __restrict__ lor, int delta) {
     extern __shared__ int32_t lds_fmin[];  // [groups * 64] when delta > 0
     const int lane = lane_id();
@@ -267,7 +278,7 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
             return near;
         };
         uint64_t nor = 0, forr = 0;
-        int cnt = 0;
+        int cnt = 0, hcnt = 0;
         for (uint64_t rest = nzw; rest;) {
             const int k0 = __builtin_ctzll(rest);
             rest &= rest - 1;
@@ -289,7 +300,7 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
                 const uint64_t hm = near & hmask[(int64_t)g * V + v];
                 if (hm) {
                     const int h0 = hrow[v], h1 = hrow[v + 1];
-                    cnt += (h1 - h0 + chunk - 1) / chunk;
+                    hcnt += (h1 - h0 + chunk - 1) / chunk;
                     edges += h1 - h0;
                     ledges += (int64_t)__popcll(hm) * (h1 - h0);
                 }
@@ -305,7 +316,20 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
             if (lane == 0) base = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ITEMS], (unsigned long long)tot);
             base = shfl64(base, 0);
         }
-        const bool room = base + tot <= itemcap;
+        bool room = base + tot <= itemcap;
+        int64_t hp = 0;  // this lane's first hidden item
+        if (hrow) {
+            int htot;
+            const int hoff = wave_excl_scan(hcnt, &htot);
+            int64_t hbase = 0;
+            if (htot > 0) {
+                if (lane == 0)
+                    hbase = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_HITEMS], (unsigned long long)htot);
+                hbase = shfl64(hbase, 0);
+            }
+            room = room && hbase + htot <= hitemcap;
+            hp = hbase + hoff;
+        }
         if (!room && lane == 0) atomicOr((unsigned long long*)&ctrl[B_OVF], 1ull);
         int64_t p = base + off;
         for (uint64_t rest = nzw; rest;) {
@@ -335,9 +359,9 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
                         const uint64_t hm = near & hmask[(int64_t)g * V + v];
                         if (hm) {
                             const int h0 = hrow[v], h1 = hrow[v + 1];
-                            for (int e = h0; e < h1; e += chunk, ++p) {
-                                items[p] = make_int4(v, e, min(e + chunk, h1), g | (1 << 16));
-                                imask[p] = hm;
+                            for (int e = h0; e < h1; e += chunk, ++hp) {
+                                hitems[hp] = make_int4(v, e, min(e + chunk, h1), g);
+                                himask[hp] = hm;
                             }
                         }
                     }
@@ -404,7 +428,8 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
         for (int gg = 0; gg < groups; ++gg) lor[2 * gg] = lor[2 * gg + 1] = 0;
         ctrl[B_DONE_E] = 0;
         int64_t ni = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ITEMS], 0ull);
-        if (ni > itemcap) { ctrl[B_OVF] = 1; ctrl[B_ITEMS] = 0; }
+        int64_t nh = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_HITEMS], 0ull);
+        if (ni > itemcap || nh > hitemcap) { ctrl[B_OVF] = 1; ctrl[B_ITEMS] = 0; ctrl[B_HITEMS] = 0; }
     }
 }
 
@@ -460,6 +485,9 @@ struct BatchState {
     uint64_t* lor = nullptr;    // [groups][2] lanes with near / far work this round
     int4* items = nullptr;
     uint64_t* imask = nullptr;
+    int4* hitems = nullptr;     // hidden-list items (pruned graphs)
+    uint64_t* himask = nullptr;
+    int64_t hitemcap = 0;
     int32_t* srcs = nullptr;
     int64_t* sums = nullptr;    // [64*groups]
     int32_t* reach = nullptr;   // [64*groups]
@@ -489,6 +517,7 @@ void batch_release(Graph* g) {
             (void)hipFree(s->thr); (void)hipFree(s->fmin); (void)hipFree(s->lor);
             (void)hipFree(s->imask); (void)hipFree(s->srcs); (void)hipFree(s->sums);
             (void)hipFree(s->reach); (void)hipFree(s->medges);
+            (void)hipFree(s->hitems); (void)hipFree(s->himask);
             for (auto& e : s->ev) if (e) (void)hipEventDestroy(e);
             s->cols.release();
             delete s;
@@ -512,15 +541,19 @@ static int ensure_batch(Graph* g, int groups) {
     }
     if (!w.h_ctrl) HIP_TRY(hipHostMalloc((void**)&w.h_ctrl, sizeof(int64_t) * B_N, hipHostMallocDefault));
     // Each (v,g) is activated at most once per round: at most
-    // ceil(deg/chunk) + ceil(hdeg/chunk) <= 2 + (deg + hdeg)/chunk items.
-    const int64_t needi = groups * (2 * V + (g->E + g->HE) / chunk) + 64;
-    if (groups > s->groups || s->itemcap < needi) {
+    // ceil(deg/chunk) <= 1 + deg/chunk live items and as many hidden ones.
+    const int64_t needi = groups * (V + g->E / chunk) + 64;
+    const bool pruned = g->d_hrow != nullptr;
+    const int64_t needh = pruned ? groups * (V + g->HE / chunk) + 64 : 0;
+    if (groups > s->groups || s->itemcap < needi || s->hitemcap < needh) {
         if (w.exec) { (void)hipGraphExecDestroy(w.exec); w.exec = nullptr; }
         const int G = std::max(groups, s->groups);
         (void)hipFree(w.dist); (void)hipFree(w.mask[0]); (void)hipFree(w.hmask);
         (void)hipFree(s->abits); (void)hipFree(s->items); (void)hipFree(s->imask); (void)hipFree(s->srcs);
         (void)hipFree(s->sums); (void)hipFree(s->reach); (void)hipFree(s->medges);
         (void)hipFree(s->thr); (void)hipFree(s->fmin); (void)hipFree(s->lor);
+        (void)hipFree(s->hitems); (void)hipFree(s->himask);
+        s->hitems = nullptr; s->himask = nullptr; s->hitemcap = 0;
         w.dist = nullptr; w.mask[0] = nullptr; w.hmask = nullptr;
         s->abits = nullptr; s->items = nullptr; s->imask = nullptr; s->srcs = nullptr;
         s->sums = nullptr; s->reach = nullptr; s->medges = nullptr;
@@ -535,11 +568,18 @@ static int ensure_batch(Graph* g, int groups) {
         HIP_TRY(hipMalloc(&w.dist, sizeof(int32_t) * (size_t)G * V * 64 + 16));
         HIP_TRY(hipMalloc(&w.mask[0], sizeof(uint64_t) * (size_t)G * V + 8));
         if (g->d_hrow) HIP_TRY(hipMalloc(&w.hmask, sizeof(uint64_t) * (size_t)G * V + 8));
-        s->itemcap = G * (2 * V + (g->E + g->HE) / chunk) + 64;
+        const int64_t itemcap = G * (V + g->E / chunk) + 64;
         HIP_TRY(hipMalloc(&s->abits, sizeof(uint32_t) * s->nw * G));
         HIP_TRY(hipMemset(s->abits, 0, sizeof(uint32_t) * s->nw * G));
-        HIP_TRY(hipMalloc(&s->items, sizeof(int4) * s->itemcap));
-        HIP_TRY(hipMalloc(&s->imask, sizeof(uint64_t) * s->itemcap));
+        HIP_TRY(hipMalloc(&s->items, sizeof(int4) * itemcap));
+        HIP_TRY(hipMalloc(&s->imask, sizeof(uint64_t) * itemcap));
+        if (pruned) {
+            const int64_t hcap = G * (V + g->HE / chunk) + 64;
+            HIP_TRY(hipMalloc(&s->hitems, sizeof(int4) * hcap));
+            HIP_TRY(hipMalloc(&s->himask, sizeof(uint64_t) * hcap));
+            s->hitemcap = hcap;
+        }
+        s->itemcap = itemcap;
         HIP_TRY(hipMalloc(&s->srcs, sizeof(int32_t) * 64 * G));
         HIP_TRY(hipMalloc(&s->sums, sizeof(int64_t) * 64 * G));
         HIP_TRY(hipMalloc(&s->reach, sizeof(int32_t) * 64 * G));
@@ -563,9 +603,9 @@ static int expand_b(Graph* g, int groups, hipStream_t strm) {
     const size_t lds = delta > 0 ? sizeof(int32_t) * 64 * groups : 0;
     hipLaunchKernelGGL(expand_b_k, dim3(egrid), dim3(256), lds, strm, g->d_row,
                        pruned ? g->d_hrow : (const int32_t*)nullptr, w.mask[0],
-                       pruned ? w.hmask : (const uint64_t*)nullptr, s->abits, s->items, s->imask, w.ctrl,
-                       (int64_t)g->V, s->nw, groups, g->tune.batch_chunk, s->itemcap, w.dist, s->thr, s->fmin,
-                       s->lor, delta);
+                       pruned ? w.hmask : (const uint64_t*)nullptr, s->abits, s->items, s->imask, s->hitems,
+                       s->himask, w.ctrl, (int64_t)g->V, s->nw, groups, g->tune.batch_chunk, s->itemcap,
+                       s->hitemcap, w.dist, s->thr, s->fmin, s->lor, delta);
     return C2G_OK;
 }
 
@@ -578,14 +618,22 @@ static int launch_round_b(Graph* g, int groups, double* relax_ms, hipStream_t st
     const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
     const int grid = relax_grid_b(g);
     if (relax_ms) HIP_TRY(hipEventRecord(s->ev[0], strm));
-    if (pruned)
-        hipLaunchKernelGGL(relax_b_k<true>, dim3(grid), dim3(256), 0, strm, g->d_adj, g->d_hadj,
-                           w.dist, w.mask[0], w.hmask, s->items, s->imask, s->abits, w.ctrl,
-                           (int64_t)g->V, s->nw);
-    else
-        hipLaunchKernelGGL(relax_b_k<false>, dim3(grid), dim3(256), 0, strm, g->d_adj,
-                           (const int2*)nullptr, w.dist, w.mask[0], (uint64_t*)nullptr, s->items,
-                           s->imask, s->abits, w.ctrl, (int64_t)g->V, s->nw);
+    auto relax = [&](auto hidden, auto block) {
+        constexpr bool H = decltype(hidden)::value;
+        constexpr int B = decltype(block)::value;
+        hipLaunchKernelGGL((relax_b_k<H, B>), dim3(grid), dim3(B), 0, strm, H ? g->d_hadj : g->d_adj, w.dist,
+                           w.mask[0], H ? w.hmask : (uint64_t*)nullptr, H ? s->hitems : s->items,
+                           H ? s->himask : s->imask, s->abits, w.ctrl, (int64_t)g->V, s->nw);
+    };
+    auto relax_tb = [&](auto hidden) {
+        switch (g->tune.block) {
+            case 128: relax(hidden, std::integral_constant<int, 128>()); break;
+            case 512: relax(hidden, std::integral_constant<int, 512>()); break;
+            default: relax(hidden, std::integral_constant<int, 256>()); break;
+        }
+    };
+    if (pruned) relax_tb(std::true_type());  // hidden-list items first (see relax_b_k's round accounting)
+    relax_tb(std::false_type());
     if (relax_ms) {
         HIP_TRY(hipEventRecord(s->ev[1], strm));
         HIP_TRY(hipEventSynchronize(s->ev[1]));
@@ -602,7 +650,7 @@ static int ensure_exec_b(Graph* g, int groups) {
     BatchWs& w = g->batch;
     const int K = g->tune.iters_per_graph;
     if (w.exec && w.exec_iters == K && w.exec_grid == relax_grid_b(g) &&
-        w.exec_chunk == g->tune.batch_chunk && w.exec_block == groups)
+        w.exec_chunk == g->tune.batch_chunk && w.exec_groups == groups && w.exec_tb == g->tune.block)
         return C2G_OK;
     if (w.exec) { (void)hipGraphExecDestroy(w.exec); w.exec = nullptr; }
     hipGraph_t graph;
@@ -624,7 +672,8 @@ static int ensure_exec_b(Graph* g, int groups) {
     w.exec_iters = K;
     w.exec_grid = relax_grid_b(g);
     w.exec_chunk = g->tune.batch_chunk;
-    w.exec_block = groups;  // the captured expand scans this many groups
+    w.exec_groups = groups;  // the captured expand scans this many groups
+    w.exec_tb = g->tune.block;
     return C2G_OK;
 }
 
@@ -664,7 +713,7 @@ static int batch_run(Graph* g, const int32_t* srcs, int32_t n, c2g_stats* st, do
         HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, sizeof(int64_t) * B_N, hipMemcpyDeviceToHost, g->stream));
         HIP_TRY(hipStreamSynchronize(g->stream));
         if (w.h_ctrl[B_OVF]) return fail(C2G_EDEVICE, "batched frontier capacity exceeded (internal)");
-        if (w.h_ctrl[B_ITEMS] == 0 && w.h_ctrl[B_FAR] == 0) break;
+        if (w.h_ctrl[B_ITEMS] == 0 && w.h_ctrl[B_HITEMS] == 0 && w.h_ctrl[B_FAR] == 0) break;
     }
     if (st) {
         st->lane = C2G_LANE_GPU;
@@ -824,15 +873,18 @@ int64_t bserve_ws_bytes(Graph* g) {
     const int64_t V = std::max<int64_t>(g->V, 1);
     const int chunk = std::max(1, g->tune.batch_chunk);
     const int64_t nw = (((V + 31) / 32 + 1) + 63) & ~(int64_t)63;
-    const int64_t needi = (2 * V + (g->E + g->HE) / chunk) + 64;
+    const bool pruned = !g->hvl.empty();
+    const int64_t needi = V + g->E / chunk + 64;
+    const int64_t needh = pruned ? V + g->HE / chunk + 64 : 0;
     int64_t b = 0;
     {
         std::lock_guard<std::mutex> lk(g_bs_mu);
         bool have = false;
         for (auto& p : g_bs)
-            if (p.first == g && p.second->groups >= 1 && p.second->itemcap >= needi) have = true;
+            if (p.first == g && p.second->groups >= 1 && p.second->itemcap >= needi && p.second->hitemcap >= needh)
+                have = true;
         if (!have)
-            b += 256 * V + 8 * V * (g->hvl.empty() ? 1 : 2) + 4 * nw + 24 * needi + 64 * (4 + 4 + 8 + 8 + 4 + 8) +
+            b += 256 * V + 8 * V * (pruned ? 2 : 1) + 4 * nw + 24 * (needi + needh) + 64 * (4 + 4 + 8 + 8 + 4 + 8) +
                  8 * B_N;
     }
     {

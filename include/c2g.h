@@ -93,7 +93,10 @@ typedef struct {
 
 /* Tunables picked by the autotuner (c2g_autotune) or set by hand. */
 typedef struct {
-    int32_t block;          /* threads per workgroup for relax kernels         */
+    int32_t block;          /* threads per workgroup of the batched SSSP relax
+                               kernel (and the single-source one while
+                               block_single is 0): 128, 256 or 512 (the
+                               reference compiles in 512, core/Util.cuh:31-35) */
     int32_t grid_per_cu;    /* workgroups per CU for grid-stride kernels       */
     int32_t chunk;          /* max edges per single-query frontier item        */
     int32_t batch_chunk;    /* max edges per batched frontier item             */
@@ -112,7 +115,9 @@ typedef struct {
                                vertices renumbered by descending in-degree
                                (rows gathered most often share cache lines);
                                results come back in the caller's ids: 0 / 1 */
-    int32_t reserved[5];
+    int32_t block_single;   /* threads per workgroup of the single-source relax
+                               kernel (0 = block): 128, 256 or 512            */
+    int32_t reserved[4];
 } c2g_tune;
 
 /* ======================================================================= */

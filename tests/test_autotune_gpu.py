@@ -16,7 +16,8 @@ def test_autotune_keeps_results(gpu, batched):
     srcs = np.flatnonzero(deg > 0)[:70].astype(np.int32)
     with c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], device=0) as g:
         t = g.autotune(srcs[:8] if not batched else srcs, batched=batched)
-        assert t.grid_per_cu in (2, 4, 8, 16) and t.iters_per_graph in (4, 8, 16)
+        assert t.grid_per_cu in (2, 4, 8, 16) and t.iters_per_graph in (4, 8, 16) and t.block in (128, 256, 512)
+        assert t.block_single in (0, 128, 256, 512)
         assert g.tune().as_dict() == t.as_dict()
         d, _ = g.sssp(int(srcs[0]))
         db, _ = g.sssp_batch(srcs[:5])
@@ -39,3 +40,33 @@ def test_autotune_ppr_switch_point(gpu):
         with pytest.raises(c2g.C2GError):
             g.set_tune(ppr_pull=1001)
     assert np.abs(p1.astype(np.float64) - p0).sum() <= 2e-5
+
+
+@pytest.mark.parametrize("block", [128, 256, 512])
+def test_block_shapes_keep_results(gpu, block):
+    # every relax workgroup size the tuner may pick, single-source, batched
+    # and on a pruned graph, gives Dijkstra's distances
+    from conftest import random_graph
+    G = rmat_graph(14, 1, wmax=64)
+    deg = np.diff(G["vlist"])
+    srcs = np.concatenate([[int(np.argmax(deg))], np.flatnonzero(deg == 1)[:3]]).astype(np.int32)
+    refs = oracle.dijkstra_many(G["vlist"], G["elist"], G["wlist"], srcs)
+    with c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], device=0) as g:
+        g.set_tune(block=block, block_single=0)
+        db, _ = g.sssp_batch(srcs)
+        for q, s in enumerate(srcs):
+            assert np.array_equal(g.sssp(int(s))[0], refs[q]) and np.array_equal(db[q], refs[q])
+        g.set_tune(block=256, block_single=block)
+        assert np.array_equal(g.sssp(int(srcs[0]))[0], refs[0])
+        with pytest.raises(c2g.C2GError):
+            g.set_tune(block=64)
+    H = random_graph(200, 500, seed=4, wmax=9)
+    P = c2g.prune_sssp_mem(H["vlist"], H["elist"], H["wlist"])
+    with c2g.Graph.from_csr(P["vlist"], P["elist"], P["wlist"], hvlist=P["hvlist"], helist=P["helist"],
+                            hwlist=P["hwlist"], variant=c2g.VARIANT_SSSP_PURN, device=0) as g:
+        g.set_tune(block=block)
+        ss = np.arange(0, 200, 7, dtype=np.int32)
+        db, _ = g.sssp_batch(ss)
+        for q, s in enumerate(ss):
+            ref = oracle.dijkstra(H["vlist"], H["elist"], H["wlist"], int(s))
+            assert np.array_equal(db[q], ref) and np.array_equal(g.sssp(int(s))[0], ref), int(s)

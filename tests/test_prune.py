@@ -171,3 +171,43 @@ def test_loader_on_reference_dataset_layouts(tmp_path):
     assert e.value.code == c2g.EFORMAT and "compacted" in c2g.lib().c2g_last_error().decode()
     with c2g.Graph.open(ds, variant=c2g.VARIANT_PPR_PURN, no_device=True) as g:
         assert g.V == V
+
+
+def _site_graph(scale, seed):
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    return bench.site_graph(scale, seed=seed)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "purn-sssp")), reason="reference binaries not built")
+def test_prune_matches_live_reference_where_pruning_bites(tmp_path):
+    # the web-like site graph, where about a quarter of the edges go (the
+    # reference's web-BerkStan run kept 0.691 / 0.721): same bytes as the
+    # reference's own purn-sssp and purn-ppr, built from its sources
+    G = _site_graph(12, seed=7)
+    ds = str(tmp_path / "ds")
+    c2g.write_dataset(ds, G["vlist"], G["elist"], G["wlist"], G["flist"])
+    subprocess.run([os.path.join(REF, "purn-sssp"), ds], check=True, capture_output=True)
+    subprocess.run([os.path.join(REF, "purn-ppr"), ds], check=True, capture_output=True)
+    P = c2g.prune_sssp_mem(G["vlist"], G["elist"], G["wlist"])
+    assert len(P["elist"]) < 0.8 * G["E"]
+    for f in ("vlist", "elist", "wlist"):
+        assert P[f].tobytes() == np.fromfile(f"{ds}/sssp/{f}.bin", np.int32).tobytes(), f
+    Q = c2g.prune_ppr_mem(G["vlist"], G["elist"], G["flist"])
+    assert len(Q["elist"]) < 0.8 * G["E"]
+    for f in ("vlist", "elist", "wlist", "hvlist", "helist", "hwlist"):
+        assert Q[f].tobytes() == open(f"{ds}/ppr/{f}.bin", "rb").read(), f
+
+
+def test_pruning_guarantee_site_graph_host_lane():
+    # pruned + hidden seeding == full-graph Dijkstra where pruning bites
+    G = _site_graph(10, seed=3)
+    P, g = _pruned_graph(G, no_device=True)
+    assert len(P["elist"]) < 0.8 * G["E"]
+    srcs = list(range(0, G["V"], 17)) + [int(x) for x in P["porder"][:20]]
+    with g:
+        for s in srcs:
+            d, _ = g.host_sssp(s)
+            assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], s)), s

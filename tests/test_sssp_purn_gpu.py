@@ -69,3 +69,26 @@ def test_scale20_pruned_certificate(gpu):
             check_sssp_certificate(G["vlist"], G["elist"], G["wlist"], int(srcs[q]), d)
             fin = d != np.iinfo(np.int32).max
             assert sums[q] == d[fin].astype(np.int64).sum() and reached[q] == fin.sum()
+
+
+def test_site_graph_where_pruning_bites(gpu):
+    # the web-like site graph keeps about 0.74 of its edges under purn-sssp:
+    # single-source and batched pruned distances equal full-graph Dijkstra
+    import importlib.util
+    import os
+    from conftest import ROOT
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    G = bench.site_graph(16, seed=9)
+    P, g = _pruned(G)
+    assert len(P["elist"]) < 0.8 * G["E"]
+    deg = np.diff(G["vlist"])
+    rng = np.random.default_rng(4)
+    srcs = np.concatenate([P["porder"][:32], rng.choice(np.flatnonzero(deg > 0), 32, replace=False)]).astype(np.int32)
+    refs = oracle.dijkstra_many(G["vlist"], G["elist"], G["wlist"], srcs)
+    with g:
+        db, _ = g.sssp_batch(srcs)
+        for q, s in enumerate(srcs):
+            assert np.array_equal(db[q], refs[q]), (int(s), "batched")
+            assert np.array_equal(g.sssp(int(s))[0], refs[q]), int(s)
