@@ -164,7 +164,7 @@ def measured_traffic(kernel, args):
 
 def ingest_counters(dirs, out_path):
     """Sums rocprofv3 --pmc CSV outputs (one directory per counter pass) per
-    dispatch of relax_b_k<false> and writes the traffic summary: read bytes
+    dispatch of relax_b_k<false, B> and writes the traffic summary: read bytes
     = 32*RDREQ_32B + 64*RDREQ_64B + 128*RDREQ_128B, write bytes = 32*(WRREQ -
     WRREQ_64B) + 64*WRREQ_64B, per launch averaged over the dispatches."""
     import csv
@@ -172,13 +172,16 @@ def ingest_counters(dirs, out_path):
     # per[kind][counter] = list of per-dispatch sums (a counter may come in
     # several rows per dispatch, one per hardware instance)
     per = {"relax": {}, "copy": {}}
+    kname = None
     for d in dirs:
         for fn in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
             acc = {}
             with open(fn) as f:
                 for r in csv.DictReader(f):
                     k = r["Kernel_Name"]
-                    kind = "relax" if "relax_b_k<false>" in k else "copy" if "diag_copy_k" in k else None
+                    kind = "relax" if "relax_b_k<false," in k else "copy" if "diag_copy_k" in k else None
+                    if kind == "relax":
+                        kname = k[k.index("relax_b_k<"):k.index(">") + 1]
                     if kind:
                         key = (kind, r["Dispatch_Id"], r["Counter_Name"])
                         acc[key] = acc.get(key, 0.0) + float(r["Counter_Value"])
@@ -201,7 +204,7 @@ def ingest_counters(dirs, out_path):
                         "rdreq_dram": avg(dct, "TCC_EA0_RDREQ_DRAM")}
 
     rd, wb, cnt = traffic(per)
-    out = {"kernel": "relax_b_k<false>", "workload": "R-MAT scale 24 seed 4, 256 sources",
+    out = {"kernel": kname, "workload": "R-MAT scale 24 seed 4, 256 sources",
            "dispatches": len(per.get("TCC_EA0_RDREQ_128B", per.get("TCC_EA0_RDREQ_128B_sum", []))),
            "read_bytes_per_launch": int(rd), "write_bytes_per_launch": int(wb),
            "traffic_bytes_per_launch": int(rd + wb), "counters": {k: v for k, v in cnt.items()},
@@ -464,7 +467,7 @@ def c2_phase(args):
     # same recipe at scale 20 (about a minute; scale 22 would take several),
     # beside c2g_prune_sssp, and its pruned CSR compared byte for byte.
     ref = os.path.join(ROOT, "oracle", "_ref", "purn-sssp")
-    if os.path.exists(ref):
+    if os.path.exists(ref) and not args.no_ref_prune:
         import shutil
         import subprocess
         import tempfile
@@ -600,6 +603,11 @@ def main():
     ap.add_argument("--aux-only", action="store_true", help="only the PPR and adsorption timings")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the profiled pass (for rocprofv3: it then sees exactly those launches)")
+    ap.add_argument("--each-only", type=int, default=0, metavar="N",
+                    help="only the per-query pass (single-source kernels) on the first N headline sources "
+                         "(for rocprofv3)")
+    ap.add_argument("--no-ref-prune", action="store_true",
+                    help="--c2 without the reference purn-sssp timing (for rocprofv3)")
     ap.add_argument("--set-tune", default="", metavar="K=V,...",
                     help="pin tunables (e.g. grid_per_cu=16,block=256) instead of running the autotuner, so a "
                          "profiled run repeats the timed run's launch shapes")
@@ -658,7 +666,7 @@ def main():
     if args.set_tune:
         g.set_tune(**{k: int(v) for k, v in (kv.split("=") for kv in args.set_tune.split(","))})
         log(f"tunables pinned: {g.tune().as_dict()}")
-    elif not args.no_autotune and not args.profile_only:
+    elif not args.no_autotune and not args.profile_only and not args.each_only:
         # the workload's launch shapes and bin thresholds come from the
         # autotuner, on a sample of the same recipe drawn with another seed
         # (not the timed sources): single-source settings on 16 sources, then
@@ -687,6 +695,18 @@ def main():
                         "rounds": stn.iterations, "csr_edges": stn.csr_edges})
             log(f"batch sweep: {res[-1]}")
         print(json.dumps({"batch_sweep": res}), flush=True)
+        return
+    if args.each_only:
+        # the per-query pass alone: half hubs, half out-degree-1 sources
+        half = args.each_only // 2
+        es = np.concatenate([srcs[:half], srcs[args.queries // 2:args.queries // 2 + args.each_only - half]])
+        g.sssp_each_resident(es[:2])
+        t0 = time.perf_counter()
+        sums_e, reach_e, st_e = g.sssp_each_resident(es)
+        te = time.perf_counter() - t0
+        print(json.dumps({"each_only": {"queries": len(es), "ms_total": round(te * 1e3, 1),
+                                        "ms_per_query": round(te / len(es) * 1e3, 3),
+                                        "counters": st_e.as_dict()}}), flush=True)
         return
     if args.profile_only:
         args.warmup = args.steps = 0
@@ -744,13 +764,14 @@ def main():
         g.set_tune(profile=0)
         algo_bytes = 8 * sp.csr_edges + 8 * sp.frontier_total + 4 * sp.edges_relaxed + 4 * sp.improvements
         achieved = algo_bytes / (sp.kernel_ms * 1e-3) / 1e9
-        traffic, prov = measured_traffic("relax_b_k<false>", args)
+        kname = f"relax_b_k<false, {g.tune().block}>"
+        traffic, prov = measured_traffic(kname, args)
         algo_per_launch = algo_bytes / max(1, sp.iterations)
         extra["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
             "frac": round(achieved / 8000.0, 4), "traffic": traffic, "traffic_source": prov,
             "traffic_over_algorithmic": round(traffic / algo_per_launch, 3) if traffic else None,
-            "kernel": "relax_b_k<false>", "launches": sp.iterations,
+            "kernel": kname, "launches": sp.iterations,
             "avg_launch_us": round(sp.kernel_ms * 1e3 / max(1, sp.iterations), 2),
             "algo_bytes_per_launch": int(algo_per_launch),
             "lane_relaxations_per_reached_edge": round(sp.edges_relaxed / max(1, sp.reached_edges), 3),
