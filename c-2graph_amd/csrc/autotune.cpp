@@ -8,8 +8,8 @@
 // distances (the tunables change scheduling only), so tuning never changes
 // results.  Search: coordinate descent over grid_per_cu_single and chunk
 // (single-source) or grid_per_cu and batch_chunk (batched), then
-// iters_per_graph, then the relax kernels' workgroup size (block); two
-// passes, best of two runs per candidate.
+// iters_per_graph, then the relax kernels' workgroup size (block) and, for
+// single-source, the XCD split; two passes, best of two runs per candidate.
 #include "graph.h"
 #include <cstring>
 
@@ -111,15 +111,19 @@ extern "C" int c2g_autotune(c2g_graph* h, const int32_t* srcs, int32_t n, int32_
     const std::vector<int> chunks = b ? std::vector<int>{16, 32, 64, 128} : std::vector<int>{16, 32, 64, 128, 256};
     const std::vector<int> iters = {4, 8, 16};
     const std::vector<int> blocks = {128, 256, 512};
+    const std::vector<int> splits = {0, 1};
+    const int dims = b ? 4 : 5;  // single-source also tries the XCD split
     for (int pass = 0; pass < 2 && rc == C2G_OK; ++pass) {
-        for (int which = 0; which < 4 && rc == C2G_OK; ++which) {
-            const std::vector<int>& vals = which == 0 ? grids : which == 1 ? chunks : which == 2 ? iters : blocks;
+        for (int which = 0; which < dims && rc == C2G_OK; ++which) {
+            const std::vector<int>& vals =
+                which == 0 ? grids : which == 1 ? chunks : which == 2 ? iters : which == 3 ? blocks : splits;
             for (int v : vals) {
                 c2g_tune t = best;
                 if (which == 0) (b ? t.grid_per_cu : t.grid_per_cu_single) = v;
                 else if (which == 1) (b ? t.batch_chunk : t.chunk) = v;
                 else if (which == 2) t.iters_per_graph = v;
-                else (b ? t.block : t.block_single) = v;
+                else if (which == 3) (b ? t.block : t.block_single) = v;
+                else t.xcd_split = v;
                 if (!memcmp(&t, &best, sizeof t)) continue;
                 set_tune_locked(g, t);
                 const double tt = measure(g, srcs, m, b, &rc);

@@ -82,7 +82,7 @@ struct SsspWs {
     int64_t* h_ctrl = nullptr;   // pinned mirror
     uint8_t* hid = nullptr;      // [V] 1 = distance was set through a hidden edge
     hipGraphExec_t exec = nullptr;
-    int exec_iters = 0, exec_grid = 0, exec_chunk = 0, exec_tb = 0;
+    int exec_iters = 0, exec_grid = 0, exec_chunk = 0, exec_tb = 0, exec_split = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
     HostStage out;               // distance copies to the host
 };
@@ -118,6 +118,10 @@ struct Graph {
     int32_t* d_hrow = nullptr;
     int2* d_hadj = nullptr;
     int2* d_hadjf = nullptr;
+    // split mode of single-source SSSP: per-vertex offsets of the target
+    // partitions of each row, [V+1][8] (built on first use)
+    int32_t* d_split = nullptr;
+    int32_t* d_hsplit = nullptr;
     SsspWs sssp;
     BatchWs batch;
     c2g_tune tune{};

@@ -194,6 +194,8 @@ void device_release(Graph* g) {
     if (g->batch.h_ctrl) (void)hipHostFree(g->batch.h_ctrl);
     dfree(g->d_row); dfree(g->d_adj); dfree(g->d_adjf); dfree(g->d_hrow); dfree(g->d_hadj);
     dfree(g->d_hadjf);
+    dfree(g->d_split);
+    dfree(g->d_hsplit);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     g->stream = nullptr;
     g->on_device = false;
@@ -465,6 +467,7 @@ int c2g_set_tune(c2g_graph* h, const c2g_tune* t) {
         return fail(C2G_EINVAL, "block (threads per workgroup of the relax kernels) must be 128, 256 or 512");
     if (t->block_single != 0 && t->block_single != 128 && t->block_single != 256 && t->block_single != 512)
         return fail(C2G_EINVAL, "block_single must be 0, 128, 256 or 512");
+    if (t->xcd_split != 0 && t->xcd_split != 1) return fail(C2G_EINVAL, "xcd_split must be 0 or 1");
     if (t->grid_per_cu < 1 || t->grid_per_cu > 64) return fail(C2G_EINVAL, "grid_per_cu in [1,64]");
     if (t->grid_per_cu_single < 0 || t->grid_per_cu_single > 64)
         return fail(C2G_EINVAL, "grid_per_cu_single in [0,64]");

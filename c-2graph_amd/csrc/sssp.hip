@@ -57,8 +57,18 @@ enum : int {
     C_NEAR = 13,  // vertices expanded this round
     C_FAR = 14,   // 1 while deferred vertices remain
     C_SLOTS = 15, // slot pools: bit s set when slot s has items queued for the next round
-    C_N = 16
+    C_QN = 16,    // split mode: items of partition p for the next round at C_QN + p (8 entries)
+    C_N = 24
 };
+
+// Split mode (c2g_tune.xcd_split): the edges of a frontier row are cut by
+// the partition of their target, p = v / ceil(V/8), into eight queues, and
+// the relax workgroups with blockIdx % 8 == p take queue p.  Workgroups are
+// handed to the eight XCDs round-robin, so each XCD's L2 gathers only its
+// eighth of `dist`.  Rows are sorted by target, so a row's edges into
+// partition p are one contiguous range: split[8u + p] is its first edge and
+// split[8u + 8] = split[8(u+1)] the end of the row.
+constexpr int kParts = 8;
 
 // Device edge entries carry a flag in bit 31 of the target id: set when the
 // target has out-edges (or hidden edges), i.e. when activating it can lead
@@ -80,15 +90,47 @@ __global__ void sssp_init_k(int32_t* __restrict__ dist, uint32_t* __restrict__ b
     }
 }
 
-// Queues the source: live items and (pruned graphs) hidden items.
+// Queues the source: live items and (pruned graphs) hidden items.  In split
+// mode (split non-null) thread p < 8 writes partition p's items of the source
+// into queue p (q + p * qseg).
 __global__ void sssp_seed_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
                             int32_t* __restrict__ dist, uint8_t* __restrict__ hid,
                             int4* __restrict__ q, int64_t* __restrict__ ctrl, int32_t src,
-                            int chunk, int delta) {
+                            int chunk, int delta, const int32_t* __restrict__ split,
+                            const int32_t* __restrict__ hsplit, int64_t qseg) {
     const int r0 = row[src], r1 = row[src + 1];
     const int nl = (r1 - r0 + chunk - 1) / chunk;
     int h0 = 0, h1 = 0, nh = 0;
     if (hrow) { h0 = hrow[src]; h1 = hrow[src + 1]; nh = (h1 - h0 + chunk - 1) / chunk; }
+    if (split) {
+        __shared__ int64_t s_n[kParts];
+        if (threadIdx.x < kParts) {
+            const int p = threadIdx.x;
+            int4* qp = q + p * qseg;
+            int64_t k = 0;
+            for (int b = split[(int64_t)src * 8 + p], e = split[(int64_t)src * 8 + p + 1]; b < e; b += chunk)
+                qp[k++] = make_int4(src, b, min(b + chunk, e), 0);
+            if (hsplit)
+                for (int b = hsplit[(int64_t)src * 8 + p], e = hsplit[(int64_t)src * 8 + p + 1]; b < e; b += chunk)
+                    qp[k++] = make_int4(src, b, min(b + chunk, e), 1);
+            s_n[p] = k;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            dist[src] = 0;
+            if (hid) hid[src] = 1;
+            for (int i = 0; i < C_N; ++i) ctrl[i] = 0;
+            ctrl[C_THR] = delta > 0 ? delta : INT32_MAX;
+            ctrl[C_FMIN] = INT32_MAX;
+            int64_t tot = 0;
+            for (int p = 0; p < kParts; ++p) { ctrl[C_QN + p] = s_n[p]; tot += s_n[p]; }
+            ctrl[C_CUR] = tot;
+            ctrl[C_FRONT] = 1;
+            ctrl[C_EDGES] = r1 - r0;
+            ctrl[C_HEDGES] = h1 - h0;
+        }
+        return;
+    }
     for (int j = threadIdx.x; j < nl + nh; j += blockDim.x) {
         if (j < nl) {
             int b = r0 + j * chunk;
@@ -115,16 +157,22 @@ __global__ void sssp_seed_k(const int32_t* __restrict__ row, const int32_t* __re
 // slot 0; a slot pool runs several queries side by side, slot s owning
 // dist[s*V..], hid[s*V..] and the bitmap words [s*nwp..] of each map.
 // BLOCK: threads per workgroup (128, 256 or 512; c2g_tune.block).
-template <bool PRUNED, int BLOCK>
+// SPLIT: the workgroup takes queue p = blockIdx % 8 of the split mode (the
+// grid is a multiple of 8).
+template <bool PRUNED, int BLOCK, bool SPLIT>
 __global__ void __launch_bounds__(BLOCK)
 sssp_relax_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
              int32_t* __restrict__ dist, uint32_t* __restrict__ bits,
              uint32_t* __restrict__ hbits, uint8_t* __restrict__ hid,
-             const int4* __restrict__ q, int64_t* __restrict__ ctrl, int64_t V, int64_t nwp) {
-    const int64_t n = ctrl[C_CUR];
+             const int4* __restrict__ q0, int64_t* __restrict__ ctrl, int64_t V, int64_t nwp, int64_t qseg) {
+    const int part = SPLIT ? (int)(blockIdx.x & (kParts - 1)) : 0;
+    const int64_t n = SPLIT ? ctrl[C_QN + part] : ctrl[C_CUR];
+    const int4* __restrict__ q = q0 + (SPLIT ? part * qseg : 0);
     const int lane = lane_id();
-    const int64_t wave = wave_index();
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t wave = SPLIT ? (int64_t)(blockIdx.x >> 3) * (BLOCK >> 6) +
+                                     __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)
+                               : wave_index();
+    const int64_t nwaves = SPLIT ? (int64_t)(gridDim.x >> 3) * (BLOCK >> 6) : ((int64_t)gridDim.x * BLOCK) >> 6;
     int64_t impr = 0;
 
     for (int64_t base = wave * kWave; base < n; base += nwaves * kWave) {
@@ -192,8 +240,13 @@ sssp_relax_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
         unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[C_DONE], 1ull);
         if (prev == (unsigned long long)gridDim.x - 1) {
             __threadfence();
-            if (n > 0) ctrl[C_ITER] += 1;
-            ctrl[C_ITEMS] += n;
+            int64_t ntot = n;
+            if (SPLIT) {
+                ntot = 0;
+                for (int p = 0; p < kParts; ++p) { ntot += ctrl[C_QN + p]; ctrl[C_QN + p] = 0; }
+            }
+            if (ntot > 0) ctrl[C_ITER] += 1;
+            ctrl[C_ITEMS] += ntot;
             ctrl[C_CUR] = 0;
             ctrl[C_DONE] = 0;
             ctrl[C_SLOTS] = 0;
@@ -208,12 +261,16 @@ sssp_relax_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
 // vertices with dist < ctrl[C_THR] are expanded; the others keep their bit
 // for a later round, and when a round expands nothing the last workgroup
 // moves the bound to the bucket of the smallest deferred distance.
-template <bool PRUNED>
+template <bool PRUNED, bool SPLIT>
 __global__ void __launch_bounds__(256)
 sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
               uint32_t* __restrict__ bits, uint32_t* __restrict__ hbits, int4* __restrict__ q,
               int64_t* __restrict__ ctrl, int64_t V, int chunk, int64_t qcap,
-              const int32_t* __restrict__ dist, int delta, int slots, int64_t nwp) {
+              const int32_t* __restrict__ dist, int delta, int slots, int64_t nwp,
+              const int32_t* __restrict__ split, const int32_t* __restrict__ hsplit) {
+    // NP queues: one, or in split mode one per target partition (queue p at
+    // q + p * qcap, qcap items each)
+    constexpr int NP = SPLIT ? kParts : 1;
     const int lane = lane_id();
     const int64_t nw = (V + 31) >> 5;
     const int64_t nwr = (int64_t)slots * nwp;  // nwp is a multiple of 64: a wave stays in one slot
@@ -221,10 +278,22 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
     const int32_t T = (int32_t)ctrl[C_THR];
     int64_t front = 0, edges = 0, hedges = 0, near_n = 0;
     int32_t fmin = INT32_MAX;
+    // Edge range of partition p of v's live (or hidden) list.
+    auto range = [&](bool hidden, int v, int p, int& b, int& e) {
+        if (SPLIT) {
+            const int32_t* sp = (hidden ? hsplit : split) + (int64_t)v * kParts;
+            b = sp[p];
+            e = sp[p + 1];
+        } else {
+            const int32_t* r = hidden ? hrow : row;
+            b = r[v];
+            e = r[v + 1];
+        }
+    };
     // The loop's trip count is the same for every thread of a workgroup, so
     // its barriers are reached by all of them.
-    __shared__ int64_t s_tot[16];
-    __shared__ int64_t s_base;
+    __shared__ int64_t s_tot[16][NP];
+    __shared__ int64_t s_base[NP];
     const int wid = threadIdx.x >> 6;
     const int nwv = blockDim.x >> 6;
     for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < nwr; i0 += stride) {
@@ -253,53 +322,87 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
             if (PRUNED && wh != (wh & ~nh)) hbits[i] = wh & ~nh;
         }
         const int tag = slot << 1;
-        int cnt = 0;
+        int cnt[NP];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) cnt[p] = 0;
         for (uint32_t m = nl; m; m &= m - 1) {
             const int v = (int)(wi * 32 + __builtin_ctz(m));
-            const int d = row[v + 1] - row[v];
-            cnt += (d + chunk - 1) / chunk;
             front += 1;
-            edges += d;
+            edges += row[v + 1] - row[v];
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                int b, e;
+                range(false, v, p, b, e);
+                cnt[p] += (e - b + chunk - 1) / chunk;
+            }
         }
         if (PRUNED)
             for (uint32_t m = nh; m; m &= m - 1) {
                 const int v = (int)(wi * 32 + __builtin_ctz(m));
-                const int d = hrow[v + 1] - hrow[v];
-                cnt += (d + chunk - 1) / chunk;
-                hedges += d;
+                hedges += hrow[v + 1] - hrow[v];
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    int b, e;
+                    range(true, v, p, b, e);
+                    cnt[p] += (e - b + chunk - 1) / chunk;
+                }
             }
         near_n += __popc(nl | nh);
-        int tot;
-        const int off = wave_excl_scan(cnt, &tot);
-        // one queue reservation per workgroup: wave totals meet in LDS
-        if (lane == 0) s_tot[wid] = tot;
+        int off[NP], tot[NP];
+        int any = 0;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            off[p] = wave_excl_scan(cnt[p], &tot[p]);
+            any |= tot[p];
+        }
+        // one queue reservation per workgroup and queue: wave totals meet in LDS
+        if (lane == 0)
+            for (int p = 0; p < NP; ++p) s_tot[wid][p] = tot[p];
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if ((int)threadIdx.x < NP) {
+            const int p = threadIdx.x;
             int64_t sum = 0;
-            for (int w = 0; w < nwv; ++w) sum += s_tot[w];
-            s_base = sum > 0 ? (int64_t)atomicAdd((unsigned long long*)&ctrl[C_CUR], (unsigned long long)sum) : 0;
+            for (int w = 0; w < nwv; ++w) sum += s_tot[w][p];
+            s_base[p] = sum > 0 ? (int64_t)atomicAdd((unsigned long long*)&ctrl[SPLIT ? C_QN + p : C_CUR],
+                                                     (unsigned long long)sum)
+                                : 0;
         }
         __syncthreads();
-        int64_t base = s_base;
-        for (int w = 0; w < wid; ++w) base += s_tot[w];
+        int64_t pos[NP];
+        bool room = true;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            int64_t base = s_base[p];
+            for (int w = 0; w < wid; ++w) base += s_tot[w][p];
+            room = room && base + tot[p] <= qcap;
+            pos[p] = base + off[p];
+        }
         __syncthreads();  // s_tot and s_base are rewritten in the next iteration
-        if (tot == 0) continue;
+        if (any == 0) continue;
         if (lane == 0) atomicOr((unsigned long long*)&ctrl[C_SLOTS], 1ull << slot);
-        if (base + tot > qcap) {
+        if (!room) {
             if (lane == 0) atomicOr((unsigned long long*)&ctrl[C_OVF], 1ull);
             continue;
         }
-        int64_t p = base + off;
         for (uint32_t m = nl; m; m &= m - 1) {
             const int v = (int)(wi * 32 + __builtin_ctz(m));
-            const int r0 = row[v], r1 = row[v + 1];
-            for (int b = r0; b < r1; b += chunk) q[p++] = make_int4(v, b, min(b + chunk, r1), tag);
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                int b0, e;
+                range(false, v, p, b0, e);
+                for (int b = b0; b < e; b += chunk) q[p * qcap + pos[p]++] = make_int4(v, b, min(b + chunk, e), tag);
+            }
         }
         if (PRUNED)
             for (uint32_t m = nh; m; m &= m - 1) {
                 const int v = (int)(wi * 32 + __builtin_ctz(m));
-                const int h0 = hrow[v], h1 = hrow[v + 1];
-                for (int b = h0; b < h1; b += chunk) q[p++] = make_int4(v, b, min(b + chunk, h1), tag | 1);
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    int b0, e;
+                    range(true, v, p, b0, e);
+                    for (int b = b0; b < e; b += chunk)
+                        q[p * qcap + pos[p]++] = make_int4(v, b, min(b + chunk, e), tag | 1);
+                }
             }
     }
     front = wave_sum64(front);
@@ -331,11 +434,20 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
                 ctrl[C_THR] = min((int64_t)INT32_MAX, (fm / delta + 1) * delta);
             ctrl[C_FMIN] = INT32_MAX;
             ctrl[C_NEAR] = 0;
-            int64_t nn = (int64_t)atomicAdd((unsigned long long*)&ctrl[C_CUR], 0ull);
-            if (nn > qcap) {  // items past the queue were dropped: stop, report
-                ctrl[C_OVF] = 1;
-                ctrl[C_CUR] = 0;
+            // items past a queue's end were dropped: stop, report
+            bool ovf = false;
+            int64_t nn = 0;
+            for (int p = 0; p < NP; ++p) {
+                const int64_t np = (int64_t)atomicAdd((unsigned long long*)&ctrl[SPLIT ? C_QN + p : C_CUR], 0ull);
+                ovf = ovf || np > qcap;
+                nn += np;
             }
+            if (ovf) {
+                ctrl[C_OVF] = 1;
+                for (int p = 0; p < NP; ++p) ctrl[SPLIT ? C_QN + p : C_CUR] = 0;
+                nn = 0;
+            }
+            ctrl[C_CUR] = nn;
         }
     }
 }
@@ -366,13 +478,65 @@ __global__ void dist_checksum_k(const int32_t* __restrict__ row, const int32_t* 
 // Words per activation bitmap of one slot, padded to whole waves.
 static int64_t bitmap_words(int64_t V) { return (((V + 31) / 32 + 1) + 63) & ~(int64_t)63; }
 
+// Queue capacity of one partition in split mode: each vertex is activated at
+// most once per round and has at most ceil(n/chunk) <= n/chunk + 1 items per
+// partition of each list.
+static int64_t split_qseg(const Graph* g) {
+    const int chunk = std::max(1, g->tune.chunk);
+    return 2 * std::max<int64_t>(g->V, 1) + (g->E + g->HE) / chunk + 64;
+}
+
+// The per-vertex partition offsets of the live (and hidden) rows, built on
+// the host the first time split mode runs: split[8u + p] = first edge of row
+// u whose target lies in partition p (targets sorted, p = v / ceil(V/8)),
+// split[8V] = end of the last row.
+static int ensure_split(Graph* g) {
+    if (g->d_split && (!g->d_hrow || g->d_hsplit)) return C2G_OK;
+    const int64_t V = g->V;
+    const int64_t pw = (V + kParts - 1) / kParts;
+    auto build = [&](const std::vector<int32_t>& row, const std::vector<int32_t>& el, int32_t** dst) -> int {
+        std::vector<int32_t> sp((size_t)(V + 1) * kParts);
+        parallel_for(V, [&](int64_t lo, int64_t hi, int) {
+            for (int64_t u = lo; u < hi; ++u) {
+                const int32_t* b = el.data() + row[u];
+                const int32_t* e = el.data() + row[u + 1];
+                sp[u * kParts] = row[u];
+                // a row not sorted by target (a caller's CSR may not be) goes
+                // whole into partition 0: the eight ranges must tile the row
+                const bool sorted = std::is_sorted(b, e);
+                for (int p = 1; p < kParts; ++p)
+                    sp[u * kParts + p] =
+                        sorted ? (int32_t)(std::lower_bound(b, e, (int32_t)std::min<int64_t>(p * pw, INT32_MAX)) -
+                                           el.data())
+                               : row[u + 1];
+            }
+        }, 0, 1 << 14);
+        for (int p = 0; p < kParts; ++p) sp[V * kParts + p] = row[V];
+        HIP_TRY(hipMalloc(dst, sizeof(int32_t) * sp.size()));
+        HIP_TRY(hipMemcpy(*dst, sp.data(), sizeof(int32_t) * sp.size(), hipMemcpyHostToDevice));
+        return C2G_OK;
+    };
+    HIP_TRY(hipSetDevice(g->device));
+    int rc;
+    if (!g->d_split && (rc = build(g->vl, g->el, &g->d_split))) return rc;
+    if (g->d_hrow && !g->d_hsplit && (rc = build(g->hvl, g->hel, &g->d_hsplit))) return rc;
+    return C2G_OK;
+}
+
+static bool split_mode(const Graph* g) { return g->tune.xcd_split != 0; }
+
 int ensure_sssp_ws(Graph* g) {
     SsspWs& w = g->sssp;
     const int64_t V = g->V;
     const int chunk = std::max(1, g->tune.chunk);
     // Each vertex is activated at most once per round for its live list and
     // once for its hidden list: at most ceil(d/chunk) <= d/chunk + 1 items each.
-    const int64_t need = 2 * V + (g->E + g->HE) / chunk + 64;
+    // Split mode: kParts queues of split_qseg items.
+    const int64_t need = split_mode(g) ? kParts * split_qseg(g) : 2 * V + (g->E + g->HE) / chunk + 64;
+    if (split_mode(g)) {
+        int rc = ensure_split(g);
+        if (rc) return rc;
+    }
     const int64_t nwp = bitmap_words(V);
     HIP_TRY(hipSetDevice(g->device));
     // Each piece is checked on its own: after a failed allocation the next
@@ -397,7 +561,8 @@ int ensure_sssp_ws(Graph* g) {
 
 static int relax_grid(const Graph* g) {
     const int per_cu = g->tune.grid_per_cu_single > 0 ? g->tune.grid_per_cu_single : g->tune.grid_per_cu;
-    return std::max(1, g->num_cus) * per_cu;
+    const int grid = std::max(1, g->num_cus) * per_cu;
+    return split_mode(g) ? (grid + kParts - 1) / kParts * kParts : grid;  // split: a multiple of 8
 }
 
 static bool is_pruned(const Graph* g) { return g->d_hrow != nullptr && g->d_hadj != nullptr; }
@@ -409,34 +574,51 @@ struct SsspBufs {
     uint32_t* bits;    // live [nwp], then hidden [nwp]
     uint8_t* hid;      // [V] (pruned graphs)
     int4* q;           // frontier items
-    int64_t qcap;
+    int64_t qcap;      // items per queue (split mode: per partition)
     int64_t* ctrl;     // [C_N]
+    const int32_t* split = nullptr;   // split mode: partition offsets (null: one queue)
+    const int32_t* hsplit = nullptr;
 };
 
-static SsspBufs ws_bufs(Graph* g) {
+// The workspace's buffers; split mode when the handle's tunables ask for it
+// and `allow_split` (budgeted runs read back one queue).
+static SsspBufs ws_bufs(Graph* g, bool allow_split = true) {
     SsspWs& w = g->sssp;
-    return SsspBufs{w.dist, w.bits, w.hid, w.q[0], w.qcap, w.ctrl};
+    SsspBufs b{w.dist, w.bits, w.hid, w.q[0], w.qcap, w.ctrl};
+    if (allow_split && split_mode(g) && g->d_split) {
+        b.split = g->d_split;
+        b.hsplit = g->d_hsplit;
+        b.qcap = split_qseg(g);
+    }
+    return b;
 }
 
 // One relax launch with the tuned workgroup size; hbits/hid null on full
 // graphs.  Items may belong to several slots (slot pools).
+// qseg > 0: split mode, kParts queues of qseg items (grid a multiple of 8).
 static void launch_relax(Graph* g, hipStream_t strm, int grid, int32_t* dist, uint32_t* bits, uint32_t* hbits,
-                         uint8_t* hid, const int4* q, int64_t* ctrl, int64_t nwp) {
-    auto go = [&](auto pruned, auto block) {
+                         uint8_t* hid, const int4* q, int64_t* ctrl, int64_t nwp, int64_t qseg = 0) {
+    auto go = [&](auto pruned, auto block, auto split) {
         constexpr bool P = decltype(pruned)::value;
         constexpr int B = decltype(block)::value;
-        hipLaunchKernelGGL((sssp_relax_k<P, B>), dim3(grid), dim3(B), 0, strm, g->d_adj,
-                           P ? g->d_hadj : (const int2*)nullptr, dist, bits, hbits, hid, q, ctrl, (int64_t)g->V, nwp);
+        constexpr bool S = decltype(split)::value;
+        hipLaunchKernelGGL((sssp_relax_k<P, B, S>), dim3(grid), dim3(B), 0, strm, g->d_adj,
+                           P ? g->d_hadj : (const int2*)nullptr, dist, bits, hbits, hid, q, ctrl, (int64_t)g->V, nwp,
+                           qseg);
     };
-    auto tb = [&](auto pruned) {
+    auto tb = [&](auto pruned, auto split) {
         switch (g->tune.block_single > 0 ? g->tune.block_single : g->tune.block) {
-            case 128: go(pruned, std::integral_constant<int, 128>()); break;
-            case 512: go(pruned, std::integral_constant<int, 512>()); break;
-            default: go(pruned, std::integral_constant<int, 256>()); break;
+            case 128: go(pruned, std::integral_constant<int, 128>(), split); break;
+            case 512: go(pruned, std::integral_constant<int, 512>(), split); break;
+            default: go(pruned, std::integral_constant<int, 256>(), split); break;
         }
     };
-    if (hbits) tb(std::true_type());
-    else tb(std::false_type());
+    auto sp = [&](auto pruned) {
+        if (qseg > 0) tb(pruned, std::true_type());
+        else tb(pruned, std::false_type());
+    };
+    if (hbits) sp(std::true_type());
+    else sp(std::false_type());
 }
 
 // Launches k rounds (relax + expand) on `b`.  With relax_ms non-null
@@ -450,19 +632,26 @@ static int launch_rounds_on(Graph* g, const SsspBufs& b, int k, double* relax_ms
     uint32_t* bits = b.bits;
     uint32_t* hbits = bits + nwp;
     const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (nwp + 255) / 256));
+    const bool split = b.split != nullptr;
     for (int r = 0; r < k; ++r) {
         if (relax_ms) HIP_TRY(hipEventRecord(w.ev[0], g->stream));
         launch_relax(g, g->stream, grid, b.dist, bits, pruned ? hbits : nullptr, pruned ? b.hid : nullptr, b.q, b.ctrl,
-                     nwp);
+                     nwp, split ? b.qcap : 0);
         if (relax_ms) HIP_TRY(hipEventRecord(w.ev[1], g->stream));
+        auto expand = [&](auto pr, auto sp) {
+            constexpr bool P = decltype(pr)::value;
+            constexpr bool S = decltype(sp)::value;
+            hipLaunchKernelGGL((sssp_expand_k<P, S>), dim3(egrid), dim3(256), 0, g->stream, g->d_row,
+                               P ? g->d_hrow : (const int32_t*)nullptr, bits, P ? hbits : (uint32_t*)nullptr, b.q,
+                               b.ctrl, (int64_t)g->V, g->tune.chunk, b.qcap, (const int32_t*)b.dist, g->tune.delta, 1,
+                               nwp, b.split, P ? b.hsplit : (const int32_t*)nullptr);
+        };
         if (pruned) {
-            hipLaunchKernelGGL(sssp_expand_k<true>, dim3(egrid), dim3(256), 0, g->stream, g->d_row,
-                               g->d_hrow, bits, hbits, b.q, b.ctrl, (int64_t)g->V, g->tune.chunk, b.qcap,
-                               (const int32_t*)b.dist, g->tune.delta, 1, nwp);
+            if (split) expand(std::true_type(), std::true_type());
+            else expand(std::true_type(), std::false_type());
         } else {
-            hipLaunchKernelGGL(sssp_expand_k<false>, dim3(egrid), dim3(256), 0, g->stream, g->d_row,
-                               (const int32_t*)nullptr, bits, (uint32_t*)nullptr, b.q, b.ctrl,
-                               (int64_t)g->V, g->tune.chunk, b.qcap, (const int32_t*)b.dist, g->tune.delta, 1, nwp);
+            if (split) expand(std::false_type(), std::true_type());
+            else expand(std::false_type(), std::false_type());
         }
         if (relax_ms) {
             HIP_TRY(hipEventSynchronize(w.ev[1]));
@@ -475,8 +664,8 @@ static int launch_rounds_on(Graph* g, const SsspBufs& b, int k, double* relax_ms
     return C2G_OK;
 }
 
-static int launch_rounds(Graph* g, int k, double* relax_ms = nullptr) {
-    return launch_rounds_on(g, ws_bufs(g), k, relax_ms);
+static int launch_rounds(Graph* g, int k, double* relax_ms = nullptr, bool allow_split = true) {
+    return launch_rounds_on(g, ws_bufs(g, allow_split), k, relax_ms);
 }
 
 // Clears `b` for a query from src and queues the source's items.
@@ -489,7 +678,8 @@ static void launch_seed(Graph* g, const SsspBufs& b, int32_t src) {
                        (int64_t)g->V);
     hipLaunchKernelGGL(sssp_seed_k, dim3(1), dim3(256), 0, g->stream, g->d_row,
                        pruned ? g->d_hrow : (const int32_t*)nullptr, b.dist,
-                       pruned ? b.hid : (uint8_t*)nullptr, b.q, b.ctrl, src, g->tune.chunk, g->tune.delta);
+                       pruned ? b.hid : (uint8_t*)nullptr, b.q, b.ctrl, src, g->tune.chunk, g->tune.delta, b.split,
+                       pruned ? b.hsplit : (const int32_t*)nullptr, b.qcap);
 }
 
 // A hipGraph holding `iters_per_graph` rounds.
@@ -497,7 +687,8 @@ static int ensure_exec(Graph* g) {
     SsspWs& w = g->sssp;
     const int K = g->tune.iters_per_graph;
     if (w.exec && w.exec_iters == K && w.exec_grid == relax_grid(g) && w.exec_chunk == g->tune.chunk &&
-        w.exec_tb == (g->tune.block_single > 0 ? g->tune.block_single : g->tune.block))
+        w.exec_tb == (g->tune.block_single > 0 ? g->tune.block_single : g->tune.block) &&
+        w.exec_split == g->tune.xcd_split)
         return C2G_OK;
     if (w.exec) { (void)hipGraphExecDestroy(w.exec); w.exec = nullptr; }
     hipGraph_t graph;
@@ -519,6 +710,7 @@ static int ensure_exec(Graph* g) {
     w.exec_grid = relax_grid(g);
     w.exec_chunk = g->tune.chunk;
     w.exec_tb = g->tune.block_single > 0 ? g->tune.block_single : g->tune.block;
+    w.exec_split = g->tune.xcd_split;
     return C2G_OK;
 }
 
@@ -554,7 +746,8 @@ int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* 
     const int64_t nwp = bitmap_words(g->V);
     uint32_t* bits = w.bits;
     const double t0 = now_ms();
-    launch_seed(g, ws_bufs(g), src);
+    const bool allow_split = budget <= 0;  // a budgeted run reads its queue back as one list
+    launch_seed(g, ws_bufs(g, allow_split), src);
     HIP_TRY(hipGetLastError());
     const bool prof = g->tune.profile != 0;
     const bool use_graph = g->tune.use_graph && budget <= 0 && !prof;
@@ -566,7 +759,7 @@ int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* 
         if (budget > 0) k = std::min(k, budget - done_rounds);
         if (k <= 0) break;
         if (use_graph) HIP_TRY(hipGraphLaunch(w.exec, g->stream));
-        else if ((rc = launch_rounds(g, k, prof ? &relax_ms : nullptr))) return rc;
+        else if ((rc = launch_rounds(g, k, prof ? &relax_ms : nullptr, allow_split))) return rc;
         done_rounds += k;
         HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, sizeof(int64_t) * C_N, hipMemcpyDeviceToHost,
                                g->stream));
@@ -783,13 +976,14 @@ int qserve_step(Graph* g, hipStream_t strm, uint64_t* slots_with_work) {
     launch_relax(g, strm, grid, s->dist, s->bits, pruned ? hb : nullptr, pruned ? s->hid : nullptr, s->q, s->ctrl,
                  s->nwp);
     if (pruned) {
-        hipLaunchKernelGGL(sssp_expand_k<true>, dim3(egrid), dim3(256), 0, strm, g->d_row, g->d_hrow, s->bits, hb,
-                           s->q, s->ctrl, V, std::max(1, g->tune.chunk), s->qcap, (const int32_t*)s->dist, 0,
-                           s->slots, s->nwp);
+        hipLaunchKernelGGL((sssp_expand_k<true, false>), dim3(egrid), dim3(256), 0, strm, g->d_row, g->d_hrow, s->bits,
+                           hb, s->q, s->ctrl, V, std::max(1, g->tune.chunk), s->qcap, (const int32_t*)s->dist, 0,
+                           s->slots, s->nwp, (const int32_t*)nullptr, (const int32_t*)nullptr);
     } else {
-        hipLaunchKernelGGL(sssp_expand_k<false>, dim3(egrid), dim3(256), 0, strm, g->d_row, (const int32_t*)nullptr,
-                           s->bits, (uint32_t*)nullptr, s->q, s->ctrl, V, std::max(1, g->tune.chunk), s->qcap,
-                           (const int32_t*)s->dist, 0, s->slots, s->nwp);
+        hipLaunchKernelGGL((sssp_expand_k<false, false>), dim3(egrid), dim3(256), 0, strm, g->d_row,
+                           (const int32_t*)nullptr, s->bits, (uint32_t*)nullptr, s->q, s->ctrl, V,
+                           std::max(1, g->tune.chunk), s->qcap, (const int32_t*)s->dist, 0, s->slots, s->nwp,
+                           (const int32_t*)nullptr, (const int32_t*)nullptr);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(s->h_ctrl, s->ctrl, sizeof(int64_t) * C_N, hipMemcpyDeviceToHost, strm));

@@ -117,7 +117,12 @@ typedef struct {
                                results come back in the caller's ids: 0 / 1 */
     int32_t block_single;   /* threads per workgroup of the single-source relax
                                kernel (0 = block): 128, 256 or 512            */
-    int32_t reserved[4];
+    int32_t xcd_split;      /* single-source SSSP: 1 = cut each frontier row by
+                               the partition (one eighth of the vertex range)
+                               of its targets into eight queues, relaxed by the
+                               workgroups of one XCD each, so each XCD's L2
+                               holds one eighth of the distances: 0 / 1     */
+    int32_t reserved[3];
 } c2g_tune;
 
 /* ======================================================================= */

@@ -165,3 +165,65 @@ def test_distances_near_int32_max(gpu):
     assert np.array_equal(db[0], ref)
     assert db[1].tolist() == [INT_MAX, 0, big, big + 5]
     assert db[2].tolist() == [INT_MAX, INT_MAX, 0, 5]
+
+
+# ---- split mode (c2g_tune.xcd_split): frontier rows cut by the target's
+# partition into eight queues, one per XCD's workgroups
+
+@pytest.mark.parametrize("delta,block,chunk", [(0, 256, 32), (0, 512, 8), (16, 128, 64), (64, 256, 1)])
+def test_split_mode_equals_dijkstra(gpu, delta, block, chunk):
+    G = rmat_graph(16, 1, wmax=64)
+    deg = np.diff(G["vlist"])
+    rng = np.random.default_rng(delta + block)
+    srcs = [int(np.argmax(deg))] + [int(x) for x in rng.choice(np.flatnonzero(deg > 0), 5)]
+    refs = oracle.dijkstra_many(G["vlist"], G["elist"], G["wlist"], srcs)
+    with c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], device=0) as g:
+        g.set_tune(xcd_split=1, delta=delta, block_single=block, chunk=chunk)
+        for s, ref in zip(srcs, refs):
+            d, st = g.sssp(s)
+            assert np.array_equal(d, ref), s
+        sums, reach, _ = g.sssp_each_resident(np.array(srcs, np.int32))
+        for k, ref in enumerate(refs):
+            fin = ref != INT_MAX
+            assert sums[k] == ref[fin].astype(np.int64).sum() and reach[k] == fin.sum()
+        # budgeted runs read their queue back and run unsplit
+        d, fr, _ = g.sssp_budget(srcs[0], 2)
+        d2, _ = g.sssp_migrate(srcs[0], 3)
+        assert np.array_equal(d2, refs[0])
+
+
+def test_split_mode_pruned_and_unsorted_rows(gpu):
+    from conftest import random_graph
+    for seed in range(4):
+        G = random_graph(500, 1600, seed=seed, wmax=9)
+        P = c2g.prune_sssp_mem(G["vlist"], G["elist"], G["wlist"])
+        with c2g.Graph.from_csr(P["vlist"], P["elist"], P["wlist"], hvlist=P["hvlist"], helist=P["helist"],
+                                hwlist=P["hwlist"], variant=c2g.VARIANT_SSSP_PURN, device=0) as g:
+            g.set_tune(xcd_split=1)
+            for s in list(range(0, 500, 23)) + [int(x) for x in P["porder"][:10]]:
+                assert np.array_equal(g.sssp(s)[0], oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], s)), (seed, s)
+    # rows whose targets are not sorted go whole into one partition
+    G = random_graph(3000, 20000, seed=7, wmax=20)
+    el, wl = G["elist"].copy(), G["wlist"].copy()
+    rng = np.random.default_rng(1)
+    for u in range(0, 3000, 3):
+        b, e = G["vlist"][u], G["vlist"][u + 1]
+        perm = rng.permutation(e - b)
+        el[b:e], wl[b:e] = el[b:e][perm], wl[b:e][perm]
+    with c2g.Graph.from_csr(G["vlist"], el, wl, device=0) as g:
+        g.set_tune(xcd_split=1)
+        for s in (0, 5, 777, 2999):
+            assert np.array_equal(g.sssp(s)[0], oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], s)), s
+
+
+@pytest.mark.slow
+def test_split_mode_scale20(gpu):
+    G = rmat_graph(20, 2, wmax=255)
+    deg = np.diff(G["vlist"])
+    rng = np.random.default_rng(2)
+    srcs = [int(np.argmax(deg))] + [int(x) for x in rng.choice(np.flatnonzero(deg > 0), 3)]
+    refs = oracle.dijkstra_many(G["vlist"], G["elist"], G["wlist"], srcs)
+    with c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], device=0) as g:
+        g.set_tune(xcd_split=1)
+        for s, ref in zip(srcs, refs):
+            assert np.array_equal(g.sssp(s)[0], ref), s
