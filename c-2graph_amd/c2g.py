@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libc2g.so")
 
 OK, EINVAL, EIO, EFORMAT, ENOMEM, EDEVICE, ENOTSUP, EBUSY, ETIMEOUT = 0, -1, -2, -3, -4, -5, -6, -7, -8
 VARIANT_ORIGIN, VARIANT_SSSP_PURN, VARIANT_PPR_PURN = 0, 1, 2
+ADS_PUSH_BASE, ADS_PUSH_PERK = 0, 1
 LANE_GPU, LANE_HOST, LANE_MIXED = 0, 1, 2
 Q_SSSP, Q_PPR, Q_ADS = 0, 1, 2
 INT_MAX = 2**31 - 1
@@ -38,6 +39,14 @@ class Stats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+class Round(C.Structure):
+    _fields_ = [("frontier", C.c_int64), ("edges", C.c_int64), ("lane_edges", C.c_int64),
+                ("improvements", C.c_int64), ("items", C.c_int64), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Opts(C.Structure):
@@ -368,6 +377,24 @@ class Graph:
         check(lib().c2g_sssp(self._h, int(src), _ptr(d, C.c_int32), C.byref(st)))
         return d, st
 
+    def sssp_rounds(self, src, cap=4096):
+        """c2g_sssp_rounds: distances, stats and the per-round trace (list of dicts)."""
+        d = np.empty(self.V, np.int32)
+        rs = (Round * cap)()
+        n = C.c_int32()
+        st = Stats()
+        check(lib().c2g_sssp_rounds(self._h, int(src), _ptr(d, C.c_int32), rs, cap, C.byref(n), C.byref(st)))
+        return d, st, [rs[i].as_dict() for i in range(min(n.value, cap))]
+
+    def sssp_batch_rounds(self, srcs, cap=4096):
+        srcs = np.ascontiguousarray(srcs, np.int32)
+        rs = (Round * cap)()
+        n = C.c_int32()
+        st = Stats()
+        check(lib().c2g_sssp_batch_rounds(self._h, _ptr(srcs, C.c_int32), len(srcs), rs, cap, C.byref(n),
+                                          C.byref(st)))
+        return st, [rs[i].as_dict() for i in range(min(n.value, cap))]
+
     def sssp_noalloc(self, src, dbuf):
         """c2g_sssp_noalloc: distances into the DeviceBuffer dbuf (int32[V])."""
         if dbuf.nbytes < 4 * self.V:
@@ -473,6 +500,15 @@ class Graph:
         r = np.empty(self.V, np.float32)
         st = Stats()
         check(lib().c2g_adsorption_push(self._h, int(src), _ptr(p, C.c_float), _ptr(r, C.c_float), C.byref(st)))
+        return p, r, st
+
+    def adsorption_push_mode(self, src, mode):
+        """c2g_adsorption_push_mode: 0 = w-base (w-purn on a PPR_PURN graph), 1 = w-base-perk."""
+        p = np.empty(self.V, np.float32)
+        r = np.empty(self.V, np.float32)
+        st = Stats()
+        check(lib().c2g_adsorption_push_mode(self._h, int(src), int(mode), _ptr(p, C.c_float), _ptr(r, C.c_float),
+                                             C.byref(st)))
         return p, r, st
 
     def adsorption(self, seed_v, seed_l, L=16, p_inj=0.25, p_cont=0.70, p_abnd=0.05, iters=20):

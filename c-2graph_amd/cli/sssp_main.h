@@ -4,7 +4,7 @@
 
 // mode: 0 base, 1 purn, 2 base-M, 3 purn-M
 inline int sssp_main(int argc, char** argv, int mode, const char* usage) {
-    Args a = parse_args(argc, argv);
+    Args a = parse_args(argc, argv, {"rounds"});
     const bool mig = mode >= 2;
     if (a.pos.size() < (mig ? 3u : 2u)) {
         fprintf(stderr, "usage: %s\n", usage);
@@ -64,6 +64,18 @@ inline int sssp_main(int argc, char** argv, int mode, const char* usage) {
     for (int i = 0; i < 10 && i < V; i++) std::cout << i << "\t" << dist[i] << std::endl;
     std::cout << "gpu runtime: " << (t1 - t0) << " seconds" << std::endl;
     std::cout << "源顶点source = " << src << std::endl;
+    if (a.has("rounds") && !mig) {
+        // per-iteration frontier sizes, the reference's iteration_act_num
+        // (sssp/base/Main.cu:87-93 prints them in a commented-out loop)
+        std::vector<c2g_round> rs(100000);
+        int32_t nr = 0;
+        int rc2 = c2g_sssp_rounds(g, src, dist.data(), rs.data(), (int32_t)rs.size(), &nr, &st);
+        if (rc2) return die(rc2);
+        std::cout << "iteration\tfrontier\tedges\timprovements\tms" << std::endl;
+        for (int i = 0; i < nr && i < (int)rs.size(); ++i)
+            std::cout << i << "\t" << rs[i].frontier << "\t" << rs[i].edges << "\t" << rs[i].improvements << "\t"
+                      << rs[i].ms << std::endl;
+    }
     int rc = maybe_write(a, dist.data(), V, false);
     c2g_close(g);
     return rc;

@@ -152,11 +152,43 @@ int host_bf(const Graph* g, int32_t* dist, const int32_t* frontier, int32_t coun
 // device must be current); once per device, at upload.
 int sssp_scratch_init(int device);
 
+// Per-round trace of a query (c2g_sssp_rounds / c2g_sssp_batch_rounds):
+// rounds run one at a time and each is recorded, up to `cap`.
+struct RoundLog {
+    c2g_round* out = nullptr;
+    int32_t cap = 0;
+    int32_t n = 0;
+    int64_t prev[4] = {0, 0, 0, 0};  // cumulative frontier, edges, lane edges at the last two snapshots
+    int64_t prev2[4] = {0, 0, 0, 0};
+    int64_t impr = 0, items = 0;     // cumulative improvements and items at the last snapshot
+    // Records the round that just ran from cumulative counters: frontier,
+    // edges and lane edges are counted when a round's work is queued (by the
+    // previous round), improvements and items when it runs.
+    void add(const int64_t* q3, int64_t impr_now, int64_t items_now, double ms) {
+        if (n < cap) {
+            c2g_round& r = out[n];
+            r.frontier = prev[0] - prev2[0];
+            r.edges = prev[1] - prev2[1];
+            r.lane_edges = prev[2] - prev2[2];
+            r.improvements = impr_now - impr;
+            r.items = items_now - items;
+            r.ms = ms;
+        }
+        ++n;
+        for (int k = 0; k < 3; ++k) { prev2[k] = prev[k]; prev[k] = q3[k]; }
+        impr = impr_now;
+        items = items_now;
+    }
+    void start(const int64_t* q3) {
+        for (int k = 0; k < 3; ++k) { prev2[k] = 0; prev[k] = q3[k]; }
+    }
+};
+
 // Used by the server and batched callers.
 int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* frontier_out,
-             int32_t* count_out, uint8_t* hid_out, c2g_stats* st, bool copy_out);
+             int32_t* count_out, uint8_t* hid_out, c2g_stats* st, bool copy_out, RoundLog* log = nullptr);
 int sssp_batch_group(Graph* g, const int32_t* srcs, int32_t n, int32_t* dist_out,
-                     int64_t* sums, int32_t* reached, c2g_stats* st);
+                     int64_t* sums, int32_t* reached, c2g_stats* st, RoundLog* log = nullptr);
 int ppr_single(Graph* g, hipStream_t strm, int32_t src, float alpha, float eps, float* p_out,
                float* r_out, c2g_stats* st);
 // Batched PPR (64 queries per lane group) on `strm`; query q's p (and r)

@@ -219,7 +219,8 @@ ppr_push_k(const int2* __restrict__ adjf, float* __restrict__ m, uint32_t* __res
 
 // Threshold pass over the touched vertices only (the touched bitmap, one
 // 32-vertex word per lane): r[v] += m[v]; if r[v] > eps*outdeg(v) (eps
-// alone when !degscale; always in the `force` first round), p[v] += alpha*r[v], r[v] = 0 and v's items join
+// alone when !degscale; always in the `force` first round), p[v] += credit*r[v]
+// (credit = alpha for PPR), r[v] = 0 and v's items join
 // the next round carrying the residual to push.  Two passes per word: count
 // the items of the activated vertices, take the queue range with one atomic
 // per wave, then apply the updates and write the items.  With ru set, each
@@ -231,7 +232,7 @@ ppr_push_k(const int2* __restrict__ adjf, float* __restrict__ m, uint32_t* __res
 __global__ void __launch_bounds__(256)
 ppr_expand_k(const int32_t* __restrict__ row, float* __restrict__ p, float* __restrict__ r,
              float* __restrict__ m, uint32_t* __restrict__ tb, int4* __restrict__ q, int64_t* __restrict__ ctrl,
-             int64_t V, float alpha, float eps, int chunk, int64_t qcap, int force, int degscale,
+             int64_t V, float credit, float eps, int chunk, int64_t qcap, int force, int degscale,
              int2* __restrict__ ru, uint32_t* __restrict__ stamp, int64_t pull_edges, int can_pull) {
     const int lane = lane_id();
     const int S1 = ru ? (int)*stamp + 1 : 0;
@@ -270,7 +271,7 @@ ppr_expand_k(const int32_t* __restrict__ row, float* __restrict__ p, float* __re
             const float rv = r[v] + mv;
             const int r0 = row[v], r1 = row[v + 1];
             if (force || rv > (degscale ? (float)(r1 - r0) * eps : eps)) {
-                p[v] += alpha * rv;
+                p[v] += credit * rv;
                 r[v] = 0.0f;
                 if (ru) ru[v] = make_int2(S1, __float_as_int(rv));
                 front += 1;
@@ -710,6 +711,8 @@ struct PushSpec {
     int degscale = 1;
     float r0 = 1.0f;
     float p_src = 0.0f;
+    float credit = -1.0f;     // share of an activated residual credited to p (< 0: alpha)
+    bool hidden_seed = true;  // pruned graphs: seed H(s) and N'(s) (ppr-purn)
 };
 
 static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, float eps, int32_t budget,
@@ -733,6 +736,7 @@ static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, f
     PprState* s = pstate(g);
     if ((rc = ensure_ppr_single(g, s))) return rc;
     const bool pruned = ppr_pruned(g);
+    const float credit = spec.credit < 0.0f ? alpha : spec.credit;
     const int64_t V = g->V;
     const int grid = std::max(1, g->num_cus) * g->tune.grid_per_cu;
     const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (V + 255) / 256));
@@ -746,11 +750,12 @@ static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, f
     const int64_t pull_edges = use_pull ? std::max<int64_t>(1, g->E * g->tune.ppr_pull / 1000) : 0;
     const double t0 = now_ms();
     hipLaunchKernelGGL(ppr_init_k, dim3(egrid), dim3(256), 0, strm, s->p, s->r, s->m, s->tb, V);
+    const bool hseed = pruned && spec.hidden_seed;
     hipLaunchKernelGGL(ppr_seed_k, dim3(1), dim3(256), 0, strm, g->d_row, g->d_adjf,
-                       pruned ? g->d_hrow : (const int32_t*)nullptr, pruned ? g->d_hadjf : (const int2*)nullptr,
-                       s->p, s->m, s->tb, s->ctrl, src, alpha, pruned ? 1 : 0, spec.r0, spec.p_src);
+                       hseed ? g->d_hrow : (const int32_t*)nullptr, hseed ? g->d_hadjf : (const int2*)nullptr,
+                       s->p, s->m, s->tb, s->ctrl, src, alpha, hseed ? 1 : 0, spec.r0, spec.p_src);
     hipLaunchKernelGGL(ppr_expand_k, dim3(wgrid), dim3(256), 0, strm, g->d_row, s->p, s->r, s->m, s->tb, s->q,
-                       s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 1, spec.degscale, ru, s->stamp, pull_edges,
+                       s->ctrl, V, credit, eps, g->tune.chunk, s->qcap, 1, spec.degscale, ru, s->stamp, pull_edges,
                        can_pull);
     HIP_TRY(hipGetLastError());
     if (use_pull) s->stamp_est += 1;
@@ -762,7 +767,7 @@ static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, f
             hipLaunchKernelGGL(ppr_push_k, dim3(grid), dim3(256), 0, strm, g->d_adjf, s->m, s->tb, s->q,
                                s->ctrl, alpha, s->tadjf, s->titems, can_pull ? s->nt : 0, s->ru, s->stamp);
             hipLaunchKernelGGL(ppr_expand_k, dim3(wgrid), dim3(256), 0, strm, g->d_row, s->p, s->r, s->m,
-                               s->tb, s->q, s->ctrl, V, alpha, eps, g->tune.chunk, s->qcap, 0, spec.degscale,
+                               s->tb, s->q, s->ctrl, V, credit, eps, g->tune.chunk, s->qcap, 0, spec.degscale,
                                ru, s->stamp, pull_edges, can_pull);
         }
         HIP_TRY(hipGetLastError());
@@ -963,22 +968,34 @@ int c2g_ppr_migrate(c2g_graph* h, int32_t src, float alpha, float eps, int32_t b
     return C2G_OK;
 }
 
-int c2g_adsorption_push(c2g_graph* h, int32_t src, float* p_out, float* r_out, c2g_stats* st) {
+int c2g_adsorption_push_mode(c2g_graph* h, int32_t src, int32_t mode, float* p_out, float* r_out, c2g_stats* st) {
     if (!h || !p_out) return fail(C2G_EINVAL, "null argument");
+    if (mode != C2G_ADS_PUSH_BASE && mode != C2G_ADS_PUSH_PERK) return fail(C2G_EINVAL, "unknown push mode %d", mode);
     C2G_HANDLE_LOCK(h);
     Graph* g = &h->g;
     if (st) memset(st, 0, sizeof *st);
-    if (ppr_pruned(g)) return fail(C2G_ENOTSUP, "the adsorption push runs on the full graph (origin variant)");
-    // src/adsorption/w-base/Main.cu:63-64 (alpha 0.2, rmax 1/V), :149-152
-    // (r[s] = 10, p[s] += 0.8 * r[s]), :248 (message (1-alpha) r w), :351-355
-    // (r += m; r > rmax activates with p += alpha r).
+    // alpha 0.2, r[s] = 10, messages (1-alpha) r w (adsorption/w-base/Main.cu:63-64,
+    // :149-152, :248).  BASE: p[s] += 0.8 * r[s] and activations credit
+    // p += alpha r (:351-355), rmax 1/V; on a purn-ppr graph this is
+    // adsorption/w-purn, the same push on the live pruned CSR with no hidden
+    // seeding (its seeding is commented out, w-purn/Main.cu:157-169) and rmax
+    // 1/(V+1) (:67).  PERK: p[s] += alpha * r[s] and activations credit
+    // p += (1-alpha) r (adsorption/w-base-perk/Main.cu:306-309, :555-557),
+    // rmax 1/V (Graph.h:117).
     const float alpha = 0.2f;
     PushSpec spec;
     spec.degscale = 0;
     spec.r0 = 10.0f;
-    spec.p_src = (1.0f - 2.0f * alpha) * spec.r0;  // the forced first activation adds alpha*r0
-    const float rmax = 1.0f / (float)std::max(g->V, 1);
+    spec.hidden_seed = false;
+    spec.credit = mode == C2G_ADS_PUSH_PERK ? 1.0f - alpha : alpha;
+    // the source's forced first activation adds credit*r0 to p_src
+    spec.p_src = ((mode == C2G_ADS_PUSH_PERK ? alpha : 0.8f) - spec.credit) * spec.r0;
+    const float rmax = 1.0f / (float)(std::max(g->V, 1) + (mode == C2G_ADS_PUSH_BASE && ppr_pruned(g) ? 1 : 0));
     return ppr_single_ex(g, g->stream, src, alpha, rmax, 0, 0, p_out, r_out, nullptr, nullptr, st, spec);
+}
+
+int c2g_adsorption_push(c2g_graph* h, int32_t src, float* p_out, float* r_out, c2g_stats* st) {
+    return c2g_adsorption_push_mode(h, src, C2G_ADS_PUSH_BASE, p_out, r_out, st);
 }
 
 int c2g_ppr_batch(c2g_graph* h, const int32_t* srcs, int32_t n, float alpha, float eps, float* p_out,

@@ -735,7 +735,7 @@ static int64_t reached_edges(const Graph* g, const int32_t* dist) {
 // Runs one query on the device.  budget > 0 stops after `budget` rounds and
 // leaves the remaining frontier in frontier_out (vertices, deduplicated).
 int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* frontier_out,
-             int32_t* count_out, uint8_t* hid_out, c2g_stats* st, bool copy_out) {
+             int32_t* count_out, uint8_t* hid_out, c2g_stats* st, bool copy_out, RoundLog* log) {
     if (!g->on_device) return fail(C2G_EDEVICE, "graph has no device image (no HIP device)");
     if (!g->d_adj) return fail(C2G_ENOTSUP, "graph has no int weights (wlist) for SSSP");
     if (src < 0 || src >= g->V) return fail(C2G_EINVAL, "source %d out of range [0,%d)", src, g->V);
@@ -750,21 +750,43 @@ int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* 
     launch_seed(g, ws_bufs(g, allow_split), src);
     HIP_TRY(hipGetLastError());
     const bool prof = g->tune.profile != 0;
-    const bool use_graph = g->tune.use_graph && budget <= 0 && !prof;
+    const bool one = prof || log;  // one round per host check
+    const bool use_graph = g->tune.use_graph && budget <= 0 && !one;
     if (use_graph && (rc = ensure_exec(g))) return rc;
+    auto snap = [&](int64_t* q3) {
+        q3[0] = w.h_ctrl[C_FRONT];
+        q3[1] = w.h_ctrl[C_EDGES] + w.h_ctrl[C_HEDGES];
+        q3[2] = q3[1];
+    };
+    if (log) {
+        HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, sizeof(int64_t) * C_N, hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        int64_t q3[3];
+        snap(q3);
+        log->start(q3);
+    }
     int done_rounds = 0;
     double relax_ms = 0;
     for (;;) {
-        int k = prof ? 1 : g->tune.iters_per_graph;
+        int k = one ? 1 : g->tune.iters_per_graph;
         if (budget > 0) k = std::min(k, budget - done_rounds);
         if (k <= 0) break;
+        if (log) HIP_TRY(hipEventRecord(w.ev[0], g->stream));
         if (use_graph) HIP_TRY(hipGraphLaunch(w.exec, g->stream));
         else if ((rc = launch_rounds(g, k, prof ? &relax_ms : nullptr, allow_split))) return rc;
+        if (log) HIP_TRY(hipEventRecord(w.ev[1], g->stream));
         done_rounds += k;
         HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, sizeof(int64_t) * C_N, hipMemcpyDeviceToHost,
                                g->stream));
         HIP_TRY(hipStreamSynchronize(g->stream));
         if (w.h_ctrl[C_OVF]) return fail(C2G_EDEVICE, "frontier queue overflow (internal)");
+        if (log) {
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, w.ev[0], w.ev[1]));
+            int64_t q3[3];
+            snap(q3);
+            log->add(q3, w.h_ctrl[C_IMPR], w.h_ctrl[C_ITEMS], ms);
+        }
         if (w.h_ctrl[C_CUR] == 0 && w.h_ctrl[C_FAR] == 0) break;
     }
     const double t1 = now_ms();
@@ -1124,6 +1146,19 @@ int c2g_sssp_noalloc(c2g_graph* h, int32_t src, int32_t* d_dist, c2g_stats* st) 
     C2G_HANDLE_LOCK(h);
     if (st) memset(st, 0, sizeof *st);
     return sssp_noalloc_run(&h->g, src, d_dist, st);
+}
+
+int c2g_sssp_rounds(c2g_graph* h, int32_t src, int32_t* dist_out, c2g_round* rounds, int32_t cap,
+                    int32_t* nrounds, c2g_stats* st) {
+    if (!h || !dist_out || (cap > 0 && !rounds) || cap < 0) return fail(C2G_EINVAL, "c2g_sssp_rounds: bad argument");
+    C2G_HANDLE_LOCK(h);
+    if (st) memset(st, 0, sizeof *st);
+    RoundLog log;
+    log.out = rounds;
+    log.cap = cap;
+    int rc = sssp_run(&h->g, src, 0, dist_out, nullptr, nullptr, nullptr, st, true, &log);
+    if (nrounds) *nrounds = log.n;
+    return rc;
 }
 
 int c2g_sssp_budget(c2g_graph* h, int32_t src, int32_t budget, int32_t* dist_out,

@@ -678,7 +678,8 @@ static int ensure_exec_b(Graph* g, int groups) {
 }
 
 // Runs n queries: lane q of group q/64 starts at srcs[q].
-static int batch_run(Graph* g, const int32_t* srcs, int32_t n, c2g_stats* st, double* relax_ms) {
+static int batch_run(Graph* g, const int32_t* srcs, int32_t n, c2g_stats* st, double* relax_ms,
+                     RoundLog* log = nullptr) {
     BatchWs& w = g->batch;
     BatchState* s = bstate(g);
     const int groups = (n + 63) / 64;
@@ -697,22 +698,51 @@ static int batch_run(Graph* g, const int32_t* srcs, int32_t n, c2g_stats* st, do
     // Turn the seeded activations into the first round's items.
     expand_b(g, groups, g->stream);
     HIP_TRY(hipGetLastError());
-    const bool use_graph = g->tune.use_graph && !relax_ms;
+    const bool one = relax_ms || log;  // one round per host check
+    const bool use_graph = g->tune.use_graph && !one;
     int rc;
     if (use_graph && (rc = ensure_exec_b(g, groups))) return rc;
+    auto snap = [&](int64_t* q3) {
+        q3[0] = w.h_ctrl[B_FRONT];
+        q3[1] = w.h_ctrl[B_EDGES];
+        q3[2] = w.h_ctrl[B_LEDGES];
+    };
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (log) {
+        HIP_TRY(hipEventCreate(&ev[0]));
+        if (hipEventCreate(&ev[1]) != hipSuccess) { (void)hipEventDestroy(ev[0]); return hip_fail(hipErrorOutOfMemory, "hipEventCreate"); }
+        HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, sizeof(int64_t) * B_N, hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        int64_t q3[3];
+        snap(q3);
+        log->start(q3);
+    }
+    struct EvGuard {
+        hipEvent_t* e;
+        ~EvGuard() { for (int k = 0; k < 2; ++k) if (e[k]) (void)hipEventDestroy(e[k]); }
+    } evg{ev};
     for (;;) {
+        if (log) HIP_TRY(hipEventRecord(ev[0], g->stream));
         if (use_graph) {
             HIP_TRY(hipGraphLaunch(w.exec, g->stream));
         } else {
             // Profiled runs check after every round so that no empty round
             // is launched (launch count == rounds with work).
-            const int k = relax_ms ? 1 : g->tune.iters_per_graph;
+            const int k = one ? 1 : g->tune.iters_per_graph;
             for (int r = 0; r < k; ++r)
                 if ((rc = launch_round_b(g, groups, relax_ms))) return rc;
         }
+        if (log) HIP_TRY(hipEventRecord(ev[1], g->stream));
         HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, sizeof(int64_t) * B_N, hipMemcpyDeviceToHost, g->stream));
         HIP_TRY(hipStreamSynchronize(g->stream));
         if (w.h_ctrl[B_OVF]) return fail(C2G_EDEVICE, "batched frontier capacity exceeded (internal)");
+        if (log) {
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            int64_t q3[3];
+            snap(q3);
+            log->add(q3, w.h_ctrl[B_IMPR], w.h_ctrl[B_ITEMS_TOT], ms);
+        }
         if (w.h_ctrl[B_ITEMS] == 0 && w.h_ctrl[B_HITEMS] == 0 && w.h_ctrl[B_FAR] == 0) break;
     }
     if (st) {
@@ -768,7 +798,7 @@ static int check_srcs(const Graph* g, const int32_t* srcs, int32_t n) {
 }
 
 int sssp_batch_group(Graph* g, const int32_t* srcs, int32_t n, int32_t* dist_out, int64_t* sums,
-                     int32_t* reached, c2g_stats* st) {
+                     int32_t* reached, c2g_stats* st, RoundLog* log) {
     int rc = check_srcs(g, srcs, n);
     if (rc) return rc;
     const int groups = (n + 63) / 64;
@@ -776,7 +806,7 @@ int sssp_batch_group(Graph* g, const int32_t* srcs, int32_t n, int32_t* dist_out
     const double t0 = now_ms();
     double relax_ms = 0;
     const bool prof = g->tune.profile != 0;
-    if ((rc = batch_run(g, srcs, n, st, prof ? &relax_ms : nullptr))) return rc;
+    if ((rc = batch_run(g, srcs, n, st, prof ? &relax_ms : nullptr, log))) return rc;
     const double t1 = now_ms();
     int64_t medges = 0;
     if ((rc = batch_checksums(g, n, sums, reached, &medges))) return rc;
@@ -981,6 +1011,19 @@ int c2g_sssp_batch(c2g_graph* h, const int32_t* srcs, int32_t n, int32_t* dist_o
     C2G_HANDLE_LOCK(h);
     if (st) memset(st, 0, sizeof *st);
     return sssp_batch_group(&h->g, srcs, n, dist_out, nullptr, nullptr, st);
+}
+
+int c2g_sssp_batch_rounds(c2g_graph* h, const int32_t* srcs, int32_t n, c2g_round* rounds, int32_t cap,
+                          int32_t* nrounds, c2g_stats* st) {
+    if (!h || !srcs || (cap > 0 && !rounds) || cap < 0) return fail(C2G_EINVAL, "c2g_sssp_batch_rounds: bad argument");
+    C2G_HANDLE_LOCK(h);
+    if (st) memset(st, 0, sizeof *st);
+    RoundLog log;
+    log.out = rounds;
+    log.cap = cap;
+    int rc = sssp_batch_group(&h->g, srcs, n, nullptr, nullptr, nullptr, st, &log);
+    if (nrounds) *nrounds = log.n;
+    return rc;
 }
 
 int c2g_sssp_batch_resident(c2g_graph* h, const int32_t* srcs, int32_t n, int64_t* sums,

@@ -91,6 +91,18 @@ typedef struct {
     int32_t pad;
 } c2g_stats;
 
+/* One frontier round of a traced query (c2g_sssp_rounds, c2g_sssp_batch_rounds):
+ * the reference records the frontier size of every iteration in
+ * iteration_act_num (src/sssp/base/Main.cu:345-350). */
+typedef struct {
+    int64_t frontier;     /* vertices relaxed this round (batched: (vertex, group) pairs) */
+    int64_t edges;        /* CSR edges streamed (hidden edges included)                   */
+    int64_t lane_edges;   /* (query, edge) relaxations: = edges for one query            */
+    int64_t improvements; /* relaxations that lowered a distance (writes issued)         */
+    int64_t items;        /* work items                                                   */
+    double  ms;           /* device time of the round (relax + expand kernels)           */
+} c2g_round;
+
 /* Tunables picked by the autotuner (c2g_autotune) or set by hand. */
 typedef struct {
     int32_t block;          /* threads per workgroup of the batched SSSP relax
@@ -279,6 +291,18 @@ int c2g_sssp_each_resident(c2g_graph* g, const int32_t* srcs, int32_t n,
  * frontier_out has room for V entries. */
 int c2g_sssp_budget(c2g_graph* g, int32_t src, int32_t budget, int32_t* dist_out,
                     int32_t* frontier_out, int32_t* count_out, c2g_stats* st);
+/* c2g_sssp with a per-round trace (CalcuSSSP's iteration_act_num,
+ * src/sssp/base/Main.cu:345-350; printed by the commented-out loop at
+ * :87-93).  Rounds run one launch pair at a time with a host read-back after
+ * each, so the query is slower than c2g_sssp; the distances are the same.
+ * Up to `cap` rounds are written to rounds[]; *nrounds gets the count of all
+ * rounds (may exceed cap). */
+int c2g_sssp_rounds(c2g_graph* g, int32_t src, int32_t* dist_out, c2g_round* rounds, int32_t cap,
+                    int32_t* nrounds, c2g_stats* st);
+/* c2g_sssp_batch_resident with the same per-round trace (no reference
+ * counterpart: the reference runs one query per launch). */
+int c2g_sssp_batch_rounds(c2g_graph* g, const int32_t* srcs, int32_t n, c2g_round* rounds, int32_t cap,
+                          int32_t* nrounds, c2g_stats* st);
 /* c2g_sssp without any allocation (no reference counterpart: it is for callers
  * that cannot allocate, such as tight serving loops or code that must not
  * call hipMalloc).  The distances land in the caller's DEVICE array
@@ -346,11 +370,29 @@ int c2g_ppr_batch(c2g_graph* g, const int32_t* srcs, int32_t n, float alpha,
 /* ======================================================================= */
 /* The reference's own "adsorption" kernel, kept as a reference mode:
  * calcuatePPR of src/adsorption/w-base/Main.cu:123-395, a single-source push
- * on the full graph with alpha = 0.2, r[s] = 10, p[s] = 0.8 * r[s]
+ * with alpha = 0.2, r[s] = 10, p[s] = 0.8 * r[s]
  * (:63-64, :149-152), messages (1-alpha) r_u f(u,v) (:248) and a constant
  * threshold rmax = 1/V (:351-355).  Rounds run to convergence (the reference
  * caps them at 2000, :186).  p_out, r_out: float[V] (r_out nullable). */
 int c2g_adsorption_push(c2g_graph* g, int32_t src, float* p_out, float* r_out, c2g_stats* st);
+
+/* The reference's other adsorption push variants (same seeding r[s] = 10,
+ * alpha 0.2, messages (1-alpha) r w):
+ *   C2G_ADS_PUSH_BASE  p[s] += 0.8 r[s], activation credits alpha r, rmax 1/V:
+ *                      c2g_adsorption_push.  On a PPR_PURN graph this is
+ *                      adsorption/w-purn: the push on the live pruned CSR,
+ *                      no hidden seeding (commented out in the reference,
+ *                      src/adsorption/w-purn/Main.cu:157-169), rmax 1/(V+1)
+ *                      (:67).
+ *   C2G_ADS_PUSH_PERK  adsorption/w-base-perk: p[s] += alpha r[s]
+ *                      (src/adsorption/w-base-perk/Main.cu:306-309),
+ *                      activation credits (1-alpha) r (:555-557), rmax 1/V
+ *                      (Graph.h:117); runs on either graph variant (the
+ *                      reference's loader reads ppr/).                   */
+#define C2G_ADS_PUSH_BASE 0
+#define C2G_ADS_PUSH_PERK 1
+int c2g_adsorption_push_mode(c2g_graph* g, int32_t src, int32_t mode, float* p_out, float* r_out,
+                             c2g_stats* st);
 
 /* Multi-label adsorption (reference: adsorption/w-base/Main.cu:123-395 is a
  * single-source push; the multi-label recurrence here follows Baluja et al.

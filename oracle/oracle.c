@@ -260,8 +260,9 @@ int orc_ppr_power(int32_t V, const int32_t* vl, const int32_t* el, const float* 
  * r[v] > eps*outdeg(v) and a zero-initialised flag.  p, r updated in place
  * from the active list `frontier` (count entries); FP32 state like the
  * reference.  Returns the number of rounds. */
+/* credit: the share of an activated residual credited to p (alpha for PPR). */
 static int push_rounds(int32_t V, const int32_t* vl, const int32_t* el, const float* fl,
-                       float alpha, float eps, int degscale, float* p, float* r,
+                       float alpha, float eps, int degscale, float credit, float* p, float* r,
                        const int32_t* frontier, int32_t count, int32_t max_rounds,
                        int64_t* push_out) {
     if (count <= 0) return 0;
@@ -295,7 +296,7 @@ static int push_rounds(int32_t V, const int32_t* vl, const int32_t* el, const fl
             msg[v] = 0.0f;
             flag[v] = 0;
             if (r[v] > (degscale ? (float)(vl[v + 1] - vl[v]) * eps : eps)) {
-                p[v] += r[v] * alpha;
+                p[v] += r[v] * credit;
                 act[na++] = v;
             }
         }
@@ -309,7 +310,7 @@ int orc_forward_push(int32_t V, const int32_t* vl, const int32_t* el, const floa
                      float alpha, float eps, float* p, float* r,
                      const int32_t* frontier, int32_t count, int32_t max_rounds,
                      int64_t* push_out) {
-    return push_rounds(V, vl, el, fl, alpha, eps, 1, p, r, frontier, count, max_rounds, push_out);
+    return push_rounds(V, vl, el, fl, alpha, eps, 1, alpha, p, r, frontier, count, max_rounds, push_out);
 }
 
 /* The reference's adsorption kernel (src/adsorption/w-base/Main.cu): alpha
@@ -322,7 +323,27 @@ int orc_adsorption_push(int32_t V, const int32_t* vl, const int32_t* el, const f
     for (int32_t v = 0; v < V; ++v) { p[v] = 0.0f; r[v] = 0.0f; }
     r[src] = 10.0f;
     p[src] += 0.8f * r[src];
-    return push_rounds(V, vl, el, fl, alpha, rmax, 0, p, r, &src, 1, max_rounds, push_out);
+    return push_rounds(V, vl, el, fl, alpha, rmax, 0, alpha, p, r, &src, 1, max_rounds, push_out);
+}
+
+/* The reference's other adsorption variants.  mode 0 with rmax 1/(V+1):
+ * adsorption/w-purn (w-base's push on the purn-ppr graph's live CSR, whose
+ * hidden-list seeding is commented out, w-purn/Main.cu:67, :157-169,
+ * :183-186).  mode 1: adsorption/w-base-perk, whose source gets
+ * p[s] += alpha*r[s] (:306-309) and whose activation credits
+ * p[v] += (1-alpha)*r[v] (:555-557), rmax 1/V (w-base-perk/Graph.h:117),
+ * alpha 0.2 (PPR.cpp:20).  Pinned (test_oracle.py) by the push invariant
+ * rewritten for the credit rule: with A = total residual activated,
+ * |10*pi_s - alpha*A|_1 = sum(r), where A = p / credit after the source's
+ * seeding credit is taken out. */
+int orc_adsorption_push_mode(int32_t V, const int32_t* vl, const int32_t* el, const float* fl, int32_t src,
+                             int32_t mode, float rmax, float* p, float* r, int32_t max_rounds, int64_t* push_out) {
+    const float alpha = 0.2f;
+    for (int32_t v = 0; v < V; ++v) { p[v] = 0.0f; r[v] = 0.0f; }
+    r[src] = 10.0f;
+    p[src] += (mode == 1 ? alpha : 0.8f) * r[src];
+    return push_rounds(V, vl, el, fl, alpha, rmax, 0, mode == 1 ? 1.0f - alpha : alpha, p, r, &src, 1, max_rounds,
+                       push_out);
 }
 
 /* Seeds r[s]=1, p[s]=alpha (src/ppr/base/Main.cu:146-150) then pushes. */

@@ -40,6 +40,8 @@ def lib():
         L.orc_ppr_push.argtypes = [C.c_int32, i32, i32, f32, C.c_int32, C.c_float, C.c_float, f32, f32,
                                    C.c_int32, i64]
         L.orc_adsorption_push.argtypes = [C.c_int32, i32, i32, f32, C.c_int32, f32, f32, C.c_int32, i64]
+        L.orc_adsorption_push_mode.argtypes = [C.c_int32, i32, i32, f32, C.c_int32, C.c_int32, C.c_float, f32, f32,
+                                               C.c_int32, i64]
         L.orc_adsorption.argtypes = [C.c_int32, i32, i32, f32, i32, i32, C.c_int32, C.c_int32, C.c_double,
                                      C.c_double, C.c_int32, f64]
         L.orc_adsorption.restype = None
@@ -119,6 +121,19 @@ def adsorption_push(vlist, elist, flist, src, max_rounds=1 << 30):
     pushes = np.zeros(1, np.int64)
     rounds = lib().orc_adsorption_push(V, _c(vlist, np.int32), _c(elist, np.int32), _c(flist, np.float32),
                                        int(src), p, r, int(max_rounds), pushes)
+    return p, r, rounds, int(pushes[0])
+
+
+def adsorption_push_mode(vlist, elist, flist, src, mode, rmax, max_rounds=1 << 30):
+    """The reference's adsorption variants: mode 0 = w-base's push with the
+    given rmax (w-purn: 1/(V+1) on the pruned CSR), mode 1 = w-base-perk's
+    credit rule.  Returns p, r, rounds, pushes."""
+    V = len(vlist) - 1
+    p = np.empty(V, np.float32)
+    r = np.empty(V, np.float32)
+    pushes = np.zeros(1, np.int64)
+    rounds = lib().orc_adsorption_push_mode(V, _c(vlist, np.int32), _c(elist, np.int32), _c(flist, np.float32),
+                                            int(src), int(mode), float(rmax), p, r, int(max_rounds), pushes)
     return p, r, rounds, int(pushes[0])
 
 

@@ -79,6 +79,16 @@ def no_dangling_graph(V, E, seed):
     return dict(V=V, E=len(v), vlist=vl, elist=v.astype(np.int32), flist=f)
 
 
+def site_graph(scale, seed):
+    """The web-like site graph of bench.py (where purn-sssp / purn-ppr drop
+    about a quarter of the edges)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    return bench.site_graph(scale, seed=seed)
+
+
 def check_sssp_certificate(vl, el, wl, src, d):
     """Exactness certificate for int SSSP with positive weights (Graph500-style):
     d[s]=0; d[v] <= d[u]+w on every edge out of a reached u; every reached v != s

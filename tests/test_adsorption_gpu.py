@@ -8,7 +8,7 @@ import pytest
 
 import c2g
 import oracle
-from conftest import no_dangling_graph, random_graph, rmat_graph
+from conftest import no_dangling_graph, random_graph, rmat_graph, site_graph
 
 pytestmark = pytest.mark.gpu
 
@@ -95,14 +95,67 @@ def test_reference_push_mode(gpu):
             assert st.iterations > 1
 
 
-def test_reference_push_mode_refuses_pruned_graph(gpu):
-    G = random_graph(300, 900, seed=4)
+def _push_invariant_l1(p, s, pi, mode, alpha=0.2, r0=10.0):
+    # with A the residual activated in total, |r0 pi_s - alpha A|_1 = sum(r);
+    # A follows from p through the mode's credits (seeding 0.8 r0 or alpha r0,
+    # activations alpha r or (1-alpha) r)
+    q = p.astype(np.float64)
+    q[s] -= (0.8 if mode == c2g.ADS_PUSH_BASE else alpha) * r0
+    aA = alpha * q / (alpha if mode == c2g.ADS_PUSH_BASE else 1.0 - alpha)
+    aA[s] += alpha * r0
+    return np.abs(r0 * pi - aA).sum()
+
+
+def test_perk_push_mode(gpu):
+    # adsorption/w-base-perk: p[s] = alpha*r[s] at seeding, activations credit
+    # (1-alpha) r, rmax 1/V; against the oracle's serial run (L1 <= 2e-4) and
+    # the push invariant rewritten for that credit rule
+    G = no_dangling_graph(3000, 30000, seed=5)
+    V = G["V"]
+    with c2g.Graph.from_csr(G["vlist"], G["elist"], flist=G["flist"], device=0) as g:
+        for s in (1, 77, V - 2):
+            p, r, st = g.adsorption_push_mode(s, c2g.ADS_PUSH_PERK)
+            po, ro, _, _ = oracle.adsorption_push_mode(G["vlist"], G["elist"], G["flist"], s, 1, 1.0 / V)
+            assert np.abs(p.astype(np.float64) - po).sum() <= 2e-4, s
+            assert r.max() <= 1.0 / V + 1e-7 and st.iterations > 1
+            pi = oracle.ppr_power(G["vlist"], G["elist"], G["flist"], s, alpha=0.2, tol=1e-15)
+            l1 = _push_invariant_l1(p, s, pi, c2g.ADS_PUSH_PERK)
+            assert abs(l1 - r.astype(np.float64).sum()) <= 1e-3, s
+        # mode 0 is c2g_adsorption_push
+        pa, ra, _ = g.adsorption_push(9)
+        pb, rb, _ = g.adsorption_push_mode(9, c2g.ADS_PUSH_BASE)
+        assert np.abs(pa.astype(np.float64) - pb).sum() <= 2e-4
+        with pytest.raises(c2g.C2GError):
+            g.adsorption_push_mode(0, 2)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_pruned_push_modes(gpu, mode):
+    # adsorption/w-purn on a purn-ppr graph: the push on the live pruned CSR
+    # with no hidden seeding and rmax 1/(V+1) (mode 0); w-base-perk's rule on
+    # the same CSR with rmax 1/V (mode 1).  Against the oracle on the live CSR.
+    G = site_graph(11, seed=7)
     P = c2g.prune_ppr_mem(G["vlist"], G["elist"], G["flist"])
+    V = G["V"]
+    assert len(P["elist"]) < 0.8 * G["E"]
+    assert np.diff(P["hvlist"])[V - 1] > 0  # a source with hidden edges, which are not seeded
+    rmax = 1.0 / (V + 1) if mode == 0 else 1.0 / V
     with c2g.Graph.from_csr(P["vlist"], P["elist"], flist=P["wlist"], hvlist=P["hvlist"], helist=P["helist"],
                             hwf=P["hwlist"], variant=c2g.VARIANT_PPR_PURN, device=0) as g:
-        with pytest.raises(c2g.C2GError) as e:
-            g.adsorption_push(0)
-        assert e.value.code == c2g.ENOTSUP
+        for s in (0, 13, V - 1):
+            p, r, st = g.adsorption_push_mode(s, mode)
+            po, ro, _, _ = oracle.adsorption_push_mode(P["vlist"], P["elist"], P["wlist"], s, mode, rmax)
+            assert np.abs(p.astype(np.float64) - po).sum() <= 2e-4, s
+            assert r.max() <= rmax * (1 + 1e-6), s
+            # no hidden seeding: with the full seed at s, p[s] starts from
+            # the seeding credit plus the source's activation
+            assert p[s] >= (0.8 if mode == 0 else 0.2) * 10.0
+    if mode == 0:
+        # threshold 1/(V+1) differs from w-base's 1/V: the runs stop at
+        # different points on the same CSR
+        pb, _, _, _ = oracle.adsorption_push_mode(P["vlist"], P["elist"], P["wlist"], 13, 0, 1.0 / V)
+        assert not np.array_equal(pb, oracle.adsorption_push_mode(P["vlist"], P["elist"], P["wlist"], 13, 0,
+                                                                  1.0 / (V + 1))[0])
 
 
 @pytest.mark.slow

@@ -51,6 +51,14 @@ def test_convert2binary_tool(tmp_path):
     assert c2g.read_i32(d + "/origin/elist.bin").tolist() == [1, 2, 0, 1, 3]
 
 
+@pytest.mark.gpu
+def test_sssp_base_prints_rounds(gpu, ds):
+    # --rounds: the per-iteration frontier sizes (iteration_act_num)
+    out = run("sssp-base", ds, 0, "--reps", 1, "--rounds").stdout
+    lines = out[out.index("iteration\tfrontier"):].strip().splitlines()[1:]
+    assert len(lines) >= 2 and lines[0].split("\t")[1] == "1"
+
+
 def test_usage_errors():
     assert run("sssp-base", check=False).returncode == 2
     assert run("purn-sssp", check=False).returncode == 2

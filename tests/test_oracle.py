@@ -219,3 +219,36 @@ def test_dijkstra_many_equals_single():
     many = oracle.dijkstra_many(G["vlist"], G["elist"], G["wlist"], srcs, threads=3)
     for s, d in zip(srcs, many):
         assert np.array_equal(d, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], s))
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_adsorption_push_modes_invariant(mode):
+    # w-purn's threshold (mode 0, rmax 1/(V+1)) and w-base-perk's credit rule
+    # (mode 1: p[s] = alpha*r0 at seeding, p[v] += (1-alpha)*r[v] at activation).
+    # With A the total residual activated (the source's r0 included), the
+    # push invariant |r0*pi_s - alpha*A|_1 = sum(r) holds for either rule;
+    # A follows from p through the rule's credits.
+    G = no_dangling_graph(400, 3000, seed=11)
+    alpha, r0 = 0.2, 10.0
+    V = G["V"]
+    rmax = 1.0 / (V + 1) if mode == 0 else 1.0 / V
+    for s in (3, 250):
+        p, r, rounds, pushes = oracle.adsorption_push_mode(G["vlist"], G["elist"], G["flist"], s, mode, rmax)
+        pi = oracle.ppr_power(G["vlist"], G["elist"], G["flist"], s, alpha=alpha, tol=1e-15)
+        q = p.astype(np.float64)
+        seed_credit = 0.8 if mode == 0 else alpha
+        credit = alpha if mode == 0 else 1.0 - alpha
+        q[s] -= seed_credit * r0
+        aA = alpha * q / credit
+        aA[s] += alpha * r0
+        l1 = np.abs(r0 * pi - aA).sum()
+        assert abs(l1 - r.astype(np.float64).sum()) <= 1e-3, (mode, s, l1, r.sum())
+        assert r.max() <= rmax * (1 + 1e-6) and rounds > 1 and pushes > 0
+        # the other rule's credit breaks the invariant
+        wrong = alpha * q / (1.0 - credit)
+        wrong[s] += alpha * r0
+        assert abs(np.abs(r0 * pi - wrong).sum() - r.astype(np.float64).sum()) > 0.5
+    # mode 0 with rmax 1/V is the w-base push exactly
+    p0, r0v, _, _ = oracle.adsorption_push_mode(G["vlist"], G["elist"], G["flist"], 3, 0, 1.0 / V)
+    pb, rb, _, _ = oracle.adsorption_push(G["vlist"], G["elist"], G["flist"], 3)
+    assert np.array_equal(p0, pb) and np.array_equal(r0v, rb)

@@ -227,3 +227,30 @@ def test_split_mode_scale20(gpu):
         g.set_tune(xcd_split=1)
         for s, ref in zip(srcs, refs):
             assert np.array_equal(g.sssp(s)[0], ref), s
+
+
+def test_round_trace_single_and_batched(gpu):
+    # per-round counters (the reference's iteration_act_num): the rounds add up
+    # to the query's totals, the first round relaxes the source alone, and
+    # the distances equal the untraced query's
+    G = rmat_graph(16, 1, wmax=64)
+    s = int(np.argmax(np.diff(G["vlist"])))
+    with c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], device=0) as g:
+        d0, st0 = g.sssp(s)
+        d, st, rs = g.sssp_rounds(s)
+        assert np.array_equal(d, d0) and len(rs) == st.iterations
+        assert rs[0]["frontier"] == 1 and rs[0]["edges"] == G["vlist"][s + 1] - G["vlist"][s]
+        assert sum(r["frontier"] for r in rs) == st.frontier_total
+        assert sum(r["edges"] for r in rs) == st.edges_relaxed
+        assert sum(r["improvements"] for r in rs) == st.improvements
+        assert sum(r["items"] for r in rs) == st.items and all(r["ms"] > 0 for r in rs)
+        srcs = np.flatnonzero(np.diff(G["vlist"]) > 0)[:100].astype(np.int32)
+        stb, rb = g.sssp_batch_rounds(srcs)
+        sums, reach, st2 = g.sssp_batch_resident(srcs)
+        assert len(rb) >= stb.iterations
+        assert sum(r["frontier"] for r in rb) == stb.frontier_total
+        assert sum(r["lane_edges"] for r in rb) == stb.edges_relaxed
+        assert sum(r["edges"] for r in rb) == stb.csr_edges
+        assert sum(r["improvements"] for r in rb) == stb.improvements
+        _, _, rs2 = g.sssp_rounds(s, cap=2)
+        assert len(rs2) == 2  # the trace is cut at cap
