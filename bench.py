@@ -381,6 +381,13 @@ def ppr_purn_study(args):
         mx.append(float(d.max()))
         hidden_src += int(P["hvlist"][s_ + 1] > P["hvlist"][s_])
     sr = [float(gf.ppr(int(s_))[1].astype(np.float64).sum()) for s_ in srcs[:8]]  # the push's own truncation
+    # the reference's adsorption push on both graphs (w-base on the full graph,
+    # w-purn on the pruned one; r[s] = 10, so L1 is divided by 10)
+    al1 = []
+    for s_ in srcs[:8]:
+        pf, _, _ = gf.adsorption_push_mode(int(s_), c2g.ADS_PUSH_BASE)
+        pp, _, _ = gp.adsorption_push_mode(int(s_), c2g.ADS_PUSH_BASE)
+        al1.append(float(np.abs(pp.astype(np.float64) - pf).sum() / 10.0))
     gf.close()
     gp.close()
     out = {"workload": f"{gname} (V={G['V']}, E={G['E']}), 32 sources, alpha 0.15, eps 1e-7*outdeg",
@@ -388,6 +395,7 @@ def ppr_purn_study(args):
            "prune_s": round(t_prune, 1), "sources_with_hidden_list": hidden_src,
            "l1_pruned_vs_full": {"mean": float(np.mean(l1)), "max": float(np.max(l1))},
            "maxabs_pruned_vs_full": float(np.max(mx)), "push_sum_r_full_mean": float(np.mean(sr)),
+           "adsorption_push_l1_per_unit_w_purn_vs_w_base": {"mean": float(np.mean(al1)), "max": float(np.max(al1))},
            "ms_per_query": {"full": round(tf / len(srcs), 3), "pruned": round(tp / len(srcs), 3)}}
     # purn-sssp on the same graph: pruned SSSP must equal the full graph's
     Q = c2g.prune_sssp_mem(G["vlist"], G["elist"], G["wlist"])
