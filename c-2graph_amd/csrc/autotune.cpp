@@ -8,8 +8,10 @@
 // distances (the tunables change scheduling only), so tuning never changes
 // results.  Search: coordinate descent over grid_per_cu_single and chunk
 // (single-source) or grid_per_cu and batch_chunk (batched), then
-// iters_per_graph, then the relax kernels' workgroup size (block) and, for
-// single-source, the XCD split; two passes, best of two runs per candidate.
+// iters_per_graph, then the relax kernels' workgroup size (block), the
+// light-edge bound (off, or 1/2, 3/4, 1, 3/2, 2 times the default) and, for
+// single-source, the
+// XCD split; two passes, best of two runs per candidate.
 #include "graph.h"
 #include <cstring>
 
@@ -17,13 +19,15 @@ using namespace c2g;
 
 namespace {
 
-// Captured launch sequences hold tunables as kernel arguments.
-void set_tune_locked(Graph* g, const c2g_tune& t) {
+// Captured launch sequences hold tunables as kernel arguments; a new light
+// bound rebuilds the light rows.
+int set_tune_locked(Graph* g, const c2g_tune& t) {
     if (memcmp(&g->tune, &t, sizeof t) != 0) {
         if (g->sssp.exec) { (void)hipGraphExecDestroy(g->sssp.exec); g->sssp.exec = nullptr; }
         if (g->batch.exec) { (void)hipGraphExecDestroy(g->batch.exec); g->batch.exec = nullptr; }
     }
     g->tune = t;
+    return light_build(g);
 }
 
 double time_once(Graph* g, const int32_t* srcs, int32_t n, bool batched, int* rc) {
@@ -72,8 +76,8 @@ extern "C" int c2g_autotune_ppr(c2g_graph* h, const int32_t* srcs, int32_t n, fl
     const int32_t m = std::min<int32_t>(n, 16);
     c2g_tune best = g->tune;
     best.profile = 0;
-    set_tune_locked(g, best);
-    int rc = C2G_OK;
+    int rc = set_tune_locked(g, best);
+    if (rc) return rc;
     (void)time_ppr(g, srcs, 1, alpha, eps, &rc);  // warm: allocations, in-edge rows
     double tbest = time_ppr(g, srcs, m, alpha, eps, &rc);
     for (int v : {0, 25, 50, 100, 200, 400, 800}) {
@@ -81,13 +85,14 @@ extern "C" int c2g_autotune_ppr(c2g_graph* h, const int32_t* srcs, int32_t n, fl
         c2g_tune t = best;
         t.ppr_pull = v;
         if (!memcmp(&t, &best, sizeof t)) continue;
-        set_tune_locked(g, t);
+        if ((rc = set_tune_locked(g, t))) break;
         const double tt = time_ppr(g, srcs, m, alpha, eps, &rc);
         if (rc) break;
         if (tt < tbest * 0.98) { tbest = tt; best = t; }
     }
-    set_tune_locked(g, best);
+    const int rs = set_tune_locked(g, best);
     if (rc) return rc;
+    if (rs) return rs;
     if (best_out) *best_out = best;
     return C2G_OK;
 }
@@ -103,38 +108,50 @@ extern "C" int c2g_autotune(c2g_graph* h, const int32_t* srcs, int32_t n, int32_
     const int32_t m = b ? std::min<int32_t>(n, 64 * 4) : std::min<int32_t>(n, 16);
     c2g_tune best = g->tune;
     best.profile = 0;
-    set_tune_locked(g, best);
-    int rc = C2G_OK;
+    int rc = set_tune_locked(g, best);
+    if (rc) return rc;
     (void)time_once(g, srcs, m, b, &rc);  // warm allocations and code objects
     double tbest = measure(g, srcs, m, b, &rc);
     const std::vector<int> grids = {2, 4, 8, 16};
     const std::vector<int> chunks = b ? std::vector<int>{16, 32, 64, 128} : std::vector<int>{16, 32, 64, 128, 256};
     const std::vector<int> iters = {4, 8, 16};
     const std::vector<int> blocks = {128, 256, 512};
+    const int L0 = best.light > 0 ? best.light : default_light(g);
+    std::vector<int> lights = {0};
+    if (L0 > 0)
+        for (int v : {L0 / 2, 3 * L0 / 4, L0, 3 * L0 / 2, 2 * L0})
+            if (v > 0 && std::find(lights.begin(), lights.end(), v) == lights.end()) lights.push_back(v);
     const std::vector<int> splits = {0, 1};
-    const int dims = b ? 4 : 5;  // single-source also tries the XCD split
+    const int dims = b ? 5 : 6;  // single-source also tries the XCD split
     for (int pass = 0; pass < 2 && rc == C2G_OK; ++pass) {
         for (int which = 0; which < dims && rc == C2G_OK; ++which) {
-            const std::vector<int>& vals =
-                which == 0 ? grids : which == 1 ? chunks : which == 2 ? iters : which == 3 ? blocks : splits;
+            const std::vector<int>& vals = which == 0   ? grids
+                                           : which == 1 ? chunks
+                                           : which == 2 ? iters
+                                           : which == 3 ? blocks
+                                           : which == 4 ? lights
+                                                        : splits;
             for (int v : vals) {
                 c2g_tune t = best;
                 if (which == 0) (b ? t.grid_per_cu : t.grid_per_cu_single) = v;
                 else if (which == 1) (b ? t.batch_chunk : t.chunk) = v;
                 else if (which == 2) t.iters_per_graph = v;
                 else if (which == 3) (b ? t.block : t.block_single) = v;
+                else if (which == 4) t.light = v;
                 else t.xcd_split = v;
                 if (!memcmp(&t, &best, sizeof t)) continue;
-                set_tune_locked(g, t);
+                if ((rc = set_tune_locked(g, t))) break;
                 const double tt = measure(g, srcs, m, b, &rc);
                 if (rc) break;
                 if (tt < tbest * 0.98) { tbest = tt; best = t; }
             }
-            set_tune_locked(g, best);
+            const int rs = set_tune_locked(g, best);
+            if (!rc) rc = rs;
         }
     }
-    set_tune_locked(g, best);
+    const int rs = set_tune_locked(g, best);
     if (rc) return rc;
+    if (rs) return rs;
     if (best_out) *best_out = best;
     return C2G_OK;
 }

@@ -93,6 +93,7 @@ struct BatchWs {
     int32_t* dist = nullptr;     // [group][V][64]
     uint64_t* mask[1] = {nullptr};  // [group][V] lanes pending expansion
     uint64_t* hmask = nullptr;   // [group][V] lanes whose value came through a hidden edge
+    uint64_t* ph = nullptr;      // [group][V] lanes whose heavy edges (c2g_tune.light) are pending
     int64_t* ctrl = nullptr;
     int64_t* h_ctrl = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -122,6 +123,13 @@ struct Graph {
     // partitions of each row, [V+1][8] (built on first use)
     int32_t* d_split = nullptr;
     int32_t* d_hsplit = nullptr;
+    // Light-edge rows of SSSP (c2g_tune.light): when built, d_adj holds the
+    // E live entries followed by each vertex's edges of weight < light_built,
+    // row by row, and d_lrow[v] .. d_lrow[v+1] (absolute indices >= E) is
+    // v's light row.
+    int32_t* d_lrow = nullptr;
+    int32_t light_built = 0;
+    int64_t EL = 0;
     SsspWs sssp;
     BatchWs batch;
     c2g_tune tune{};
@@ -142,7 +150,19 @@ void ppr_release(Graph* g);
 void ads_release(Graph* g);
 void serve_release(Graph* g);
 void default_tune(Graph* g);
+int default_light(const Graph* g);
 int ensure_sssp_ws(Graph* g);
+// Builds (or drops) the light-edge rows for g->tune.light; captured launch
+// sequences are invalidated when d_adj moves.
+int light_build(Graph* g);
+// The light rows when the SSSP kernels should use them (light > 0, built,
+// delta == 0, xcd_split == 0), else null.
+inline const int32_t* light_rows(const Graph* g) {
+    return g->tune.light > 0 && g->light_built == g->tune.light && g->d_lrow && g->tune.delta == 0 &&
+                   g->tune.xcd_split == 0
+               ? g->d_lrow
+               : nullptr;
+}
 
 // Host-side helpers shared by lanes
 int host_bf(const Graph* g, int32_t* dist, const int32_t* frontier, int32_t count,

@@ -64,6 +64,96 @@ void default_tune(Graph* g) {
     t.iters_per_graph = 8;
     t.use_graph = 1;
     t.delta = 0;
+    t.light = default_light(g);
+}
+
+// Light-edge bound: about a sixteenth of the mean weight (on R-MAT with
+// uniform weights U[1,255] and U[1,1023] the measured optimum of batched SSSP
+// lies at 1/16 to 1/21, of single-source SSSP at 1/8 to 1/21).  0 when almost
+// no edge would be light (a vertex would then only wait a round).
+int default_light(const Graph* g) {
+    const int64_t E = (int64_t)g->wl.size();
+    int light = 0;
+    if (E > 0) {
+        const int nt = std::max(1, std::min(64, host_threads()));
+        std::vector<double> part(64, 0.0);
+        parallel_for(E, [&](int64_t lo, int64_t hi, int tid) {
+            double a = 0;
+            for (int64_t i = lo; i < hi; ++i) a += (double)g->wl[i];
+            part[tid] += a;
+        }, nt, 1 << 20);
+        double sum = 0;
+        for (double x : part) sum += x;
+        const int64_t L = std::min<int64_t>(std::max<int64_t>(1, std::llround(sum / E / 16.0)), INT32_MAX);
+        std::vector<int64_t> cnt(64, 0);
+        parallel_for(E, [&](int64_t lo, int64_t hi, int tid) {
+            int64_t c = 0;
+            for (int64_t i = lo; i < hi; ++i) c += g->wl[i] < L;
+            cnt[tid] += c;
+        }, nt, 1 << 20);
+        int64_t nl = 0;
+        for (int64_t x : cnt) nl += x;
+        light = nl * 100 >= E ? (int)L : 0;
+    }
+    return light;
+}
+
+// Light-edge rows for tune.light (see Graph::d_lrow).  The light entries are
+// packed like the live ones (bit 31 of the target: the target has edges) and
+// appended after them in a new d_adj; the old buffer is freed.
+int light_build(Graph* g) {
+    const int L = g->tune.light;
+    if (!g->on_device || g->wl.empty() || g->E == 0 || L <= 0) {
+        g->light_built = 0;
+        return C2G_OK;
+    }
+    if (L == g->light_built && g->d_lrow) return C2G_OK;
+    HIP_TRY(hipSetDevice(g->device));
+    const int64_t V = g->V, E = g->E;
+    std::vector<int32_t> lr((size_t)V + 1);
+    parallel_for(V, [&](int64_t lo, int64_t hi, int) {
+        for (int64_t v = lo; v < hi; ++v) {
+            int32_t c = 0;
+            for (int32_t e = g->vl[v]; e < g->vl[v + 1]; ++e) c += g->wl[e] < L;
+            lr[v + 1] = c;
+        }
+    }, 0, 1 << 14);
+    lr[0] = 0;
+    for (int64_t v = 0; v < V; ++v) lr[v + 1] += lr[v];
+    const int64_t EL = lr[V];
+    if (E + EL > INT32_MAX) return fail(C2G_ENOTSUP, "light rows would need more than 2^31 edge entries");
+    std::vector<int2> tail((size_t)std::max<int64_t>(EL, 1));
+    parallel_for(V, [&](int64_t lo, int64_t hi, int) {
+        for (int64_t v = lo; v < hi; ++v) {
+            int64_t k = lr[v];
+            for (int32_t e = g->vl[v]; e < g->vl[v + 1]; ++e) {
+                if (g->wl[e] >= L) continue;
+                const int32_t t = g->el[e];
+                const bool nz = g->vl[t + 1] > g->vl[t] || (!g->hvl.empty() && g->hvl[t + 1] > g->hvl[t]);
+                tail[k++] = make_int2(nz ? (int32_t)((uint32_t)t | 0x80000000u) : t, g->wl[e]);
+            }
+        }
+    }, 0, 1 << 14);
+    for (int64_t v = 0; v <= V; ++v) lr[v] = (int32_t)(lr[v] + E);
+    int2* nb = nullptr;
+    HIP_TRY(hipMalloc(&nb, sizeof(int2) * (size_t)(E + std::max<int64_t>(EL, 1))));
+    hipError_t e1 = hipMemcpy(nb, g->d_adj, sizeof(int2) * (size_t)E, hipMemcpyDeviceToDevice);
+    if (e1 == hipSuccess && EL > 0) e1 = hipMemcpy(nb + E, tail.data(), sizeof(int2) * EL, hipMemcpyHostToDevice);
+    if (e1 == hipSuccess && !g->d_lrow) e1 = hipMalloc(&g->d_lrow, sizeof(int32_t) * (size_t)(V + 1));
+    if (e1 == hipSuccess) e1 = hipMemcpy(g->d_lrow, lr.data(), sizeof(int32_t) * (size_t)(V + 1), hipMemcpyHostToDevice);
+    if (e1 != hipSuccess) {
+        (void)hipFree(nb);
+        g->light_built = 0;
+        return hip_fail(e1, "light rows");
+    }
+    // captured launch sequences hold the old d_adj
+    if (g->sssp.exec) { (void)hipGraphExecDestroy(g->sssp.exec); g->sssp.exec = nullptr; }
+    if (g->batch.exec) { (void)hipGraphExecDestroy(g->batch.exec); g->batch.exec = nullptr; }
+    (void)hipFree(g->d_adj);
+    g->d_adj = nb;
+    g->EL = EL;
+    g->light_built = L;
+    return C2G_OK;
 }
 
 static int load_dir(Graph* g, const std::string& ds, int variant, const c2g_opts* o) {
@@ -157,7 +247,7 @@ int device_upload(Graph* g, int device) {
     }
     (void)align_up;
     g->on_device = true;
-    return C2G_OK;
+    return light_build(g);
 }
 
 template <class T>
@@ -190,12 +280,14 @@ void device_release(Graph* g) {
     if (g->sssp.h_ctrl) (void)hipHostFree(g->sssp.h_ctrl);
     g->sssp.out.release();
     for (auto& e : g->sssp.ev) if (e) (void)hipEventDestroy(e);
-    dfree(g->batch.dist); dfree(g->batch.mask[0]); dfree(g->batch.hmask); dfree(g->batch.ctrl);
+    dfree(g->batch.dist); dfree(g->batch.mask[0]); dfree(g->batch.hmask); dfree(g->batch.ph); dfree(g->batch.ctrl);
     if (g->batch.h_ctrl) (void)hipHostFree(g->batch.h_ctrl);
     dfree(g->d_row); dfree(g->d_adj); dfree(g->d_adjf); dfree(g->d_hrow); dfree(g->d_hadj);
     dfree(g->d_hadjf);
     dfree(g->d_split);
     dfree(g->d_hsplit);
+    dfree(g->d_lrow);
+    g->light_built = 0;
     if (g->stream) (void)hipStreamDestroy(g->stream);
     g->stream = nullptr;
     g->on_device = false;
@@ -478,6 +570,7 @@ int c2g_set_tune(c2g_graph* h, const c2g_tune* t) {
     if (t->iters_per_graph < 2 || t->iters_per_graph > 256 || (t->iters_per_graph & 1))
         return fail(C2G_EINVAL, "iters_per_graph must be even, in [2,256]");
     if (t->delta < 0) return fail(C2G_EINVAL, "delta >= 0");
+    if (t->light < 0) return fail(C2G_EINVAL, "light >= 0");
     C2G_HANDLE_LOCK(h);
     if (memcmp(&h->g.tune, t, sizeof *t) != 0) {
         // Captured launch sequences hold the old values as kernel arguments.
@@ -485,7 +578,7 @@ int c2g_set_tune(c2g_graph* h, const c2g_tune* t) {
         if (h->g.batch.exec) { (void)hipGraphExecDestroy(h->g.batch.exec); h->g.batch.exec = nullptr; }
     }
     h->g.tune = *t;
-    return C2G_OK;
+    return light_build(&h->g);
 }
 
 int c2g_get_tune(const c2g_graph* h, c2g_tune* t) {

@@ -30,6 +30,14 @@
 // A slot pool runs several queries side by side with the same kernels: slot s
 // owns dist[s*V..], its own bitmap words and hidden flags (server lane).
 //
+// Light-edge mode (c2g_tune.light = L > 0, delta 0, no split): a vertex
+// lowered in the last round relaxes only its light row (edges of weight < L,
+// kept after the E live entries of `adj`) and is marked in the pending
+// bitmap; the first round in which it is not lowered again relaxes its full
+// row and clears the mark.  Heavy edges are then relaxed with a settled
+// distance rather than after every improvement: about half the relaxations
+// on R-MAT with uniform weights, for a few more rounds.
+//
 // Pruned graphs: a vertex's hidden list is relaxed when the vertex is the
 // source or when its distance was ever lowered through a hidden edge
 // (flag `hid`).  Hidden edges stand for real paths, so extra relaxation can
@@ -77,7 +85,8 @@ constexpr int kIdMask = 0x7fffffff;
 constexpr int kSteps = 4;  // 64-edge steps per lane whose loads are in flight together
 
 __global__ void sssp_init_k(int32_t* __restrict__ dist, uint32_t* __restrict__ bits,
-                            uint32_t* __restrict__ hbits, uint8_t* __restrict__ hid, int64_t V) {
+                            uint32_t* __restrict__ hbits, uint8_t* __restrict__ hid, int64_t V,
+                            uint32_t* __restrict__ pbits) {
     const int64_t nw = (V + 31) >> 5;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < V;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -86,6 +95,7 @@ __global__ void sssp_init_k(int32_t* __restrict__ dist, uint32_t* __restrict__ b
         if (i < nw) {
             bits[i] = 0;
             if (hbits) hbits[i] = 0;
+            if (pbits) pbits[i] = 0;
         }
     }
 }
@@ -261,13 +271,18 @@ sssp_relax_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
 // vertices with dist < ctrl[C_THR] are expanded; the others keep their bit
 // for a later round, and when a round expands nothing the last workgroup
 // moves the bound to the bucket of the smallest deferred distance.
+// Light-edge mode (lrow and pbits non-null; delta 0, not split): a set bit
+// queues the vertex's light row and moves the bit to pbits; a pbits bit
+// without a bits bit queues the full row and is cleared.  Pending bits keep
+// ctrl[C_FAR] (and the slot's C_SLOTS bit) set.
 template <bool PRUNED, bool SPLIT>
 __global__ void __launch_bounds__(256)
 sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
               uint32_t* __restrict__ bits, uint32_t* __restrict__ hbits, int4* __restrict__ q,
               int64_t* __restrict__ ctrl, int64_t V, int chunk, int64_t qcap,
               const int32_t* __restrict__ dist, int delta, int slots, int64_t nwp,
-              const int32_t* __restrict__ split, const int32_t* __restrict__ hsplit) {
+              const int32_t* __restrict__ split, const int32_t* __restrict__ hsplit,
+              const int32_t* __restrict__ lrow, uint32_t* __restrict__ pbits) {
     // NP queues: one, or in split mode one per target partition (queue p at
     // q + p * qcap, qcap items each)
     constexpr int NP = SPLIT ? kParts : 1;
@@ -276,6 +291,7 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
     const int64_t nwr = (int64_t)slots * nwp;  // nwp is a multiple of 64: a wave stays in one slot
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int32_t T = (int32_t)ctrl[C_THR];
+    const bool lt = !SPLIT && lrow != nullptr && pbits != nullptr;
     int64_t front = 0, edges = 0, hedges = 0, near_n = 0;
     int32_t fmin = INT32_MAX;
     // Edge range of partition p of v's live (or hidden) list.
@@ -301,11 +317,16 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
         const bool inr = i < nwr;
         const int slot = inr ? (int)(i / nwp) : 0;
         const int64_t wi = inr ? i - (int64_t)slot * nwp : nw;
-        uint32_t wl = 0, wh = 0;
+        uint32_t wl = 0, wh = 0, wp = 0;
         if (wi < nw) {
             wl = bits[i];
             if (PRUNED) wh = hbits[i];
+            if (lt) wp = pbits[i];
         }
+        // light mode: full rows of the pending vertices not lowered again
+        const uint32_t hv = lt ? wp & ~wl : 0u;
+        if (lt && wi < nw && wp != wl) pbits[i] = wl;
+        if (lt && wl) fmin = 0;  // pending vertices remain (ctrl[C_FAR])
         // Split by distance: near bits are expanded now, far bits stay.
         uint32_t nl = wl, nh = wh;
         if (delta > 0 && (wl | wh)) {
@@ -328,6 +349,11 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
         for (uint32_t m = nl; m; m &= m - 1) {
             const int v = (int)(wi * 32 + __builtin_ctz(m));
             front += 1;
+            if (lt) {
+                edges += lrow[v + 1] - lrow[v];
+                cnt[0] += (lrow[v + 1] - lrow[v] + chunk - 1) / chunk;
+                continue;
+            }
             edges += row[v + 1] - row[v];
 #pragma unroll
             for (int p = 0; p < NP; ++p) {
@@ -335,6 +361,12 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
                 range(false, v, p, b, e);
                 cnt[p] += (e - b + chunk - 1) / chunk;
             }
+        }
+        for (uint32_t m = hv; m; m &= m - 1) {
+            const int v = (int)(wi * 32 + __builtin_ctz(m));
+            front += 1;
+            edges += row[v + 1] - row[v];
+            cnt[0] += (row[v + 1] - row[v] + chunk - 1) / chunk;
         }
         if (PRUNED)
             for (uint32_t m = nh; m; m &= m - 1) {
@@ -347,7 +379,7 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
                     cnt[p] += (e - b + chunk - 1) / chunk;
                 }
             }
-        near_n += __popc(nl | nh);
+        near_n += __popc(nl | nh | hv);
         int off[NP], tot[NP];
         int any = 0;
 #pragma unroll
@@ -378,20 +410,31 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
             pos[p] = base + off[p];
         }
         __syncthreads();  // s_tot and s_base are rewritten in the next iteration
+        // a slot with pending vertices (light mode) still has work
+        const bool pend = __ballot(lt && wl != 0) != 0;
+        if ((any || pend) && lane == 0) atomicOr((unsigned long long*)&ctrl[C_SLOTS], 1ull << slot);
         if (any == 0) continue;
-        if (lane == 0) atomicOr((unsigned long long*)&ctrl[C_SLOTS], 1ull << slot);
         if (!room) {
             if (lane == 0) atomicOr((unsigned long long*)&ctrl[C_OVF], 1ull);
             continue;
         }
         for (uint32_t m = nl; m; m &= m - 1) {
             const int v = (int)(wi * 32 + __builtin_ctz(m));
+            if (lt) {
+                for (int b = lrow[v], e = lrow[v + 1]; b < e; b += chunk)
+                    q[pos[0]++] = make_int4(v, b, min(b + chunk, e), tag);
+                continue;
+            }
 #pragma unroll
             for (int p = 0; p < NP; ++p) {
                 int b0, e;
                 range(false, v, p, b0, e);
                 for (int b = b0; b < e; b += chunk) q[p * qcap + pos[p]++] = make_int4(v, b, min(b + chunk, e), tag);
             }
+        }
+        for (uint32_t m = hv; m; m &= m - 1) {
+            const int v = (int)(wi * 32 + __builtin_ctz(m));
+            for (int b = row[v], e = row[v + 1]; b < e; b += chunk) q[pos[0]++] = make_int4(v, b, min(b + chunk, e), tag);
         }
         if (PRUNED)
             for (uint32_t m = nh; m; m &= m - 1) {
@@ -542,8 +585,8 @@ int ensure_sssp_ws(Graph* g) {
     // Each piece is checked on its own: after a failed allocation the next
     // call allocates what is missing instead of using it.
     if (!w.dist) HIP_TRY(hipMalloc(&w.dist, sizeof(int32_t) * std::max<int64_t>(V, 1)));
-    // live and hidden activation bitmaps
-    if (!w.bits) HIP_TRY(hipMalloc(&w.bits, sizeof(uint32_t) * nwp * 2));
+    // live, hidden and pending (light mode) activation bitmaps
+    if (!w.bits) HIP_TRY(hipMalloc(&w.bits, sizeof(uint32_t) * nwp * 3));
     if (!w.ctrl) HIP_TRY(hipMalloc(&w.ctrl, sizeof(int64_t) * C_N));
     if (!w.ev[0]) HIP_TRY(hipEventCreate(&w.ev[0]));
     if (!w.ev[1]) HIP_TRY(hipEventCreate(&w.ev[1]));
@@ -571,7 +614,7 @@ static bool is_pruned(const Graph* g) { return g->d_hrow != nullptr && g->d_hadj
 // the caller's distance array plus the static scratch (c2g_sssp_noalloc).
 struct SsspBufs {
     int32_t* dist;     // [V]
-    uint32_t* bits;    // live [nwp], then hidden [nwp]
+    uint32_t* bits;    // live [nwp], then hidden [nwp], then pending (light mode) [nwp]
     uint8_t* hid;      // [V] (pruned graphs)
     int4* q;           // frontier items
     int64_t qcap;      // items per queue (split mode: per partition)
@@ -633,6 +676,8 @@ static int launch_rounds_on(Graph* g, const SsspBufs& b, int k, double* relax_ms
     uint32_t* hbits = bits + nwp;
     const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (nwp + 255) / 256));
     const bool split = b.split != nullptr;
+    const int32_t* lrow = split ? nullptr : light_rows(g);
+    uint32_t* pbits = lrow ? bits + 2 * nwp : nullptr;
     for (int r = 0; r < k; ++r) {
         if (relax_ms) HIP_TRY(hipEventRecord(w.ev[0], g->stream));
         launch_relax(g, g->stream, grid, b.dist, bits, pruned ? hbits : nullptr, pruned ? b.hid : nullptr, b.q, b.ctrl,
@@ -644,7 +689,7 @@ static int launch_rounds_on(Graph* g, const SsspBufs& b, int k, double* relax_ms
             hipLaunchKernelGGL((sssp_expand_k<P, S>), dim3(egrid), dim3(256), 0, g->stream, g->d_row,
                                P ? g->d_hrow : (const int32_t*)nullptr, bits, P ? hbits : (uint32_t*)nullptr, b.q,
                                b.ctrl, (int64_t)g->V, g->tune.chunk, b.qcap, (const int32_t*)b.dist, g->tune.delta, 1,
-                               nwp, b.split, P ? b.hsplit : (const int32_t*)nullptr);
+                               nwp, b.split, P ? b.hsplit : (const int32_t*)nullptr, lrow, pbits);
         };
         if (pruned) {
             if (split) expand(std::true_type(), std::true_type());
@@ -675,7 +720,7 @@ static void launch_seed(Graph* g, const SsspBufs& b, int32_t src) {
     const int tb = 256, gridv = (int)std::min<int64_t>((g->V + tb - 1) / tb, 65535);
     hipLaunchKernelGGL(sssp_init_k, dim3(std::max(gridv, 1)), dim3(tb), 0, g->stream, b.dist, b.bits,
                        pruned ? b.bits + nwp : (uint32_t*)nullptr, pruned ? b.hid : (uint8_t*)nullptr,
-                       (int64_t)g->V);
+                       (int64_t)g->V, b.bits + 2 * nwp);
     hipLaunchKernelGGL(sssp_seed_k, dim3(1), dim3(256), 0, g->stream, g->d_row,
                        pruned ? g->d_hrow : (const int32_t*)nullptr, b.dist,
                        pruned ? b.hid : (uint8_t*)nullptr, b.q, b.ctrl, src, g->tune.chunk, g->tune.delta, b.split,
@@ -805,11 +850,13 @@ int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* 
         std::vector<uint8_t> seen((size_t)g->V, 0);
         for (auto& it : items)
             if (!seen[it.x]) { seen[it.x] = 1; if (frontier_out) frontier_out[m] = it.x; ++m; }
-        // vertices deferred by delta-stepping are still pending
+        // vertices deferred by delta-stepping, or whose full row is pending
+        // (light mode), are still pending
         const int64_t nwd = (g->V + 31) / 32;
-        std::vector<uint32_t> bw((size_t)nwd * (pruned ? 2 : 1));
+        std::vector<uint32_t> bw((size_t)nwd * 3, 0u);
         HIP_TRY(hipMemcpy(bw.data(), bits, sizeof(uint32_t) * nwd, hipMemcpyDeviceToHost));
         if (pruned) HIP_TRY(hipMemcpy(bw.data() + nwd, bits + nwp, sizeof(uint32_t) * nwd, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(bw.data() + 2 * nwd, bits + 2 * nwp, sizeof(uint32_t) * nwd, hipMemcpyDeviceToHost));
         for (int64_t k = 0; k < (int64_t)bw.size(); ++k)
             for (uint32_t x = bw[k]; x; x &= x - 1) {
                 const int32_t v = (int32_t)((k % nwd) * 32 + __builtin_ctz(x));
@@ -833,13 +880,14 @@ int sssp_run(Graph* g, int32_t src, int32_t budget, int32_t* dist_out, int32_t* 
 // ------------------------------------------------------------ slot pool
 // Clears slot s before a query joins it.
 __global__ void slot_reset_k(int32_t* __restrict__ dist, uint8_t* __restrict__ hid, uint32_t* __restrict__ bits,
-                             uint32_t* __restrict__ hbits, int64_t V, int64_t nw) {
+                             uint32_t* __restrict__ hbits, int64_t V, int64_t nw, uint32_t* __restrict__ pbits) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < V; i += (int64_t)gridDim.x * blockDim.x) {
         dist[i] = INT32_MAX;
         if (hid) hid[i] = 0;
         if (i < nw) {
             bits[i] = 0;
             if (hbits) hbits[i] = 0;
+            pbits[i] = 0;
         }
     }
 }
@@ -878,7 +926,7 @@ struct SlotPool {
     int slots = 0;
     int64_t nwp = 0, qcap = 0;
     int32_t* dist = nullptr;   // [slots][V]
-    uint32_t* bits = nullptr;  // live [slots][nwp], then hidden [slots][nwp]
+    uint32_t* bits = nullptr;  // live [slots][nwp], then hidden [slots][nwp], then pending [slots][nwp]
     uint8_t* hid = nullptr;    // [slots][V] (pruned graphs)
     int4* q = nullptr;
     int64_t* ctrl = nullptr;
@@ -917,7 +965,7 @@ static int64_t qserve_bytes(const Graph* g, int slots) {
     const int64_t V = std::max<int64_t>(g->V, 1);
     const int chunk = std::max(1, g->tune.chunk);
     const int64_t qcap = (int64_t)slots * (2 * V + (g->E + g->HE) / chunk) + 64;
-    return 4 * slots * V + 8 * slots * bitmap_words(V) + (g->hvl.empty() ? 0 : slots * V) + 16 * qcap +
+    return 4 * slots * V + 12 * slots * bitmap_words(V) + (g->hvl.empty() ? 0 : slots * V) + 16 * qcap +
            8 * C_N;
 }
 
@@ -941,7 +989,7 @@ int qserve_init(Graph* g, hipStream_t strm, int slots) {
     HIP_TRY(hipSetDevice(g->device));
     const int64_t needq = (int64_t)slots * (2 * std::max<int64_t>(g->V, 1) + (g->E + g->HE) / std::max(1, g->tune.chunk)) + 64;
     if (s->slots == slots && s->qcap >= needq) {
-        HIP_TRY(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * 2 * slots * s->nwp, strm));
+        HIP_TRY(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * 3 * slots * s->nwp, strm));
         HIP_TRY(hipMemsetAsync(s->ctrl, 0, sizeof(int64_t) * C_N, strm));
         HIP_TRY(hipStreamSynchronize(strm));
         return C2G_OK;
@@ -958,8 +1006,8 @@ int qserve_init(Graph* g, hipStream_t strm, int slots) {
     // per slot and round: each vertex at most once per list, ceil(d/chunk) <= d/chunk + 1 items
     s->qcap = (int64_t)slots * (2 * V + (g->E + g->HE) / chunk) + 64;
     HIP_TRY(hipMalloc(&s->dist, sizeof(int32_t) * slots * V));
-    HIP_TRY(hipMalloc(&s->bits, sizeof(uint32_t) * 2 * slots * s->nwp));
-    HIP_TRY(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * 2 * slots * s->nwp, strm));
+    HIP_TRY(hipMalloc(&s->bits, sizeof(uint32_t) * 3 * slots * s->nwp));
+    HIP_TRY(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * 3 * slots * s->nwp, strm));
     if (g->d_hrow) HIP_TRY(hipMalloc(&s->hid, (size_t)slots * V));
     HIP_TRY(hipMalloc(&s->q, sizeof(int4) * s->qcap));
     HIP_TRY(hipMalloc(&s->ctrl, sizeof(int64_t) * C_N));
@@ -978,7 +1026,8 @@ int qserve_admit(Graph* g, hipStream_t strm, int slot, int32_t src) {
     const int grid = (int)std::min<int64_t>((V + 255) / 256, (int64_t)std::max(1, g->num_cus) * 8);
     hipLaunchKernelGGL(slot_reset_k, dim3(std::max(grid, 1)), dim3(256), 0, strm, s->dist + slot * V,
                        pruned ? s->hid + slot * V : (uint8_t*)nullptr, s->bits + slot * s->nwp,
-                       pruned ? s->bits + (s->slots + slot) * s->nwp : (uint32_t*)nullptr, V, (V + 31) / 32);
+                       pruned ? s->bits + (s->slots + slot) * s->nwp : (uint32_t*)nullptr, V, (V + 31) / 32,
+                       s->bits + (2 * s->slots + slot) * s->nwp);
     hipLaunchKernelGGL(slot_seed_k, dim3(1), dim3(256), 0, strm, g->d_row,
                        pruned ? g->d_hrow : (const int32_t*)nullptr, s->dist + slot * V,
                        pruned ? s->hid + slot * V : (uint8_t*)nullptr, s->q, s->ctrl, s->qcap, slot, src,
@@ -995,17 +1044,20 @@ int qserve_step(Graph* g, hipStream_t strm, uint64_t* slots_with_work) {
     const int64_t words = (int64_t)s->slots * s->nwp;
     const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (words + 255) / 256));
     uint32_t* hb = s->bits + (int64_t)s->slots * s->nwp;
+    // light mode needs delta 0, which the pool always runs with
+    const int32_t* lrow = g->tune.light > 0 && g->light_built == g->tune.light ? g->d_lrow : nullptr;
+    uint32_t* pb = lrow ? s->bits + 2 * (int64_t)s->slots * s->nwp : nullptr;
     launch_relax(g, strm, grid, s->dist, s->bits, pruned ? hb : nullptr, pruned ? s->hid : nullptr, s->q, s->ctrl,
                  s->nwp);
     if (pruned) {
         hipLaunchKernelGGL((sssp_expand_k<true, false>), dim3(egrid), dim3(256), 0, strm, g->d_row, g->d_hrow, s->bits,
                            hb, s->q, s->ctrl, V, std::max(1, g->tune.chunk), s->qcap, (const int32_t*)s->dist, 0,
-                           s->slots, s->nwp, (const int32_t*)nullptr, (const int32_t*)nullptr);
+                           s->slots, s->nwp, (const int32_t*)nullptr, (const int32_t*)nullptr, lrow, pb);
     } else {
         hipLaunchKernelGGL((sssp_expand_k<false, false>), dim3(egrid), dim3(256), 0, strm, g->d_row,
                            (const int32_t*)nullptr, s->bits, (uint32_t*)nullptr, s->q, s->ctrl, V,
                            std::max(1, g->tune.chunk), s->qcap, (const int32_t*)s->dist, 0, s->slots, s->nwp,
-                           (const int32_t*)nullptr, (const int32_t*)nullptr);
+                           (const int32_t*)nullptr, (const int32_t*)nullptr, lrow, pb);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(s->h_ctrl, s->ctrl, sizeof(int64_t) * C_N, hipMemcpyDeviceToHost, strm));
@@ -1069,7 +1121,7 @@ static ScratchLayout scratch_layout(const Graph* g) {
     L.ctrl = 0;
     L.sums = up(sizeof(int64_t) * C_N);
     L.bits = L.sums + up(sizeof(int64_t) * 4);
-    L.hid = L.bits + up(sizeof(uint32_t) * 2 * bitmap_words(V));
+    L.hid = L.bits + up(sizeof(uint32_t) * 3 * bitmap_words(V));
     L.q = L.hid + (is_pruned(g) ? up((size_t)V) : 0);
     L.total = L.q + sizeof(int4) * (size_t)L.qcap;
     return L;

@@ -134,7 +134,18 @@ typedef struct {
                                of its targets into eight queues, relaxed by the
                                workgroups of one XCD each, so each XCD's L2
                                holds one eighth of the distances: 0 / 1     */
-    int32_t reserved[3];
+    int32_t light;          /* SSSP light-edge bound L (0 = off): a vertex whose
+                               distance was lowered in the last round relaxes
+                               only its edges of weight < L; its other edges
+                               wait until a round in which it was not lowered
+                               again (batched: in which none of the group's
+                               queries lowered it), so they are relaxed with a
+                               settled distance.  Same distances, fewer
+                               relaxations.  Applies while delta == 0 and
+                               xcd_split == 0.  Default: about 1/16 of the
+                               mean edge weight, 0 when fewer than 1 % of the
+                               edges would be light (e.g. unit weights).  */
+    int32_t reserved[2];
 } c2g_tune;
 
 /* ======================================================================= */

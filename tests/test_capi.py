@@ -80,3 +80,17 @@ def test_tune_validation():
             g.set_tune(iters_per_graph=3)
         with pytest.raises(c2g.C2GError):
             g.set_tune(chunk=0)
+
+
+@pytest.mark.parametrize("wmax,want", [(1023, 32), (255, 8), (64, 2), (1, 0)])
+def test_default_light_bound(wmax, want):
+    # the light-edge bound starts at about 1/16 of the mean weight (mean of
+    # U[1,wmax] is (wmax+1)/2), and off when under 1 % of the edges would be
+    # light (unit weights: none is)
+    G = c2g.rmat(12, 16, seed=3, wmax=wmax)
+    with c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], no_device=True) as g:
+        assert g.tune().light == want
+        g.set_tune(light=5)
+        assert g.tune().light == 5
+        with pytest.raises(c2g.C2GError):
+            g.set_tune(light=-1)
