@@ -310,8 +310,17 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
     // its barriers are reached by all of them.
     __shared__ int64_t s_tot[16][NP];
     __shared__ int64_t s_base[NP];
+    // workgroup totals of the round's counters: one global atomic each per
+    // workgroup (per-wave atomics on the control block's two cache lines
+    // serialised the kernel)
+    __shared__ unsigned long long s_acc[5];  // front, edges, hidden edges, near, slot bits
+    __shared__ int s_fmin;
+    if (threadIdx.x < 5) s_acc[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_fmin = INT32_MAX;
+    __syncthreads();
     const int wid = threadIdx.x >> 6;
     const int nwv = blockDim.x >> 6;
+    uint64_t slotbits = 0;  // slots with items or pending vertices (wave-uniform)
     for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < nwr; i0 += stride) {
         const int64_t i = i0 + threadIdx.x;
         const bool inr = i < nwr;
@@ -412,7 +421,7 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
         __syncthreads();  // s_tot and s_base are rewritten in the next iteration
         // a slot with pending vertices (light mode) still has work
         const bool pend = __ballot(lt && wl != 0) != 0;
-        if ((any || pend) && lane == 0) atomicOr((unsigned long long*)&ctrl[C_SLOTS], 1ull << slot);
+        if (any || pend) slotbits |= 1ull << slot;
         if (any == 0) continue;
         if (!room) {
             if (lane == 0) atomicOr((unsigned long long*)&ctrl[C_OVF], 1ull);
@@ -455,16 +464,23 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) fmin = min(fmin, __shfl_xor(fmin, d, 64));
     if (lane == 0) {
-        if (front | hedges) {
-            atomicAdd((unsigned long long*)&ctrl[C_FRONT], (unsigned long long)front);
-            atomicAdd((unsigned long long*)&ctrl[C_EDGES], (unsigned long long)edges);
-            if (PRUNED) atomicAdd((unsigned long long*)&ctrl[C_HEDGES], (unsigned long long)hedges);
-        }
-        if (near_n) atomicAdd((unsigned long long*)&ctrl[C_NEAR], (unsigned long long)near_n);
-        if (fmin != INT32_MAX) atomicMin((long long*)&ctrl[C_FMIN], (long long)fmin);
+        if (front) atomicAdd(&s_acc[0], (unsigned long long)front);
+        if (edges) atomicAdd(&s_acc[1], (unsigned long long)edges);
+        if (hedges) atomicAdd(&s_acc[2], (unsigned long long)hedges);
+        if (near_n) atomicAdd(&s_acc[3], (unsigned long long)near_n);
+        if (slotbits) atomicOr(&s_acc[4], (unsigned long long)slotbits);
+        if (fmin != INT32_MAX) atomicMin(&s_fmin, fmin);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        if (s_acc[0] | s_acc[2]) {
+            atomicAdd((unsigned long long*)&ctrl[C_FRONT], s_acc[0]);
+            atomicAdd((unsigned long long*)&ctrl[C_EDGES], s_acc[1]);
+            if (PRUNED) atomicAdd((unsigned long long*)&ctrl[C_HEDGES], s_acc[2]);
+        }
+        if (s_acc[3]) atomicAdd((unsigned long long*)&ctrl[C_NEAR], s_acc[3]);
+        if (s_acc[4]) atomicOr((unsigned long long*)&ctrl[C_SLOTS], s_acc[4]);
+        if (s_fmin != INT32_MAX) atomicMin((long long*)&ctrl[C_FMIN], (long long)s_fmin);
         __threadfence();
         unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[C_DONE_E], 1ull);
         if (prev == (unsigned long long)gridDim.x - 1) {

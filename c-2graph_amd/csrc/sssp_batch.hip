@@ -65,6 +65,7 @@ enum : int {
 };
 
 constexpr int kJ = 8;  // edges whose row loads are in flight together
+constexpr int kS = 2;  // expand: vertices per lane whose loads are in flight together
 constexpr int kIdMaskB = 0x7fffffff;  // bit 31 of an edge target = target has edges
 
 __global__ void batch_fill_k(int32_t* __restrict__ dist, int64_t n_dist, uint64_t* __restrict__ mask,
@@ -169,6 +170,9 @@ relax_b_k(const int2* __restrict__ adj, int32_t* __restrict__ dist, uint64_t* __
     uint64_t* __restrict__ bits = s_bits[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
     const uint32_t loff = (uint32_t)lane * 4u;  // this lane's byte offset in a 256-byte row
     int64_t impr = 0;
+    __shared__ unsigned long long s_impr;  // workgroup total: one global atomic per workgroup
+    if (threadIdx.x == 0) s_impr = 0;
+    __syncthreads();
 
     for (int64_t i = wave; i < n; i += nwaves) {
         const int4 it = items[i];
@@ -207,9 +211,10 @@ relax_b_k(const int2* __restrict__ adj, int32_t* __restrict__ dist, uint64_t* __
         }
     }
     impr = wave_sum64(impr);
-    if (lane == 0 && impr) atomicAdd((unsigned long long*)&ctrl[B_IMPR], (unsigned long long)impr);
+    if (lane == 0 && impr) atomicAdd(&s_impr, (unsigned long long)impr);
     __syncthreads();
     if (threadIdx.x == 0) {
+        if (s_impr) atomicAdd((unsigned long long*)&ctrl[B_IMPR], s_impr);
         __threadfence();
         unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[HIDDEN ? B_DONE_H : B_DONE_R], 1ull);
         if (prev == (unsigned long long)gridDim.x - 1) {
@@ -253,20 +258,70 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
     int64_t front = 0, edges = 0, ledges = 0;
     // Work unit: a block of 64 bitmap words (2048 vertices of one group; nw
     // is a multiple of 64).  The wave loads the block's words in one step and
-    // walks its nonzero words two at a time, lanes 0-31 on one word and 32-63
-    // on the other, so each active vertex has a lane of its own and its mask
-    // and row offsets arrive in coalesced loads.  Pass 1 counts each lane's
+    // walks its nonzero words 2*kS at a time, lanes 0-31 on one word and 32-63
+    // on the other in each of kS slots, so each active vertex has a lane of
+    // its own, the loads of a lane's kS vertices are in flight together, and
+    // masks and row offsets arrive in coalesced loads.  Pass 1 counts each lane's
     // items, one atomic reserves the block's item range, pass 2 writes the
     // items and takes the expanded lanes out of the mask.
     const int64_t nblk = (nw * groups) >> 6;
     const int64_t nwav = ((int64_t)gridDim.x * blockDim.x) >> 6;
     const int half = lane >> 5, bit = lane & 31;
-    for (int64_t blk = wave_index(); blk < nblk; blk += nwav) {
+    // Each wave takes a contiguous range of blocks (mostly of one group): its
+    // lane summaries go out once per group, and one atomic reserves the item
+    // ranges of all its blocks (pass 1 counts over the whole range, pass 2
+    // writes).
+    const int64_t per = (nblk + nwav - 1) / nwav;
+    const int64_t bb0 = min(nblk, (int64_t)wave_index() * per), bb1 = min(nblk, bb0 + per);
+    // Workgroup totals (counters, item reservations, lane summaries of the
+    // first two groups the workgroup's blocks touch) go out with one global
+    // atomic each per workgroup: per-wave atomics on the same few cache lines
+    // serialise in the L2.
+    __shared__ unsigned long long s_acc[3];  // front, edges, lane edges
+    __shared__ unsigned long long s_lor[2][2];
+    __shared__ int64_t s_res[2][8];          // per wave: items, hidden items
+    __shared__ int64_t s_base[2];
+    const int wid = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    const int g0 = (int)((min(nblk - 1, (int64_t)blockIdx.x * nwv * per) << 6) / nw);
+    if (threadIdx.x < 3) s_acc[threadIdx.x] = 0;
+    if (threadIdx.x < 4) s_lor[threadIdx.x >> 1][threadIdx.x & 1] = 0;
+    __syncthreads();
+    uint64_t nor = 0, forr = 0;
+    int gcur = -1;
+    auto flush = [&]() {
+        if (gcur < 0) return;
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) {
+            nor |= (uint64_t)shfl64((int64_t)nor, lane ^ d);
+            forr |= (uint64_t)shfl64((int64_t)forr, lane ^ d);
+        }
+        if (lane == 0 && (nor | forr)) {
+            const int k = gcur - g0;
+            if (k >= 0 && k < 2) {
+                if (nor) atomicOr(&s_lor[k][0], (unsigned long long)nor);
+                if (forr) atomicOr(&s_lor[k][1], (unsigned long long)forr);
+            } else {
+                if (nor) atomicOr((unsigned long long*)&lor[2 * gcur], (unsigned long long)nor);
+                if (forr) atomicOr((unsigned long long*)&lor[2 * gcur + 1], (unsigned long long)forr);
+            }
+        }
+        nor = forr = 0;
+    };
+    int cnt = 0, hcnt = 0;
+    int64_t p = 0, hp = 0;  // this lane's next item and hidden item (pass 2)
+    bool room = true;
+    // One block: pass 1 (write false) counts items and records lane
+    // summaries, pass 2 (write true) writes items and updates the masks.
+    auto block = [&](int64_t blk, bool write) {
         const int64_t w0 = blk << 6;
         const uint32_t myword = abits[w0 + lane];
         const uint64_t nzw = __ballot(myword != 0);
-        if (!nzw) continue;
+        if (!nzw) return;
         const int g = (int)(w0 / nw);
+        if (!write && g != gcur) {
+            flush();
+            gcur = g;
+        }
         const int64_t vb0 = (w0 - (int64_t)g * nw) * 32;  // first vertex of the block
         uint64_t* __restrict__ mg = mask + (int64_t)g * V;
         uint64_t* __restrict__ pg = lrow ? ph + (int64_t)g * V : nullptr;
@@ -293,132 +348,215 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
         };
         // The live items of v: edge range [b, e) and lanes; near = the
         // lowered lanes expanded now (their hidden lists go with them),
-        // pend = lanes v keeps for a later round.
+        // pend = lanes v keeps for a later round.  m, h, r0/r1 (row) and
+        // l0/l1 (light row) are v's loads, issued for all slots of a step
+        // before any is used.
         struct Work { int b, e; uint64_t lanes, near, pend; };
-        auto work = [&](int v, uint64_t m, bool record) -> Work {
+        auto work = [&](int v, uint64_t m, uint64_t h, int r0, int r1, int l0, int l1, bool record) -> Work {
             if (pg) {
-                const uint64_t h = pg[v];
-                if (m) return Work{lrow[v], lrow[v + 1], m, m, h | m};
-                return Work{row[v], row[v + 1], h, 0, 0};
+                if (m) return Work{l0, l1, m, m, h | m};
+                return Work{r0, r1, h, 0, 0};
             }
             const uint64_t near = split(v, m, record);
-            return Work{row[v], row[v + 1], near, near, m & ~near};
+            return Work{r0, r1, near, near, m & ~near};
         };
-        uint64_t nor = 0, forr = 0;
-        int cnt = 0, hcnt = 0;
-        for (uint64_t rest = nzw; rest;) {
-            const int k0 = __builtin_ctzll(rest);
-            rest &= rest - 1;
-            const int k1 = rest ? __builtin_ctzll(rest) : k0;
-            const uint32_t wb = rest ? (uint32_t)__builtin_amdgcn_readlane((int)myword, k1) : 0u;
-            if (rest) rest &= rest - 1;
-            const uint32_t wa = (uint32_t)__builtin_amdgcn_readlane((int)myword, k0);
-            const uint32_t word = half ? wb : wa;
-            if (!((word >> bit) & 1u)) continue;
-            const int v = (int)(vb0 + (int64_t)(half ? k1 : k0) * 32 + bit);
-            const Work wk = work(v, mg[v], true);
-            forr |= wk.pend;
-            if (!wk.lanes) continue;
-            nor |= wk.lanes;
-            if (wk.e > wk.b) cnt += (wk.e - wk.b + chunk - 1) / chunk;
-            if (hrow && wk.near) {
-                const uint64_t hm = wk.near & hmask[(int64_t)g * V + v];
-                if (hm) {
-                    const int h0 = hrow[v], h1 = hrow[v + 1];
-                    hcnt += (h1 - h0 + chunk - 1) / chunk;
-                    edges += h1 - h0;
-                    ledges += (int64_t)__popcll(hm) * (h1 - h0);
-                }
+        // A step takes the next 2*kS nonzero words: slot s of lanes 0-31
+        // walks word 2s, of lanes 32-63 word 2s+1.
+        struct Step {
+            int k[2 * kS];
+            uint32_t w[2 * kS];
+            bool has[2 * kS];
+        };
+        auto next_step = [&](uint64_t& rest) -> Step {
+            Step st;
+#pragma unroll
+            for (int j = 0; j < 2 * kS; ++j) {
+                st.has[j] = rest != 0;
+                st.k[j] = st.has[j] ? __builtin_ctzll(rest) : 0;
+                if (st.has[j]) rest &= rest - 1;
+                st.w[j] = st.has[j] ? (uint32_t)__builtin_amdgcn_readlane((int)myword, st.k[j]) : 0u;
             }
-            front += 1;
-            edges += wk.e - wk.b;
-            ledges += (int64_t)__popcll(wk.lanes) * (wk.e - wk.b);
-        }
-        int tot;
-        const int off = wave_excl_scan(cnt, &tot);
-        int64_t base = 0;
-        if (tot > 0) {
-            if (lane == 0) base = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ITEMS], (unsigned long long)tot);
-            base = shfl64(base, 0);
-        }
-        bool room = base + tot <= itemcap;
-        int64_t hp = 0;  // this lane's first hidden item
-        if (hrow) {
-            int htot;
-            const int hoff = wave_excl_scan(hcnt, &htot);
-            int64_t hbase = 0;
-            if (htot > 0) {
-                if (lane == 0)
-                    hbase = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_HITEMS], (unsigned long long)htot);
-                hbase = shfl64(hbase, 0);
-            }
-            room = room && hbase + htot <= hitemcap;
-            hp = hbase + hoff;
-        }
-        if (!room && lane == 0) atomicOr((unsigned long long*)&ctrl[B_OVF], 1ull);
-        int64_t p = base + off;
-        for (uint64_t rest = nzw; rest;) {
-            const int k0 = __builtin_ctzll(rest);
-            rest &= rest - 1;
-            const bool two = rest != 0;
-            const int k1 = two ? __builtin_ctzll(rest) : k0;
-            const uint32_t wb = two ? (uint32_t)__builtin_amdgcn_readlane((int)myword, k1) : 0u;
-            if (two) rest &= rest - 1;
-            const uint32_t wa = (uint32_t)__builtin_amdgcn_readlane((int)myword, k0);
-            const uint32_t word = half ? wb : wa;
-            const bool on = (word >> bit) & 1u;
-            bool keep = false;  // v stays active: lanes left for a later bucket or a heavy pass
-            if (on) {
-                const int v = (int)(vb0 + (int64_t)(half ? k1 : k0) * 32 + bit);
-                const uint64_t m = mg[v];
-                const Work wk = work(v, m, false);
-                if (pg) {
-                    if (m) mg[v] = 0;
-                    if (pg[v] != wk.pend) pg[v] = wk.pend;
-                } else {
-                    mg[v] = wk.pend;
-                }
-                keep = wk.pend != 0;
-                if (wk.lanes && room) {
-                    for (int e = wk.b; e < wk.e; e += chunk, ++p) {
-                        items[p] = make_int4(v, e, min(e + chunk, wk.e), g);
-                        imask[p] = wk.lanes;
+            return st;
+        };
+        if (!write) for (uint64_t rest = nzw; rest;) {
+            const Step st = next_step(rest);
+            bool on[kS];
+            int vv[kS], r0[kS], r1[kS], l0[kS], l1[kS];
+            uint64_t mm[kS], hh[kS];
+#pragma unroll
+            for (int sl = 0; sl < kS; ++sl) {
+                const uint32_t word = half ? st.w[2 * sl + 1] : st.w[2 * sl];
+                on[sl] = (word >> bit) & 1u;
+                vv[sl] = (int)(vb0 + (int64_t)(half ? st.k[2 * sl + 1] : st.k[2 * sl]) * 32 + bit);
+                mm[sl] = hh[sl] = 0;
+                r0[sl] = r1[sl] = l0[sl] = l1[sl] = 0;
+                if (on[sl]) {
+                    const int v = vv[sl];
+                    mm[sl] = mg[v];
+                    if (pg) {
+                        hh[sl] = pg[v];
+                        l0[sl] = lrow[v];
+                        l1[sl] = lrow[v + 1];
                     }
-                    if (hrow && wk.near) {
-                        const uint64_t hm = wk.near & hmask[(int64_t)g * V + v];
-                        if (hm) {
-                            const int h0 = hrow[v], h1 = hrow[v + 1];
-                            for (int e = h0; e < h1; e += chunk, ++hp) {
-                                hitems[hp] = make_int4(v, e, min(e + chunk, h1), g);
-                                himask[hp] = hm;
+                    r0[sl] = row[v];
+                    r1[sl] = row[v + 1];
+                }
+            }
+#pragma unroll
+            for (int sl = 0; sl < kS; ++sl) {
+                if (!on[sl]) continue;
+                const int v = vv[sl];
+                const Work wk = work(v, mm[sl], hh[sl], r0[sl], r1[sl], l0[sl], l1[sl], true);
+                forr |= wk.pend;
+                if (!wk.lanes) continue;
+                nor |= wk.lanes;
+                if (wk.e > wk.b) cnt += (wk.e - wk.b + chunk - 1) / chunk;
+                if (hrow && wk.near) {
+                    const uint64_t hm = wk.near & hmask[(int64_t)g * V + v];
+                    if (hm) {
+                        const int h0 = hrow[v], h1 = hrow[v + 1];
+                        hcnt += (h1 - h0 + chunk - 1) / chunk;
+                        edges += h1 - h0;
+                        ledges += (int64_t)__popcll(hm) * (h1 - h0);
+                    }
+                }
+                front += 1;
+                edges += wk.e - wk.b;
+                ledges += (int64_t)__popcll(wk.lanes) * (wk.e - wk.b);
+            }
+        }
+        if (write) for (uint64_t rest = nzw; rest;) {
+            const Step st = next_step(rest);
+            bool on[kS];
+            int vv[kS], r0[kS], r1[kS], l0[kS], l1[kS];
+            uint64_t mm[kS], hh[kS];
+#pragma unroll
+            for (int sl = 0; sl < kS; ++sl) {
+                const uint32_t word = half ? st.w[2 * sl + 1] : st.w[2 * sl];
+                on[sl] = (word >> bit) & 1u;
+                vv[sl] = (int)(vb0 + (int64_t)(half ? st.k[2 * sl + 1] : st.k[2 * sl]) * 32 + bit);
+                mm[sl] = hh[sl] = 0;
+                r0[sl] = r1[sl] = l0[sl] = l1[sl] = 0;
+                if (on[sl]) {
+                    const int v = vv[sl];
+                    mm[sl] = mg[v];
+                    if (pg) {
+                        hh[sl] = pg[v];
+                        l0[sl] = lrow[v];
+                        l1[sl] = lrow[v + 1];
+                    }
+                    r0[sl] = row[v];
+                    r1[sl] = row[v + 1];
+                }
+            }
+#pragma unroll
+            for (int sl = 0; sl < kS; ++sl) {
+                bool keep = false;  // v stays active: lanes left for a later bucket or a full-row pass
+                Work wk{0, 0, 0, 0, 0};
+                int c = 0;  // this lane's live items in the slot
+                const int v = vv[sl];
+                if (on[sl]) {
+                    const uint64_t m = mm[sl];
+                    wk = work(v, m, hh[sl], r0[sl], r1[sl], l0[sl], l1[sl], false);
+                    if (pg) {
+                        if (m) mg[v] = 0;
+                        if (hh[sl] != wk.pend) pg[v] = wk.pend;
+                    } else {
+                        mg[v] = wk.pend;
+                    }
+                    keep = wk.pend != 0;
+                    if (wk.lanes && room) {
+                        if (wk.e > wk.b) c = (wk.e - wk.b + chunk - 1) / chunk;
+                        if (hrow && wk.near) {
+                            const uint64_t hm = wk.near & hmask[(int64_t)g * V + v];
+                            if (hm) {
+                                const int h0 = hrow[v], h1 = hrow[v + 1];
+                                for (int e = h0; e < h1; e += chunk, ++hp) {
+                                    hitems[hp] = make_int4(v, e, min(e + chunk, h1), g);
+                                    himask[hp] = hm;
+                                }
                             }
                         }
                     }
                 }
+                // The slot's live items, written by the whole wave: item j
+                // belongs to the lane whose range of the slot's prefix holds
+                // it, so a hub row's thousands of items take thousands/64
+                // steps instead of one lane's serial loop.
+                int ctot;
+                const int cpre = wave_excl_scan(c, &ctot);
+                for (int j0 = 0; j0 < ctot; j0 += kWave) {
+                    const int j = j0 + lane;
+                    const int o = wave_owner(cpre, min(j, ctot - 1));
+                    const int k = j - __shfl(cpre, o, 64);
+                    const int vo = __shfl(v, o, 64), bo = __shfl(wk.b, o, 64), eo = __shfl(wk.e, o, 64);
+                    const int64_t po = shfl64(p, o);
+                    const uint64_t lo = (uint64_t)shfl64((int64_t)wk.lanes, o);
+                    if (j < ctot) {
+                        const int e = bo + k * chunk;
+                        items[po + k] = make_int4(vo, e, min(e + chunk, eo), g);
+                        imask[po + k] = lo;
+                    }
+                }
+                p += c;
+                // the slot's two words keep only the bits of vertices with
+                // lanes left
+                const uint64_t kb = __ballot(keep);
+                if (lane == 0 && st.has[2 * sl] && (uint32_t)kb != st.w[2 * sl]) abits[w0 + st.k[2 * sl]] = (uint32_t)kb;
+                if (lane == 32 && st.has[2 * sl + 1] && (uint32_t)(kb >> 32) != st.w[2 * sl + 1])
+                    abits[w0 + st.k[2 * sl + 1]] = (uint32_t)(kb >> 32);
             }
-            // the two words keep only the bits of vertices with far lanes
-            const uint64_t kb = __ballot(keep);
-            if (lane == 0 && (uint32_t)kb != wa) abits[w0 + k0] = (uint32_t)kb;
-            if (two && lane == 32 && (uint32_t)(kb >> 32) != wb) abits[w0 + k1] = (uint32_t)(kb >> 32);
         }
-        // Lane bookkeeping of this group.
-#pragma unroll
-        for (int d = 32; d > 0; d >>= 1) {
-            nor |= (uint64_t)shfl64((int64_t)nor, lane ^ d);
-            forr |= (uint64_t)shfl64((int64_t)forr, lane ^ d);
-        }
-        if (lane == 0) {
-            if (nor) atomicOr((unsigned long long*)&lor[2 * g], (unsigned long long)nor);
-            if (forr) atomicOr((unsigned long long*)&lor[2 * g + 1], (unsigned long long)forr);
-        }
+    };
+    for (int64_t blk = bb0; blk < bb1; ++blk) block(blk, false);
+    // Item ranges: wave totals meet in LDS, one atomic per queue reserves
+    // the workgroup's range, each wave takes its part in wave order.
+    int tot, htot = 0;
+    const int off = wave_excl_scan(cnt, &tot);
+    const int hoff = hrow ? wave_excl_scan(hcnt, &htot) : 0;
+    if (lane == 0) {
+        s_res[0][wid] = tot;
+        s_res[1][wid] = htot;
     }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int k = threadIdx.x;
+        int64_t sum = 0;
+        for (int w = 0; w < nwv; ++w) sum += s_res[k][w];
+        s_base[k] = sum > 0 ? (int64_t)atomicAdd((unsigned long long*)&ctrl[k ? B_HITEMS : B_ITEMS],
+                                                 (unsigned long long)sum)
+                            : 0;
+    }
+    __syncthreads();
+    int64_t base = s_base[0], hbase = s_base[1];
+    for (int w = 0; w < wid; ++w) {
+        base += s_res[0][w];
+        hbase += s_res[1][w];
+    }
+    room = base + tot <= itemcap && (!hrow || hbase + htot <= hitemcap);
+    if (!room && lane == 0) atomicOr((unsigned long long*)&ctrl[B_OVF], 1ull);
+    p = base + off;
+    hp = hbase + hoff;
+    for (int64_t blk = bb0; blk < bb1; ++blk) block(blk, true);
+    flush();  // lane bookkeeping of the last group
     front = wave_sum64(front);
     edges = wave_sum64(edges);
     ledges = wave_sum64(ledges);
     if (lane == 0 && front) {
-        atomicAdd((unsigned long long*)&ctrl[B_FRONT], (unsigned long long)front);
-        atomicAdd((unsigned long long*)&ctrl[B_EDGES], (unsigned long long)edges);
-        atomicAdd((unsigned long long*)&ctrl[B_LEDGES], (unsigned long long)ledges);
+        atomicAdd(&s_acc[0], (unsigned long long)front);
+        atomicAdd(&s_acc[1], (unsigned long long)edges);
+        atomicAdd(&s_acc[2], (unsigned long long)ledges);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_acc[0]) {
+        atomicAdd((unsigned long long*)&ctrl[B_FRONT], s_acc[0]);
+        atomicAdd((unsigned long long*)&ctrl[B_EDGES], s_acc[1]);
+        atomicAdd((unsigned long long*)&ctrl[B_LEDGES], s_acc[2]);
+    }
+    if (threadIdx.x < 4) {
+        const int k = threadIdx.x >> 1, f = threadIdx.x & 1;
+        if (s_lor[k][f] && g0 + k < groups)
+            atomicOr((unsigned long long*)&lor[2 * (g0 + k) + f], s_lor[k][f]);
     }
     if (delta > 0) {
         __syncthreads();
