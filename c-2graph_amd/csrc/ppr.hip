@@ -237,9 +237,15 @@ ppr_expand_k(const int32_t* __restrict__ row, float* __restrict__ p, float* __re
     const int lane = lane_id();
     const int S1 = ru ? (int)*stamp + 1 : 0;
     const int64_t nw = (V + 31) >> 5;
-    const int64_t nwr = (nw + 63) & ~(int64_t)63;  // uniform trip count per wave
+    // workgroup-wide queue reservations and counter totals: one global
+    // atomic each per workgroup rather than per wave
+    __shared__ int64_t s_tot[4], s_base;
+    __shared__ unsigned long long s_acc[2];
+    if (threadIdx.x < 2) s_acc[threadIdx.x] = 0;
     int64_t front = 0, edges = 0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nwr; i += (int64_t)gridDim.x * blockDim.x) {
+    // uniform trip count per workgroup (wg_reserve holds barriers)
+    for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x; i0 < nw; i0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = i0 + threadIdx.x;
         uint32_t word = 0;
         if (i < nw) {
             word = tb[i];
@@ -253,16 +259,10 @@ ppr_expand_k(const int32_t* __restrict__ row, float* __restrict__ p, float* __re
             const int d = row[v + 1] - row[v];
             if (force || r[v] + mv > (degscale ? (float)d * eps : eps)) cnt += (d + chunk - 1) / chunk;
         }
-        int tot;
-        const int off = wave_excl_scan(cnt, &tot);
-        int64_t base = 0;
-        if (tot > 0) {
-            if (lane == 0) base = (int64_t)atomicAdd((unsigned long long*)&ctrl[P_CUR], (unsigned long long)tot);
-            base = shfl64(base, 0);
-        }
-        const bool room = base + tot <= qcap;
+        int64_t wend;
+        int64_t pos = wg_reserve(cnt, (unsigned long long*)&ctrl[P_CUR], s_tot, &s_base, &wend);
+        const bool room = wend <= qcap;
         if (!room && lane == 0) atomicOr((unsigned long long*)&ctrl[P_OVF], 1ull);
-        int64_t pos = base + off;
         for (uint32_t b = word; b; b &= b - 1) {
             const int64_t v = i * 32 + __builtin_ctz(b);
             const float mv = m[v];
@@ -286,13 +286,18 @@ ppr_expand_k(const int32_t* __restrict__ row, float* __restrict__ p, float* __re
     }
     front = wave_sum64(front);
     edges = wave_sum64(edges);
+    __syncthreads();  // s_acc was zeroed before the loop's first barrier or here
     if (lane == 0 && front) {
-        atomicAdd((unsigned long long*)&ctrl[P_FRONT], (unsigned long long)front);
-        atomicAdd((unsigned long long*)&ctrl[P_EDGES], (unsigned long long)edges);
-        atomicAdd((unsigned long long*)&ctrl[P_REDGES], (unsigned long long)edges);
+        atomicAdd(&s_acc[0], (unsigned long long)front);
+        atomicAdd(&s_acc[1], (unsigned long long)edges);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        if (s_acc[0]) {
+            atomicAdd((unsigned long long*)&ctrl[P_FRONT], s_acc[0]);
+            atomicAdd((unsigned long long*)&ctrl[P_EDGES], s_acc[1]);
+            atomicAdd((unsigned long long*)&ctrl[P_REDGES], s_acc[1]);
+        }
         __threadfence();
         unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[P_DONE_E], 1ull);
         if (prev == (unsigned long long)gridDim.x - 1) {
@@ -446,10 +451,14 @@ pprb_emit_k(const int32_t* __restrict__ row, uint64_t* __restrict__ mask, uint32
             int4* __restrict__ items, uint64_t* __restrict__ imask, int64_t* __restrict__ ctrl, int64_t V,
             int64_t nw, int groups, int chunk, int64_t itemcap) {
     const int lane = lane_id();
+    __shared__ int64_t s_tot[4], s_base;
+    __shared__ unsigned long long s_acc[2];
+    if (threadIdx.x < 2) s_acc[threadIdx.x] = 0;
     int64_t front = 0, edges = 0;
     const int64_t n = nw * groups;
-    const int64_t nn = (n + 63) & ~(int64_t)63;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nn; i += (int64_t)gridDim.x * blockDim.x) {
+    // uniform trip count per workgroup (wg_reserve holds barriers)
+    for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x; i0 < n; i0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = i0 + threadIdx.x;
         uint32_t word = 0;
         int g = 0;
         int64_t vb = 0;
@@ -469,16 +478,10 @@ pprb_emit_k(const int32_t* __restrict__ row, uint64_t* __restrict__ mask, uint32
                 edges += d;
             }
         }
-        int tot;
-        const int off = wave_excl_scan(cnt, &tot);
-        int64_t base = 0;
-        if (tot > 0) {
-            if (lane == 0) base = (int64_t)atomicAdd((unsigned long long*)&ctrl[P_CUR], (unsigned long long)tot);
-            base = shfl64(base, 0);
-        }
-        const bool room = base + tot <= itemcap;
+        int64_t wend;
+        int64_t pos = wg_reserve(cnt, (unsigned long long*)&ctrl[P_CUR], s_tot, &s_base, &wend);
+        const bool room = wend <= itemcap;
         if (!room && lane == 0) atomicOr((unsigned long long*)&ctrl[P_OVF], 1ull);
-        int64_t pos = base + off;
         for (uint32_t b = word; b; b &= b - 1) {
             const int64_t v = vb + __builtin_ctz(b);
             uint64_t* mp = &mask[(int64_t)g * V + v];
@@ -495,12 +498,17 @@ pprb_emit_k(const int32_t* __restrict__ row, uint64_t* __restrict__ mask, uint32
     }
     front = wave_sum64(front);
     edges = wave_sum64(edges);
+    __syncthreads();
     if (lane == 0 && front) {
-        atomicAdd((unsigned long long*)&ctrl[P_FRONT], (unsigned long long)front);
-        atomicAdd((unsigned long long*)&ctrl[P_EDGES], (unsigned long long)edges);
+        atomicAdd(&s_acc[0], (unsigned long long)front);
+        atomicAdd(&s_acc[1], (unsigned long long)edges);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        if (s_acc[0]) {
+            atomicAdd((unsigned long long*)&ctrl[P_FRONT], s_acc[0]);
+            atomicAdd((unsigned long long*)&ctrl[P_EDGES], s_acc[1]);
+        }
         __threadfence();
         unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[P_DONE_E], 1ull);
         if (prev == (unsigned long long)gridDim.x - 1) {

@@ -242,10 +242,15 @@ sssp_relax_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
         }
     }
     impr = wave_sum64(impr);
-    if (lane == 0 && impr) atomicAdd((unsigned long long*)&ctrl[C_IMPR], (unsigned long long)impr);
+    // workgroup total, then one global atomic per workgroup
+    __shared__ unsigned long long s_impr;
+    if (threadIdx.x == 0) s_impr = 0;
+    __syncthreads();
+    if (lane == 0 && impr) atomicAdd(&s_impr, (unsigned long long)impr);
     // The last workgroup to finish retires the queue (expand refills it).
     __syncthreads();
     if (threadIdx.x == 0) {
+        if (s_impr) atomicAdd((unsigned long long*)&ctrl[C_IMPR], s_impr);
         __threadfence();
         unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[C_DONE], 1ull);
         if (prev == (unsigned long long)gridDim.x - 1) {

@@ -87,4 +87,30 @@ __device__ __forceinline__ int wave_owner(int pre, int k) {
     return o;
 }
 
+// Queue reservation for a whole workgroup with one atomic: each lane asks
+// for `cnt` entries; returns the lane's first position, and *wave_end the end
+// of its wave's range (for a capacity check).  Wave totals meet in s_tot
+// (one entry per wave, at least blockDim.x/64) and the base in *s_base.
+// Every thread of the workgroup must call it (it holds barriers).
+__device__ __forceinline__ int64_t wg_reserve(int cnt, unsigned long long* ctr, int64_t* s_tot, int64_t* s_base,
+                                              int64_t* wave_end) {
+    const int lane = lane_id();
+    const int wid = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    int tot;
+    const int off = wave_excl_scan(cnt, &tot);
+    if (lane == 0) s_tot[wid] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t sum = 0;
+        for (int w = 0; w < nwv; ++w) sum += s_tot[w];
+        *s_base = sum > 0 ? (int64_t)atomicAdd(ctr, (unsigned long long)sum) : 0;
+    }
+    __syncthreads();
+    int64_t base = *s_base;
+    for (int w = 0; w < wid; ++w) base += s_tot[w];
+    __syncthreads();  // s_tot and s_base may be rewritten by the next call
+    *wave_end = base + tot;
+    return base + off;
+}
+
 }  // namespace c2g
