@@ -70,7 +70,7 @@ constexpr int kIdMaskB = 0x7fffffff;  // bit 31 of an edge target = target has e
 
 __global__ void batch_fill_k(int32_t* __restrict__ dist, int64_t n_dist, uint64_t* __restrict__ mask,
                              uint64_t* __restrict__ hmask, int64_t n_mask, uint32_t* __restrict__ abits,
-                             int64_t n_bits, uint64_t* __restrict__ ph) {
+                             int64_t n_bits, uint64_t* __restrict__ ph, uint32_t* __restrict__ kbits) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     int4* d4 = reinterpret_cast<int4*>(dist);
@@ -81,7 +81,10 @@ __global__ void batch_fill_k(int32_t* __restrict__ dist, int64_t n_dist, uint64_
         if (hmask) hmask[i] = 0;
         if (ph) ph[i] = 0;
     }
-    for (int64_t i = t; i < n_bits; i += stride) abits[i] = 0;
+    for (int64_t i = t; i < n_bits; i += stride) {
+        abits[i] = 0;
+        if (kbits) kbits[i] = 0;
+    }
 }
 
 // Seeds lane q of group q/64 at its source (one thread per query).
@@ -248,7 +251,7 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
            int64_t* __restrict__ ctrl, int64_t V, int64_t nw, int groups,
            int chunk, int64_t itemcap, int64_t hitemcap, const int32_t* __restrict__ dist, int32_t* __restrict__ thr,
            int32_t* __restrict__ fmin, uint64_t* __restrict__ lor, int delta, const int32_t* __restrict__ lrow,
-           uint64_t* __restrict__ ph) {
+           uint64_t* __restrict__ ph, uint32_t* __restrict__ kbits) {
     extern __shared__ int32_t lds_fmin[];  // [groups * 64] when delta > 0
     const int lane = lane_id();
     if (delta > 0) {
@@ -310,9 +313,141 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
     int cnt = 0, hcnt = 0;
     int64_t p = 0, hp = 0;  // this lane's next item and hidden item (pass 2)
     bool room = true;
-    // One block: pass 1 (write false) counts items and records lane
-    // summaries, pass 2 (write true) writes items and updates the masks.
-    auto block = [&](int64_t blk, bool write) {
+    // One block without delta-stepping (plain rounds, or light-edge mode
+    // with kbits): abits marks vertices lowered in the last round, kbits
+    // (light mode) vertices kept for their full row.  The bitmaps alone fix
+    // every vertex's edge range, so pass 1 (write false) reads only row
+    // offsets (and, on pruned graphs, the masks of lowered vertices for their
+    // hidden lists); pass 2 reads the lane masks, writes the items, updates
+    // mask, ph and both bitmaps, and records the lane summaries.
+    auto block_bits = [&](int64_t blk, bool write) {
+        const int64_t w0 = blk << 6;
+        const uint32_t da = abits[w0 + lane];
+        const uint32_t dk = kbits ? kbits[w0 + lane] : 0u;
+        const uint32_t myword = da | dk;
+        const uint64_t nzw = __ballot(myword != 0);
+        if (!nzw) return;
+        const int g = (int)(w0 / nw);
+        if (write && g != gcur) {
+            flush();
+            gcur = g;
+        }
+        const int64_t vb0 = (w0 - (int64_t)g * nw) * 32;  // first vertex of the block
+        uint64_t* __restrict__ mg = mask + (int64_t)g * V;
+        uint64_t* __restrict__ pg = kbits ? ph + (int64_t)g * V : nullptr;
+        for (uint64_t rest = nzw; rest;) {
+            int k[2 * kS];
+            uint32_t wd[2 * kS], wk[2 * kS];
+            bool has[2 * kS];
+#pragma unroll
+            for (int j = 0; j < 2 * kS; ++j) {
+                has[j] = rest != 0;
+                k[j] = has[j] ? __builtin_ctzll(rest) : 0;
+                if (has[j]) rest &= rest - 1;
+                wd[j] = has[j] ? (uint32_t)__builtin_amdgcn_readlane((int)da, k[j]) : 0u;
+                wk[j] = has[j] ? (uint32_t)__builtin_amdgcn_readlane((int)dk, k[j]) : 0u;
+            }
+            bool on[kS], dirty[kS];
+            int vv[kS], r0[kS], r1[kS], l0[kS], l1[kS];
+            uint64_t mm[kS], hh[kS];
+#pragma unroll
+            for (int sl = 0; sl < kS; ++sl) {
+                const int jw = 2 * sl + half;
+                dirty[sl] = (wd[jw] >> bit) & 1u;
+                on[sl] = dirty[sl] || ((wk[jw] >> bit) & 1u);
+                vv[sl] = (int)(vb0 + (int64_t)k[jw] * 32 + bit);
+                mm[sl] = hh[sl] = 0;
+                r0[sl] = r1[sl] = l0[sl] = l1[sl] = 0;
+                if (on[sl]) {
+                    const int v = vv[sl];
+                    if (write || (hrow && dirty[sl])) mm[sl] = mg[v];
+                    if (write && pg) hh[sl] = pg[v];
+                    if (pg) {
+                        l0[sl] = lrow[v];
+                        l1[sl] = lrow[v + 1];
+                    }
+                    r0[sl] = row[v];
+                    r1[sl] = row[v + 1];
+                }
+            }
+#pragma unroll
+            for (int sl = 0; sl < kS; ++sl) {
+                const int v = vv[sl];
+                // light mode: a lowered vertex takes its light row, a kept
+                // one its full row
+                const bool lt = pg && dirty[sl];
+                const int b = lt ? l0[sl] : r0[sl], e = lt ? l1[sl] : r1[sl];
+                int c = on[sl] && e > b ? (e - b + chunk - 1) / chunk : 0;
+                uint64_t hm = 0;
+                if (hrow && dirty[sl]) hm = mm[sl] & hmask[(int64_t)g * V + v];
+                const int hc = hm ? (hrow[v + 1] - hrow[v] + chunk - 1) / chunk : 0;
+                if (!write) {
+                    cnt += c;
+                    hcnt += hc;
+                    continue;
+                }
+                bool keep = false;
+                uint64_t lanes = 0;
+                if (on[sl]) {
+                    const uint64_t m = mm[sl], h = hh[sl];
+                    lanes = dirty[sl] ? m : h;
+                    const uint64_t pend = lt ? h | m : 0;
+                    if (dirty[sl] && m) mg[v] = 0;
+                    if (pg && pend != h) pg[v] = pend;
+                    keep = pend != 0;
+                    nor |= lanes;
+                    forr |= pend;
+                    if (lanes) front += 1;
+                    edges += e - b;
+                    ledges += (int64_t)__popcll(lanes) * (e - b);
+                    if (hm) {
+                        const int h0 = hrow[v], h1 = hrow[v + 1];
+                        edges += h1 - h0;
+                        ledges += (int64_t)__popcll(hm) * (h1 - h0);
+                        if (room)
+                            for (int x = h0; x < h1; x += chunk, ++hp) {
+                                hitems[hp] = make_int4(v, x, min(x + chunk, h1), g);
+                                himask[hp] = hm;
+                            }
+                    }
+                }
+                if (!room) c = 0;
+                // the slot's live items, written by the whole wave (a hub
+                // row's items take rows/64 steps, not one lane's loop)
+                int ctot;
+                const int cpre = wave_excl_scan(c, &ctot);
+                for (int j0 = 0; j0 < ctot; j0 += kWave) {
+                    const int j = j0 + lane;
+                    const int o = wave_owner(cpre, min(j, ctot - 1));
+                    const int kk = j - __shfl(cpre, o, 64);
+                    const int vo = __shfl(v, o, 64), bo = __shfl(b, o, 64), eo = __shfl(e, o, 64);
+                    const int64_t po = shfl64(p, o);
+                    const uint64_t lo = (uint64_t)shfl64((int64_t)lanes, o);
+                    if (j < ctot) {
+                        const int x = bo + kk * chunk;
+                        items[po + kk] = make_int4(vo, x, min(x + chunk, eo), g);
+                        imask[po + kk] = lo;
+                    }
+                }
+                p += c;
+                // every lowered vertex is consumed; kbits keeps the vertices
+                // whose full row is pending
+                const uint64_t kb = __ballot(keep);
+                if (lane == 0 && has[2 * sl]) {
+                    if (wd[2 * sl]) abits[w0 + k[2 * sl]] = 0u;
+                    if (kbits && (uint32_t)kb != wk[2 * sl]) kbits[w0 + k[2 * sl]] = (uint32_t)kb;
+                }
+                if (lane == 32 && has[2 * sl + 1]) {
+                    if (wd[2 * sl + 1]) abits[w0 + k[2 * sl + 1]] = 0u;
+                    if (kbits && (uint32_t)(kb >> 32) != wk[2 * sl + 1]) kbits[w0 + k[2 * sl + 1]] = (uint32_t)(kb >> 32);
+                }
+            }
+        }
+    };
+    // One block with delta-stepping: pass 1 (write false) counts items and
+    // records lane summaries, pass 2 (write true) writes items and updates
+    // the masks.
+    auto block_delta = [&](int64_t blk, bool write) {
         const int64_t w0 = blk << 6;
         const uint32_t myword = abits[w0 + lane];
         const uint64_t nzw = __ballot(myword != 0);
@@ -324,7 +459,6 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
         }
         const int64_t vb0 = (w0 - (int64_t)g * nw) * 32;  // first vertex of the block
         uint64_t* __restrict__ mg = mask + (int64_t)g * V;
-        uint64_t* __restrict__ pg = lrow ? ph + (int64_t)g * V : nullptr;
         const int32_t* dG = dist + (int64_t)g * V * 64;
         const int32_t* tg = thr + g * 64;
         // Near lanes of v (and far lanes, recorded once in pass 1).
@@ -348,15 +482,10 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
         };
         // The live items of v: edge range [b, e) and lanes; near = the
         // lowered lanes expanded now (their hidden lists go with them),
-        // pend = lanes v keeps for a later round.  m, h, r0/r1 (row) and
-        // l0/l1 (light row) are v's loads, issued for all slots of a step
-        // before any is used.
+        // pend = lanes v keeps for a later bucket.  m and r0/r1 are v's
+        // loads, issued for all slots of a step before any is used.
         struct Work { int b, e; uint64_t lanes, near, pend; };
-        auto work = [&](int v, uint64_t m, uint64_t h, int r0, int r1, int l0, int l1, bool record) -> Work {
-            if (pg) {
-                if (m) return Work{l0, l1, m, m, h | m};
-                return Work{r0, r1, h, 0, 0};
-            }
+        auto work = [&](int v, uint64_t m, int r0, int r1, bool record) -> Work {
             const uint64_t near = split(v, m, record);
             return Work{r0, r1, near, near, m & ~near};
         };
@@ -381,23 +510,18 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
         if (!write) for (uint64_t rest = nzw; rest;) {
             const Step st = next_step(rest);
             bool on[kS];
-            int vv[kS], r0[kS], r1[kS], l0[kS], l1[kS];
-            uint64_t mm[kS], hh[kS];
+            int vv[kS], r0[kS], r1[kS];
+            uint64_t mm[kS];
 #pragma unroll
             for (int sl = 0; sl < kS; ++sl) {
                 const uint32_t word = half ? st.w[2 * sl + 1] : st.w[2 * sl];
                 on[sl] = (word >> bit) & 1u;
                 vv[sl] = (int)(vb0 + (int64_t)(half ? st.k[2 * sl + 1] : st.k[2 * sl]) * 32 + bit);
-                mm[sl] = hh[sl] = 0;
-                r0[sl] = r1[sl] = l0[sl] = l1[sl] = 0;
+                mm[sl] = 0;
+                r0[sl] = r1[sl] = 0;
                 if (on[sl]) {
                     const int v = vv[sl];
                     mm[sl] = mg[v];
-                    if (pg) {
-                        hh[sl] = pg[v];
-                        l0[sl] = lrow[v];
-                        l1[sl] = lrow[v + 1];
-                    }
                     r0[sl] = row[v];
                     r1[sl] = row[v + 1];
                 }
@@ -406,7 +530,7 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
             for (int sl = 0; sl < kS; ++sl) {
                 if (!on[sl]) continue;
                 const int v = vv[sl];
-                const Work wk = work(v, mm[sl], hh[sl], r0[sl], r1[sl], l0[sl], l1[sl], true);
+                const Work wk = work(v, mm[sl], r0[sl], r1[sl], true);
                 forr |= wk.pend;
                 if (!wk.lanes) continue;
                 nor |= wk.lanes;
@@ -428,42 +552,32 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
         if (write) for (uint64_t rest = nzw; rest;) {
             const Step st = next_step(rest);
             bool on[kS];
-            int vv[kS], r0[kS], r1[kS], l0[kS], l1[kS];
-            uint64_t mm[kS], hh[kS];
+            int vv[kS], r0[kS], r1[kS];
+            uint64_t mm[kS];
 #pragma unroll
             for (int sl = 0; sl < kS; ++sl) {
                 const uint32_t word = half ? st.w[2 * sl + 1] : st.w[2 * sl];
                 on[sl] = (word >> bit) & 1u;
                 vv[sl] = (int)(vb0 + (int64_t)(half ? st.k[2 * sl + 1] : st.k[2 * sl]) * 32 + bit);
-                mm[sl] = hh[sl] = 0;
-                r0[sl] = r1[sl] = l0[sl] = l1[sl] = 0;
+                mm[sl] = 0;
+                r0[sl] = r1[sl] = 0;
                 if (on[sl]) {
                     const int v = vv[sl];
                     mm[sl] = mg[v];
-                    if (pg) {
-                        hh[sl] = pg[v];
-                        l0[sl] = lrow[v];
-                        l1[sl] = lrow[v + 1];
-                    }
                     r0[sl] = row[v];
                     r1[sl] = row[v + 1];
                 }
             }
 #pragma unroll
             for (int sl = 0; sl < kS; ++sl) {
-                bool keep = false;  // v stays active: lanes left for a later bucket or a full-row pass
+                bool keep = false;  // v stays active: lanes left for a later bucket
                 Work wk{0, 0, 0, 0, 0};
                 int c = 0;  // this lane's live items in the slot
                 const int v = vv[sl];
                 if (on[sl]) {
                     const uint64_t m = mm[sl];
-                    wk = work(v, m, hh[sl], r0[sl], r1[sl], l0[sl], l1[sl], false);
-                    if (pg) {
-                        if (m) mg[v] = 0;
-                        if (hh[sl] != wk.pend) pg[v] = wk.pend;
-                    } else {
-                        mg[v] = wk.pend;
-                    }
+                    wk = work(v, m, r0[sl], r1[sl], false);
+                    mg[v] = wk.pend;
                     keep = wk.pend != 0;
                     if (wk.lanes && room) {
                         if (wk.e > wk.b) c = (wk.e - wk.b + chunk - 1) / chunk;
@@ -507,6 +621,10 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
                     abits[w0 + st.k[2 * sl + 1]] = (uint32_t)(kb >> 32);
             }
         }
+    };
+    auto block = [&](int64_t blk, bool write) {
+        if (delta > 0) block_delta(blk, write);
+        else block_bits(blk, write);
     };
     for (int64_t blk = bb0; blk < bb1; ++blk) block(blk, false);
     // Item ranges: wave totals meet in LDS, one atomic per queue reserves
@@ -646,7 +764,8 @@ __global__ void batch_checksum_k(const int32_t* __restrict__ row, const int32_t*
 struct BatchState {
     int groups = 0;
     int64_t nw = 0, itemcap = 0;
-    uint32_t* abits = nullptr;  // [groups][nw] activation bitmaps (nw padded to 64 words)
+    uint32_t* abits = nullptr;  // [groups][nw] activation bitmaps (nw padded to 64 words), then
+                                // [groups][nw] kept-vertex bitmaps (light mode)
     int32_t* thr = nullptr;     // [groups][64] bucket bound per lane (delta-stepping)
     int32_t* fmin = nullptr;    // [groups][64] smallest deferred distance this round
     uint64_t* lor = nullptr;    // [groups][2] lanes with near / far work this round
@@ -737,8 +856,8 @@ static int ensure_batch(Graph* g, int groups) {
         HIP_TRY(hipMalloc(&w.mask[0], sizeof(uint64_t) * (size_t)G * V + 8));
         if (g->d_hrow) HIP_TRY(hipMalloc(&w.hmask, sizeof(uint64_t) * (size_t)G * V + 8));
         const int64_t itemcap = G * (V + g->E / chunk) + 64;
-        HIP_TRY(hipMalloc(&s->abits, sizeof(uint32_t) * s->nw * G));
-        HIP_TRY(hipMemset(s->abits, 0, sizeof(uint32_t) * s->nw * G));
+        HIP_TRY(hipMalloc(&s->abits, sizeof(uint32_t) * s->nw * G * 2));
+        HIP_TRY(hipMemset(s->abits, 0, sizeof(uint32_t) * s->nw * G * 2));
         HIP_TRY(hipMalloc(&s->items, sizeof(int4) * itemcap));
         HIP_TRY(hipMalloc(&s->imask, sizeof(uint64_t) * itemcap));
         if (pruned) {
@@ -766,6 +885,11 @@ static int ensure_batch(Graph* g, int groups) {
 
 // ph when the light-edge mode runs (light rows built and ph allocated).
 static uint64_t* light_ph(Graph* g) { return light_rows(g) ? g->batch.ph : nullptr; }
+// The kept-vertex bitmaps of the light-edge mode (null when it is off).
+static uint32_t* light_kbits(Graph* g) {
+    BatchState* s = bstate(g);
+    return light_ph(g) ? s->abits + s->nw * s->groups : nullptr;
+}
 
 static int relax_grid_b(const Graph* g) { return std::max(1, g->num_cus) * g->tune.grid_per_cu; }
 
@@ -782,7 +906,7 @@ static int expand_b(Graph* g, int groups, hipStream_t strm) {
                        pruned ? w.hmask : (const uint64_t*)nullptr, s->abits, s->items, s->imask, s->hitems,
                        s->himask, w.ctrl, (int64_t)g->V, s->nw, groups, g->tune.batch_chunk, s->itemcap,
                        s->hitemcap, w.dist, s->thr, s->fmin, s->lor, delta,
-                       light_ph(g) ? light_rows(g) : (const int32_t*)nullptr, light_ph(g));
+                       light_ph(g) ? light_rows(g) : (const int32_t*)nullptr, light_ph(g), light_kbits(g));
     return C2G_OK;
 }
 
@@ -866,7 +990,7 @@ static int batch_run(Graph* g, const int32_t* srcs, int32_t n, c2g_stats* st, do
     const int fill_grid = std::max(1, g->num_cus) * 16;
     hipLaunchKernelGGL(batch_fill_k, dim3(fill_grid), dim3(256), 0, g->stream, w.dist,
                        (int64_t)groups * V * 64, w.mask[0], pruned ? w.hmask : (uint64_t*)nullptr,
-                       (int64_t)groups * V, s->abits, s->nw * groups, light_ph(g));
+                       (int64_t)groups * V, s->abits, s->nw * groups, light_ph(g), light_kbits(g));
     hipLaunchKernelGGL(batch_thr_init_k, dim3((64 * groups + 255) / 256), dim3(256), 0, g->stream, s->thr, s->fmin,
                        s->lor, 64 * groups, g->tune.delta);
     hipLaunchKernelGGL(batch_seed_k, dim3((n + 255) / 256), dim3(256), 0, g->stream, s->srcs, n, g->d_row,
@@ -1091,7 +1215,7 @@ int64_t bserve_ws_bytes(Graph* g) {
             if (p.first == g && p.second->groups >= 1 && p.second->itemcap >= needi && p.second->hitemcap >= needh)
                 have = true;
         if (!have)
-            b += 256 * V + 8 * V * (pruned ? 2 : 1) + (light_rows(g) ? 8 * V : 0) + 4 * nw + 24 * (needi + needh) +
+            b += 256 * V + 8 * V * (pruned ? 2 : 1) + (light_rows(g) ? 8 * V : 0) + 8 * nw + 24 * (needi + needh) +
                  64 * (4 + 4 + 8 + 8 + 4 + 8) + 8 * B_N;
     }
     {
@@ -1122,7 +1246,7 @@ int bserve_init(Graph* g, hipStream_t strm) {
     hipLaunchKernelGGL(batch_thr_init_k, dim3(1), dim3(64), 0, strm, s->thr, s->fmin, s->lor, 64, g->tune.delta);
     hipLaunchKernelGGL(batch_fill_k, dim3(std::max(1, g->num_cus) * 16), dim3(256), 0, strm, w.dist,
                        (int64_t)g->V * 64, w.mask[0], pruned ? w.hmask : (uint64_t*)nullptr, (int64_t)g->V,
-                       s->abits, s->nw, light_ph(g));
+                       s->abits, s->nw, light_ph(g), light_kbits(g));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(strm));
     return C2G_OK;
