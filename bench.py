@@ -142,12 +142,13 @@ def _sha256(path):
         return hashlib.sha256(f.read()).hexdigest()
 
 
-def measured_traffic(kernel, args):
+def measured_traffic(kernel, args, tune):
     """HBM bytes per launch of `kernel` on this workload, from the committed
     rocprofv3 counter summary, with its provenance.  The summary records the
-    sha256 of the kernel's source file at the counter run; when the current
-    file differs, the figure no longer describes this kernel and traffic is
-    None (the note says why)."""
+    sha256 of the kernel's source file at the counter run and the light-edge
+    bound the run used (the library default); when the current file differs,
+    or this run's bound differs, the figure no longer describes this kernel
+    and traffic is None (the note says why)."""
     if (args.scale, args.seed, args.queries, args.wmax) != (24, 4, 256, 1023) or not os.path.exists(TRAFFIC_JSON):
         return None, {"note": "no counter summary for this workload"}
     with open(TRAFFIC_JSON) as f:
@@ -159,10 +160,14 @@ def measured_traffic(kernel, args):
     if t.get("kernel_source_sha256") != _sha256(KERNEL_SRC):
         prov["note"] = "stale: the kernel source changed since the counter run"
         return None, prov
+    if t.get("light") != tune.get("light"):
+        prov["note"] = f"counter run at light {t.get('light')}, this run at light {tune.get('light')}"
+        return None, prov
+    prov["light"] = t.get("light")
     return t["traffic_bytes_per_launch"], prov
 
 
-def ingest_counters(dirs, out_path):
+def ingest_counters(dirs, out_path, light):
     """Sums rocprofv3 --pmc CSV outputs (one directory per counter pass) per
     dispatch of relax_b_k<false, B> and writes the traffic summary: read bytes
     = 32*RDREQ_32B + 64*RDREQ_64B + 128*RDREQ_128B, write bytes = 32*(WRREQ -
@@ -208,7 +213,7 @@ def ingest_counters(dirs, out_path):
            "dispatches": len(per.get("TCC_EA0_RDREQ_128B", per.get("TCC_EA0_RDREQ_128B_sum", []))),
            "read_bytes_per_launch": int(rd), "write_bytes_per_launch": int(wb),
            "traffic_bytes_per_launch": int(rd + wb), "counters": {k: v for k, v in cnt.items()},
-           "kernel_source_sha256": _sha256(KERNEL_SRC),
+           "kernel_source_sha256": _sha256(KERNEL_SRC), "light": light,
            "source": "profiles/relax_b_c4_scale24_hbm_traffic.md"}
     if copy:
         crd, cwb, ccnt = traffic(copy)
@@ -619,13 +624,16 @@ def main():
     ap.add_argument("--set-tune", default="", metavar="K=V,...",
                     help="pin tunables (e.g. grid_per_cu=16,block=256) instead of running the autotuner, so a "
                          "profiled run repeats the timed run's launch shapes")
+    ap.add_argument("--ingest-light", type=int, default=32,
+                    help="light-edge bound of the --profile-only counter runs (the library default on this "
+                         "workload: 1/16 of the mean weight 512)")
     ap.add_argument("--ingest-counters", nargs="+", metavar="DIR",
                     help="summarise rocprofv3 --pmc CSV directories of --profile-only (and --copy-calib) "
                          "runs into profiles/relax_b_c4_scale24_traffic.json, then exit")
     args = ap.parse_args()
 
     if args.ingest_counters:
-        print(json.dumps(ingest_counters(args.ingest_counters, TRAFFIC_JSON)), flush=True)
+        print(json.dumps(ingest_counters(args.ingest_counters, TRAFFIC_JSON, args.ingest_light)), flush=True)
         return
 
     if args.gpus != 1:
@@ -773,7 +781,7 @@ def main():
         algo_bytes = 8 * sp.csr_edges + 8 * sp.frontier_total + 4 * sp.edges_relaxed + 4 * sp.improvements
         achieved = algo_bytes / (sp.kernel_ms * 1e-3) / 1e9
         kname = f"relax_b_k<false, {g.tune().block}>"
-        traffic, prov = measured_traffic(kname, args)
+        traffic, prov = measured_traffic(kname, args, g.tune().as_dict())
         algo_per_launch = algo_bytes / max(1, sp.iterations)
         extra["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
