@@ -912,12 +912,20 @@ static int expand_b(Graph* g, int groups, hipStream_t strm) {
 
 // One round = relax_b + expand_b.  When relax_ms is non-null, events bracket
 // the relax kernel and its duration is added to *relax_ms (profiling runs).
-static int launch_round_b(Graph* g, int groups, double* relax_ms, hipStream_t strm = nullptr) {
+// skip_relax: the host knows both queues are empty (a round that only
+// expands vertices kept for their full rows), so no relax is launched.
+static int launch_round_b(Graph* g, int groups, double* relax_ms, hipStream_t strm = nullptr,
+                          bool skip_relax = false) {
     if (!strm) strm = g->stream;
     BatchWs& w = g->batch;
     BatchState* s = bstate(g);
     const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
     const int grid = relax_grid_b(g);
+    if (skip_relax) {
+        expand_b(g, groups, strm);
+        HIP_TRY(hipGetLastError());
+        return C2G_OK;
+    }
     if (relax_ms) HIP_TRY(hipEventRecord(s->ev[0], strm));
     auto relax = [&](auto hidden, auto block) {
         constexpr bool H = decltype(hidden)::value;
@@ -1009,11 +1017,14 @@ static int batch_run(Graph* g, const int32_t* srcs, int32_t n, c2g_stats* st, do
         q3[2] = w.h_ctrl[B_LEDGES];
     };
     hipEvent_t ev[2] = {nullptr, nullptr};
+    if (one) {
+        // the host checks every round: start from the seeded round's counters
+        HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, sizeof(int64_t) * B_N, hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    }
     if (log) {
         HIP_TRY(hipEventCreate(&ev[0]));
         if (hipEventCreate(&ev[1]) != hipSuccess) { (void)hipEventDestroy(ev[0]); return hip_fail(hipErrorOutOfMemory, "hipEventCreate"); }
-        HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, sizeof(int64_t) * B_N, hipMemcpyDeviceToHost, g->stream));
-        HIP_TRY(hipStreamSynchronize(g->stream));
         int64_t q3[3];
         snap(q3);
         log->start(q3);
@@ -1027,11 +1038,12 @@ static int batch_run(Graph* g, const int32_t* srcs, int32_t n, c2g_stats* st, do
         if (use_graph) {
             HIP_TRY(hipGraphLaunch(w.exec, g->stream));
         } else {
-            // Profiled runs check after every round so that no empty round
+            // Profiled runs check after every round so that no empty relax
             // is launched (launch count == rounds with work).
             const int k = one ? 1 : g->tune.iters_per_graph;
+            const bool empty = one && w.h_ctrl[B_ITEMS] == 0 && w.h_ctrl[B_HITEMS] == 0;
             for (int r = 0; r < k; ++r)
-                if ((rc = launch_round_b(g, groups, relax_ms))) return rc;
+                if ((rc = launch_round_b(g, groups, relax_ms, nullptr, empty))) return rc;
         }
         if (log) HIP_TRY(hipEventRecord(ev[1], g->stream));
         HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, sizeof(int64_t) * B_N, hipMemcpyDeviceToHost, g->stream));

@@ -238,7 +238,9 @@ def test_round_trace_single_and_batched(gpu):
     with c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], device=0) as g:
         d0, st0 = g.sssp(s)
         d, st, rs = g.sssp_rounds(s)
-        assert np.array_equal(d, d0) and len(rs) == st.iterations
+        # a round whose queue is empty (light-edge mode: only vertices kept
+        # for their full rows) is traced but does not count as an iteration
+        assert np.array_equal(d, d0) and len(rs) >= st.iterations
         assert rs[0]["frontier"] == 1 and rs[0]["edges"] == G["vlist"][s + 1] - G["vlist"][s]
         assert sum(r["frontier"] for r in rs) == st.frontier_total
         assert sum(r["edges"] for r in rs) == st.edges_relaxed
