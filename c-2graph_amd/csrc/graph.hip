@@ -65,6 +65,7 @@ void default_tune(Graph* g) {
     t.use_graph = 1;
     t.delta = 0;
     t.light = default_light(g);
+    t.batch_order = 1;
 }
 
 // Light-edge bound: about a sixteenth of the mean weight (on R-MAT with
@@ -571,6 +572,7 @@ int c2g_set_tune(c2g_graph* h, const c2g_tune* t) {
         return fail(C2G_EINVAL, "iters_per_graph must be even, in [2,256]");
     if (t->delta < 0) return fail(C2G_EINVAL, "delta >= 0");
     if (t->light < 0) return fail(C2G_EINVAL, "light >= 0");
+    if (t->batch_order != 0 && t->batch_order != 1) return fail(C2G_EINVAL, "batch_order must be 0 or 1");
     C2G_HANDLE_LOCK(h);
     if (memcmp(&h->g.tune, t, sizeof *t) != 0) {
         // Captured launch sequences hold the old values as kernel arguments.

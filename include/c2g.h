@@ -145,7 +145,12 @@ typedef struct {
                                xcd_split == 0.  Default: about 1/16 of the
                                mean edge weight, 0 when fewer than 1 % of the
                                edges would be light (e.g. unit weights).  */
-    int32_t reserved[2];
+    int32_t batch_order;    /* batched SSSP: 1 = assign queries to 64-lane
+                               groups in order of their sources' two-hop
+                               reach, so that each group's lanes are pending
+                               in the same rounds (results in the caller's
+                               order); 0 = the caller's order           */
+    int32_t reserved[1];
 } c2g_tune;
 
 /* ======================================================================= */

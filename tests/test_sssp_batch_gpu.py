@@ -142,3 +142,31 @@ def test_batch_duplicate_edges(gpu):
     assert np.array_equal(ds, d[0])
     for q, s in enumerate(srcs):
         assert np.array_equal(d[q], oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], int(s))), q
+
+
+@pytest.mark.parametrize("light", [0, 16])
+def test_lane_order_keeps_caller_order(gpu, light):
+    # batch_order 1 assigns queries to lanes by their sources' two-hop reach;
+    # distances, checksums and reached counts come back in the caller's order,
+    # identical to the caller's lane order (ragged 150 sources, 3 groups)
+    G = rmat_graph(15, 9, wmax=255)
+    deg = np.diff(G["vlist"])
+    rng = np.random.default_rng(9)
+    srcs = np.concatenate([np.argsort(-deg)[:50], rng.choice(np.flatnonzero(deg > 0), 100, replace=False)])
+    srcs = rng.permutation(srcs).astype(np.int32)
+    with c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], device=0) as g:
+        assert g.tune().batch_order == 1
+        g.set_tune(light=light)
+        d1, s1 = g.sssp_batch(srcs)
+        sums1, reach1, _ = g.sssp_batch_resident(srcs)
+        g.set_tune(batch_order=0)
+        d0, s0 = g.sssp_batch(srcs)
+        sums0, reach0, _ = g.sssp_batch_resident(srcs)
+        with pytest.raises(c2g.C2GError):
+            g.set_tune(batch_order=2)
+    assert np.array_equal(d1, d0)
+    assert np.array_equal(sums1, sums0) and np.array_equal(reach1, reach0)
+    assert s1.reached == s0.reached == int(reach0.sum())
+    refs = oracle.dijkstra_many(G["vlist"], G["elist"], G["wlist"], srcs[[0, 75, 149]])
+    for k, q in enumerate((0, 75, 149)):
+        assert np.array_equal(d1[q], refs[k])
