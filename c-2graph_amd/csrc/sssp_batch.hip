@@ -77,7 +77,7 @@ __global__ void batch_fill_k(int32_t* __restrict__ dist, int64_t n_dist, uint64_
     const int64_t n4 = n_dist / 4;
     for (int64_t i = t; i < n4; i += stride) d4[i] = make_int4(INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX);
     for (int64_t i = t; i < n_mask; i += stride) {
-        mask[i] = 0;
+        if (mask) mask[i] = 0;
         if (hmask) hmask[i] = 0;
         if (ph) ph[i] = 0;
     }
@@ -728,11 +728,34 @@ __global__ void batch_thr_init_k(int32_t* __restrict__ thr, int32_t* __restrict_
     if (t < 2 * (n / 64)) lor[t] = 0;
 }
 
+// Resets the distance rows of the vertices in live[0..nlive) (those with an
+// in-edge, live or hidden) in groups [0, groups): the only rows a traversal
+// can write apart from its sources' entries.  16 threads per 256-byte row.
+__global__ void batch_fill_rows_k(int32_t* __restrict__ dist, const int32_t* __restrict__ live, int64_t nlive,
+                                  int groups, int64_t V) {
+    const int64_t n = (int64_t)groups * nlive * 16;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += stride) {
+        const int64_t r = t >> 4, part = t & 15;
+        const int64_t g = r / nlive, v = live[r - g * nlive];
+        reinterpret_cast<int4*>(dist + (g * V + v) * 64)[part] = make_int4(INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX);
+    }
+}
+
+// Resets the entries the previous traversal seeded: lane q%64 of group q/64
+// at srcs[q], for q < n.
+__global__ void batch_reset_srcs_k(int32_t* __restrict__ dist, const int32_t* __restrict__ srcs, int n, int64_t V) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) dist[((int64_t)(q >> 6) * V + srcs[q]) * 64 + (q & 63)] = INT32_MAX;
+}
+
 // Per-lane checksums of one group: sum of finite distances, reached count,
-// and out-degree sum of reached vertices (m_s), for lanes [0, nlanes).
+// and out-degree sum of reached vertices (m_s), for lanes [0, nlanes).  With
+// `live` non-null only its nlive rows are read (the vertices a traversal
+// can reach other than the sources themselves); V is then nlive.
 __global__ void batch_checksum_k(const int32_t* __restrict__ row, const int32_t* __restrict__ dist,
                                  int64_t V, int64_t* __restrict__ sums, int32_t* __restrict__ reached,
-                                 int64_t* __restrict__ medges) {
+                                 int64_t* __restrict__ medges, const int32_t* __restrict__ live) {
     const int lane = lane_id();
     const int64_t wave = wave_index();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -743,9 +766,10 @@ __global__ void batch_checksum_k(const int32_t* __restrict__ row, const int32_t*
         int d[4], deg[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const int64_t v = v0 + k;
-            d[k] = v < V ? dist[v * 64 + lane] : INT32_MAX;
-            deg[k] = v < V ? row[v + 1] - row[v] : 0;
+            const int64_t i = v0 + k;
+            const int64_t v = i < V ? (live ? live[i] : i) : 0;
+            d[k] = i < V ? dist[v * 64 + lane] : INT32_MAX;
+            deg[k] = i < V ? row[v + 1] - row[v] : 0;
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k)
@@ -780,6 +804,17 @@ struct BatchState {
     int64_t* medges = nullptr;  // [64*groups]
     hipEvent_t ev[2] = {nullptr, nullptr};
     ColStage cols;              // per-query distance rows to the host
+    // Rows a traversal can write: vertices with an in-edge (live or hidden).
+    // After a traversal that converged, groups [0, clean_groups) hold
+    // INT32_MAX in every other row except the entries of the sources in
+    // srcs[0..nsrcs) (device copy of the last traversal's sources), and their
+    // masks, pending lanes and bitmaps are zero: the next traversal resets
+    // only the live rows and those entries.
+    int32_t* live = nullptr;
+    int64_t nlive = 0;
+    std::vector<uint8_t> has_in;  // [V] 1 = vertex has an in-edge
+    int clean_groups = 0;
+    int nsrcs = 0;
 };
 
 static std::mutex g_bs_mu;
@@ -803,7 +838,7 @@ void batch_release(Graph* g) {
             (void)hipFree(s->thr); (void)hipFree(s->fmin); (void)hipFree(s->lor);
             (void)hipFree(s->imask); (void)hipFree(s->srcs); (void)hipFree(s->sums);
             (void)hipFree(s->reach); (void)hipFree(s->medges);
-            (void)hipFree(s->hitems); (void)hipFree(s->himask);
+            (void)hipFree(s->hitems); (void)hipFree(s->himask); (void)hipFree(s->live);
             for (auto& e : s->ev) if (e) (void)hipEventDestroy(e);
             s->cols.release();
             delete s;
@@ -847,7 +882,7 @@ static int ensure_batch(Graph* g, int groups) {
         s->thr = nullptr; s->fmin = nullptr; s->lor = nullptr;
         // until every piece is in place the workspace counts as empty, so a
         // failed allocation is retried instead of used
-        s->groups = 0; s->itemcap = 0; w.groups = 0;
+        s->groups = 0; s->itemcap = 0; w.groups = 0; s->clean_groups = 0;
         s->nw = (((V + 31) / 32 + 1) + 63) & ~(int64_t)63;  // whole waves per group
         HIP_TRY(hipMalloc(&s->thr, sizeof(int32_t) * 64 * G));
         HIP_TRY(hipMalloc(&s->fmin, sizeof(int32_t) * 64 * G));
@@ -879,6 +914,21 @@ static int ensure_batch(Graph* g, int groups) {
         if (w.exec) { (void)hipGraphExecDestroy(w.exec); w.exec = nullptr; }
         HIP_TRY(hipMalloc(&w.ph, sizeof(uint64_t) * (size_t)s->groups * V + 8));
         HIP_TRY(hipMemset(w.ph, 0, sizeof(uint64_t) * (size_t)s->groups * V + 8));
+    }
+    if (!s->live && V > 0) {
+        // vertices with an in-edge: on R-MAT about 44 % of V, the only rows
+        // besides the sources' that a traversal writes
+        s->has_in.assign((size_t)V, 0);
+        for (int32_t t : g->el) s->has_in[t] = 1;
+        for (int32_t t : g->hel) s->has_in[t] = 1;
+        std::vector<int32_t> lv;
+        lv.reserve((size_t)V);
+        for (int64_t v = 0; v < V; ++v)
+            if (s->has_in[v]) lv.push_back((int32_t)v);
+        HIP_TRY(hipMalloc(&s->live, sizeof(int32_t) * std::max<size_t>(lv.size(), 1)));
+        if (!lv.empty())
+            HIP_TRY(hipMemcpy(s->live, lv.data(), sizeof(int32_t) * lv.size(), hipMemcpyHostToDevice));
+        s->nlive = (int64_t)lv.size();
     }
     return C2G_OK;
 }
@@ -994,11 +1044,29 @@ static int batch_run(Graph* g, const int32_t* srcs, int32_t n, c2g_stats* st, do
     const int groups = (n + 63) / 64;
     const int64_t V = g->V;
     const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
-    HIP_TRY(hipMemcpyAsync(s->srcs, srcs, sizeof(int32_t) * n, hipMemcpyHostToDevice, g->stream));
     const int fill_grid = std::max(1, g->num_cus) * 16;
-    hipLaunchKernelGGL(batch_fill_k, dim3(fill_grid), dim3(256), 0, g->stream, w.dist,
-                       (int64_t)groups * V * 64, w.mask[0], pruned ? w.hmask : (uint64_t*)nullptr,
-                       (int64_t)groups * V, s->abits, s->nw * groups, light_ph(g), light_kbits(g));
+    if (groups <= s->clean_groups && s->live) {
+        // after a converged traversal: the last sources' entries and the
+        // live rows (masks and bitmaps are already zero; hidden marks are
+        // cleared in full)
+        if (s->nsrcs > 0)
+            hipLaunchKernelGGL(batch_reset_srcs_k, dim3((s->nsrcs + 255) / 256), dim3(256), 0, g->stream, w.dist,
+                               s->srcs, s->nsrcs, V);
+        hipLaunchKernelGGL(batch_fill_rows_k, dim3(fill_grid), dim3(256), 0, g->stream, w.dist, s->live, s->nlive,
+                           groups, V);
+        if (pruned)
+            hipLaunchKernelGGL(batch_fill_k, dim3(fill_grid), dim3(256), 0, g->stream, (int32_t*)nullptr,
+                               (int64_t)0, (uint64_t*)nullptr, w.hmask, (int64_t)groups * V, s->abits, (int64_t)0,
+                               (uint64_t*)nullptr, (uint32_t*)nullptr);
+    } else {
+        hipLaunchKernelGGL(batch_fill_k, dim3(fill_grid), dim3(256), 0, g->stream, w.dist,
+                           (int64_t)groups * V * 64, w.mask[0], pruned ? w.hmask : (uint64_t*)nullptr,
+                           (int64_t)groups * V, s->abits, s->nw * groups, light_ph(g), light_kbits(g));
+    }
+    const int clean_before = groups <= s->clean_groups ? s->clean_groups : 0;
+    s->clean_groups = 0;  // until this traversal converges
+    HIP_TRY(hipMemcpyAsync(s->srcs, srcs, sizeof(int32_t) * n, hipMemcpyHostToDevice, g->stream));
+    s->nsrcs = n;
     hipLaunchKernelGGL(batch_thr_init_k, dim3((64 * groups + 255) / 256), dim3(256), 0, g->stream, s->thr, s->fmin,
                        s->lor, 64 * groups, g->tune.delta);
     hipLaunchKernelGGL(batch_seed_k, dim3((n + 255) / 256), dim3(256), 0, g->stream, s->srcs, n, g->d_row,
@@ -1058,6 +1126,8 @@ static int batch_run(Graph* g, const int32_t* srcs, int32_t n, c2g_stats* st, do
         }
         if (w.h_ctrl[B_ITEMS] == 0 && w.h_ctrl[B_HITEMS] == 0 && w.h_ctrl[B_FAR] == 0) break;
     }
+    // converged: every mask, pending lane and bitmap of these groups is zero
+    s->clean_groups = std::max(clean_before, groups);
     if (st) {
         st->lane = C2G_LANE_GPU;
         st->iterations = (int32_t)w.h_ctrl[B_ITER];
@@ -1070,7 +1140,10 @@ static int batch_run(Graph* g, const int32_t* srcs, int32_t n, c2g_stats* st, do
     return C2G_OK;
 }
 
-static int batch_checksums(Graph* g, int n, int64_t* sums, int32_t* reached, int64_t* medges_total) {
+// srcs: the lanes' sources (lane order); a source without in-edges is the
+// only row outside the live list a lane can reach (its own, at distance 0).
+static int batch_checksums(Graph* g, const int32_t* srcs, int n, int64_t* sums, int32_t* reached,
+                           int64_t* medges_total) {
     BatchWs& w = g->batch;
     BatchState* s = bstate(g);
     const int groups = (n + 63) / 64;
@@ -1080,8 +1153,8 @@ static int batch_checksums(Graph* g, int n, int64_t* sums, int32_t* reached, int
     HIP_TRY(hipMemsetAsync(s->medges, 0, sizeof(int64_t) * 64 * groups, g->stream));
     for (int gi = 0; gi < groups; ++gi)
         hipLaunchKernelGGL(batch_checksum_k, dim3(std::max(1, g->num_cus) * 8), dim3(256), 0, g->stream,
-                           g->d_row, w.dist + (int64_t)gi * V * 64, V, s->sums + 64 * gi,
-                           s->reach + 64 * gi, s->medges + 64 * gi);
+                           g->d_row, w.dist + (int64_t)gi * V * 64, s->live ? s->nlive : V, s->sums + 64 * gi,
+                           s->reach + 64 * gi, s->medges + 64 * gi, (const int32_t*)s->live);
     HIP_TRY(hipGetLastError());
     std::vector<int64_t> hs(64 * groups), hm(64 * groups);
     std::vector<int32_t> hr(64 * groups);
@@ -1091,6 +1164,10 @@ static int batch_checksums(Graph* g, int n, int64_t* sums, int32_t* reached, int
     HIP_TRY(hipStreamSynchronize(g->stream));
     int64_t mt = 0;
     for (int q = 0; q < n; ++q) {
+        if (s->live && !s->has_in[srcs[q]]) {  // the source's own row, at distance 0
+            hr[q] += 1;
+            hm[q] += g->vl[srcs[q] + 1] - g->vl[srcs[q]];
+        }
         if (sums) sums[q] = hs[q];
         if (reached) reached[q] = hr[q];
         mt += hm[q];
@@ -1151,7 +1228,7 @@ int sssp_batch_group(Graph* g, const int32_t* srcs, int32_t n, int32_t* dist_out
     int64_t medges = 0;
     std::vector<int64_t> lsum((size_t)n);
     std::vector<int32_t> lreach((size_t)n);
-    if ((rc = batch_checksums(g, n, lsum.data(), lreach.data(), &medges))) return rc;
+    if ((rc = batch_checksums(g, lsrc.data(), n, lsum.data(), lreach.data(), &medges))) return rc;
     int32_t rtot = 0;
     for (int32_t k = 0; k < n; ++k) {
         if (sums) sums[perm[k]] = lsum[k];
@@ -1285,6 +1362,7 @@ int bserve_init(Graph* g, hipStream_t strm) {
     BatchWs& w = g->batch;
     BatchState* s = bstate(g);
     const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
+    s->clean_groups = 0;  // the server's lanes leave group 0 in no known state
     HIP_TRY(hipMemsetAsync(w.ctrl, 0, sizeof(int64_t) * B_N, strm));
     hipLaunchKernelGGL(batch_thr_init_k, dim3(1), dim3(64), 0, strm, s->thr, s->fmin, s->lor, 64, g->tune.delta);
     hipLaunchKernelGGL(batch_fill_k, dim3(std::max(1, g->num_cus) * 16), dim3(256), 0, strm, w.dist,

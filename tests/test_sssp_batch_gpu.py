@@ -170,3 +170,41 @@ def test_lane_order_keeps_caller_order(gpu, light):
     refs = oracle.dijkstra_many(G["vlist"], G["elist"], G["wlist"], srcs[[0, 75, 149]])
     for k, q in enumerate((0, 75, 149)):
         assert np.array_equal(d1[q], refs[k])
+
+
+def test_consecutive_batches_reset_only_what_they_wrote(gpu):
+    # after a converged batch only the rows of vertices with an in-edge and
+    # the last sources' entries are reset: consecutive batches with fewer and
+    # more groups, sources without in-edges (reachable only as sources), and a
+    # batch after a served query, all against Dijkstra
+    G = rmat_graph(14, 12, wmax=200)
+    V = G["V"]
+    indeg = np.bincount(G["elist"], minlength=V)
+    deg = np.diff(G["vlist"])
+    no_in = np.flatnonzero((indeg == 0) & (deg > 0))
+    assert len(no_in) > 100
+    rng = np.random.default_rng(12)
+    runs = [np.concatenate([no_in[:40], rng.choice(V, 90, replace=False)]),
+            rng.choice(V, 30, replace=False),
+            np.concatenate([no_in[40:140], rng.choice(np.flatnonzero(deg > 0), 100, replace=False)])]
+    with c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], device=0) as g:
+        for srcs in runs:
+            srcs = srcs.astype(np.int32)
+            d, st = g.sssp_batch(srcs)
+            sums, reach, _ = g.sssp_batch_resident(srcs)
+            refs = oracle.dijkstra_many(G["vlist"], G["elist"], G["wlist"], srcs)
+            for q in range(len(srcs)):
+                assert np.array_equal(d[q], refs[q]), (len(srcs), q)
+                fin = refs[q] != INT_MAX
+                assert sums[q] == int(refs[q][fin].astype(np.int64).sum()) and reach[q] == int(fin.sum()), q
+            m = sum(int(deg[refs[q] != INT_MAX].sum()) for q in range(len(srcs)))
+            assert st.reached_edges == m
+        # the server's lanes use group 0 of the same workspace
+        with c2g.Server(g, sssp_slots=-1) as srv:  # lane group only
+            qid = srv.submit(c2g.Q_SSSP, int(no_in[5]))
+            _, out = srv.poll(qid, -1)
+        srcs = runs[0].astype(np.int32)
+        d, _ = g.sssp_batch(srcs)
+    refs = oracle.dijkstra_many(G["vlist"], G["elist"], G["wlist"], srcs)
+    assert all(np.array_equal(d[q], refs[q]) for q in range(len(srcs)))
+    assert np.array_equal(out, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], int(no_in[5])))
