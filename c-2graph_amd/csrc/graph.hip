@@ -122,7 +122,11 @@ int light_build(Graph* g) {
     lr[0] = 0;
     for (int64_t v = 0; v < V; ++v) lr[v + 1] += lr[v];
     const int64_t EL = lr[V];
-    if (E + EL > INT32_MAX) return fail(C2G_ENOTSUP, "light rows would need more than 2^31 edge entries");
+    if (E + EL > INT32_MAX) {
+        // edge offsets are int32: such a graph runs without the mode
+        g->light_built = 0;
+        return C2G_OK;
+    }
     std::vector<int2> tail((size_t)std::max<int64_t>(EL, 1));
     parallel_for(V, [&](int64_t lo, int64_t hi, int) {
         for (int64_t v = lo; v < hi; ++v) {
