@@ -269,6 +269,31 @@ sssp_relax_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
     }
 }
 
+// Calls f(v, b, e) for each set bit of m (vertex base + bit) with v's row
+// [r[v], r[v+1]), four vertices at a time: their offset loads are issued
+// together instead of one dependent pair per vertex.
+template <class F>
+__device__ __forceinline__ void walk_rows4(uint32_t m, int64_t base, const int32_t* __restrict__ r, F&& f) {
+    while (m) {
+        int v[4], b[4], e[4];
+        bool ok[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            ok[k] = m != 0;
+            v[k] = ok[k] ? (int)(base + __builtin_ctz(m)) : 0;
+            if (ok[k]) m &= m - 1;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            b[k] = ok[k] ? r[v[k]] : 0;
+            e[k] = ok[k] ? r[v[k] + 1] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (ok[k]) f(v[k], b[k], e[k]);
+    }
+}
+
 // Compacts the activation bitmaps into the next round's items: each lane owns
 // one 32-vertex word, counts the items of its set bits (ceil(deg/chunk) per
 // live list, the same for hidden lists), the wave scans the counts and takes
@@ -360,6 +385,20 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
         int cnt[NP];
 #pragma unroll
         for (int p = 0; p < NP; ++p) cnt[p] = 0;
+        if constexpr (!SPLIT) {
+            auto count = [&](int, int b, int e) {
+                front += 1;
+                edges += e - b;
+                cnt[0] += (e - b + chunk - 1) / chunk;
+            };
+            walk_rows4(nl, wi * 32, lt ? lrow : row, count);
+            walk_rows4(hv, wi * 32, row, count);
+            if (PRUNED)
+                walk_rows4(nh, wi * 32, hrow, [&](int, int b, int e) {
+                    hedges += e - b;
+                    cnt[0] += (e - b + chunk - 1) / chunk;
+                });
+        } else {
         for (uint32_t m = nl; m; m &= m - 1) {
             const int v = (int)(wi * 32 + __builtin_ctz(m));
             front += 1;
@@ -393,6 +432,7 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
                     cnt[p] += (e - b + chunk - 1) / chunk;
                 }
             }
+        }
         near_n += __popc(nl | nh | hv);
         int off[NP], tot[NP];
         int any = 0;
@@ -432,13 +472,45 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
             if (lane == 0) atomicOr((unsigned long long*)&ctrl[C_OVF], 1ull);
             continue;
         }
+        if constexpr (!SPLIT) {
+            // The wave writes its items together, one vertex per lane per
+            // step (each lane takes the next vertex of its own word), item j
+            // of a step by lane j % 64: consecutive items from consecutive
+            // lanes, where per-lane runs scattered every store.  The wave's
+            // items fill [wave base, wave base + total) in step order.
+            int64_t pw = pos[0] - off[0];  // the wave's base (uniform)
+            auto emit = [&](uint32_t mm, const int32_t* __restrict__ r, int kind) {
+                while (__ballot(mm != 0)) {
+                    int v = 0, b = 0, e = 0;
+                    if (mm) {
+                        v = (int)(wi * 32 + __builtin_ctz(mm));
+                        mm &= mm - 1;
+                        b = r[v];
+                        e = r[v + 1];
+                    }
+                    const int c = e > b ? (e - b + chunk - 1) / chunk : 0;
+                    int ctot;
+                    const int cpre = wave_excl_scan(c, &ctot);
+                    for (int j0 = 0; j0 < ctot; j0 += kWave) {
+                        const int j = j0 + lane;
+                        const int o = wave_owner(cpre, min(j, ctot - 1));
+                        const int kk = j - __shfl(cpre, o, 64);
+                        const int vo = __shfl(v, o, 64), bo = __shfl(b, o, 64), eo = __shfl(e, o, 64);
+                        if (j < ctot) {
+                            const int x = bo + kk * chunk;
+                            q[pw + j] = make_int4(vo, x, min(x + chunk, eo), tag | kind);
+                        }
+                    }
+                    pw += ctot;
+                }
+            };
+            emit(nl, lt ? lrow : row, 0);
+            emit(hv, row, 0);
+            if (PRUNED) emit(nh, hrow, 1);
+            continue;
+        }
         for (uint32_t m = nl; m; m &= m - 1) {
             const int v = (int)(wi * 32 + __builtin_ctz(m));
-            if (lt) {
-                for (int b = lrow[v], e = lrow[v + 1]; b < e; b += chunk)
-                    q[pos[0]++] = make_int4(v, b, min(b + chunk, e), tag);
-                continue;
-            }
 #pragma unroll
             for (int p = 0; p < NP; ++p) {
                 int b0, e;
