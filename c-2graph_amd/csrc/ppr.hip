@@ -380,7 +380,6 @@ pprb_push_k(const int2* __restrict__ adjf, float* __restrict__ M, const float* _
             // exec), so per lane there is one multiply and the atomic.
             if (act) {
                 gchar* MgS = sgpr_ptr(Mg);  // the group's rows, base in SGPRs
-#pragma unroll 8
                 for (int j = 0; j < ne; ++j) {
                     const int v = __builtin_amdgcn_readlane(my.x, j);
                     const float f = __int_as_float(__builtin_amdgcn_readlane(my.y, j));
