@@ -60,11 +60,11 @@ double time_ppr(Graph* g, const int32_t* srcs, int32_t n, float alpha, float eps
 
 }  // namespace
 
-// PPR push/pull switch point: the share of all out-edges (in thousandths)
-// above which a round's frontier is propagated by a pull sweep over in-edges
-// instead of scattered pushes, timed on the caller's sources, alpha and eps.
-// Both forms compute the same round, so the choice changes FP32 summation
-// order only.
+// Single-source PPR: workgroups per CU (1, 2, 4, 8), then the push/pull
+// switch point: the share of all out-edges (in thousandths) above which a
+// round's frontier is propagated by a pull sweep over in-edges instead of
+// scattered pushes, timed on the caller's sources, alpha and eps.  Both forms
+// compute the same round, so the choices change FP32 summation order only.
 extern "C" int c2g_autotune_ppr(c2g_graph* h, const int32_t* srcs, int32_t n, float alpha, float eps,
                                 c2g_tune* best_out) {
     if (!h || !srcs || n < 1) return fail(C2G_EINVAL, "c2g_autotune_ppr: need at least one sample source");
@@ -80,6 +80,17 @@ extern "C" int c2g_autotune_ppr(c2g_graph* h, const int32_t* srcs, int32_t n, fl
     if (rc) return rc;
     (void)time_ppr(g, srcs, 1, alpha, eps, &rc);  // warm: allocations, in-edge rows
     double tbest = time_ppr(g, srcs, m, alpha, eps, &rc);
+    for (int gpc : {1, 2, 4, 8}) {
+        if (rc) break;
+        c2g_tune t = best;
+        t.ppr_grid_per_cu = gpc;
+        if (!memcmp(&t, &best, sizeof t)) continue;
+        if ((rc = set_tune_locked(g, t))) break;
+        const double tt = time_ppr(g, srcs, m, alpha, eps, &rc);
+        if (rc) break;
+        if (tt < tbest * 0.98) { tbest = tt; best = t; }
+    }
+    if (!rc) rc = set_tune_locked(g, best);
     for (int v : {0, 25, 50, 100, 200, 400, 800}) {
         if (rc) break;
         c2g_tune t = best;

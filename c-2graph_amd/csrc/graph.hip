@@ -66,6 +66,7 @@ void default_tune(Graph* g) {
     t.delta = 0;
     t.light = default_light(g);
     t.batch_order = 1;
+    t.ppr_grid_per_cu = 1;
 }
 
 // Light-edge bound: about a sixteenth of the mean weight (on R-MAT with
@@ -577,6 +578,7 @@ int c2g_set_tune(c2g_graph* h, const c2g_tune* t) {
     if (t->delta < 0) return fail(C2G_EINVAL, "delta >= 0");
     if (t->light < 0) return fail(C2G_EINVAL, "light >= 0");
     if (t->batch_order != 0 && t->batch_order != 1) return fail(C2G_EINVAL, "batch_order must be 0 or 1");
+    if (t->ppr_grid_per_cu < 0 || t->ppr_grid_per_cu > 64) return fail(C2G_EINVAL, "ppr_grid_per_cu in [0,64]");
     C2G_HANDLE_LOCK(h);
     if (memcmp(&h->g.tune, t, sizeof *t) != 0) {
         // Captured launch sequences hold the old values as kernel arguments.

@@ -100,16 +100,21 @@ __global__ void ppr_seed_k(const int32_t* __restrict__ row, const int2* __restri
 
 // Pushes the queued residuals along their edges: m[v] += (1-alpha) r_u f(u,v)
 // (FP32 atomics), and marks v in the touched bitmap for the expand.
+// form: 0 = the round's form as the expand chose it; 1 = push rounds only,
+// 2 = pull rounds only (a launch of the other form returns at once and
+// leaves the round to its partner launch, so push rounds can run on a
+// small grid and pull sweeps on a large one).
 __global__ void __launch_bounds__(256)
 ppr_push_k(const int2* __restrict__ adjf, float* __restrict__ m, uint32_t* __restrict__ tb,
            const int4* __restrict__ q, int64_t* __restrict__ ctrl, float alpha,
            const int2* __restrict__ tadjf, const int4* __restrict__ titems, int64_t nt,
-           const int2* __restrict__ ru, const uint32_t* __restrict__ stamp) {
+           const int2* __restrict__ ru, const uint32_t* __restrict__ stamp, int form) {
+    const bool pull = ctrl[P_MODE] != 0;
+    if ((form == 1 && pull) || (form == 2 && !pull)) return;  // the partner launch's round
     const int64_t n = ctrl[P_CUR];
     const int64_t wave = wave_index();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     const int lane = lane_id();
-    const bool pull = ctrl[P_MODE] != 0;
     if (pull) {
         // Pull sweep over the in-edge items (v, e_begin, e_end): the edges of
         // 64 items are walked 64 at a time; lanes of one item form a segment
@@ -745,7 +750,10 @@ static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, f
     const bool pruned = ppr_pruned(g);
     const float credit = spec.credit < 0.0f ? alpha : spec.credit;
     const int64_t V = g->V;
-    const int grid = std::max(1, g->num_cus) * g->tune.grid_per_cu;
+    // push rounds and the expand on the small single-source grid; pull
+    // sweeps (dense rounds) on the full grid
+    const int grid = std::max(1, g->num_cus) * std::max(1, g->tune.ppr_grid_per_cu);
+    const int pgrid = std::max(1, g->num_cus) * g->tune.grid_per_cu;
     const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, (V + 255) / 256));
     const int wgrid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, ((V + 31) / 32 + 255) / 256));
     // Pull rounds: a round whose frontier holds more than ppr_pull/1000 of E
@@ -771,8 +779,12 @@ static int ppr_single_ex(Graph* g, hipStream_t strm, int32_t src, float alpha, f
     for (;;) {
         const int K = budget > 0 ? 1 : g->tune.iters_per_graph;
         for (int k = 0; k < K; ++k) {
+            if (can_pull)  // the pull form first: it returns at once in a push round
+                hipLaunchKernelGGL(ppr_push_k, dim3(pgrid), dim3(256), 0, strm, g->d_adjf, s->m, s->tb, s->q,
+                                   s->ctrl, alpha, s->tadjf, s->titems, s->nt, s->ru, s->stamp, 2);
             hipLaunchKernelGGL(ppr_push_k, dim3(grid), dim3(256), 0, strm, g->d_adjf, s->m, s->tb, s->q,
-                               s->ctrl, alpha, s->tadjf, s->titems, can_pull ? s->nt : 0, s->ru, s->stamp);
+                               s->ctrl, alpha, s->tadjf, s->titems, can_pull ? s->nt : 0, s->ru, s->stamp,
+                               can_pull ? 1 : 0);
             hipLaunchKernelGGL(ppr_expand_k, dim3(wgrid), dim3(256), 0, strm, g->d_row, s->p, s->r, s->m,
                                s->tb, s->q, s->ctrl, V, credit, eps, g->tune.chunk, s->qcap, 0, spec.degscale,
                                ru, s->stamp, pull_edges, can_pull);

@@ -150,7 +150,10 @@ typedef struct {
                                reach, so that each group's lanes are pending
                                in the same rounds (results in the caller's
                                order); 0 = the caller's order           */
-    int32_t reserved[1];
+    int32_t ppr_grid_per_cu;/* workgroups per CU of single-source PPR and the
+                               adsorption push modes (0 = 1): their rounds
+                               are small, and fewer workgroups launch and
+                               retire faster (1..64)                      */
 } c2g_tune;
 
 /* ======================================================================= */

@@ -34,7 +34,7 @@ def test_autotune_ppr_switch_point(gpu):
     with c2g.Graph.from_csr(G["vlist"], G["elist"], flist=G["flist"], device=0) as g:
         p0, r0, _ = g.ppr(int(srcs[0]), eps=1e-9)
         t = g.autotune_ppr(srcs, eps=1e-9)
-        assert t.ppr_pull in (0, 25, 50, 100, 200, 400, 800)
+        assert t.ppr_pull in (0, 25, 50, 100, 200, 400, 800) and t.ppr_grid_per_cu in (1, 2, 4, 8)
         assert g.tune().as_dict() == t.as_dict()
         p1, r1, _ = g.ppr(int(srcs[0]), eps=1e-9)
         with pytest.raises(c2g.C2GError):

@@ -89,7 +89,7 @@ def test_default_light_bound(wmax, want):
     # light (unit weights: none is)
     G = c2g.rmat(12, 16, seed=3, wmax=wmax)
     with c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], no_device=True) as g:
-        assert g.tune().light == want and g.tune().batch_order == 1
+        assert g.tune().light == want and g.tune().batch_order == 1 and g.tune().ppr_grid_per_cu == 1
         g.set_tune(light=5)
         assert g.tune().light == 5
         with pytest.raises(c2g.C2GError):
