@@ -265,28 +265,52 @@ ppr_expand_k(const int32_t* __restrict__ row, float* __restrict__ p, float* __re
             if (force || r[v] + mv > (degscale ? (float)d * eps : eps)) cnt += (d + chunk - 1) / chunk;
         }
         int64_t wend;
-        int64_t pos = wg_reserve(cnt, (unsigned long long*)&ctrl[P_CUR], s_tot, &s_base, &wend);
+        const int64_t pos = wg_reserve(cnt, (unsigned long long*)&ctrl[P_CUR], s_tot, &s_base, &wend);
         const bool room = wend <= qcap;
         if (!room && lane == 0) atomicOr((unsigned long long*)&ctrl[P_OVF], 1ull);
-        for (uint32_t b = word; b; b &= b - 1) {
-            const int64_t v = i * 32 + __builtin_ctz(b);
-            const float mv = m[v];
-            if (mv == 0.0f) continue;
-            m[v] = 0.0f;
-            const float rv = r[v] + mv;
-            const int r0 = row[v], r1 = row[v + 1];
-            if (force || rv > (degscale ? (float)(r1 - r0) * eps : eps)) {
-                p[v] += credit * rv;
-                r[v] = 0.0f;
-                if (ru) ru[v] = make_int2(S1, __float_as_int(rv));
-                front += 1;
-                edges += r1 - r0;
-                if (room)
-                    for (int e = r0; e < r1; e += chunk)
-                        q[pos++] = make_int4((int)v, e, min(e + chunk, r1), __float_as_int(rv));
-            } else {
-                r[v] = rv;
+        // One vertex per lane per step (each lane takes the next bit of its
+        // own word); the step's items are written by the whole wave, item j
+        // by lane j % 64, from the wave's base (lane 0's position) on.
+        int64_t pw = shfl64(pos, 0);
+        for (uint32_t b = word; __ballot(b != 0);) {
+            int64_t v = 0;
+            int r0 = 0, r1 = 0, c = 0;
+            float rv = 0.0f;
+            if (b) {
+                v = i * 32 + __builtin_ctz(b);
+                b &= b - 1;
+                const float mv = m[v];
+                if (mv != 0.0f) {
+                    m[v] = 0.0f;
+                    rv = r[v] + mv;
+                    r0 = row[v];
+                    r1 = row[v + 1];
+                    if (force || rv > (degscale ? (float)(r1 - r0) * eps : eps)) {
+                        p[v] += credit * rv;
+                        r[v] = 0.0f;
+                        if (ru) ru[v] = make_int2(S1, __float_as_int(rv));
+                        front += 1;
+                        edges += r1 - r0;
+                        if (room) c = (r1 - r0 + chunk - 1) / chunk;
+                    } else {
+                        r[v] = rv;
+                    }
+                }
             }
+            int ctot;
+            const int cpre = wave_excl_scan(c, &ctot);
+            for (int j0 = 0; j0 < ctot; j0 += kWave) {
+                const int j = j0 + lane;
+                const int o = wave_owner(cpre, min(j, ctot - 1));
+                const int kk = j - __shfl(cpre, o, 64);
+                const int vo = __shfl((int)v, o, 64), bo = __shfl(r0, o, 64), eo = __shfl(r1, o, 64);
+                const int ro = __shfl(__float_as_int(rv), o, 64);
+                if (j < ctot) {
+                    const int e = bo + kk * chunk;
+                    q[pw + j] = make_int4(vo, e, min(e + chunk, eo), ro);
+                }
+            }
+            pw += ctot;
         }
     }
     front = wave_sum64(front);
