@@ -507,21 +507,44 @@ pprb_emit_k(const int32_t* __restrict__ row, uint64_t* __restrict__ mask, uint32
             }
         }
         int64_t wend;
-        int64_t pos = wg_reserve(cnt, (unsigned long long*)&ctrl[P_CUR], s_tot, &s_base, &wend);
+        const int64_t pos = wg_reserve(cnt, (unsigned long long*)&ctrl[P_CUR], s_tot, &s_base, &wend);
         const bool room = wend <= itemcap;
         if (!room && lane == 0) atomicOr((unsigned long long*)&ctrl[P_OVF], 1ull);
-        for (uint32_t b = word; b; b &= b - 1) {
-            const int64_t v = vb + __builtin_ctz(b);
-            uint64_t* mp = &mask[(int64_t)g * V + v];
-            const uint64_t mk = *mp;
-            if (!mk) continue;
-            *mp = 0;
-            if (!room) continue;
-            const int r0 = row[v], r1 = row[v + 1];
-            for (int e = r0; e < r1; e += chunk, ++pos) {
-                items[pos] = make_int4((int)v, e, min(e + chunk, r1), g);
-                imask[pos] = mk;
+        // one vertex per lane per step, the step's items written by the
+        // whole wave from the wave's base (lane 0's position) on
+        int64_t pw = shfl64(pos, 0);
+        for (uint32_t b = word; __ballot(b != 0);) {
+            int64_t v = 0;
+            int r0 = 0, r1 = 0, c = 0;
+            uint64_t mk = 0;
+            if (b) {
+                v = vb + __builtin_ctz(b);
+                b &= b - 1;
+                uint64_t* mp = &mask[(int64_t)g * V + v];
+                mk = *mp;
+                if (mk) {
+                    *mp = 0;
+                    r0 = row[v];
+                    r1 = row[v + 1];
+                    if (room) c = (r1 - r0 + chunk - 1) / chunk;
+                }
             }
+            int ctot;
+            const int cpre = wave_excl_scan(c, &ctot);
+            for (int j0 = 0; j0 < ctot; j0 += kWave) {
+                const int j = j0 + lane;
+                const int o = wave_owner(cpre, min(j, ctot - 1));
+                const int kk = j - __shfl(cpre, o, 64);
+                const int vo = __shfl((int)v, o, 64), bo = __shfl(r0, o, 64), eo = __shfl(r1, o, 64);
+                const int go = __shfl(g, o, 64);
+                const uint64_t mo = (uint64_t)shfl64((int64_t)mk, o);
+                if (j < ctot) {
+                    const int e = bo + kk * chunk;
+                    items[pw + j] = make_int4(vo, e, min(e + chunk, eo), go);
+                    imask[pw + j] = mo;
+                }
+            }
+            pw += ctot;
         }
     }
     front = wave_sum64(front);
