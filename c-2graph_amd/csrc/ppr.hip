@@ -45,14 +45,15 @@
 namespace c2g {
 
 enum : int {
-    P_CUR = 0, P_ITER = 1, P_DONE = 2, P_DONE_E = 3, P_FRONT = 4, P_EDGES = 5, P_ITEMS = 6,
+    P_CUR = 0, P_ITER = 1, P_FRONT = 4, P_EDGES = 5, P_ITEMS = 6,
     P_OVF = 7,
     P_MODE = 8,    // single source: 1 = the next round runs as a pull sweep
     P_REDGES = 9,  // single source: out-edges of the vertices the running expand activates
     P_PULLS = 10,  // single source: rounds run as pull sweeps
     P_WANT = 11,   // single source: some round qualified for a pull sweep
-    P_N = 16
+    P_N = 16       // fields; the block is kCtrlWords long (completion tree after them)
 };
+static_assert(P_N <= kCtrlTree, "control fields overlap the completion tree");
 
 constexpr int kPullChunk = 64;  // in-edges per pull item
 constexpr int kPullWin = 4;     // 64-edge windows of a pull step
@@ -76,8 +77,10 @@ __global__ void ppr_seed_k(const int32_t* __restrict__ row, const int2* __restri
                            float* __restrict__ p, float* __restrict__ m, uint32_t* __restrict__ tb,
                            int64_t* __restrict__ ctrl, int32_t src, float alpha, int pruned, float res0,
                            float p_src) {
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         for (int i = 0; i < P_N; ++i) ctrl[i] = 0;
+        done_tree_clear(ctrl);
+    }
     if (!pruned) {
         // residual res0 at the source; its first (forced) activation adds alpha*res0 to p_src
         if (threadIdx.x == 0) {
@@ -104,7 +107,7 @@ __global__ void ppr_seed_k(const int32_t* __restrict__ row, const int2* __restri
 // 2 = pull rounds only (a launch of the other form returns at once and
 // leaves the round to its partner launch, so push rounds can run on a
 // small grid and pull sweeps on a large one).
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) C2G_OCC8_SGPR
 ppr_push_k(const int2* __restrict__ adjf, float* __restrict__ m, uint32_t* __restrict__ tb,
            const int4* __restrict__ q, int64_t* __restrict__ ctrl, float alpha,
            const int2* __restrict__ tadjf, const int4* __restrict__ titems, int64_t nt,
@@ -209,15 +212,11 @@ ppr_push_k(const int2* __restrict__ adjf, float* __restrict__ m, uint32_t* __res
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
-        unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[P_DONE], 1ull);
-        if (prev == (unsigned long long)gridDim.x - 1) {
-            __threadfence();
+        if (wg_last(ctrl + kCtrlTree)) {
             if (n > 0) ctrl[P_ITER] += 1;
             if (n > 0 && pull) ctrl[P_PULLS] += 1;
             ctrl[P_ITEMS] += n;
             ctrl[P_CUR] = 0;
-            ctrl[P_DONE] = 0;
         }
     }
 }
@@ -234,7 +233,7 @@ ppr_push_k(const int2* __restrict__ adjf, float* __restrict__ m, uint32_t* __res
 // vertices hold more than pull_edges out-edges and the in-edge rows exist
 // (can_pull); a qualifying round without them sets P_WANT, and the host
 // builds them at its next check.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) C2G_OCC8_SGPR
 ppr_expand_k(const int32_t* __restrict__ row, float* __restrict__ p, float* __restrict__ r,
              float* __restrict__ m, uint32_t* __restrict__ tb, int4* __restrict__ q, int64_t* __restrict__ ctrl,
              int64_t V, float credit, float eps, int chunk, int64_t qcap, int force, int degscale,
@@ -327,11 +326,8 @@ ppr_expand_k(const int32_t* __restrict__ row, float* __restrict__ p, float* __re
             atomicAdd((unsigned long long*)&ctrl[P_EDGES], s_acc[1]);
             atomicAdd((unsigned long long*)&ctrl[P_REDGES], s_acc[1]);
         }
-        __threadfence();
-        unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[P_DONE_E], 1ull);
-        if (prev == (unsigned long long)gridDim.x - 1) {
-            __threadfence();
-            ctrl[P_DONE_E] = 0;
+        wait_vmem();  // this workgroup's counts are performed before it counts off
+        if (wg_last(ctrl + kCtrlTree)) {
             int64_t c = (int64_t)atomicAdd((unsigned long long*)&ctrl[P_CUR], 0ull);
             if (c > qcap) { ctrl[P_OVF] = 1; ctrl[P_CUR] = 0; }
             const int64_t re = (int64_t)atomicAdd((unsigned long long*)&ctrl[P_REDGES], 0ull);
@@ -360,8 +356,10 @@ __global__ void pprb_seed_k(const int32_t* __restrict__ srcs, int n, const int32
                             uint32_t* __restrict__ tbits, int64_t* __restrict__ ctrl, int64_t V, int64_t nw,
                             float alpha) {
     const int q = blockIdx.x;
-    if (q == 0 && threadIdx.x == 0)
+    if (q == 0 && threadIdx.x == 0) {
         for (int i = 0; i < P_N; ++i) ctrl[i] = 0;
+        done_tree_clear(ctrl);
+    }
     if (q >= n) return;
     const int g = q >> 6, lane = q & 63;
     const int s = srcs[q];
@@ -423,14 +421,10 @@ pprb_push_k(const int2* __restrict__ adjf, float* __restrict__ M, const float* _
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
-        unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[P_DONE], 1ull);
-        if (prev == (unsigned long long)gridDim.x - 1) {
-            __threadfence();
+        if (wg_last(ctrl + kCtrlTree)) {
             if (n > 0) ctrl[P_ITER] += 1;
             ctrl[P_ITEMS] += n;
             ctrl[P_CUR] = 0;
-            ctrl[P_DONE] = 0;
         }
     }
 }
@@ -474,7 +468,7 @@ pprb_activate_k(const int32_t* __restrict__ row, float* __restrict__ P, float* _
     }
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) C2G_OCC8_SGPR
 pprb_emit_k(const int32_t* __restrict__ row, uint64_t* __restrict__ mask, uint32_t* __restrict__ tbits,
             int4* __restrict__ items, uint64_t* __restrict__ imask, int64_t* __restrict__ ctrl, int64_t V,
             int64_t nw, int groups, int chunk, int64_t itemcap) {
@@ -560,11 +554,8 @@ pprb_emit_k(const int32_t* __restrict__ row, uint64_t* __restrict__ mask, uint32
             atomicAdd((unsigned long long*)&ctrl[P_FRONT], s_acc[0]);
             atomicAdd((unsigned long long*)&ctrl[P_EDGES], s_acc[1]);
         }
-        __threadfence();
-        unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[P_DONE_E], 1ull);
-        if (prev == (unsigned long long)gridDim.x - 1) {
-            __threadfence();
-            ctrl[P_DONE_E] = 0;
+        wait_vmem();  // this workgroup's counts are performed before it counts off
+        if (wg_last(ctrl + kCtrlTree)) {
             int64_t c = (int64_t)atomicAdd((unsigned long long*)&ctrl[P_CUR], 0ull);
             if (c > itemcap) { ctrl[P_OVF] = 1; ctrl[P_CUR] = 0; }
         }
@@ -641,7 +632,7 @@ int64_t ppr_lane_ws_bytes(Graph* g) {
     const int64_t V = std::max<int64_t>(g->V, 1);
     const int64_t nw = (V + 31) / 32 + 1;
     int64_t b = 0;
-    if (!s->p) b += 12 * V + 4 * nw + 8 * P_N;
+    if (!s->p) b += 12 * V + 4 * nw + 8 * kCtrlWords;
     const int64_t needq = V + g->E / std::max(1, g->tune.chunk) + 64;
     if (s->qcap < needq) b += 16 * needq;
     const int64_t needi = (V + g->E / std::max(1, g->tune.batch_chunk)) + 64;
@@ -671,7 +662,7 @@ static int ensure_ppr_single(Graph* g, PprState* s) {
     if (!s->r) HIP_TRY(hipMalloc(&s->r, sizeof(float) * V));
     if (!s->m) HIP_TRY(hipMalloc(&s->m, sizeof(float) * V));
     if (!s->tb) HIP_TRY(hipMalloc(&s->tb, sizeof(uint32_t) * ((V + 31) / 32 + 1)));
-    if (!s->ctrl) HIP_TRY(hipMalloc(&s->ctrl, sizeof(int64_t) * P_N));
+    if (!s->ctrl) HIP_TRY(hipMalloc(&s->ctrl, sizeof(int64_t) * kCtrlWords));
     if (!s->h_ctrl) HIP_TRY(hipHostMalloc((void**)&s->h_ctrl, sizeof(int64_t) * P_N, hipHostMallocDefault));
     if (s->qcap < need) {
         if (s->q) (void)hipFree(s->q);
@@ -893,7 +884,7 @@ static int ensure_ppr_batch(Graph* g, PprState* s, int groups) {
     const int64_t V = std::max<int64_t>(g->V, 1);
     const int chunk = std::max(1, g->tune.batch_chunk);
     const int64_t needi = groups * (V + g->E / chunk) + 64;
-    if (!s->ctrl) HIP_TRY(hipMalloc(&s->ctrl, sizeof(int64_t) * P_N));
+    if (!s->ctrl) HIP_TRY(hipMalloc(&s->ctrl, sizeof(int64_t) * kCtrlWords));
     if (!s->h_ctrl) HIP_TRY(hipHostMalloc((void**)&s->h_ctrl, sizeof(int64_t) * P_N, hipHostMallocDefault));
     if (groups > s->groups || s->itemcap < needi) {
         const int G = std::max(groups, s->groups);

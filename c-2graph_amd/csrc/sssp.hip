@@ -52,13 +52,11 @@ namespace c2g {
 enum : int {
     C_CUR = 0,    // items in the current round's queue
     C_ITER = 2,   // rounds with work
-    C_DONE = 3,   // workgroups finished (relax)
     C_FRONT = 4,  // sum of frontier vertices over rounds
     C_EDGES = 5,  // edges queued for relaxation (R)
     C_IMPR = 6,   // improvement writes issued (U): candidate below the value read
     C_OVF = 7,    // queue overflow guard tripped (must stay 0)
     C_HEDGES = 8, // hidden edges queued
-    C_DONE_E = 9, // workgroups finished (expand)
     C_ITEMS = 10, // items processed
     C_THR = 11,   // delta-stepping: current bucket bound (INT32_MAX = plain rounds)
     C_FMIN = 12,  // smallest deferred distance this round
@@ -66,8 +64,9 @@ enum : int {
     C_FAR = 14,   // 1 while deferred vertices remain
     C_SLOTS = 15, // slot pools: bit s set when slot s has items queued for the next round
     C_QN = 16,    // split mode: items of partition p for the next round at C_QN + p (8 entries)
-    C_N = 24
+    C_N = 24      // fields; the block is kCtrlWords long (completion tree after them)
 };
+static_assert(C_N <= kCtrlTree, "control fields overlap the completion tree");
 
 // Split mode (c2g_tune.xcd_split): the edges of a frontier row are cut by
 // the partition of their target, p = v / ceil(V/8), into eight queues, and
@@ -130,6 +129,7 @@ __global__ void sssp_seed_k(const int32_t* __restrict__ row, const int32_t* __re
             dist[src] = 0;
             if (hid) hid[src] = 1;
             for (int i = 0; i < C_N; ++i) ctrl[i] = 0;
+            done_tree_clear(ctrl);
             ctrl[C_THR] = delta > 0 ? delta : INT32_MAX;
             ctrl[C_FMIN] = INT32_MAX;
             int64_t tot = 0;
@@ -154,6 +154,7 @@ __global__ void sssp_seed_k(const int32_t* __restrict__ row, const int32_t* __re
         dist[src] = 0;
         if (hid) hid[src] = 1;  // the source's hidden list is always relaxed
         for (int i = 0; i < C_N; ++i) ctrl[i] = 0;
+        done_tree_clear(ctrl);
         ctrl[C_THR] = delta > 0 ? delta : INT32_MAX;
         ctrl[C_FMIN] = INT32_MAX;
         ctrl[C_CUR] = nl + nh;
@@ -251,10 +252,7 @@ sssp_relax_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
     __syncthreads();
     if (threadIdx.x == 0) {
         if (s_impr) atomicAdd((unsigned long long*)&ctrl[C_IMPR], s_impr);
-        __threadfence();
-        unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[C_DONE], 1ull);
-        if (prev == (unsigned long long)gridDim.x - 1) {
-            __threadfence();
+        if (wg_last(ctrl + kCtrlTree)) {
             int64_t ntot = n;
             if (SPLIT) {
                 ntot = 0;
@@ -263,7 +261,6 @@ sssp_relax_k(const int2* __restrict__ adj, const int2* __restrict__ hadj,
             if (ntot > 0) ctrl[C_ITER] += 1;
             ctrl[C_ITEMS] += ntot;
             ctrl[C_CUR] = 0;
-            ctrl[C_DONE] = 0;
             ctrl[C_SLOTS] = 0;
         }
     }
@@ -558,11 +555,8 @@ sssp_expand_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow,
         if (s_acc[3]) atomicAdd((unsigned long long*)&ctrl[C_NEAR], s_acc[3]);
         if (s_acc[4]) atomicOr((unsigned long long*)&ctrl[C_SLOTS], s_acc[4]);
         if (s_fmin != INT32_MAX) atomicMin((long long*)&ctrl[C_FMIN], (long long)s_fmin);
-        __threadfence();
-        unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[C_DONE_E], 1ull);
-        if (prev == (unsigned long long)gridDim.x - 1) {
-            __threadfence();
-            ctrl[C_DONE_E] = 0;
+        wait_vmem();  // this workgroup's counts are performed before it counts off
+        if (wg_last(ctrl + kCtrlTree)) {
             const int64_t fm = (int64_t)atomicAdd((unsigned long long*)&ctrl[C_FMIN], 0ull);
             const int64_t nearc = (int64_t)atomicAdd((unsigned long long*)&ctrl[C_NEAR], 0ull);
             ctrl[C_FAR] = fm != INT32_MAX ? 1 : 0;
@@ -680,7 +674,7 @@ int ensure_sssp_ws(Graph* g) {
     if (!w.dist) HIP_TRY(hipMalloc(&w.dist, sizeof(int32_t) * std::max<int64_t>(V, 1)));
     // live, hidden and pending (light mode) activation bitmaps
     if (!w.bits) HIP_TRY(hipMalloc(&w.bits, sizeof(uint32_t) * nwp * 3));
-    if (!w.ctrl) HIP_TRY(hipMalloc(&w.ctrl, sizeof(int64_t) * C_N));
+    if (!w.ctrl) HIP_TRY(hipMalloc(&w.ctrl, sizeof(int64_t) * kCtrlWords));
     if (!w.ev[0]) HIP_TRY(hipEventCreate(&w.ev[0]));
     if (!w.ev[1]) HIP_TRY(hipEventCreate(&w.ev[1]));
     if (!w.h_ctrl) HIP_TRY(hipHostMalloc((void**)&w.h_ctrl, sizeof(int64_t) * C_N, hipHostMallocDefault));
@@ -711,7 +705,7 @@ struct SsspBufs {
     uint8_t* hid;      // [V] (pruned graphs)
     int4* q;           // frontier items
     int64_t qcap;      // items per queue (split mode: per partition)
-    int64_t* ctrl;     // [C_N]
+    int64_t* ctrl;     // [kCtrlWords]
     const int32_t* split = nullptr;   // split mode: partition offsets (null: one queue)
     const int32_t* hsplit = nullptr;
 };
@@ -1059,7 +1053,7 @@ static int64_t qserve_bytes(const Graph* g, int slots) {
     const int chunk = std::max(1, g->tune.chunk);
     const int64_t qcap = (int64_t)slots * (2 * V + (g->E + g->HE) / chunk) + 64;
     return 4 * slots * V + 12 * slots * bitmap_words(V) + (g->hvl.empty() ? 0 : slots * V) + 16 * qcap +
-           8 * C_N;
+           8 * kCtrlWords;
 }
 
 // Device bytes a slot pool of `slots` would add to what the handle holds.
@@ -1083,7 +1077,7 @@ int qserve_init(Graph* g, hipStream_t strm, int slots) {
     const int64_t needq = (int64_t)slots * (2 * std::max<int64_t>(g->V, 1) + (g->E + g->HE) / std::max(1, g->tune.chunk)) + 64;
     if (s->slots == slots && s->qcap >= needq) {
         HIP_TRY(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * 3 * slots * s->nwp, strm));
-        HIP_TRY(hipMemsetAsync(s->ctrl, 0, sizeof(int64_t) * C_N, strm));
+        HIP_TRY(hipMemsetAsync(s->ctrl, 0, sizeof(int64_t) * kCtrlWords, strm));
         HIP_TRY(hipStreamSynchronize(strm));
         return C2G_OK;
     }
@@ -1103,8 +1097,8 @@ int qserve_init(Graph* g, hipStream_t strm, int slots) {
     HIP_TRY(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * 3 * slots * s->nwp, strm));
     if (g->d_hrow) HIP_TRY(hipMalloc(&s->hid, (size_t)slots * V));
     HIP_TRY(hipMalloc(&s->q, sizeof(int4) * s->qcap));
-    HIP_TRY(hipMalloc(&s->ctrl, sizeof(int64_t) * C_N));
-    HIP_TRY(hipMemsetAsync(s->ctrl, 0, sizeof(int64_t) * C_N, strm));
+    HIP_TRY(hipMalloc(&s->ctrl, sizeof(int64_t) * kCtrlWords));
+    HIP_TRY(hipMemsetAsync(s->ctrl, 0, sizeof(int64_t) * kCtrlWords, strm));
     HIP_TRY(hipHostMalloc((void**)&s->h_ctrl, sizeof(int64_t) * C_N, hipHostMallocDefault));
     HIP_TRY(hipStreamSynchronize(strm));
     s->slots = slots;
@@ -1212,7 +1206,7 @@ static ScratchLayout scratch_layout(const Graph* g) {
     const int chunk = std::max(1, g->tune.chunk);
     L.qcap = 2 * V + (g->E + g->HE) / chunk + 64;  // the workspace's own bound (ensure_sssp_ws)
     L.ctrl = 0;
-    L.sums = up(sizeof(int64_t) * C_N);
+    L.sums = up(sizeof(int64_t) * kCtrlWords);
     L.bits = L.sums + up(sizeof(int64_t) * 4);
     L.hid = L.bits + up(sizeof(uint32_t) * 3 * bitmap_words(V));
     L.q = L.hid + (is_pruned(g) ? up((size_t)V) : 0);

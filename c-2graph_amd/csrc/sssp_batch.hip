@@ -49,8 +49,6 @@ enum : int {
     B_SPARE = 0,
     B_ITEMS = 1,   // items for the next relax round
     B_ITER = 2,    // relax rounds with work
-    B_DONE_R = 3,  // workgroups finished (relax)
-    B_DONE_E = 4,  // workgroups finished (expand)
     B_FRONT = 5,   // sum over rounds of activated (v,g)
     B_EDGES = 6,   // CSR edges streamed (per item, once for all lanes)
     B_IMPR = 7,    // improvement writes issued (U): candidate below the value read
@@ -60,12 +58,15 @@ enum : int {
     B_LANEOR = 11,     // server lane: lanes of group 0 with pending work after expand
     B_FAR = 12,        // 1 while some lane has deferred (far) work
     B_HITEMS = 13,     // hidden-list items for the next relax round (pruned graphs)
-    B_DONE_H = 14,     // workgroups finished (hidden relax)
-    B_N = 16
+    B_N = 16           // fields; the block is kCtrlWords long (completion tree after them)
 };
+static_assert(B_N <= kCtrlTree, "control fields overlap the completion tree");
 
 constexpr int kJ = 8;  // edges whose row loads are in flight together
-constexpr int kS = 2;  // expand: vertices per lane whose loads are in flight together
+// expand: vertices per lane whose loads are in flight together.  One, so
+// that the kernel fits 64 VGPRs and runs 8 waves per SIMD (two slots need 77
+// VGPRs, 6 waves; both take the same time within 4 %).
+constexpr int kS = 1;
 constexpr int kIdMaskB = 0x7fffffff;  // bit 31 of an edge target = target has edges
 
 __global__ void batch_fill_k(int32_t* __restrict__ dist, int64_t n_dist, uint64_t* __restrict__ mask,
@@ -93,8 +94,10 @@ __global__ void batch_seed_k(const int32_t* __restrict__ srcs, int n, const int3
                              uint64_t* __restrict__ mask, uint64_t* __restrict__ hmask,
                              uint32_t* __restrict__ abits, int64_t* __restrict__ ctrl, int64_t V, int64_t nw) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q == 0)
+    if (q == 0) {
         for (int i = 0; i < B_N; ++i) ctrl[i] = 0;
+        done_tree_clear(ctrl);
+    }
     if (q >= n) return;
     const int g = q >> 6, lane = q & 63;
     const int32_t s = srcs[q];
@@ -157,7 +160,7 @@ __device__ __forceinline__ void relax_step_b(const int2 my, int j0, int ne, int 
 // items of any graph run with HIDDEN = false, which never touches hmask.
 // BLOCK: threads per workgroup (128, 256 or 512; c2g_tune.block).
 template <bool HIDDEN, int BLOCK>
-__global__ void __launch_bounds__(BLOCK)
+__global__ void __launch_bounds__(BLOCK) C2G_OCC8_SGPR
 relax_b_k(const int2* __restrict__ adj, int32_t* __restrict__ dist, uint64_t* __restrict__ mask,
           uint64_t* __restrict__ hmask, const int4* __restrict__ items, const uint64_t* __restrict__ imask,
           uint32_t* __restrict__ abits, int64_t* __restrict__ ctrl, int64_t V, int64_t nw) {
@@ -218,16 +221,12 @@ relax_b_k(const int2* __restrict__ adj, int32_t* __restrict__ dist, uint64_t* __
     __syncthreads();
     if (threadIdx.x == 0) {
         if (s_impr) atomicAdd((unsigned long long*)&ctrl[B_IMPR], s_impr);
-        __threadfence();
-        unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[HIDDEN ? B_DONE_H : B_DONE_R], 1ull);
-        if (prev == (unsigned long long)gridDim.x - 1) {
-            __threadfence();
+        if (wg_last(ctrl + kCtrlTree)) {
             // a round counts once: the hidden pass runs first and counts only
             // rounds whose live queue is empty
             if (HIDDEN ? (n > 0 && ctrl[B_ITEMS] == 0) : n > 0) ctrl[B_ITER] += 1;
             ctrl[B_ITEMS_TOT] += n;
             ctrl[HIDDEN ? B_HITEMS : B_ITEMS] = 0;
-            ctrl[HIDDEN ? B_DONE_H : B_DONE_R] = 0;
         }
     }
 }
@@ -243,8 +242,11 @@ relax_b_k(const int2* __restrict__ adj, int32_t* __restrict__ dist, uint64_t* __
 // near (plain Bellman-Ford rounds).  Light-edge mode (lrow non-null, delta
 // 0): lowered lanes m of v get items over v's light row [lrow[v], lrow[v+1])
 // and join ph[g][v], and v stays active; a v with no lowered lane gets items
-// over its full row for the lanes in ph[g][v], which it clears.
-__global__ void __launch_bounds__(256)
+// over its full row for the lanes in ph[g][v], which it clears.  DELTA:
+// compiled for delta > 0 (the plain and light-edge forms leave the
+// delta-stepping code out, which keeps them within 64 VGPRs).
+template <bool DELTA>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) C2G_OCC8_SGPR
 expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, uint64_t* __restrict__ mask,
            const uint64_t* __restrict__ hmask, uint32_t* __restrict__ abits, int4* __restrict__ items,
            uint64_t* __restrict__ imask, int4* __restrict__ hitems, uint64_t* __restrict__ himask,
@@ -623,7 +625,7 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
         }
     };
     auto block = [&](int64_t blk, bool write) {
-        if (delta > 0) block_delta(blk, write);
+        if constexpr (DELTA) block_delta(blk, write);
         else block_bits(blk, write);
     };
     for (int64_t blk = bb0; blk < bb1; ++blk) block(blk, false);
@@ -682,16 +684,13 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
             if (lds_fmin[t] != INT32_MAX) atomicMin(&fmin[t], lds_fmin[t]);
     }
     // The last workgroup closes the round: bucket moves and lane summaries.
+    // Every wave's atomics are performed before the workgroup counts off.
     __shared__ int is_last;
+    wait_vmem();
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        unsigned long long prev = atomicAdd((unsigned long long*)&ctrl[B_DONE_E], 1ull);
-        is_last = prev == (unsigned long long)gridDim.x - 1;
-    }
+    if (threadIdx.x == 0) is_last = wg_last(ctrl + kCtrlTree);
     __syncthreads();
     if (!is_last) return;
-    __threadfence();
     for (int t = threadIdx.x; t < groups * 64; t += blockDim.x) {
         const int gg = t >> 6, q = t & 63;
         const uint64_t nb = (uint64_t)atomicOr((unsigned long long*)&lor[2 * gg], 0ull);
@@ -711,7 +710,6 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
         ctrl[B_LANEOR] = (int64_t)((uint64_t)atomicOr((unsigned long long*)&lor[0], 0ull) |
                                    (uint64_t)atomicOr((unsigned long long*)&lor[1], 0ull));
         for (int gg = 0; gg < groups; ++gg) lor[2 * gg] = lor[2 * gg + 1] = 0;
-        ctrl[B_DONE_E] = 0;
         int64_t ni = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ITEMS], 0ull);
         int64_t nh = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_HITEMS], 0ull);
         if (ni > itemcap || nh > hitemcap) { ctrl[B_OVF] = 1; ctrl[B_ITEMS] = 0; ctrl[B_HITEMS] = 0; }
@@ -857,8 +855,8 @@ static int ensure_batch(Graph* g, int groups) {
     if (!s->ev[0]) HIP_TRY(hipEventCreate(&s->ev[0]));
     if (!s->ev[1]) HIP_TRY(hipEventCreate(&s->ev[1]));
     if (!w.ctrl) {
-        HIP_TRY(hipMalloc(&w.ctrl, sizeof(int64_t) * B_N));
-        HIP_TRY(hipMemset(w.ctrl, 0, sizeof(int64_t) * B_N));
+        HIP_TRY(hipMalloc(&w.ctrl, sizeof(int64_t) * kCtrlWords));
+        HIP_TRY(hipMemset(w.ctrl, 0, sizeof(int64_t) * kCtrlWords));
     }
     if (!w.h_ctrl) HIP_TRY(hipHostMalloc((void**)&w.h_ctrl, sizeof(int64_t) * B_N, hipHostMallocDefault));
     // Each (v,g) is activated at most once per round: at most
@@ -951,7 +949,7 @@ static int expand_b(Graph* g, int groups, hipStream_t strm) {
     const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(relax_grid_b(g), (words + 255) / 256));
     const int delta = g->tune.delta;
     const size_t lds = delta > 0 ? sizeof(int32_t) * 64 * groups : 0;
-    hipLaunchKernelGGL(expand_b_k, dim3(egrid), dim3(256), lds, strm, g->d_row,
+    hipLaunchKernelGGL(delta > 0 ? expand_b_k<true> : expand_b_k<false>, dim3(egrid), dim3(256), lds, strm, g->d_row,
                        pruned ? g->d_hrow : (const int32_t*)nullptr, w.mask[0],
                        pruned ? w.hmask : (const uint64_t*)nullptr, s->abits, s->items, s->imask, s->hitems,
                        s->himask, w.ctrl, (int64_t)g->V, s->nw, groups, g->tune.batch_chunk, s->itemcap,
@@ -1336,7 +1334,7 @@ int64_t bserve_ws_bytes(Graph* g) {
                 have = true;
         if (!have)
             b += 256 * V + 8 * V * (pruned ? 2 : 1) + (light_rows(g) ? 8 * V : 0) + 8 * nw + 24 * (needi + needh) +
-                 64 * (4 + 4 + 8 + 8 + 4 + 8) + 8 * B_N;
+                 64 * (4 + 4 + 8 + 8 + 4 + 8) + 8 * kCtrlWords;
     }
     {
         std::lock_guard<std::mutex> lk(g_ss_mu);
@@ -1363,7 +1361,7 @@ int bserve_init(Graph* g, hipStream_t strm) {
     BatchState* s = bstate(g);
     const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
     s->clean_groups = 0;  // the server's lanes leave group 0 in no known state
-    HIP_TRY(hipMemsetAsync(w.ctrl, 0, sizeof(int64_t) * B_N, strm));
+    HIP_TRY(hipMemsetAsync(w.ctrl, 0, sizeof(int64_t) * kCtrlWords, strm));
     hipLaunchKernelGGL(batch_thr_init_k, dim3(1), dim3(64), 0, strm, s->thr, s->fmin, s->lor, 64, g->tune.delta);
     hipLaunchKernelGGL(batch_fill_k, dim3(std::max(1, g->num_cus) * 16), dim3(256), 0, strm, w.dist,
                        (int64_t)g->V * 64, w.mask[0], pruned ? w.hmask : (uint64_t*)nullptr, (int64_t)g->V,
