@@ -728,15 +728,17 @@ __global__ void batch_thr_init_k(int32_t* __restrict__ thr, int32_t* __restrict_
 
 // Resets the distance rows of the vertices in live[0..nlive) (those with an
 // in-edge, live or hidden) in groups [0, groups): the only rows a traversal
-// can write apart from its sources' entries.  16 threads per 256-byte row.
+// can write apart from its sources' entries.  16 threads per 256-byte row,
+// each writing its 16 bytes of the vertex's row in every group.
 __global__ void batch_fill_rows_k(int32_t* __restrict__ dist, const int32_t* __restrict__ live, int64_t nlive,
                                   int groups, int64_t V) {
-    const int64_t n = (int64_t)groups * nlive * 16;
+    const int64_t n = nlive * 16;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int4 inf4 = make_int4(INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX);
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += stride) {
-        const int64_t r = t >> 4, part = t & 15;
-        const int64_t g = r / nlive, v = live[r - g * nlive];
-        reinterpret_cast<int4*>(dist + (g * V + v) * 64)[part] = make_int4(INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX);
+        const int64_t v = live[t >> 4];
+        const int part = (int)(t & 15);
+        for (int g = 0; g < groups; ++g) reinterpret_cast<int4*>(dist + ((int64_t)g * V + v) * 64)[part] = inf4;
     }
 }
 
@@ -747,39 +749,70 @@ __global__ void batch_reset_srcs_k(int32_t* __restrict__ dist, const int32_t* __
     if (q < n) dist[((int64_t)(q >> 6) * V + srcs[q]) * 64 + (q & 63)] = INT32_MAX;
 }
 
-// Per-lane checksums of one group: sum of finite distances, reached count,
-// and out-degree sum of reached vertices (m_s), for lanes [0, nlanes).  With
-// `live` non-null only its nlive rows are read (the vertices a traversal
-// can reach other than the sources themselves); V is then nlive.
-__global__ void batch_checksum_k(const int32_t* __restrict__ row, const int32_t* __restrict__ dist,
-                                 int64_t V, int64_t* __restrict__ sums, int32_t* __restrict__ reached,
-                                 int64_t* __restrict__ medges, const int32_t* __restrict__ live) {
+// Per-lane checksums of groups [g0, g0 + ng), ng <= kCkG: sum of finite
+// distances, reached count and out-degree sum of reached vertices (m_s), for
+// all 64 lanes of each group.  With `live` non-null only its nrows rows are
+// read (the vertices a traversal can reach other than the sources
+// themselves); otherwise rows 0 .. nrows-1.  A wave takes two rows per step
+// and reads them in every group, so 2*ng row loads are in flight per lane;
+// the four waves of a workgroup meet in LDS, and one global atomic per
+// (group, lane, counter) and workgroup goes out.
+constexpr int kCkG = 4;
+__global__ void __launch_bounds__(256)
+batch_checksum_k(const int32_t* __restrict__ row, const int32_t* __restrict__ dist, int64_t V, int g0, int ng,
+                 int64_t* __restrict__ sums, int32_t* __restrict__ reached, int64_t* __restrict__ medges,
+                 const int32_t* __restrict__ live, int64_t nrows) {
+    __shared__ unsigned long long l_s[kCkG][64], l_m[kCkG][64];
+    __shared__ unsigned int l_c[kCkG][64];
+    for (int t = threadIdx.x; t < kCkG * 64; t += blockDim.x) {
+        l_s[t >> 6][t & 63] = 0;
+        l_m[t >> 6][t & 63] = 0;
+        l_c[t >> 6][t & 63] = 0;
+    }
+    __syncthreads();
     const int lane = lane_id();
     const int64_t wave = wave_index();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    int64_t s = 0, m = 0;
-    int c = 0;
-    // four rows per step, their loads in flight together
-    for (int64_t v0 = wave * 4; v0 < V; v0 += nwaves * 4) {
-        int d[4], deg[4];
+    int64_t s[kCkG] = {}, m[kCkG] = {};
+    int c[kCkG] = {};
+    const int32_t* __restrict__ d0 = dist + (int64_t)g0 * V * 64 + lane;
+    for (int64_t i0 = wave * 2; i0 < nrows; i0 += nwaves * 2) {
+        int64_t v[2];
+        int deg[2], d[2][kCkG];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int64_t i = v0 + k;
-            const int64_t v = i < V ? (live ? live[i] : i) : 0;
-            d[k] = i < V ? dist[v * 64 + lane] : INT32_MAX;
-            deg[k] = i < V ? row[v + 1] - row[v] : 0;
+        for (int k = 0; k < 2; ++k) {
+            const int64_t i = i0 + k;
+            v[k] = i < nrows ? (live ? live[i] : i) : -1;
+            deg[k] = v[k] >= 0 ? row[v[k] + 1] - row[v[k]] : 0;
+#pragma unroll
+            for (int gg = 0; gg < kCkG; ++gg)
+                d[k][gg] = v[k] >= 0 && gg < ng ? d0[((int64_t)gg * V + v[k]) * 64] : INT32_MAX;
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (d[k] != INT32_MAX) {
-                s += d[k];
-                c += 1;
-                m += deg[k];
-            }
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+            for (int gg = 0; gg < kCkG; ++gg)
+                if (d[k][gg] != INT32_MAX) {
+                    s[gg] += d[k][gg];
+                    c[gg] += 1;
+                    m[gg] += deg[k];
+                }
     }
-    atomicAdd((unsigned long long*)&sums[lane], (unsigned long long)s);
-    atomicAdd(&reached[lane], c);
-    atomicAdd((unsigned long long*)&medges[lane], (unsigned long long)m);
+#pragma unroll
+    for (int gg = 0; gg < kCkG; ++gg)
+        if (gg < ng && c[gg]) {
+            atomicAdd(&l_s[gg][lane], (unsigned long long)s[gg]);
+            atomicAdd(&l_m[gg][lane], (unsigned long long)m[gg]);
+            atomicAdd(&l_c[gg][lane], (unsigned int)c[gg]);
+        }
+    __syncthreads();
+    for (int t = threadIdx.x; t < ng * 64; t += blockDim.x) {
+        const int gg = t >> 6, q = t & 63;
+        if (!l_c[gg][q]) continue;
+        atomicAdd((unsigned long long*)&sums[(g0 + gg) * 64 + q], l_s[gg][q]);
+        atomicAdd(&reached[(g0 + gg) * 64 + q], (int)l_c[gg][q]);
+        atomicAdd((unsigned long long*)&medges[(g0 + gg) * 64 + q], l_m[gg][q]);
+    }
 }
 
 // ------------------------------------------------------------------ host
@@ -1149,10 +1182,10 @@ static int batch_checksums(Graph* g, const int32_t* srcs, int n, int64_t* sums, 
     HIP_TRY(hipMemsetAsync(s->sums, 0, sizeof(int64_t) * 64 * groups, g->stream));
     HIP_TRY(hipMemsetAsync(s->reach, 0, sizeof(int32_t) * 64 * groups, g->stream));
     HIP_TRY(hipMemsetAsync(s->medges, 0, sizeof(int64_t) * 64 * groups, g->stream));
-    for (int gi = 0; gi < groups; ++gi)
-        hipLaunchKernelGGL(batch_checksum_k, dim3(std::max(1, g->num_cus) * 8), dim3(256), 0, g->stream,
-                           g->d_row, w.dist + (int64_t)gi * V * 64, s->live ? s->nlive : V, s->sums + 64 * gi,
-                           s->reach + 64 * gi, s->medges + 64 * gi, (const int32_t*)s->live);
+    for (int gi = 0; gi < groups; gi += kCkG)
+        hipLaunchKernelGGL(batch_checksum_k, dim3(std::max(1, g->num_cus) * 8), dim3(256), 0, g->stream, g->d_row,
+                           (const int32_t*)w.dist, V, gi, std::min(kCkG, groups - gi), s->sums, s->reach,
+                           s->medges, (const int32_t*)s->live, s->live ? s->nlive : V);
     HIP_TRY(hipGetLastError());
     std::vector<int64_t> hs(64 * groups), hm(64 * groups);
     std::vector<int32_t> hr(64 * groups);
