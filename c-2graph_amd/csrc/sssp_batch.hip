@@ -253,7 +253,7 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
            int64_t* __restrict__ ctrl, int64_t V, int64_t nw, int groups,
            int chunk, int64_t itemcap, int64_t hitemcap, const int32_t* __restrict__ dist, int32_t* __restrict__ thr,
            int32_t* __restrict__ fmin, uint64_t* __restrict__ lor, int delta, const int32_t* __restrict__ lrow,
-           uint64_t* __restrict__ ph, uint32_t* __restrict__ kbits) {
+           uint64_t* __restrict__ ph, uint32_t* __restrict__ kbits, const uint32_t* __restrict__ sbits) {
     extern __shared__ int32_t lds_fmin[];  // [groups * 64] when delta > 0
     const int lane = lane_id();
     if (delta > 0) {
@@ -313,7 +313,10 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
         nor = forr = 0;
     };
     int cnt = 0, hcnt = 0;
-    int64_t p = 0, hp = 0;  // this lane's next item and hidden item (pass 2)
+    // pass 2: the wave's next live item (wave-uniform; pass 1 fixes only the
+    // wave's total, so any lane may count any vertex) and this lane's next
+    // hidden item
+    int64_t p = 0, hp = 0;
     bool room = true;
     // One block without delta-stepping (plain rounds, or light-edge mode
     // with kbits): abits marks vertices lowered in the last round, kbits
@@ -335,6 +338,28 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
             gcur = g;
         }
         const int64_t vb0 = (w0 - (int64_t)g * nw) * 32;  // first vertex of the block
+        if (!write && sbits && !hrow) {
+            // Pass 1 from the static bitmaps: this lane's word holds 32
+            // vertices; each with a non-empty row has one item, and only the
+            // rare rows of more than `chunk` edges need their offsets.
+            const int64_t wl = w0 - (int64_t)g * nw + lane;
+            const uint32_t sl = kbits ? da : 0u;          // light rows
+            const uint32_t sf = kbits ? dk & ~da : da;    // full rows
+            if (sl | sf) {
+                int c = __popc(sl & sbits[2 * nw + wl]) + __popc(sf & sbits[wl]);
+                const int64_t vw = vb0 + (int64_t)lane * 32;
+                for (uint32_t m = sl & sbits[3 * nw + wl]; m; m &= m - 1) {
+                    const int64_t v = vw + __builtin_ctz(m);
+                    c += (lrow[v + 1] - lrow[v] + chunk - 1) / chunk - 1;
+                }
+                for (uint32_t m = sf & sbits[nw + wl]; m; m &= m - 1) {
+                    const int64_t v = vw + __builtin_ctz(m);
+                    c += (row[v + 1] - row[v] + chunk - 1) / chunk - 1;
+                }
+                cnt += c;
+            }
+            return;
+        }
         uint64_t* __restrict__ mg = mask + (int64_t)g * V;
         uint64_t* __restrict__ pg = kbits ? ph + (int64_t)g * V : nullptr;
         for (uint64_t rest = nzw; rest;) {
@@ -423,15 +448,14 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
                     const int o = wave_owner(cpre, min(j, ctot - 1));
                     const int kk = j - __shfl(cpre, o, 64);
                     const int vo = __shfl(v, o, 64), bo = __shfl(b, o, 64), eo = __shfl(e, o, 64);
-                    const int64_t po = shfl64(p, o);
                     const uint64_t lo = (uint64_t)shfl64((int64_t)lanes, o);
                     if (j < ctot) {
                         const int x = bo + kk * chunk;
-                        items[po + kk] = make_int4(vo, x, min(x + chunk, eo), g);
-                        imask[po + kk] = lo;
+                        items[p + j] = make_int4(vo, x, min(x + chunk, eo), g);
+                        imask[p + j] = lo;
                     }
                 }
-                p += c;
+                p += ctot;
                 // every lowered vertex is consumed; kbits keeps the vertices
                 // whose full row is pending
                 const uint64_t kb = __ballot(keep);
@@ -606,15 +630,14 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
                     const int o = wave_owner(cpre, min(j, ctot - 1));
                     const int k = j - __shfl(cpre, o, 64);
                     const int vo = __shfl(v, o, 64), bo = __shfl(wk.b, o, 64), eo = __shfl(wk.e, o, 64);
-                    const int64_t po = shfl64(p, o);
                     const uint64_t lo = (uint64_t)shfl64((int64_t)wk.lanes, o);
                     if (j < ctot) {
                         const int e = bo + k * chunk;
-                        items[po + k] = make_int4(vo, e, min(e + chunk, eo), g);
-                        imask[po + k] = lo;
+                        items[p + j] = make_int4(vo, e, min(e + chunk, eo), g);
+                        imask[p + j] = lo;
                     }
                 }
-                p += c;
+                p += ctot;
                 // the slot's two words keep only the bits of vertices with
                 // lanes left
                 const uint64_t kb = __ballot(keep);
@@ -632,7 +655,7 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
     // Item ranges: wave totals meet in LDS, one atomic per queue reserves
     // the workgroup's range, each wave takes its part in wave order.
     int tot, htot = 0;
-    const int off = wave_excl_scan(cnt, &tot);
+    (void)wave_excl_scan(cnt, &tot);  // the wave's live items (any lane may hold any count)
     const int hoff = hrow ? wave_excl_scan(hcnt, &htot) : 0;
     if (lane == 0) {
         s_res[0][wid] = tot;
@@ -655,7 +678,7 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
     }
     room = base + tot <= itemcap && (!hrow || hbase + htot <= hitemcap);
     if (!room && lane == 0) atomicOr((unsigned long long*)&ctrl[B_OVF], 1ull);
-    p = base + off;
+    p = base;
     hp = hbase + hoff;
     for (int64_t blk = bb0; blk < bb1; ++blk) block(blk, true);
     flush();  // lane bookkeeping of the last group
@@ -713,6 +736,28 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
         int64_t ni = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_ITEMS], 0ull);
         int64_t nh = (int64_t)atomicAdd((unsigned long long*)&ctrl[B_HITEMS], 0ull);
         if (ni > itemcap || nh > hitemcap) { ctrl[B_OVF] = 1; ctrl[B_ITEMS] = 0; ctrl[B_HITEMS] = 0; }
+    }
+}
+
+// Static per-vertex bitmaps of the batched expand for items of `chunk` edges,
+// words [k*nw, (k+1)*nw): k = 0 the vertex's row has at least one item, 1 at
+// least two, 2 its light row has at least one, 3 at least two (light words
+// zero without light rows).  The expand counts a block's items from them
+// with popcounts, reading offsets only for rows of more than `chunk` edges.
+__global__ void batch_sbits_k(const int32_t* __restrict__ row, const int32_t* __restrict__ lrow, int64_t V,
+                              int64_t nw, int chunk, uint32_t* __restrict__ sbits) {
+    const int lane = lane_id();
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t v0 = wave_index() * 64; v0 < nw * 32; v0 += nwaves * 64) {
+        const int64_t v = v0 + lane;
+        const int d = v < V ? row[v + 1] - row[v] : 0;
+        const int l = lrow && v < V ? lrow[v + 1] - lrow[v] : 0;
+        const uint64_t b0 = __ballot(d > 0), b1 = __ballot(d > chunk), b2 = __ballot(l > 0), b3 = __ballot(l > chunk);
+        if (lane < 8) {
+            const int k = lane >> 1, h = lane & 1;
+            const uint64_t b = k == 0 ? b0 : k == 1 ? b1 : k == 2 ? b2 : b3;
+            sbits[k * nw + (v0 >> 5) + h] = (uint32_t)(b >> (32 * h));
+        }
     }
 }
 
@@ -846,6 +891,10 @@ struct BatchState {
     std::vector<uint8_t> has_in;  // [V] 1 = vertex has an in-edge
     int clean_groups = 0;
     int nsrcs = 0;
+    // static expand bitmaps (batch_sbits_k) and the (item size, light bound)
+    // they were built for
+    uint32_t* sbits = nullptr;
+    int sbits_chunk = -1, sbits_light = -1;
 };
 
 static std::mutex g_bs_mu;
@@ -870,6 +919,7 @@ void batch_release(Graph* g) {
             (void)hipFree(s->imask); (void)hipFree(s->srcs); (void)hipFree(s->sums);
             (void)hipFree(s->reach); (void)hipFree(s->medges);
             (void)hipFree(s->hitems); (void)hipFree(s->himask); (void)hipFree(s->live);
+            (void)hipFree(s->sbits);
             for (auto& e : s->ev) if (e) (void)hipEventDestroy(e);
             s->cols.release();
             delete s;
@@ -974,6 +1024,32 @@ static uint32_t* light_kbits(Graph* g) {
 
 static int relax_grid_b(const Graph* g) { return std::max(1, g->num_cus) * g->tune.grid_per_cu; }
 
+// Builds the static expand bitmaps for the current item size and light
+// bound if they are missing or were built for others.
+static int ensure_sbits(Graph* g, hipStream_t strm) {
+    BatchState* s = bstate(g);
+    if (s->nw <= 0 || g->V <= 0) return C2G_OK;
+    const int chunk = g->tune.batch_chunk, lb = light_ph(g) ? g->light_built : 0;
+    if (s->sbits && s->sbits_chunk == chunk && s->sbits_light == lb) return C2G_OK;
+    if (!s->sbits) HIP_TRY(hipMalloc(&s->sbits, sizeof(uint32_t) * 4 * (size_t)s->nw));
+    s->sbits_chunk = s->sbits_light = -1;
+    hipLaunchKernelGGL(batch_sbits_k, dim3(std::max(1, g->num_cus) * 4), dim3(256), 0, strm, g->d_row,
+                       lb ? light_rows(g) : (const int32_t*)nullptr, (int64_t)g->V, s->nw, std::max(1, chunk),
+                       s->sbits);
+    HIP_TRY(hipGetLastError());
+    s->sbits_chunk = chunk;
+    s->sbits_light = lb;
+    return C2G_OK;
+}
+
+// The static expand bitmaps when they match the current item size and light
+// bound, else null (the expand then counts from row offsets).
+static const uint32_t* valid_sbits(Graph* g) {
+    BatchState* s = bstate(g);
+    const int lb = light_ph(g) ? g->light_built : 0;
+    return s->sbits && s->sbits_chunk == g->tune.batch_chunk && s->sbits_light == lb ? s->sbits : nullptr;
+}
+
 static int expand_b(Graph* g, int groups, hipStream_t strm) {
     BatchWs& w = g->batch;
     BatchState* s = bstate(g);
@@ -987,7 +1063,8 @@ static int expand_b(Graph* g, int groups, hipStream_t strm) {
                        pruned ? w.hmask : (const uint64_t*)nullptr, s->abits, s->items, s->imask, s->hitems,
                        s->himask, w.ctrl, (int64_t)g->V, s->nw, groups, g->tune.batch_chunk, s->itemcap,
                        s->hitemcap, w.dist, s->thr, s->fmin, s->lor, delta,
-                       light_ph(g) ? light_rows(g) : (const int32_t*)nullptr, light_ph(g), light_kbits(g));
+                       light_ph(g) ? light_rows(g) : (const int32_t*)nullptr, light_ph(g), light_kbits(g),
+                       pruned ? (const uint32_t*)nullptr : valid_sbits(g));
     return C2G_OK;
 }
 
@@ -1076,6 +1153,7 @@ static int batch_run(Graph* g, const int32_t* srcs, int32_t n, c2g_stats* st, do
     const int64_t V = g->V;
     const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
     const int fill_grid = std::max(1, g->num_cus) * 16;
+    if (int rs = ensure_sbits(g, g->stream)) return rs;
     if (groups <= s->clean_groups && s->live) {
         // after a converged traversal: the last sources' entries and the
         // live rows (masks and bitmaps are already zero; hidden marks are
@@ -1416,7 +1494,9 @@ int bserve_admit(Graph* g, hipStream_t strm, int lane, int32_t src) {
 }
 
 int bserve_step(Graph* g, hipStream_t strm, uint64_t* lanes_with_work) {
-    int rc = launch_round_b(g, 1, nullptr, strm);
+    int rc = ensure_sbits(g, strm);
+    if (rc) return rc;
+    rc = launch_round_b(g, 1, nullptr, strm);
     if (rc) return rc;
     BatchWs& w = g->batch;
     HIP_TRY(hipMemcpyAsync(w.h_ctrl, w.ctrl, sizeof(int64_t) * B_N, hipMemcpyDeviceToHost, strm));
