@@ -208,3 +208,24 @@ def test_consecutive_batches_reset_only_what_they_wrote(gpu):
     refs = oracle.dijkstra_many(G["vlist"], G["elist"], G["wlist"], srcs)
     assert all(np.array_equal(d[q], refs[q]) for q in range(len(srcs)))
     assert np.array_equal(out, oracle.dijkstra(G["vlist"], G["elist"], G["wlist"], int(no_in[5])))
+
+
+def test_item_size_and_light_changes_between_batches(gpu):
+    # the expand counts a block's items from per-vertex bitmaps built for one
+    # item size and light bound (rows of more than one item, light rows of
+    # more than one item); every change between batches must rebuild them
+    G = rmat_graph(15, 16, wmax=1023)
+    deg = np.diff(G["vlist"])
+    assert (deg > 200).sum() > 10  # hub rows span many items at every size below
+    rng = np.random.default_rng(21)
+    srcs = np.concatenate([np.argsort(deg)[-40:], rng.choice(np.flatnonzero(deg > 0), 60)]).astype(np.int32)
+    refs = oracle.dijkstra_many(G["vlist"], G["elist"], G["wlist"], srcs)
+    with _graph(G) as g:
+        for chunk, light in [(64, 32), (7, 32), (7, 300), (200, 0), (1, 5), (64, 32)]:
+            g.set_tune(batch_chunk=chunk, light=light)
+            d, st = g.sssp_batch(srcs)
+            sums, reach, _ = g.sssp_batch_resident(srcs)
+            for q in range(len(srcs)):
+                assert np.array_equal(d[q], refs[q]), (chunk, light, q)
+                fin = refs[q] != INT_MAX
+                assert sums[q] == int(refs[q][fin].astype(np.int64).sum()) and reach[q] == int(fin.sum())
