@@ -60,7 +60,7 @@ class Tune(C.Structure):
                 ("delta", C.c_int32), ("profile", C.c_int32), ("grid_per_cu_single", C.c_int32),
                 ("ppr_pull", C.c_int32), ("ads_relabel", C.c_int32), ("block_single", C.c_int32),
                 ("xcd_split", C.c_int32), ("light", C.c_int32), ("batch_order", C.c_int32),
-                ("ppr_grid_per_cu", C.c_int32)]
+                ("ppr_grid_per_cu", C.c_int32), ("batch_relabel", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

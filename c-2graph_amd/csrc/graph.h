@@ -58,7 +58,8 @@ struct Graph;
 // Per-query rows out of a lane-major result array [group][V][64] of 32-bit
 // words (query q = lane q%64 of group q/64): a device tile transpose of
 // kColStage columns at a time into [kColStage][V], one pinned transfer, and a
-// parallel host copy into outs[q] (null rows are skipped).  Two staging
+// parallel host copy into outs[q] (null rows are skipped).  With `perm`,
+// vertex v's entry is read from row perm[v] of A.  Two staging
 // buffers: the transpose and transfer of one chunk overlap the host copy of
 // the previous one.  One instance per lane or state (not shared between
 // threads).
@@ -68,7 +69,8 @@ struct ColStage {
     uint32_t* host[2] = {nullptr, nullptr};  // pinned mirrors
     hipEvent_t ev[2] = {nullptr, nullptr};
     int64_t cap = 0;                         // words per staging buffer
-    int copy_out(Graph* g, hipStream_t strm, const uint32_t* A, int32_t n, void* const* outs);
+    int copy_out(Graph* g, hipStream_t strm, const uint32_t* A, int32_t n, void* const* outs,
+                 const int32_t* perm = nullptr);
     void release();
 };
 

@@ -67,6 +67,7 @@ void default_tune(Graph* g) {
     t.light = default_light(g);
     t.batch_order = 1;
     t.ppr_grid_per_cu = 1;
+    t.batch_relabel = 1;
 }
 
 // Light-edge bound: about a sixteenth of the mean weight (on R-MAT with
@@ -317,18 +318,19 @@ static int finish_open(c2g_graph* h, const c2g_opts* o, c2g_graph** out) {
 constexpr size_t kStageChunk = (size_t)32 << 20;
 static_assert(64 % kColStage == 0, "column chunks must not span lane groups");
 
-// out[k][v] = A[v][l0 + k] for k < m <= kColStage: 64-vertex tiles staged
-// through LDS so both the column read (32-byte segments) and the row writes
-// are contiguous.
+// out[k][v] = A[row(v)][l0 + k] for k < m <= kColStage, row(v) = perm[v]
+// (v when perm is null): 64-vertex tiles staged through LDS so both the
+// column read (32-byte segments) and the row writes are contiguous.
 __global__ void __launch_bounds__(256)
-extract_cols_k(const uint32_t* __restrict__ A, int64_t V, int l0, int m, uint32_t* __restrict__ out) {
+extract_cols_k(const uint32_t* __restrict__ A, const int32_t* __restrict__ perm, int64_t V, int l0, int m,
+               uint32_t* __restrict__ out) {
     __shared__ uint32_t tile[64][kColStage + 1];
     const int t = threadIdx.x;
     for (int64_t v0 = (int64_t)blockIdx.x * 64; v0 < V; v0 += (int64_t)gridDim.x * 64) {
         for (int i = t; i < 64 * m; i += 256) {
             const int r = i / m, c = i - r * m;
             const int64_t v = v0 + r;
-            tile[r][c] = v < V ? A[v * 64 + l0 + c] : 0u;
+            tile[r][c] = v < V ? A[(perm ? (int64_t)perm[v] : v) * 64 + l0 + c] : 0u;
         }
         __syncthreads();
         for (int i = t; i < 64 * m; i += 256) {
@@ -340,7 +342,8 @@ extract_cols_k(const uint32_t* __restrict__ A, int64_t V, int l0, int m, uint32_
     }
 }
 
-int ColStage::copy_out(Graph* g, hipStream_t strm, const uint32_t* A, int32_t n, void* const* outs) {
+int ColStage::copy_out(Graph* g, hipStream_t strm, const uint32_t* A, int32_t n, void* const* outs,
+                       const int32_t* perm) {
     const int64_t V = g->V;
     if (V == 0 || n <= 0) return C2G_OK;
     if (cap < kColStage * V) {
@@ -378,8 +381,8 @@ int ColStage::copy_out(Graph* g, hipStream_t strm, const uint32_t* A, int32_t n,
         const int m = chunks[c].second;
         const int gi = q0 / 64, l0 = q0 % 64;  // a chunk never spans two groups
         // buffer c&1 was last used by chunk c-2, whose host copy has finished
-        hipLaunchKernelGGL(extract_cols_k, dim3(grid), dim3(256), 0, strm, A + (int64_t)gi * V * 64, V, l0, m,
-                           dev[c & 1]);
+        hipLaunchKernelGGL(extract_cols_k, dim3(grid), dim3(256), 0, strm, A + (int64_t)gi * V * 64, perm, V, l0,
+                           m, dev[c & 1]);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(host[c & 1], dev[c & 1], sizeof(uint32_t) * V * m, hipMemcpyDeviceToHost, strm));
         HIP_TRY(hipEventRecord(ev[c & 1], strm));
@@ -579,6 +582,7 @@ int c2g_set_tune(c2g_graph* h, const c2g_tune* t) {
     if (t->light < 0) return fail(C2G_EINVAL, "light >= 0");
     if (t->batch_order != 0 && t->batch_order != 1) return fail(C2G_EINVAL, "batch_order must be 0 or 1");
     if (t->ppr_grid_per_cu < 0 || t->ppr_grid_per_cu > 64) return fail(C2G_EINVAL, "ppr_grid_per_cu in [0,64]");
+    if (t->batch_relabel != 0 && t->batch_relabel != 1) return fail(C2G_EINVAL, "batch_relabel must be 0 or 1");
     C2G_HANDLE_LOCK(h);
     if (memcmp(&h->g.tune, t, sizeof *t) != 0) {
         // Captured launch sequences hold the old values as kernel arguments.

@@ -888,13 +888,32 @@ struct BatchState {
     // only the live rows and those entries.
     int32_t* live = nullptr;
     int64_t nlive = 0;
-    std::vector<uint8_t> has_in;  // [V] 1 = vertex has an in-edge
+    std::vector<uint8_t> has_in;  // [V] 1 = vertex has an in-edge (the graph's ids)
+    bool live_rl = false;         // the live list is in relabelled ids
     int clean_groups = 0;
     int nsrcs = 0;
+    // Relabelled copy of the graph for the batched kernels
+    // (c2g_tune.batch_relabel): vertices with an in-edge first, in their
+    // order, then the rest.  rrow/radj/rhrow/rhadj mirror d_row/d_adj/
+    // d_hrow/d_hadj (radj carries the light rows after its E live entries,
+    // rlrow their absolute offsets, as light_build lays out d_adj); rnew[v]
+    // is v's new id.
+    int32_t* rrow = nullptr;
+    int2* radj = nullptr;
+    int64_t radj_cap = 0;
+    int32_t* rlrow = nullptr;
+    int32_t* rhrow = nullptr;
+    int2* rhadj = nullptr;
+    int32_t* rnew = nullptr;
+    std::vector<int32_t> newid, oldid;
+    bool rl_base = false;  // rrow, radj's live part, rhrow, rhadj, rnew built
+    int rl_light = -1;     // light bound of radj's tail and rlrow (0: none)
     // static expand bitmaps (batch_sbits_k) and the (item size, light bound)
     // they were built for
     uint32_t* sbits = nullptr;
     int sbits_chunk = -1, sbits_light = -1;
+    int sbits_rl = -1;
+    int exec_rl = -1;  // relabelled mode of the captured round sequence
 };
 
 static std::mutex g_bs_mu;
@@ -920,12 +939,152 @@ void batch_release(Graph* g) {
             (void)hipFree(s->reach); (void)hipFree(s->medges);
             (void)hipFree(s->hitems); (void)hipFree(s->himask); (void)hipFree(s->live);
             (void)hipFree(s->sbits);
+            (void)hipFree(s->rrow); (void)hipFree(s->radj); (void)hipFree(s->rlrow); (void)hipFree(s->rhrow);
+            (void)hipFree(s->rhadj); (void)hipFree(s->rnew);
             for (auto& e : s->ev) if (e) (void)hipEventDestroy(e);
             s->cols.release();
             delete s;
             g_bs.erase(g_bs.begin() + i);
             return;
         }
+}
+
+// The light bound the batched kernels run with (0: light mode off).
+static int light_key(Graph* g) { return light_rows(g) ? g->light_built : 0; }
+
+// Builds (or completes) the relabelled copy for the current light bound.
+static int rl_build(Graph* g, BatchState* s) {
+    const int64_t V = g->V, E = g->E;
+    const int L = light_key(g);
+    if (s->rl_base && s->rl_light == L) return C2G_OK;
+    if (g->batch.exec) { (void)hipGraphExecDestroy(g->batch.exec); g->batch.exec = nullptr; }
+    std::vector<uint8_t> nz((size_t)V, 0);  // v has live or hidden out-edges (bit 31 of its id in edges)
+    for (int64_t v = 0; v < V; ++v)
+        nz[v] = g->vl[v + 1] > g->vl[v] || (!g->hvl.empty() && g->hvl[v + 1] > g->hvl[v]);
+    // One CSR in new order: row offsets (plus `base`) and entries with
+    // mapped targets, keeping the entries `keep` accepts; returns the count.
+    auto build = [&](const std::vector<int32_t>& rl, const std::vector<int32_t>& ev, const std::vector<int32_t>& wv,
+                     auto keep, int64_t base, std::vector<int32_t>& nrow, std::vector<int2>& nadj) -> int64_t {
+        std::vector<int64_t> cnt((size_t)V + 1, 0);
+        parallel_for(V, [&](int64_t lo, int64_t hi, int) {
+            for (int64_t x = lo; x < hi; ++x) {
+                const int32_t v = s->oldid[x];
+                int64_t c = 0;
+                for (int32_t e = rl[v]; e < rl[v + 1]; ++e) c += keep(e) ? 1 : 0;
+                cnt[x + 1] = c;
+            }
+        }, 0, 1 << 14);
+        for (int64_t x = 0; x < V; ++x) cnt[x + 1] += cnt[x];
+        nrow.resize((size_t)V + 1);
+        for (int64_t x = 0; x <= V; ++x) nrow[x] = (int32_t)(cnt[x] + base);
+        nadj.resize((size_t)std::max<int64_t>(cnt[V], 1));
+        parallel_for(V, [&](int64_t lo, int64_t hi, int) {
+            for (int64_t x = lo; x < hi; ++x) {
+                const int32_t v = s->oldid[x];
+                int64_t k = cnt[x];
+                for (int32_t e = rl[v]; e < rl[v + 1]; ++e) {
+                    if (!keep(e)) continue;
+                    const int32_t t = ev[e], nt = s->newid[t];
+                    nadj[k++] = make_int2(nz[t] ? (int32_t)((uint32_t)nt | 0x80000000u) : nt, wv[e]);
+                }
+            }
+        }, 0, 1 << 14);
+        return cnt[V];
+    };
+    const auto all = [](int32_t) { return true; };
+    std::vector<int32_t> nrow;
+    std::vector<int2> nadj;
+    if (!s->rl_base) {
+        // vertices with an in-edge first (the rows a traversal gathers)
+        std::vector<uint8_t> hin((size_t)V, 0);
+        for (int32_t t : g->el) hin[t] = 1;
+        for (int32_t t : g->hel) hin[t] = 1;
+        s->oldid.clear();
+        s->oldid.reserve((size_t)V);
+        for (int64_t v = 0; v < V; ++v) if (hin[v]) s->oldid.push_back((int32_t)v);
+        for (int64_t v = 0; v < V; ++v) if (!hin[v]) s->oldid.push_back((int32_t)v);
+        s->newid.assign((size_t)V, 0);
+        for (int64_t x = 0; x < V; ++x) s->newid[s->oldid[x]] = (int32_t)x;
+        build(g->vl, g->el, g->wl, all, 0, nrow, nadj);
+        if (!s->rrow) HIP_TRY(hipMalloc(&s->rrow, sizeof(int32_t) * (size_t)(V + 1)));
+        HIP_TRY(hipMemcpy(s->rrow, nrow.data(), sizeof(int32_t) * (size_t)(V + 1), hipMemcpyHostToDevice));
+        if (!s->rnew) HIP_TRY(hipMalloc(&s->rnew, sizeof(int32_t) * (size_t)std::max<int64_t>(V, 1)));
+        HIP_TRY(hipMemcpy(s->rnew, s->newid.data(), sizeof(int32_t) * (size_t)V, hipMemcpyHostToDevice));
+        if (g->d_hrow && g->d_hadj) {
+            std::vector<int32_t> hrow;
+            std::vector<int2> hadj;
+            build(g->hvl, g->hel, g->hwl, all, 0, hrow, hadj);
+            if (!s->rhrow) HIP_TRY(hipMalloc(&s->rhrow, sizeof(int32_t) * (size_t)(V + 1)));
+            HIP_TRY(hipMemcpy(s->rhrow, hrow.data(), sizeof(int32_t) * (size_t)(V + 1), hipMemcpyHostToDevice));
+            if (!s->rhadj) HIP_TRY(hipMalloc(&s->rhadj, sizeof(int2) * hadj.size()));
+            HIP_TRY(hipMemcpy(s->rhadj, hadj.data(), sizeof(int2) * hadj.size(), hipMemcpyHostToDevice));
+        }
+    }
+    // the light rows (edges of weight < L) after the E live entries
+    std::vector<int32_t> lrow;
+    std::vector<int2> ladj;
+    int64_t EL = 0;
+    if (L > 0) EL = build(g->vl, g->el, g->wl, [&](int32_t e) { return g->wl[e] < L; }, E, lrow, ladj);
+    const int64_t need = std::max<int64_t>(E + EL, 1);
+    if (s->radj_cap < need) {
+        int2* nb = nullptr;
+        HIP_TRY(hipMalloc(&nb, sizeof(int2) * (size_t)need));
+        if (s->rl_base && E > 0) {
+            const hipError_t e1 = hipMemcpy(nb, s->radj, sizeof(int2) * (size_t)E, hipMemcpyDeviceToDevice);
+            if (e1 != hipSuccess) { (void)hipFree(nb); return hip_fail(e1, "relabelled rows"); }
+        }
+        (void)hipFree(s->radj);
+        s->radj = nb;
+        s->radj_cap = need;
+    }
+    if (!s->rl_base && E > 0) HIP_TRY(hipMemcpy(s->radj, nadj.data(), sizeof(int2) * (size_t)E, hipMemcpyHostToDevice));
+    if (EL > 0) HIP_TRY(hipMemcpy(s->radj + E, ladj.data(), sizeof(int2) * (size_t)EL, hipMemcpyHostToDevice));
+    if (L > 0) {
+        if (!s->rlrow) HIP_TRY(hipMalloc(&s->rlrow, sizeof(int32_t) * (size_t)(V + 1)));
+        HIP_TRY(hipMemcpy(s->rlrow, lrow.data(), sizeof(int32_t) * (size_t)(V + 1), hipMemcpyHostToDevice));
+    }
+    s->rl_base = true;
+    s->rl_light = L;
+    return C2G_OK;
+}
+
+// Whether the batched kernels run on the relabelled copy, and the arrays
+// they read (the copy's, or the graph's own).
+static bool rl_on(Graph* g) {
+    BatchState* s = bstate(g);
+    return g->tune.batch_relabel && s->rl_base && s->rl_light == light_key(g);
+}
+static const int32_t* b_row(Graph* g) { return rl_on(g) ? bstate(g)->rrow : g->d_row; }
+static const int2* b_adj(Graph* g) { return rl_on(g) ? bstate(g)->radj : g->d_adj; }
+static const int32_t* b_lrow(Graph* g) {
+    if (!light_rows(g)) return nullptr;
+    return rl_on(g) ? bstate(g)->rlrow : light_rows(g);
+}
+static const int32_t* b_hrow(Graph* g) { return rl_on(g) ? bstate(g)->rhrow : g->d_hrow; }
+static const int2* b_hadj(Graph* g) { return rl_on(g) ? bstate(g)->rhadj : g->d_hadj; }
+
+// Builds the relabelled copy when the tunable asks for it, then puts the
+// live list in the ids the kernels will run with (the relabelled ids of the
+// vertices with an in-edge are 0..nlive-1).  A change of ids leaves no group
+// in a known state.
+static int rl_sync(Graph* g) {
+    BatchState* s = bstate(g);
+    if (g->tune.batch_relabel)
+        if (int rc = rl_build(g, s)) return rc;
+    const bool on = rl_on(g);
+    if (!s->live || on == s->live_rl) return C2G_OK;
+    std::vector<int32_t> lv((size_t)s->nlive);
+    if (on) {
+        for (int64_t k = 0; k < s->nlive; ++k) lv[k] = (int32_t)k;
+    } else {
+        int64_t k = 0;
+        for (int64_t v = 0; v < g->V; ++v)
+            if (s->has_in[v]) lv[k++] = (int32_t)v;
+    }
+    s->clean_groups = 0;
+    if (s->nlive > 0) HIP_TRY(hipMemcpy(s->live, lv.data(), sizeof(int32_t) * lv.size(), hipMemcpyHostToDevice));
+    s->live_rl = on;
+    return C2G_OK;
 }
 
 static int ensure_batch(Graph* g, int groups) {
@@ -1010,8 +1169,9 @@ static int ensure_batch(Graph* g, int groups) {
         if (!lv.empty())
             HIP_TRY(hipMemcpy(s->live, lv.data(), sizeof(int32_t) * lv.size(), hipMemcpyHostToDevice));
         s->nlive = (int64_t)lv.size();
+        s->live_rl = false;
     }
-    return C2G_OK;
+    return rl_sync(g);
 }
 
 // ph when the light-edge mode runs (light rows built and ph allocated).
@@ -1029,16 +1189,17 @@ static int relax_grid_b(const Graph* g) { return std::max(1, g->num_cus) * g->tu
 static int ensure_sbits(Graph* g, hipStream_t strm) {
     BatchState* s = bstate(g);
     if (s->nw <= 0 || g->V <= 0) return C2G_OK;
-    const int chunk = g->tune.batch_chunk, lb = light_ph(g) ? g->light_built : 0;
-    if (s->sbits && s->sbits_chunk == chunk && s->sbits_light == lb) return C2G_OK;
+    const int chunk = g->tune.batch_chunk, lb = light_ph(g) ? g->light_built : 0, rl = rl_on(g);
+    if (s->sbits && s->sbits_chunk == chunk && s->sbits_light == lb && s->sbits_rl == rl) return C2G_OK;
     if (!s->sbits) HIP_TRY(hipMalloc(&s->sbits, sizeof(uint32_t) * 4 * (size_t)s->nw));
-    s->sbits_chunk = s->sbits_light = -1;
-    hipLaunchKernelGGL(batch_sbits_k, dim3(std::max(1, g->num_cus) * 4), dim3(256), 0, strm, g->d_row,
-                       lb ? light_rows(g) : (const int32_t*)nullptr, (int64_t)g->V, s->nw, std::max(1, chunk),
+    s->sbits_chunk = s->sbits_light = s->sbits_rl = -1;
+    hipLaunchKernelGGL(batch_sbits_k, dim3(std::max(1, g->num_cus) * 4), dim3(256), 0, strm, b_row(g),
+                       lb ? b_lrow(g) : (const int32_t*)nullptr, (int64_t)g->V, s->nw, std::max(1, chunk),
                        s->sbits);
     HIP_TRY(hipGetLastError());
     s->sbits_chunk = chunk;
     s->sbits_light = lb;
+    s->sbits_rl = rl;
     return C2G_OK;
 }
 
@@ -1047,7 +1208,9 @@ static int ensure_sbits(Graph* g, hipStream_t strm) {
 static const uint32_t* valid_sbits(Graph* g) {
     BatchState* s = bstate(g);
     const int lb = light_ph(g) ? g->light_built : 0;
-    return s->sbits && s->sbits_chunk == g->tune.batch_chunk && s->sbits_light == lb ? s->sbits : nullptr;
+    return s->sbits && s->sbits_chunk == g->tune.batch_chunk && s->sbits_light == lb && s->sbits_rl == (int)rl_on(g)
+               ? s->sbits
+               : nullptr;
 }
 
 static int expand_b(Graph* g, int groups, hipStream_t strm) {
@@ -1058,12 +1221,12 @@ static int expand_b(Graph* g, int groups, hipStream_t strm) {
     const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(relax_grid_b(g), (words + 255) / 256));
     const int delta = g->tune.delta;
     const size_t lds = delta > 0 ? sizeof(int32_t) * 64 * groups : 0;
-    hipLaunchKernelGGL(delta > 0 ? expand_b_k<true> : expand_b_k<false>, dim3(egrid), dim3(256), lds, strm, g->d_row,
-                       pruned ? g->d_hrow : (const int32_t*)nullptr, w.mask[0],
+    hipLaunchKernelGGL(delta > 0 ? expand_b_k<true> : expand_b_k<false>, dim3(egrid), dim3(256), lds, strm, b_row(g),
+                       pruned ? b_hrow(g) : (const int32_t*)nullptr, w.mask[0],
                        pruned ? w.hmask : (const uint64_t*)nullptr, s->abits, s->items, s->imask, s->hitems,
                        s->himask, w.ctrl, (int64_t)g->V, s->nw, groups, g->tune.batch_chunk, s->itemcap,
                        s->hitemcap, w.dist, s->thr, s->fmin, s->lor, delta,
-                       light_ph(g) ? light_rows(g) : (const int32_t*)nullptr, light_ph(g), light_kbits(g),
+                       light_ph(g) ? b_lrow(g) : (const int32_t*)nullptr, light_ph(g), light_kbits(g),
                        pruned ? (const uint32_t*)nullptr : valid_sbits(g));
     return C2G_OK;
 }
@@ -1088,7 +1251,7 @@ static int launch_round_b(Graph* g, int groups, double* relax_ms, hipStream_t st
     auto relax = [&](auto hidden, auto block) {
         constexpr bool H = decltype(hidden)::value;
         constexpr int B = decltype(block)::value;
-        hipLaunchKernelGGL((relax_b_k<H, B>), dim3(grid), dim3(B), 0, strm, H ? g->d_hadj : g->d_adj, w.dist,
+        hipLaunchKernelGGL((relax_b_k<H, B>), dim3(grid), dim3(B), 0, strm, H ? b_hadj(g) : b_adj(g), w.dist,
                            w.mask[0], H ? w.hmask : (uint64_t*)nullptr, H ? s->hitems : s->items,
                            H ? s->himask : s->imask, s->abits, w.ctrl, (int64_t)g->V, s->nw);
     };
@@ -1116,8 +1279,10 @@ static int launch_round_b(Graph* g, int groups, double* relax_ms, hipStream_t st
 static int ensure_exec_b(Graph* g, int groups) {
     BatchWs& w = g->batch;
     const int K = g->tune.iters_per_graph;
+    BatchState* s = bstate(g);
     if (w.exec && w.exec_iters == K && w.exec_grid == relax_grid_b(g) &&
-        w.exec_chunk == g->tune.batch_chunk && w.exec_groups == groups && w.exec_tb == g->tune.block)
+        w.exec_chunk == g->tune.batch_chunk && w.exec_groups == groups && w.exec_tb == g->tune.block &&
+        s->exec_rl == (int)rl_on(g))
         return C2G_OK;
     if (w.exec) { (void)hipGraphExecDestroy(w.exec); w.exec = nullptr; }
     hipGraph_t graph;
@@ -1141,6 +1306,7 @@ static int ensure_exec_b(Graph* g, int groups) {
     w.exec_chunk = g->tune.batch_chunk;
     w.exec_groups = groups;  // the captured expand scans this many groups
     w.exec_tb = g->tune.block;
+    s->exec_rl = rl_on(g);
     return C2G_OK;
 }
 
@@ -1174,12 +1340,20 @@ static int batch_run(Graph* g, const int32_t* srcs, int32_t n, c2g_stats* st, do
     }
     const int clean_before = groups <= s->clean_groups ? s->clean_groups : 0;
     s->clean_groups = 0;  // until this traversal converges
-    HIP_TRY(hipMemcpyAsync(s->srcs, srcs, sizeof(int32_t) * n, hipMemcpyHostToDevice, g->stream));
+    if (rl_on(g)) {
+        // the sources in the relabelled ids (copied before the vector goes)
+        std::vector<int32_t> rs((size_t)n);
+        for (int32_t q = 0; q < n; ++q) rs[q] = s->newid[srcs[q]];
+        HIP_TRY(hipMemcpyAsync(s->srcs, rs.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+    } else {
+        HIP_TRY(hipMemcpyAsync(s->srcs, srcs, sizeof(int32_t) * n, hipMemcpyHostToDevice, g->stream));
+    }
     s->nsrcs = n;
     hipLaunchKernelGGL(batch_thr_init_k, dim3((64 * groups + 255) / 256), dim3(256), 0, g->stream, s->thr, s->fmin,
                        s->lor, 64 * groups, g->tune.delta);
-    hipLaunchKernelGGL(batch_seed_k, dim3((n + 255) / 256), dim3(256), 0, g->stream, s->srcs, n, g->d_row,
-                       pruned ? g->d_hrow : (const int32_t*)nullptr, w.dist, w.mask[0],
+    hipLaunchKernelGGL(batch_seed_k, dim3((n + 255) / 256), dim3(256), 0, g->stream, s->srcs, n, b_row(g),
+                       pruned ? b_hrow(g) : (const int32_t*)nullptr, w.dist, w.mask[0],
                        pruned ? w.hmask : (uint64_t*)nullptr, s->abits, w.ctrl, V, s->nw);
     // Turn the seeded activations into the first round's items.
     expand_b(g, groups, g->stream);
@@ -1261,7 +1435,7 @@ static int batch_checksums(Graph* g, const int32_t* srcs, int n, int64_t* sums, 
     HIP_TRY(hipMemsetAsync(s->reach, 0, sizeof(int32_t) * 64 * groups, g->stream));
     HIP_TRY(hipMemsetAsync(s->medges, 0, sizeof(int64_t) * 64 * groups, g->stream));
     for (int gi = 0; gi < groups; gi += kCkG)
-        hipLaunchKernelGGL(batch_checksum_k, dim3(std::max(1, g->num_cus) * 8), dim3(256), 0, g->stream, g->d_row,
+        hipLaunchKernelGGL(batch_checksum_k, dim3(std::max(1, g->num_cus) * 8), dim3(256), 0, g->stream, b_row(g),
                            (const int32_t*)w.dist, V, gi, std::min(kCkG, groups - gi), s->sums, s->reach,
                            s->medges, (const int32_t*)s->live, s->live ? s->nlive : V);
     HIP_TRY(hipGetLastError());
@@ -1354,7 +1528,8 @@ int sssp_batch_group(Graph* g, const int32_t* srcs, int32_t n, int32_t* dist_out
         // Device layout [g][V][64] -> caller layout [q][V] (lane k is query perm[k]).
         std::vector<void*> outs((size_t)n);
         for (int32_t k = 0; k < n; ++k) outs[k] = dist_out + (int64_t)perm[k] * g->V;
-        if ((rc = bstate(g)->cols.copy_out(g, g->stream, (const uint32_t*)g->batch.dist, n, outs.data())))
+        if ((rc = bstate(g)->cols.copy_out(g, g->stream, (const uint32_t*)g->batch.dist, n, outs.data(),
+                                           rl_on(g) ? bstate(g)->rnew : nullptr)))
             return rc;
     }
     return C2G_OK;
@@ -1381,7 +1556,8 @@ __global__ void bserve_admit_k(const int32_t* __restrict__ row, const int32_t* _
 // Copies the distance columns of `n` lanes into out[k][V] and resets them
 // (and, on pruned graphs, the lanes' hidden-edge marks).
 __global__ void bserve_extract_k(int32_t* __restrict__ dist, uint64_t* __restrict__ hmask, int64_t V,
-                                 uint64_t lanes, const int8_t* __restrict__ slot, int32_t* __restrict__ out) {
+                                 uint64_t lanes, const int8_t* __restrict__ slot, const int32_t* __restrict__ perm,
+                                 int32_t* __restrict__ out) {
     const int lane = lane_id();
     const int64_t wave = wave_index();
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -1391,7 +1567,7 @@ __global__ void bserve_extract_k(int32_t* __restrict__ dist, uint64_t* __restric
         if (hmask && lane == 0 && (hmask[v] & lanes))
             atomicAnd((unsigned long long*)&hmask[v], ~(unsigned long long)lanes);
         if (!mine) continue;
-        const int64_t x = v * 64 + lane;
+        const int64_t x = (perm ? (int64_t)perm[v] : v) * 64 + lane;  // row of vertex v
         out[(int64_t)k * V + v] = dist[x];
         dist[x] = INT32_MAX;
     }
@@ -1439,13 +1615,17 @@ int64_t bserve_ws_bytes(Graph* g) {
     int64_t b = 0;
     {
         std::lock_guard<std::mutex> lk(g_bs_mu);
-        bool have = false;
+        bool have = false, rl = false;
         for (auto& p : g_bs)
-            if (p.first == g && p.second->groups >= 1 && p.second->itemcap >= needi && p.second->hitemcap >= needh)
-                have = true;
+            if (p.first == g) {
+                have = p.second->groups >= 1 && p.second->itemcap >= needi && p.second->hitemcap >= needh;
+                rl = p.second->rl_base && p.second->rl_light == (light_rows(g) ? g->light_built : 0);
+            }
         if (!have)
             b += 256 * V + 8 * V * (pruned ? 2 : 1) + (light_rows(g) ? 8 * V : 0) + 8 * nw + 24 * (needi + needh) +
                  64 * (4 + 4 + 8 + 8 + 4 + 8) + 8 * kCtrlWords;
+        if (g->tune.batch_relabel && !rl)  // the relabelled copy of the graph (at most)
+            b += 8 * (g->E + (light_rows(g) ? g->EL : 0)) + 12 * (V + 1) + (pruned ? 4 * (V + 1) + 8 * g->HE : 0);
     }
     {
         std::lock_guard<std::mutex> lk(g_ss_mu);
@@ -1486,9 +1666,10 @@ int bserve_admit(Graph* g, hipStream_t strm, int lane, int32_t src) {
     if (lane < 0 || lane > 63 || src < 0 || src >= g->V) return fail(C2G_EINVAL, "bad lane/source");
     BatchState* s = bstate(g);
     const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
-    hipLaunchKernelGGL(bserve_admit_k, dim3(1), dim3(1), 0, strm, g->d_row,
-                       pruned ? g->d_hrow : (const int32_t*)nullptr, g->batch.dist, g->batch.mask[0],
-                       pruned ? g->batch.hmask : (uint64_t*)nullptr, s->abits, s->thr, lane, src, g->tune.delta);
+    hipLaunchKernelGGL(bserve_admit_k, dim3(1), dim3(1), 0, strm, b_row(g),
+                       pruned ? b_hrow(g) : (const int32_t*)nullptr, g->batch.dist, g->batch.mask[0],
+                       pruned ? g->batch.hmask : (uint64_t*)nullptr, s->abits, s->thr, lane,
+                       rl_on(g) ? s->newid[src] : src, g->tune.delta);
     HIP_TRY(hipGetLastError());
     return C2G_OK;
 }
@@ -1518,7 +1699,8 @@ int bserve_extract(Graph* g, hipStream_t strm, const int* lanes, int n, int32_t*
         HIP_TRY(hipMemcpyAsync(ss->slot, slot, 64, hipMemcpyHostToDevice, strm));
         const bool pruned = g->d_hrow != nullptr && g->d_hadj != nullptr;
         hipLaunchKernelGGL(bserve_extract_k, dim3(std::max(1, g->num_cus) * 8), dim3(256), 0, strm,
-                           g->batch.dist, pruned ? g->batch.hmask : (uint64_t*)nullptr, V, lm, ss->slot, ss->out);
+                           g->batch.dist, pruned ? g->batch.hmask : (uint64_t*)nullptr, V, lm, ss->slot,
+                           rl_on(g) ? bstate(g)->rnew : (const int32_t*)nullptr, ss->out);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(ss->h_out, ss->out, sizeof(int32_t) * V * m, hipMemcpyDeviceToHost, strm));
         HIP_TRY(hipStreamSynchronize(strm));

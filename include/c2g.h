@@ -154,6 +154,14 @@ typedef struct {
                                adsorption push modes (0 = 1): their rounds
                                are small, and fewer workgroups launch and
                                retire faster (1..64)                      */
+    int32_t batch_relabel;  /* batched SSSP: 1 = run on a copy of the graph
+                               renumbered so that vertices with an in-edge
+                               (the only rows a traversal gathers) come first,
+                               which packs the gathered distance rows into
+                               less address range; sources and distance rows
+                               are mapped at the boundary, results are in
+                               the caller's ids either way; 0 = the graph's
+                               own numbering                               */
 } c2g_tune;
 
 /* ======================================================================= */

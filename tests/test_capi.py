@@ -90,7 +90,10 @@ def test_default_light_bound(wmax, want):
     G = c2g.rmat(12, 16, seed=3, wmax=wmax)
     with c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], no_device=True) as g:
         assert g.tune().light == want and g.tune().batch_order == 1 and g.tune().ppr_grid_per_cu == 1
-        g.set_tune(light=5)
-        assert g.tune().light == 5
+        assert g.tune().batch_relabel == 1
+        g.set_tune(light=5, batch_relabel=0)
+        assert g.tune().light == 5 and g.tune().batch_relabel == 0
         with pytest.raises(c2g.C2GError):
             g.set_tune(light=-1)
+        with pytest.raises(c2g.C2GError):
+            g.set_tune(batch_relabel=2)

@@ -229,3 +229,62 @@ def test_item_size_and_light_changes_between_batches(gpu):
                 assert np.array_equal(d[q], refs[q]), (chunk, light, q)
                 fin = refs[q] != INT_MAX
                 assert sums[q] == int(refs[q][fin].astype(np.int64).sum()) and reach[q] == int(fin.sum())
+
+
+def test_relabel_toggles_between_batches(gpu):
+    # the batched kernels run on a copy of the graph with the vertices that
+    # have an in-edge first (batch_relabel = 1) or on the graph itself (0):
+    # toggling between batches, with light-bound and item-size changes in
+    # between, must give Dijkstra's distances, checksums and edge counts in
+    # the caller's vertex ids, and so must the server's lanes afterwards
+    G = rmat_graph(15, 12, wmax=500)
+    V = G["V"]
+    deg = np.diff(G["vlist"])
+    indeg = np.bincount(G["elist"], minlength=V)
+    no_in = np.flatnonzero((indeg == 0) & (deg > 0))
+    rng = np.random.default_rng(33)
+    srcs = np.concatenate([no_in[:30], np.argsort(deg)[-20:], rng.choice(V, 50, replace=False)]).astype(np.int32)
+    refs = oracle.dijkstra_many(G["vlist"], G["elist"], G["wlist"], srcs)
+    m = sum(int(deg[refs[q] != INT_MAX].sum()) for q in range(len(srcs)))
+    with _graph(G) as g:
+        assert g.tune().batch_relabel == 1
+        for relabel, light, chunk in [(1, 30, 64), (0, 30, 64), (1, 30, 64), (1, 0, 64), (1, 90, 9),
+                                      (0, 90, 9), (1, 30, 64)]:
+            g.set_tune(batch_relabel=relabel, light=light, batch_chunk=chunk)
+            d, st = g.sssp_batch(srcs)
+            sums, reach, _ = g.sssp_batch_resident(srcs)
+            assert st.reached_edges == m
+            for q in range(len(srcs)):
+                assert np.array_equal(d[q], refs[q]), (relabel, light, chunk, q)
+                fin = refs[q] != INT_MAX
+                assert sums[q] == int(refs[q][fin].astype(np.int64).sum()) and reach[q] == int(fin.sum())
+        with pytest.raises(c2g.C2GError):
+            g.set_tune(batch_relabel=2)
+        for relabel in (1, 0):
+            g.set_tune(batch_relabel=relabel)
+            with c2g.Server(g, sssp_slots=-1, max_batch=16) as srv:  # lane group only
+                ids = [srv.submit(c2g.Q_SSSP, int(s)) for s in srcs[:40]]
+                for k, i in enumerate(ids):
+                    _, out = srv.poll(i, -1)
+                    assert np.array_equal(out, refs[k]), (relabel, k)
+
+
+@pytest.mark.parametrize("relabel", [0, 1])
+def test_relabel_pruned_graph(gpu, relabel):
+    # hidden lists are relabelled with the live rows; processed (pruned)
+    # vertices among the sources
+    G = rmat_graph(14, 3, wmax=255)
+    P = c2g.prune_sssp_mem(G["vlist"], G["elist"], G["wlist"])
+    assert len(P["helist"]) > 0
+    deg = np.diff(G["vlist"])
+    rng = np.random.default_rng(5)
+    srcs = rng.choice(np.flatnonzero(deg > 0), 80, replace=False).astype(np.int32)
+    srcs[:10] = P["porder"][:10]
+    refs = oracle.dijkstra_many(G["vlist"], G["elist"], G["wlist"], srcs)
+    with c2g.Graph.from_csr(P["vlist"], P["elist"], P["wlist"], hvlist=P["hvlist"], helist=P["helist"],
+                            hwlist=P["hwlist"], variant=c2g.VARIANT_SSSP_PURN, device=0) as g:
+        g.set_tune(batch_relabel=relabel)
+        for _ in range(2):
+            d, _ = g.sssp_batch(srcs)
+            for q in range(len(srcs)):
+                assert np.array_equal(d[q], refs[q]), (relabel, q)
