@@ -958,6 +958,7 @@ static int rl_build(Graph* g, BatchState* s) {
     const int L = light_key(g);
     if (s->rl_base && s->rl_light == L) return C2G_OK;
     if (g->batch.exec) { (void)hipGraphExecDestroy(g->batch.exec); g->batch.exec = nullptr; }
+    s->rl_light = -1;  // the light tail and rlrow are invalid until rebuilt below
     std::vector<uint8_t> nz((size_t)V, 0);  // v has live or hidden out-edges (bit 31 of its id in edges)
     for (int64_t v = 0; v < V; ++v)
         nz[v] = g->vl[v + 1] > g->vl[v] || (!g->hvl.empty() && g->hvl[v + 1] > g->hvl[v]);
