@@ -244,8 +244,10 @@ relax_b_k(const int2* __restrict__ adj, int32_t* __restrict__ dist, uint64_t* __
 // and join ph[g][v], and v stays active; a v with no lowered lane gets items
 // over its full row for the lanes in ph[g][v], which it clears.  DELTA:
 // compiled for delta > 0 (the plain and light-edge forms leave the
-// delta-stepping code out, which keeps them within 64 VGPRs).
-template <bool DELTA>
+// delta-stepping code out, which keeps them within 64 VGPRs).  PRUNED:
+// compiled for graphs with hidden lists (hrow non-null); the other form
+// leaves the hidden-list code out.
+template <bool DELTA, bool PRUNED>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) C2G_OCC8_SGPR
 expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, uint64_t* __restrict__ mask,
            const uint64_t* __restrict__ hmask, uint32_t* __restrict__ abits, int4* __restrict__ items,
@@ -255,6 +257,12 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
            int32_t* __restrict__ fmin, uint64_t* __restrict__ lor, int delta, const int32_t* __restrict__ lrow,
            uint64_t* __restrict__ ph, uint32_t* __restrict__ kbits, const uint32_t* __restrict__ sbits) {
     extern __shared__ int32_t lds_fmin[];  // [groups * 64] when delta > 0
+    if constexpr (!PRUNED) {
+        hrow = nullptr;
+        hmask = nullptr;
+        hitems = nullptr;
+        himask = nullptr;
+    }
     const int lane = lane_id();
     if (delta > 0) {
         for (int t = threadIdx.x; t < groups * 64; t += blockDim.x) lds_fmin[t] = INT32_MAX;
@@ -285,10 +293,12 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
     __shared__ unsigned long long s_acc[3];  // front, edges, lane edges
     __shared__ unsigned long long s_lor[2][2];
     __shared__ int64_t s_res[2][8];          // per wave: items, hidden items
+    __shared__ uint32_t s_keep[4][64];       // per wave: kept bits of the block's words (pass 2)
     __shared__ int64_t s_base[2];
     const int wid = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     const int g0 = (int)((min(nblk - 1, (int64_t)blockIdx.x * nwv * per) << 6) / nw);
     if (threadIdx.x < 3) s_acc[threadIdx.x] = 0;
+    s_keep[threadIdx.x >> 6][threadIdx.x & 63] = 0;
     if (threadIdx.x < 4) s_lor[threadIdx.x >> 1][threadIdx.x & 1] = 0;
     __syncthreads();
     uint64_t nor = 0, forr = 0;
@@ -362,6 +372,85 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
         }
         uint64_t* __restrict__ mg = mask + (int64_t)g * V;
         uint64_t* __restrict__ pg = kbits ? ph + (int64_t)g * V : nullptr;
+        if (write && !PRUNED) {
+            // Pass 2 over the block's set bits 64 at a time: set bit j of the
+            // block (in vertex order) goes to lane j % 64 of step j / 64, so a
+            // block whose words hold few bits takes one step per 64 vertices
+            // rather than one per pair of nonzero words.  Kept bits meet per
+            // word in LDS; each lane then rewrites its own word.
+            uint32_t* __restrict__ kw = s_keep[wid];
+            int nbit;
+            const int bpre = wave_excl_scan(__popc(myword), &nbit);
+            for (int s0 = 0; s0 < nbit; s0 += kWave) {
+                const int j = min(s0 + lane, nbit - 1);
+                const bool on = s0 + lane < nbit;
+                const int o = wave_owner(bpre, j);  // the word (lane) holding bit j
+                const uint32_t wo = (uint32_t)__shfl((int)myword, o, 64);
+                const uint32_t dao = (uint32_t)__shfl((int)da, o, 64);
+                const int bp = nth_set_bit(wo, j - __shfl(bpre, o, 64));
+                const int v = (int)(vb0 + (int64_t)o * 32 + bp);
+                const bool dirty = (dao >> bp) & 1u;
+                uint64_t m = 0, h = 0;
+                int r0 = 0, r1 = 0, l0 = 0, l1 = 0;
+                if (on) {
+                    m = mg[v];
+                    if (pg) {
+                        h = pg[v];
+                        l0 = lrow[v];
+                        l1 = lrow[v + 1];
+                    }
+                    r0 = row[v];
+                    r1 = row[v + 1];
+                }
+                // light mode: a lowered vertex takes its light row, a kept
+                // one its full row
+                const bool lt = pg && dirty;
+                const int b = lt ? l0 : r0, e = lt ? l1 : r1;
+                int c = on && e > b ? (e - b + chunk - 1) / chunk : 0;
+                bool keep = false;
+                uint64_t lanes = 0;
+                if (on) {
+                    lanes = dirty ? m : h;
+                    const uint64_t pend = lt ? h | m : 0;
+                    if (dirty && m) mg[v] = 0;
+                    if (pg && pend != h) pg[v] = pend;
+                    keep = pend != 0;
+                    nor |= lanes;
+                    forr |= pend;
+                    if (lanes) front += 1;
+                    edges += e - b;
+                    ledges += (int64_t)__popcll(lanes) * (e - b);
+                }
+                if (!room) c = 0;
+                int ctot;
+                const int cpre = wave_excl_scan(c, &ctot);
+                for (int j0 = 0; j0 < ctot; j0 += kWave) {
+                    const int jj = j0 + lane;
+                    const int ow = wave_owner(cpre, min(jj, ctot - 1));
+                    const int kk = jj - __shfl(cpre, ow, 64);
+                    const int vo = __shfl(v, ow, 64), bo = __shfl(b, ow, 64), eo = __shfl(e, ow, 64);
+                    const uint64_t lo = (uint64_t)shfl64((int64_t)lanes, ow);
+                    if (jj < ctot) {
+                        const int x = bo + kk * chunk;
+                        items[p + jj] = make_int4(vo, x, min(x + chunk, eo), g);
+                        imask[p + jj] = lo;
+                    }
+                }
+                p += ctot;
+                if (keep) atomicOr(&kw[o], 1u << bp);
+            }
+            // the wave's LDS operations complete in order: every kept bit of
+            // this block is in kw before a lane reads its word
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t nk = kw[lane];
+            kw[lane] = 0u;
+            // every lowered vertex is consumed; kbits keeps the vertices whose
+            // full row is pending
+            if (da) abits[w0 + lane] = 0u;
+            if (kbits && nk != dk) kbits[w0 + lane] = nk;
+            return;
+        }
         for (uint64_t rest = nzw; rest;) {
             int k[2 * kS];
             uint32_t wd[2 * kS], wk[2 * kS];
@@ -1222,7 +1311,9 @@ static int expand_b(Graph* g, int groups, hipStream_t strm) {
     const int egrid = (int)std::max<int64_t>(1, std::min<int64_t>(relax_grid_b(g), (words + 255) / 256));
     const int delta = g->tune.delta;
     const size_t lds = delta > 0 ? sizeof(int32_t) * 64 * groups : 0;
-    hipLaunchKernelGGL(delta > 0 ? expand_b_k<true> : expand_b_k<false>, dim3(egrid), dim3(256), lds, strm, b_row(g),
+    auto kern = pruned ? (delta > 0 ? expand_b_k<true, true> : expand_b_k<false, true>)
+                       : (delta > 0 ? expand_b_k<true, false> : expand_b_k<false, false>);
+    hipLaunchKernelGGL(kern, dim3(egrid), dim3(256), lds, strm, b_row(g),
                        pruned ? b_hrow(g) : (const int32_t*)nullptr, w.mask[0],
                        pruned ? w.hmask : (const uint64_t*)nullptr, s->abits, s->items, s->imask, s->hitems,
                        s->himask, w.ctrl, (int64_t)g->V, s->nw, groups, g->tune.batch_chunk, s->itemcap,

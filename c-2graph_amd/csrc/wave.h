@@ -87,6 +87,21 @@ __device__ __forceinline__ int wave_owner(int pre, int k) {
     return o;
 }
 
+// Position of the r-th set bit (r = 0 for the lowest) of w; r < popcount(w).
+__device__ __forceinline__ int nth_set_bit(uint32_t w, int r) {
+    int pos = 0;
+#pragma unroll
+    for (int half = 16; half > 0; half >>= 1) {
+        const int c = __popc(w & ((1u << half) - 1u));
+        if (r >= c) {
+            r -= c;
+            w >>= half;
+            pos += half;
+        }
+    }
+    return pos;
+}
+
 // Queue reservation for a whole workgroup with one atomic: each lane asks
 // for `cnt` entries; returns the lane's first position, and *wave_end the end
 // of its wave's range (for a capacity check).  Wave totals meet in s_tot
