@@ -372,192 +372,105 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
         }
         uint64_t* __restrict__ mg = mask + (int64_t)g * V;
         uint64_t* __restrict__ pg = kbits ? ph + (int64_t)g * V : nullptr;
-        if (write && !PRUNED) {
-            // Pass 2 over the block's set bits 64 at a time: set bit j of the
-            // block (in vertex order) goes to lane j % 64 of step j / 64, so a
-            // block whose words hold few bits takes one step per 64 vertices
-            // rather than one per pair of nonzero words.  Kept bits meet per
-            // word in LDS; each lane then rewrites its own word.
-            uint32_t* __restrict__ kw = s_keep[wid];
-            int nbit;
-            const int bpre = wave_excl_scan(__popc(myword), &nbit);
-            for (int s0 = 0; s0 < nbit; s0 += kWave) {
-                const int j = min(s0 + lane, nbit - 1);
-                const bool on = s0 + lane < nbit;
-                const int o = wave_owner(bpre, j);  // the word (lane) holding bit j
-                const uint32_t wo = (uint32_t)__shfl((int)myword, o, 64);
-                const uint32_t dao = (uint32_t)__shfl((int)da, o, 64);
-                const int bp = nth_set_bit(wo, j - __shfl(bpre, o, 64));
-                const int v = (int)(vb0 + (int64_t)o * 32 + bp);
-                const bool dirty = (dao >> bp) & 1u;
-                uint64_t m = 0, h = 0;
-                int r0 = 0, r1 = 0, l0 = 0, l1 = 0;
-                if (on) {
-                    m = mg[v];
-                    if (pg) {
-                        h = pg[v];
-                        l0 = lrow[v];
-                        l1 = lrow[v + 1];
-                    }
-                    r0 = row[v];
-                    r1 = row[v + 1];
+        // The block's set bits 64 at a time: set bit j of the block (in
+        // vertex order) goes to lane j % 64 of step j / 64, so a block whose
+        // words hold few bits takes one step per 64 vertices rather than one
+        // per pair of nonzero words.  Pass 1 (write false) counts items (only
+        // the wave's totals matter); pass 2 writes them at wave-uniform
+        // positions, updates mask and ph, and collects kept bits per word in
+        // LDS, from where each lane rewrites its own word at the end.
+        uint32_t* __restrict__ kw = s_keep[wid];
+        int nbit;
+        const int bpre = wave_excl_scan(__popc(myword), &nbit);
+        for (int s0 = 0; s0 < nbit; s0 += kWave) {
+            const int j = min(s0 + lane, nbit - 1);
+            const bool on = s0 + lane < nbit;
+            const int o = wave_owner(bpre, j);  // the word (lane) holding bit j
+            const uint32_t wo = (uint32_t)__shfl((int)myword, o, 64);
+            const uint32_t dao = (uint32_t)__shfl((int)da, o, 64);
+            const int bp = nth_set_bit(wo, j - __shfl(bpre, o, 64));
+            const int v = (int)(vb0 + (int64_t)o * 32 + bp);
+            const bool dirty = (dao >> bp) & 1u;
+            // light mode: a lowered vertex takes its light row, a kept one
+            // its full row
+            const bool lt = pg && dirty;
+            uint64_t m = 0, h = 0, hm = 0;
+            int b = 0, e = 0, h0 = 0, h1 = 0;
+            if (on) {
+                if (write || (hrow && dirty)) m = mg[v];
+                if (write && pg) h = pg[v];
+                const int32_t* rr = lt ? lrow : row;
+                b = rr[v];
+                e = rr[v + 1];
+                if (hrow && dirty) {
+                    h0 = hrow[v];
+                    h1 = hrow[v + 1];
                 }
-                // light mode: a lowered vertex takes its light row, a kept
-                // one its full row
-                const bool lt = pg && dirty;
-                const int b = lt ? l0 : r0, e = lt ? l1 : r1;
-                int c = on && e > b ? (e - b + chunk - 1) / chunk : 0;
-                bool keep = false;
-                uint64_t lanes = 0;
-                if (on) {
-                    lanes = dirty ? m : h;
-                    const uint64_t pend = lt ? h | m : 0;
-                    if (dirty && m) mg[v] = 0;
-                    if (pg && pend != h) pg[v] = pend;
-                    keep = pend != 0;
-                    nor |= lanes;
-                    forr |= pend;
-                    if (lanes) front += 1;
-                    edges += e - b;
-                    ledges += (int64_t)__popcll(lanes) * (e - b);
+            }
+            if (hrow && on && dirty) hm = m & hmask[(int64_t)g * V + v];
+            int c = on && e > b ? (e - b + chunk - 1) / chunk : 0;
+            int hc = hm && h1 > h0 ? (h1 - h0 + chunk - 1) / chunk : 0;
+            if (!write) {
+                cnt += c;
+                hcnt += hc;
+                continue;
+            }
+            bool keep = false;
+            uint64_t lanes = 0;
+            if (on) {
+                lanes = dirty ? m : h;
+                const uint64_t pend = lt ? h | m : 0;
+                if (dirty && m) mg[v] = 0;
+                if (pg && pend != h) pg[v] = pend;
+                keep = pend != 0;
+                nor |= lanes;
+                forr |= pend;
+                if (lanes) front += 1;
+                edges += e - b;
+                ledges += (int64_t)__popcll(lanes) * (e - b);
+                if (hm) {
+                    edges += h1 - h0;
+                    ledges += (int64_t)__popcll(hm) * (h1 - h0);
                 }
-                if (!room) c = 0;
-                int ctot;
-                const int cpre = wave_excl_scan(c, &ctot);
-                for (int j0 = 0; j0 < ctot; j0 += kWave) {
+            }
+            if (!room) c = hc = 0;
+            // items written by the whole wave: item jj of the step belongs to
+            // the lane whose range of the step's prefix holds it, so a hub
+            // row's thousands of items take thousands/64 steps, not one
+            // lane's serial loop
+            auto emit = [&](int n, int b0, int e0, uint64_t lm, int4* __restrict__ out,
+                            uint64_t* __restrict__ outm, int64_t& pos) {
+                int ntot;
+                const int npre = wave_excl_scan(n, &ntot);
+                for (int j0 = 0; j0 < ntot; j0 += kWave) {
                     const int jj = j0 + lane;
-                    const int ow = wave_owner(cpre, min(jj, ctot - 1));
-                    const int kk = jj - __shfl(cpre, ow, 64);
-                    const int vo = __shfl(v, ow, 64), bo = __shfl(b, ow, 64), eo = __shfl(e, ow, 64);
-                    const uint64_t lo = (uint64_t)shfl64((int64_t)lanes, ow);
-                    if (jj < ctot) {
+                    const int ow = wave_owner(npre, min(jj, ntot - 1));
+                    const int kk = jj - __shfl(npre, ow, 64);
+                    const int vo = __shfl(v, ow, 64), bo = __shfl(b0, ow, 64), eo = __shfl(e0, ow, 64);
+                    const uint64_t lo = (uint64_t)shfl64((int64_t)lm, ow);
+                    if (jj < ntot) {
                         const int x = bo + kk * chunk;
-                        items[p + jj] = make_int4(vo, x, min(x + chunk, eo), g);
-                        imask[p + jj] = lo;
+                        out[pos + jj] = make_int4(vo, x, min(x + chunk, eo), g);
+                        outm[pos + jj] = lo;
                     }
                 }
-                p += ctot;
-                if (keep) atomicOr(&kw[o], 1u << bp);
-            }
-            // the wave's LDS operations complete in order: every kept bit of
-            // this block is in kw before a lane reads its word
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            const uint32_t nk = kw[lane];
-            kw[lane] = 0u;
-            // every lowered vertex is consumed; kbits keeps the vertices whose
-            // full row is pending
-            if (da) abits[w0 + lane] = 0u;
-            if (kbits && nk != dk) kbits[w0 + lane] = nk;
-            return;
+                pos += ntot;
+            };
+            emit(c, b, e, lanes, items, imask, p);
+            if (hrow) emit(hc, h0, h1, hm, hitems, himask, hp);
+            if (keep) atomicOr(&kw[o], 1u << bp);
         }
-        for (uint64_t rest = nzw; rest;) {
-            int k[2 * kS];
-            uint32_t wd[2 * kS], wk[2 * kS];
-            bool has[2 * kS];
-#pragma unroll
-            for (int j = 0; j < 2 * kS; ++j) {
-                has[j] = rest != 0;
-                k[j] = has[j] ? __builtin_ctzll(rest) : 0;
-                if (has[j]) rest &= rest - 1;
-                wd[j] = has[j] ? (uint32_t)__builtin_amdgcn_readlane((int)da, k[j]) : 0u;
-                wk[j] = has[j] ? (uint32_t)__builtin_amdgcn_readlane((int)dk, k[j]) : 0u;
-            }
-            bool on[kS], dirty[kS];
-            int vv[kS], r0[kS], r1[kS], l0[kS], l1[kS];
-            uint64_t mm[kS], hh[kS];
-#pragma unroll
-            for (int sl = 0; sl < kS; ++sl) {
-                const int jw = 2 * sl + half;
-                dirty[sl] = (wd[jw] >> bit) & 1u;
-                on[sl] = dirty[sl] || ((wk[jw] >> bit) & 1u);
-                vv[sl] = (int)(vb0 + (int64_t)k[jw] * 32 + bit);
-                mm[sl] = hh[sl] = 0;
-                r0[sl] = r1[sl] = l0[sl] = l1[sl] = 0;
-                if (on[sl]) {
-                    const int v = vv[sl];
-                    if (write || (hrow && dirty[sl])) mm[sl] = mg[v];
-                    if (write && pg) hh[sl] = pg[v];
-                    if (pg) {
-                        l0[sl] = lrow[v];
-                        l1[sl] = lrow[v + 1];
-                    }
-                    r0[sl] = row[v];
-                    r1[sl] = row[v + 1];
-                }
-            }
-#pragma unroll
-            for (int sl = 0; sl < kS; ++sl) {
-                const int v = vv[sl];
-                // light mode: a lowered vertex takes its light row, a kept
-                // one its full row
-                const bool lt = pg && dirty[sl];
-                const int b = lt ? l0[sl] : r0[sl], e = lt ? l1[sl] : r1[sl];
-                int c = on[sl] && e > b ? (e - b + chunk - 1) / chunk : 0;
-                uint64_t hm = 0;
-                if (hrow && dirty[sl]) hm = mm[sl] & hmask[(int64_t)g * V + v];
-                const int hc = hm ? (hrow[v + 1] - hrow[v] + chunk - 1) / chunk : 0;
-                if (!write) {
-                    cnt += c;
-                    hcnt += hc;
-                    continue;
-                }
-                bool keep = false;
-                uint64_t lanes = 0;
-                if (on[sl]) {
-                    const uint64_t m = mm[sl], h = hh[sl];
-                    lanes = dirty[sl] ? m : h;
-                    const uint64_t pend = lt ? h | m : 0;
-                    if (dirty[sl] && m) mg[v] = 0;
-                    if (pg && pend != h) pg[v] = pend;
-                    keep = pend != 0;
-                    nor |= lanes;
-                    forr |= pend;
-                    if (lanes) front += 1;
-                    edges += e - b;
-                    ledges += (int64_t)__popcll(lanes) * (e - b);
-                    if (hm) {
-                        const int h0 = hrow[v], h1 = hrow[v + 1];
-                        edges += h1 - h0;
-                        ledges += (int64_t)__popcll(hm) * (h1 - h0);
-                        if (room)
-                            for (int x = h0; x < h1; x += chunk, ++hp) {
-                                hitems[hp] = make_int4(v, x, min(x + chunk, h1), g);
-                                himask[hp] = hm;
-                            }
-                    }
-                }
-                if (!room) c = 0;
-                // the slot's live items, written by the whole wave (a hub
-                // row's items take rows/64 steps, not one lane's loop)
-                int ctot;
-                const int cpre = wave_excl_scan(c, &ctot);
-                for (int j0 = 0; j0 < ctot; j0 += kWave) {
-                    const int j = j0 + lane;
-                    const int o = wave_owner(cpre, min(j, ctot - 1));
-                    const int kk = j - __shfl(cpre, o, 64);
-                    const int vo = __shfl(v, o, 64), bo = __shfl(b, o, 64), eo = __shfl(e, o, 64);
-                    const uint64_t lo = (uint64_t)shfl64((int64_t)lanes, o);
-                    if (j < ctot) {
-                        const int x = bo + kk * chunk;
-                        items[p + j] = make_int4(vo, x, min(x + chunk, eo), g);
-                        imask[p + j] = lo;
-                    }
-                }
-                p += ctot;
-                // every lowered vertex is consumed; kbits keeps the vertices
-                // whose full row is pending
-                const uint64_t kb = __ballot(keep);
-                if (lane == 0 && has[2 * sl]) {
-                    if (wd[2 * sl]) abits[w0 + k[2 * sl]] = 0u;
-                    if (kbits && (uint32_t)kb != wk[2 * sl]) kbits[w0 + k[2 * sl]] = (uint32_t)kb;
-                }
-                if (lane == 32 && has[2 * sl + 1]) {
-                    if (wd[2 * sl + 1]) abits[w0 + k[2 * sl + 1]] = 0u;
-                    if (kbits && (uint32_t)(kb >> 32) != wk[2 * sl + 1]) kbits[w0 + k[2 * sl + 1]] = (uint32_t)(kb >> 32);
-                }
-            }
-        }
+        if (!write) return;
+        // the wave's LDS operations complete in order: every kept bit of
+        // this block is in kw before a lane reads its word
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t nk = kw[lane];
+        kw[lane] = 0u;
+        // every lowered vertex is consumed; kbits keeps the vertices whose
+        // full row is pending
+        if (da) abits[w0 + lane] = 0u;
+        if (kbits && nk != dk) kbits[w0 + lane] = nk;
     };
     // One block with delta-stepping: pass 1 (write false) counts items and
     // records lane summaries, pass 2 (write true) writes items and updates
@@ -768,7 +681,7 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
     room = base + tot <= itemcap && (!hrow || hbase + htot <= hitemcap);
     if (!room && lane == 0) atomicOr((unsigned long long*)&ctrl[B_OVF], 1ull);
     p = base;
-    hp = hbase + hoff;
+    hp = hbase + (DELTA ? hoff : 0);  // the bitmap walk writes hidden items per wave, the delta form per lane
     for (int64_t blk = bb0; blk < bb1; ++blk) block(blk, true);
     flush();  // lane bookkeeping of the last group
     front = wave_sum64(front);
