@@ -323,9 +323,10 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
         nor = forr = 0;
     };
     int cnt = 0, hcnt = 0;
-    // pass 2: the wave's next live item (wave-uniform; pass 1 fixes only the
-    // wave's total, so any lane may count any vertex) and this lane's next
-    // hidden item
+    // pass 2: the wave's next live item and next hidden item (wave-uniform;
+    // pass 1 fixes only the wave's totals, so any lane may count any vertex),
+    // except that the delta-stepping form writes hidden items per lane, at
+    // this lane's next hidden item
     int64_t p = 0, hp = 0;
     bool room = true;
     // One block without delta-stepping (plain rounds, or light-edge mode
