@@ -156,9 +156,10 @@ typedef struct {
                                retire faster (1..64)                      */
     int32_t batch_relabel;  /* batched SSSP: 1 = run on a copy of the graph
                                renumbered so that vertices with an in-edge
-                               (the only rows a traversal gathers) come first,
-                               which packs the gathered distance rows into
-                               less address range; sources and distance rows
+                               (the only rows a traversal writes) come first,
+                               which packs those rows and the set bits of the
+                               expand's bitmaps into the low ids (about 4 %
+                               faster on R-MAT); sources and distance rows
                                are mapped at the boundary, results are in
                                the caller's ids either way; 0 = the graph's
                                own numbering                               */
