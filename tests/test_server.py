@@ -222,3 +222,19 @@ def test_server_owns_the_handle():
         finally:
             s.stop()
         g.set_tune(chunk=16)  # released at stop
+
+
+def test_group_lane_charges_the_relabelled_copy():
+    # the 64-lane group runs on a relabelled copy of the graph (row offsets,
+    # edge entries and the new ids: 8 bytes per edge and 12 per vertex on a
+    # graph without light rows or hidden lists); admission charges it only
+    # while batch_relabel is on
+    G = rmat_graph(12, 9, wmax=64, want_f=True)
+    E, V = len(G["elist"]), G["V"]
+    charged = {}
+    for relabel in (1, 0):
+        with c2g.Graph.from_csr(G["vlist"], G["elist"], G["wlist"], G["flist"], no_device=True) as g:
+            g.set_tune(batch_relabel=relabel)
+            with c2g.Server(g, dry_device=True, fake_free_hbm=BIG, ads_labels=4, ads_iters=5) as s:
+                charged[relabel] = s.stats()["lane_bytes"][1]
+    assert charged[1] - charged[0] == 8 * E + 12 * (V + 1)
