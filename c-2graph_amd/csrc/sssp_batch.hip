@@ -63,9 +63,9 @@ enum : int {
 static_assert(B_N <= kCtrlTree, "control fields overlap the completion tree");
 
 constexpr int kJ = 8;  // edges whose row loads are in flight together
-// expand: vertices per lane whose loads are in flight together.  One, so
-// that the kernel fits 64 VGPRs and runs 8 waves per SIMD (two slots need 77
-// VGPRs, 6 waves; both take the same time within 4 %).
+// expand, delta-stepping form: vertices per lane whose loads are in flight
+// together.  One, so that the kernel fits 64 VGPRs and runs 8 waves per SIMD
+// (two slots need 77 VGPRs, 6 waves; both took the same time within 4 %).
 constexpr int kS = 1;
 constexpr int kIdMaskB = 0x7fffffff;  // bit 31 of an edge target = target has edges
 
@@ -270,13 +270,13 @@ expand_b_k(const int32_t* __restrict__ row, const int32_t* __restrict__ hrow, ui
     }
     int64_t front = 0, edges = 0, ledges = 0;
     // Work unit: a block of 64 bitmap words (2048 vertices of one group; nw
-    // is a multiple of 64).  The wave loads the block's words in one step and
-    // walks its nonzero words 2*kS at a time, lanes 0-31 on one word and 32-63
-    // on the other in each of kS slots, so each active vertex has a lane of
-    // its own, the loads of a lane's kS vertices are in flight together, and
-    // masks and row offsets arrive in coalesced loads.  Pass 1 counts each lane's
-    // items, one atomic reserves the block's item range, pass 2 writes the
-    // items and takes the expanded lanes out of the mask.
+    // is a multiple of 64).  The wave loads the block's words in one step;
+    // without delta-stepping it walks the block's set bits 64 at a time (one
+    // lane per active vertex, see block_bits), the delta-stepping form walks
+    // its nonzero words 2*kS at a time, lanes 0-31 on one word and 32-63 on
+    // the other.  Either way masks and row offsets arrive in coalesced loads.
+    // Pass 1 counts items, one atomic reserves the item range, pass 2 writes
+    // the items and takes the expanded lanes out of the mask.
     const int64_t nblk = (nw * groups) >> 6;
     const int64_t nwav = ((int64_t)gridDim.x * blockDim.x) >> 6;
     const int half = lane >> 5, bit = lane & 31;
