@@ -156,7 +156,7 @@ def measured_traffic(kernel, args, tune):
     if t.get("kernel") != kernel:
         return None, {"note": f"counter summary is for {t.get('kernel')}"}
     prov = {"profile": t.get("source"), "kernel_source_sha256": t.get("kernel_source_sha256"),
-            "counters": t.get("counters")}
+            "counters": t.get("counters"), "dispatches": t.get("dispatches")}
     if t.get("kernel_source_sha256") != _sha256(KERNEL_SRC):
         prov["note"] = "stale: the kernel source changed since the counter run"
         return None, prov
@@ -165,6 +165,18 @@ def measured_traffic(kernel, args, tune):
         return None, prov
     prov["light"] = t.get("light")
     return t["traffic_bytes_per_launch"], prov
+
+
+def traffic_for_run(traffic, prov, launches):
+    """Traffic per launch of a run with `launches` launches: the round count
+    of a traversal varies by a round or two between runs while its bytes do
+    not, so the counter runs' traffic per traversal (per-launch figure times
+    their dispatch count) is spread over this run's launches."""
+    if not traffic or not prov.get("dispatches"):
+        return traffic, prov
+    per_trav = traffic * prov["dispatches"]
+    prov = dict(prov, traffic_per_traversal=int(per_trav))
+    return int(per_trav / max(1, launches)), prov
 
 
 def ingest_counters(dirs, out_path, light):
@@ -781,7 +793,7 @@ def main():
         algo_bytes = 8 * sp.csr_edges + 8 * sp.frontier_total + 4 * sp.edges_relaxed + 4 * sp.improvements
         achieved = algo_bytes / (sp.kernel_ms * 1e-3) / 1e9
         kname = f"relax_b_k<false, {g.tune().block}>"
-        traffic, prov = measured_traffic(kname, args, g.tune().as_dict())
+        traffic, prov = traffic_for_run(*measured_traffic(kname, args, g.tune().as_dict()), sp.iterations)
         algo_per_launch = algo_bytes / max(1, sp.iterations)
         extra["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
